@@ -1,0 +1,47 @@
+"""ctypes declarations of include/chronos_amg.h."""
+import ctypes as C
+
+_VP = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+_F64 = C.c_double
+_I32P = C.POINTER(C.c_int32)
+_F64P = C.POINTER(C.c_double)
+
+
+class ChrCsr(C.Structure):
+    _fields_ = [("nrows", _I32), ("ncols", _I32), ("rowptr", _I32P), ("col", _I32P), ("val", _F64P)]
+
+
+class ChrLevel(C.Structure):
+    _fields_ = [("A", ChrCsr), ("P", ChrCsr), ("G", ChrCsr), ("omega", _F64), ("nupre", _I32),
+                ("nupost", _I32)]
+
+
+class ChrHierStats(C.Structure):
+    _fields_ = [("nlev", _I32), ("precision", _I32), ("n", _I64 * 16), ("vcycle_bytes", _F64),
+                ("level_bytes", _F64 * 16), ("pcg_iter_bytes", _F64), ("device_bytes", _F64)]
+
+
+# Every symbol declared in include/chronos_amg.h (checked by tests/test_abi.py).
+EXPORTS = {
+    "chr_last_error": (C.c_char_p, []),
+    "chr_init": (C.c_int, [_I32]),
+    "chr_hier_create": (C.c_int, [_I32, C.POINTER(ChrLevel), _I32, C.POINTER(_VP)]),
+    "chr_hier_destroy": (C.c_int, [_VP]),
+    "chr_hier_get_stats": (C.c_int, [_VP, C.POINTER(ChrHierStats)]),
+    "chr_vcycle_apply": (C.c_int, [_VP, _I32, _VP, _I64, _VP, _I64, _I32]),
+    "chr_mat_create": (C.c_int, [C.POINTER(ChrCsr), C.POINTER(_VP)]),
+    "chr_mat_destroy": (C.c_int, [_VP]),
+    "chr_spmv": (C.c_int, [_VP, _VP, _VP, _I32]),
+    "chr_pcg_solve": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _F64, _I32, _I32P, _F64P, _I32P]),
+    "chr_time_vcycle": (C.c_int, [_VP, _I32, _F64P]),
+    "chr_time_spmv": (C.c_int, [_VP, _I32, C.c_char, _I32, _F64P, _F64P]),
+}
+
+
+def declare(lib):
+    for name, (res, args) in EXPORTS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args

@@ -1,0 +1,177 @@
+"""Python consumer of the HIP solve phase, mirroring ChronosLab's solve-phase interface.
+
+Reference interface (paths relative to the reference root) and the mirror here:
+  M = @(x) AMG_Vcycle(AMG_prec, A, x)            AMGPreconditioner.__call__(b)
+      (sources/utils/computeAMG.m:49)
+  [M, T_setup] = computeAMG(A, TV0, param, verb)  computeAMG(A, param) -> (M, T_setup)
+      (sources/utils/computeAMG.m:1-51)
+  [x,iter,resvec,flag] = PCG_CJ(prodM, prodA, b, x0, itmax, tol)
+      (sources/LinearSolver/PCG_CJ.m:1-73)     PCG_CJ(M, A, b, x0, itmax, tol)
+
+All compute runs in libchronos_amg.so on the GPU; a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import hierarchy as H
+from ._amg_sigs import ChrCsr, ChrHierStats, ChrLevel
+from ._lib import load_amg_lib
+
+FP64 = 0
+FP32_CYCLE = 1
+PTR_HOST = 0
+PTR_DEVICE = 1
+
+_I32P = C.POINTER(C.c_int32)
+_F64P = C.POINTER(C.c_double)
+
+
+def _lib():
+    return load_amg_lib()
+
+
+def _check(rc: int, what: str = ""):
+    if rc != 0:
+        raise RuntimeError(f"chronos_amg error {rc} {what}: {_lib().chr_last_error().decode()}")
+
+
+def _csr(m: H.Csr | None, keep: list) -> ChrCsr:
+    if m is None:
+        return ChrCsr(0, 0, None, None, None)
+    rp = np.ascontiguousarray(m.rowptr, np.int32)
+    ci = np.ascontiguousarray(m.col, np.int32)
+    v = np.ascontiguousarray(m.val, np.float64)
+    keep.extend([rp, ci, v])
+    return ChrCsr(m.nrows, m.ncols, rp.ctypes.data_as(_I32P), ci.ctypes.data_as(_I32P),
+                  v.ctypes.data_as(_F64P))
+
+
+def _precision(p) -> int:
+    if isinstance(p, str):
+        return {"fp64": FP64, "fp32": FP32_CYCLE, "fp32_cycle": FP32_CYCLE}[p.lower()]
+    return int(p)
+
+
+def init(device: int = 0) -> None:
+    _check(_lib().chr_init(device), "chr_init")
+
+
+class DeviceMatrix:
+    """A CSR operator resident on the GPU in FP64 (the outer prodA)."""
+
+    def __init__(self, A: H.Csr):
+        keep = []
+        c = _csr(A, keep)
+        self.ptr = C.c_void_p()
+        _check(_lib().chr_mat_create(C.byref(c), C.byref(self.ptr)), "chr_mat_create")
+        self.shape = (A.nrows, A.ncols)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib().chr_mat_destroy(self.ptr)
+            self.ptr = None
+
+    def __call__(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.shape[0])
+        _check(_lib().chr_spmv(self.ptr, x.ctypes.data, y.ctypes.data, PTR_HOST), "chr_spmv")
+        return y
+
+
+class AMGPreconditioner:
+    """The uploaded hierarchy; calling it applies one V-cycle (AMG_Vcycle with x0 = 0)."""
+
+    def __init__(self, hier: H.Hierarchy, precision="fp64"):
+        self.hier = hier
+        self.precision = _precision(precision)
+        keep = []
+        nlev = hier.nlev
+        arr = (ChrLevel * nlev)()
+        for l, L in enumerate(hier.levels):
+            arr[l] = ChrLevel(_csr(L.A, keep), _csr(L.P, keep), _csr(L.G, keep), L.omega, L.nupre, L.nupost)
+        self.ptr = C.c_void_p()
+        _check(_lib().chr_hier_create(nlev, arr, self.precision, C.byref(self.ptr)), "chr_hier_create")
+        self.n = hier.levels[0].A.nrows
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib().chr_hier_destroy(self.ptr)
+            self.ptr = None
+
+    def stats(self) -> dict:
+        st = ChrHierStats()
+        _check(_lib().chr_hier_get_stats(self.ptr, C.byref(st)))
+        nl = st.nlev
+        return {"nlev": nl, "n": list(st.n[:nl]), "vcycle_bytes": st.vcycle_bytes,
+                "level_bytes": list(st.level_bytes[:nl]), "pcg_iter_bytes": st.pcg_iter_bytes,
+                "device_bytes": st.device_bytes}
+
+    def __call__(self, b: np.ndarray) -> np.ndarray:
+        b = np.asfortranarray(b, dtype=np.float64)
+        k = 1 if b.ndim == 1 else b.shape[1]
+        x = np.empty_like(b, order="F")
+        _check(_lib().chr_vcycle_apply(self.ptr, k, b.ctypes.data, self.n, x.ctypes.data, self.n, PTR_HOST),
+               "chr_vcycle_apply")
+        return x
+
+    def apply_device(self, k: int, b_ptr: int, x_ptr: int, ld: int | None = None) -> None:
+        ld = self.n if ld is None else ld
+        _check(_lib().chr_vcycle_apply(self.ptr, k, b_ptr, ld, x_ptr, ld, PTR_DEVICE), "chr_vcycle_apply")
+
+    def pcg(self, b, x0=None, itmax: int = 1000, tol: float = 1e-8, A: DeviceMatrix | None = None):
+        """PCG_CJ with this V-cycle as prodM (host arrays)."""
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64)
+        it, fl = C.c_int32(), C.c_int32()
+        resvec = np.zeros(itmax + 1)
+        _check(_lib().chr_pcg_solve(self.ptr, A.ptr if A is not None else None, b.ctypes.data, x.ctypes.data,
+                                    itmax, tol, PTR_HOST, C.byref(it), resvec.ctypes.data_as(_F64P),
+                                    C.byref(fl)), "chr_pcg_solve")
+        return x, it.value, resvec[: it.value + 1], fl.value
+
+    def pcg_device(self, b_ptr: int, x_ptr: int, itmax: int = 1000, tol: float = 1e-8,
+                   A: DeviceMatrix | None = None):
+        it, fl = C.c_int32(), C.c_int32()
+        resvec = np.zeros(itmax + 1)
+        _check(_lib().chr_pcg_solve(self.ptr, A.ptr if A is not None else None, b_ptr, x_ptr, itmax, tol,
+                                    PTR_DEVICE, C.byref(it), resvec.ctypes.data_as(_F64P), C.byref(fl)),
+               "chr_pcg_solve")
+        return it.value, resvec[: it.value + 1], fl.value
+
+    def time_vcycle(self, reps: int = 20) -> float:
+        ms = C.c_double()
+        _check(_lib().chr_time_vcycle(self.ptr, reps, C.byref(ms)), "chr_time_vcycle")
+        return ms.value
+
+    def time_spmv(self, level: int, which: str, reps: int = 20) -> tuple[float, float]:
+        ms, by = C.c_double(), C.c_double()
+        _check(_lib().chr_time_spmv(self.ptr, level, which.encode(), reps, C.byref(ms), C.byref(by)),
+               "chr_time_spmv")
+        return ms.value, by.value
+
+
+def computeAMG(A: H.Csr, param: H.SetupParams | None = None, precision="fp64", seed: int = 1):
+    """computeAMG.m:1-51: BC rule (unit diagonal -> lambda_max/10), setup, upload.
+
+    Returns (M, T_setup) with M the uploaded preconditioner (callable like the handle
+    `@(x) AMG_Vcycle(AMG_prec, A, x)`, A being the BC-modified operator)."""
+    Afix, _ = H.bc_fix(A, seed)
+    t0 = time.perf_counter()
+    hier = H.build(Afix, param)
+    T_setup = time.perf_counter() - t0
+    return AMGPreconditioner(hier, precision), T_setup
+
+
+def PCG_CJ(prodM: AMGPreconditioner, prodA, b, x0, itmax: int, tol: float):
+    """[x, iter, resvec, flag] = PCG_CJ(prodM, prodA, b, x0, itmax, tol) on the GPU.
+
+    prodA is a DeviceMatrix, a host Csr (uploaded for the call), or None for the
+    preconditioner's own level-0 operator."""
+    if isinstance(prodA, H.Csr):
+        prodA = DeviceMatrix(prodA)
+    x, it, resvec, flag = prodM.pcg(b, x0, itmax, tol, prodA)
+    return x, it, resvec, flag

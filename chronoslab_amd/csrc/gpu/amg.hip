@@ -1,0 +1,822 @@
+// Host runtime and C ABI of the MI355X AMG solve phase (include/chronos_amg.h).
+//
+// The hierarchy is uploaded once.  One V-cycle (AMG_Vcycle.m:1-54, zero initial guess on
+// every level) is the kernel sequence, per non-coarsest level l:
+//   y  = G b              x = omega G^T y          r = b - A x
+//   bc = P^T r            (level l+1)              x = x + P xc
+//   r  = b - A x          y = G r                  x = x + omega G^T y
+// and x = Ainv b on the coarsest level (FP64 dense inverse).  G^T and P^T are explicit
+// transposed CSR copies made on upload, so every product is a row-parallel gather with a
+// fixed summation order (no atomics).  In CHR_FP32_CYCLE mode the level matrices and the
+// cycle vectors are FP32; the level-0 input is read in FP64 and the output written in FP64
+// by the kernels that touch them, so no conversion pass is needed.
+#include "chronos_amg.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(CHR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+// Owned device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t b) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = b;
+    if (b == 0) return CHR_OK;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(CHR_EALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return CHR_OK;
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+int check_csr(const chr_csr &m, const char *what) {
+  if (m.nrows < 0 || m.ncols < 0 || !m.rowptr || (m.rowptr[m.nrows] > 0 && (!m.col || !m.val)))
+    return fail(CHR_EARG, std::string(what) + ": null pointer or negative size");
+  if (m.rowptr[0] != 0) return fail(CHR_EARG, std::string(what) + ": rowptr[0] != 0");
+  for (int32_t i = 0; i < m.nrows; ++i) {
+    if (m.rowptr[i + 1] < m.rowptr[i])
+      return fail(CHR_EARG, std::string(what) + ": rowptr not monotone");
+    for (int32_t e = m.rowptr[i]; e < m.rowptr[i + 1]; ++e)
+      if (m.col[e] < 0 || m.col[e] >= m.ncols)
+        return fail(CHR_EARG, std::string(what) + ": column index out of range");
+  }
+  return CHR_OK;
+}
+
+// Host CSR transpose (columns of the result come out sorted).
+void transpose_host(const chr_csr &M, std::vector<int32_t> &rp, std::vector<int32_t> &ci,
+                    std::vector<double> &v) {
+  const int64_t nnz = M.rowptr[M.nrows];
+  rp.assign((size_t)M.ncols + 1, 0);
+  ci.resize(nnz);
+  v.resize(nnz);
+  for (int64_t e = 0; e < nnz; ++e) rp[M.col[e] + 1]++;
+  for (int32_t i = 0; i < M.ncols; ++i) rp[i + 1] += rp[i];
+  std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
+  for (int32_t i = 0; i < M.nrows; ++i)
+    for (int32_t e = M.rowptr[i]; e < M.rowptr[i + 1]; ++e) {
+      int32_t p = pos[M.col[e]]++;
+      ci[p] = i;
+      v[p] = M.val[e];
+    }
+}
+
+int pick_lanes(int64_t nnz, int64_t nrows) {
+  const double avg = nrows ? (double)nnz / (double)nrows : 1.0;
+  int lanes = 2;
+  while (lanes < 64 && lanes < avg) lanes *= 2;
+  return lanes;
+}
+
+// Device CSR in FP32 or FP64.
+struct DevCsr {
+  int32_t nrows = 0, ncols = 0;
+  int64_t nnz = 0;
+  bool fp32 = false;
+  int lanes = 8;
+  DevBuf rp, ci, val;
+  int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
+             bool as_fp32) {
+    nrows = nr;
+    ncols = nc;
+    nnz = hrp[nr];
+    fp32 = as_fp32;
+    lanes = pick_lanes(nnz, nr);
+    int rc;
+    if ((rc = rp.alloc(sizeof(int32_t) * (nr + 1)))) return rc;
+    if ((rc = ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
+    if ((rc = val.alloc((as_fp32 ? 4 : 8) * std::max<int64_t>(nnz, 1)))) return rc;
+    HIPC(hipMemcpy(rp.p, hrp, sizeof(int32_t) * (nr + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+      HIPC(hipMemcpy(ci.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+      if (as_fp32) {
+        std::vector<float> f(hv, hv + nnz);
+        HIPC(hipMemcpy(val.p, f.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+      } else {
+        HIPC(hipMemcpy(val.p, hv, sizeof(double) * nnz, hipMemcpyHostToDevice));
+      }
+    }
+    return CHR_OK;
+  }
+  int upload(const chr_csr &m, bool as_fp32) {
+    return upload(m.nrows, m.ncols, m.rowptr, m.col, m.val, as_fp32);
+  }
+  // Algorithmic bytes of one product: matrix stream + row pointers.
+  double stream_bytes() const {
+    return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1);
+  }
+  double device_bytes() const { return (double)(rp.bytes + ci.bytes + val.bytes); }
+};
+
+template <typename TV, typename TX, typename Epi>
+hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
+                         hipStream_t s) {
+  if (M.nrows == 0) return hipSuccess;
+  const int64_t threads = (int64_t)M.nrows * M.lanes;
+  const unsigned grid = (unsigned)((threads + chr::kBlock - 1) / chr::kBlock);
+  const int *rp = M.rp.as<int>();
+  const int *ci = M.ci.as<int>();
+  const TV *v = M.val.as<TV>();
+#define CHR_LANES_CASE(L)                                                                  \
+  case L:                                                                                  \
+    hipLaunchKernelGGL((chr::spmv_kernel<L, TV, TX, Epi>), dim3(grid), dim3(chr::kBlock), 0, s, \
+                       M.nrows, rp, ci, v, x, epi, stop);                                  \
+    break;
+  switch (M.lanes) {
+    CHR_LANES_CASE(2)
+    CHR_LANES_CASE(4)
+    CHR_LANES_CASE(8)
+    CHR_LANES_CASE(16)
+    CHR_LANES_CASE(32)
+    CHR_LANES_CASE(64)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef CHR_LANES_CASE
+  return hipGetLastError();
+}
+
+template <typename TX, typename Epi>
+hipError_t launch_spmv(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  return M.fp32 ? launch_spmv_t<float>(M, x, epi, stop, s)
+                : launch_spmv_t<double>(M, x, epi, stop, s);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Opaque objects
+// ---------------------------------------------------------------------------------------
+struct chr_mat {
+  DevCsr A;
+};
+
+struct PcgWork {
+  int32_t n = 0, itmax_cap = 0;
+  DevBuf x, r, z, p0, p1, ap, part, state, resvec;
+  unsigned nblk = 0;
+  chr::PcgState *pinned = nullptr;  // two slots
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  const chr_mat *graph_A = nullptr;  // outer operator the cached graphs were built for
+  ~PcgWork() {
+    for (auto &g : exec)
+      if (g) (void)hipGraphExecDestroy(g);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (pinned) (void)hipHostFree(pinned);
+  }
+};
+
+struct chr_hier {
+  struct Level {
+    int32_t n = 0;
+    DevCsr A, G, GT, P, PT;
+    double omega = 1.0;
+    int32_t nupre = 1, nupost = 1;
+    DevBuf b, x, r, y;  // cycle vectors (b unused on level 0)
+  };
+  int32_t precision = CHR_FP64;
+  std::vector<Level> lev;
+  int32_t nc = 0;
+  DevBuf Ainv;  // FP64 dense inverse of the coarsest operator (row-major)
+  DevBuf io_b, io_x;  // FP64 staging for host-pointer calls (n_0)
+  chr_mat outer;     // level-0 operator in FP64 (default outer operator of PCG)
+  hipStream_t stream = nullptr;
+  std::unique_ptr<PcgWork> pcg;
+  chr_hier_stats stats{};
+  ~chr_hier() {
+    pcg.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  bool fp32() const { return precision == CHR_FP32_CYCLE; }
+};
+
+namespace {
+
+// Dense SPD inverse via Cholesky (FP64, host).  Returns false if not positive definite.
+bool spd_inverse(int32_t n, std::vector<double> &a) {
+  // a: row-major n x n; lower Cholesky in place.
+  for (int32_t j = 0; j < n; ++j) {
+    double d = a[(size_t)j * n + j];
+    for (int32_t k = 0; k < j; ++k) d -= a[(size_t)j * n + k] * a[(size_t)j * n + k];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    a[(size_t)j * n + j] = d;
+    for (int32_t i = j + 1; i < n; ++i) {
+      double s = a[(size_t)i * n + j];
+      for (int32_t k = 0; k < j; ++k) s -= a[(size_t)i * n + k] * a[(size_t)j * n + k];
+      a[(size_t)i * n + j] = s / d;
+    }
+  }
+  std::vector<double> inv((size_t)n * n, 0.0), col(n);
+  for (int32_t c = 0; c < n; ++c) {
+    std::fill(col.begin(), col.end(), 0.0);
+    col[c] = 1.0;
+    for (int32_t i = 0; i < n; ++i) {
+      double s = col[i];
+      for (int32_t k = 0; k < i; ++k) s -= a[(size_t)i * n + k] * col[k];
+      col[i] = s / a[(size_t)i * n + i];
+    }
+    for (int32_t i = n - 1; i >= 0; --i) {
+      double s = col[i];
+      for (int32_t k = i + 1; k < n; ++k) s -= a[(size_t)k * n + i] * col[k];
+      col[i] = s / a[(size_t)i * n + i];
+    }
+    for (int32_t i = 0; i < n; ++i) inv[(size_t)i * n + c] = col[i];
+  }
+  a.swap(inv);
+  return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// V-cycle enqueue.  T is the cycle type (float or double); level 0 reads b0 and writes
+// out0 in FP64.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+struct Cycle {
+  chr_hier &h;
+  const int *stop;
+  hipStream_t s;
+
+  hipError_t coarse(const void *b, void *x, bool level0) {
+    const int n = h.nc;
+    const unsigned grid = (unsigned)(((int64_t)n * chr::kWave + chr::kBlock - 1) / chr::kBlock);
+    if (level0) {
+      hipLaunchKernelGGL((chr::coarse_gemv_kernel<double>), dim3(grid), dim3(chr::kBlock), 0, s, n,
+                         h.Ainv.as<double>(), (const double *)b, (double *)x, stop);
+    } else {
+      hipLaunchKernelGGL((chr::coarse_gemv_kernel<T>), dim3(grid), dim3(chr::kBlock), 0, s, n,
+                         h.Ainv.as<double>(), (const T *)b, (T *)x, stop);
+    }
+    return hipGetLastError();
+  }
+
+  // x <- x + omega G^T (G (b - A x)) sweep, with the output written to out (may be x).
+  template <typename TB, typename TO>
+  hipError_t post_sweep(chr_hier::Level &L, const TB *b, TO *out) {
+    hipError_t e;
+    T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
+    if ((e = launch_spmv(L.A, x, chr::EpiResidual<TB, T>{b, r}, stop, s))) return e;
+    if ((e = launch_spmv(L.G, r, chr::EpiStore<T>{y}, stop, s))) return e;
+    return launch_spmv(L.GT, y, chr::EpiAxpy<T, TO>{x, out, L.omega}, stop, s);
+  }
+
+  template <typename TB, typename TO>
+  hipError_t level(int l, const TB *b, TO *out) {
+    chr_hier::Level &L = h.lev[l];
+    const bool last = (l == (int)h.lev.size() - 1);
+    if (last) return coarse(b, out, l == 0);
+    hipError_t e;
+    T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
+    // Pre-smoothing from a zero initial guess, then nupre-1 further sweeps.
+    if ((e = launch_spmv(L.G, b, chr::EpiStore<T>{y}, stop, s))) return e;
+    if ((e = launch_spmv(L.GT, y, chr::EpiScale<T>{x, L.omega}, stop, s))) return e;
+    for (int k = 2; k <= L.nupre; ++k)
+      if ((e = post_sweep(L, b, x))) return e;
+    // Residual and restriction.
+    if ((e = launch_spmv(L.A, x, chr::EpiResidual<TB, T>{b, r}, stop, s))) return e;
+    chr_hier::Level &C = h.lev[l + 1];
+    T *bc = C.b.as<T>(), *xc = C.x.as<T>();
+    if ((e = launch_spmv(L.PT, r, chr::EpiStore<T>{bc}, stop, s))) return e;
+    // Coarse correction.
+    if ((e = level<T, T>(l + 1, bc, xc))) return e;
+    if ((e = launch_spmv(L.P, xc, chr::EpiAxpy<T, T>{x, x, 1.0}, stop, s))) return e;
+    // Post-smoothing; the last sweep writes the level's output.
+    if (L.nupost == 0) {
+      const unsigned grid = (unsigned)((L.n + chr::kBlock - 1) / chr::kBlock);
+      hipLaunchKernelGGL((chr::convert_kernel<T, TO>), dim3(grid), dim3(chr::kBlock), 0, s, L.n,
+                         (const T *)x, out);
+      return hipGetLastError();
+    }
+    for (int k = 1; k < L.nupost; ++k)
+      if ((e = post_sweep(L, b, x))) return e;
+    return post_sweep(L, b, out);
+  }
+};
+
+hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop) {
+  if (h.fp32()) {
+    Cycle<float> c{h, stop, h.stream};
+    return c.level<double, double>(0, b, out);
+  }
+  Cycle<double> c{h, stop, h.stream};
+  return c.level<double, double>(0, b, out);
+}
+
+void fill_stats(chr_hier &h) {
+  chr_hier_stats &st = h.stats;
+  std::memset(&st, 0, sizeof(st));
+  st.nlev = (int32_t)h.lev.size();
+  st.precision = h.precision;
+  const double s = h.fp32() ? 4.0 : 8.0;
+  double total = 0.0, dev = 0.0;
+  for (size_t l = 0; l < h.lev.size(); ++l) {
+    auto &L = h.lev[l];
+    if (l < 16) st.n[l] = L.n;
+    double b;
+    if (l + 1 == h.lev.size()) {
+      b = 8.0 * (double)h.nc * h.nc + 2.0 * h.nc * 8.0;
+      dev += h.Ainv.bytes;
+    } else {
+      const double nc = h.lev[l + 1].n;
+      b = 2 * L.A.stream_bytes() + 2 * L.G.stream_bytes() + 2 * L.GT.stream_bytes() +
+          L.P.stream_bytes() + L.PT.stream_bytes() + 18.0 * L.n * s + 2.0 * nc * s;
+      dev += L.A.device_bytes() + L.G.device_bytes() + L.GT.device_bytes() + L.P.device_bytes() +
+             L.PT.device_bytes() + L.b.bytes + L.x.bytes + L.r.bytes + L.y.bytes;
+    }
+    if (l < 16) st.level_bytes[l] = b;
+    total += b;
+  }
+  st.vcycle_bytes = total;
+  dev += h.outer.A.device_bytes();
+  st.device_bytes = dev;
+  const auto &A0 = h.outer.A;
+  st.pcg_iter_bytes = total + ((double)A0.nnz * 12.0 + 4.0 * (A0.nrows + 1)) + 18.0 * A0.nrows * 8.0;
+}
+
+int sync_stream(hipStream_t s) {
+  HIPC(hipStreamSynchronize(s));
+  return CHR_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+extern "C" const char *chr_last_error(void) { return g_err.c_str(); }
+
+extern "C" int chr_init(int32_t device) {
+  HIPC(hipSetDevice(device));
+  return CHR_OK;
+}
+
+extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t precision,
+                               chr_hier **out) {
+  if (nlev < 1 || nlev > 16 || !levels || !out)
+    return fail(CHR_EARG, "chr_hier_create: need 1..16 levels and non-null pointers");
+  if (precision != CHR_FP64 && precision != CHR_FP32_CYCLE)
+    return fail(CHR_EARG, "chr_hier_create: unknown precision");
+  // Shape checks before any device work.
+  for (int32_t l = 0; l < nlev; ++l) {
+    int rc;
+    const chr_level &L = levels[l];
+    if ((rc = check_csr(L.A, "A"))) return rc;
+    if (L.A.nrows != L.A.ncols) return fail(CHR_EARG, "level operator must be square");
+    if (l + 1 < nlev) {
+      if ((rc = check_csr(L.P, "P"))) return rc;
+      if ((rc = check_csr(L.G, "G"))) return rc;
+      if (L.P.nrows != L.A.nrows || L.P.ncols != levels[l + 1].A.nrows)
+        return fail(CHR_EARG, "P shape does not match consecutive level sizes");
+      if (L.G.nrows != L.A.nrows || L.G.ncols != L.A.nrows)
+        return fail(CHR_EARG, "G shape does not match the level size");
+      if (L.nupre < 1 || L.nupost < 0) return fail(CHR_EARG, "nupre must be >= 1, nupost >= 0");
+    }
+  }
+  auto h = std::make_unique<chr_hier>();
+  h->precision = precision;
+  HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  const bool f32 = precision == CHR_FP32_CYCLE;
+  const size_t s = f32 ? 4 : 8;
+  h->lev.resize(nlev);
+  int rc;
+  for (int32_t l = 0; l < nlev; ++l) {
+    const chr_level &in = levels[l];
+    auto &L = h->lev[l];
+    L.n = in.A.nrows;
+    if (l + 1 < nlev) {
+      if ((rc = L.A.upload(in.A, f32))) return rc;
+      if ((rc = L.G.upload(in.G, f32))) return rc;
+      if ((rc = L.P.upload(in.P, f32))) return rc;
+      std::vector<int32_t> trp, tci;
+      std::vector<double> tv;
+      transpose_host(in.G, trp, tci, tv);
+      if ((rc = L.GT.upload(in.G.ncols, in.G.nrows, trp.data(), tci.data(), tv.data(), f32))) return rc;
+      transpose_host(in.P, trp, tci, tv);
+      if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, trp.data(), tci.data(), tv.data(), f32))) return rc;
+      L.omega = in.omega;
+      L.nupre = in.nupre;
+      L.nupost = in.nupost;
+      if ((rc = L.x.alloc(s * L.n))) return rc;
+      if ((rc = L.r.alloc(s * L.n))) return rc;
+      if ((rc = L.y.alloc(s * L.n))) return rc;
+    } else {
+      // Coarsest level: FP64 dense inverse.
+      const int32_t n = in.A.nrows;
+      if ((int64_t)n * n > (int64_t)1 << 26)
+        return fail(CHR_EARG, "coarsest level too large for a dense inverse");
+      std::vector<double> a((size_t)n * n, 0.0);
+      for (int32_t i = 0; i < n; ++i)
+        for (int32_t e = in.A.rowptr[i]; e < in.A.rowptr[i + 1]; ++e)
+          a[(size_t)i * n + in.A.col[e]] = in.A.val[e];
+      if (!spd_inverse(n, a)) return fail(CHR_ENOTSPD, "coarsest operator is not positive definite");
+      h->nc = n;
+      if ((rc = h->Ainv.alloc(sizeof(double) * a.size()))) return rc;
+      HIPC(hipMemcpy(h->Ainv.p, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice));
+      if ((rc = L.x.alloc((l == 0 ? 8 : s) * n))) return rc;
+    }
+    if (l > 0 && (rc = L.b.alloc(s * L.n))) return rc;
+  }
+  // Level-0 operator in FP64 for the outer Krylov loop.
+  if ((rc = h->outer.A.upload(levels[0].A, false))) return rc;
+  const int32_t n0 = h->lev[0].n;
+  if ((rc = h->io_b.alloc(8 * (size_t)n0))) return rc;
+  if ((rc = h->io_x.alloc(8 * (size_t)n0))) return rc;
+  fill_stats(*h);
+  *out = h.release();
+  return CHR_OK;
+}
+
+extern "C" int chr_hier_destroy(chr_hier *h) {
+  if (h) {
+    (void)hipStreamSynchronize(h->stream);
+    delete h;
+  }
+  return CHR_OK;
+}
+
+extern "C" int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st) {
+  if (!h || !st) return fail(CHR_EARG, "null");
+  *st = h->stats;
+  return CHR_OK;
+}
+
+extern "C" int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t ldb, double *x,
+                                int64_t ldx, int32_t ptr_kind) {
+  if (!h || k < 0 || (k > 0 && (!b || !x))) return fail(CHR_EARG, "chr_vcycle_apply: arguments");
+  const int32_t n = h->lev[0].n;
+  if (k > 0 && (ldb < n || ldx < n)) return fail(CHR_EARG, "chr_vcycle_apply: leading dimension < n");
+  for (int32_t c = 0; c < k; ++c) {
+    const double *bc = b + (int64_t)c * ldb;
+    double *xc = x + (int64_t)c * ldx;
+    if (ptr_kind == CHR_PTR_HOST) {
+      HIPC(hipMemcpyAsync(h->io_b.p, bc, 8 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+      HIPC(enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), nullptr));
+      HIPC(hipMemcpyAsync(xc, h->io_x.p, 8 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+      int rc = sync_stream(h->stream);
+      if (rc) return rc;
+    } else {
+      HIPC(enqueue_vcycle(*h, bc, xc, nullptr));
+    }
+  }
+  return sync_stream(h->stream);
+}
+
+extern "C" int chr_mat_create(const chr_csr *A, chr_mat **out) {
+  if (!A || !out) return fail(CHR_EARG, "null");
+  int rc = check_csr(*A, "A");
+  if (rc) return rc;
+  auto m = std::make_unique<chr_mat>();
+  if ((rc = m->A.upload(*A, false))) return rc;
+  *out = m.release();
+  return CHR_OK;
+}
+
+extern "C" int chr_mat_destroy(chr_mat *m) {
+  delete m;
+  return CHR_OK;
+}
+
+extern "C" int chr_spmv(chr_mat *A, const double *x, double *y, int32_t ptr_kind) {
+  if (!A || !x || !y) return fail(CHR_EARG, "null");
+  const DevCsr &M = A->A;
+  DevBuf dx, dy;
+  const double *px = x;
+  double *py = y;
+  int rc;
+  if (ptr_kind == CHR_PTR_HOST) {
+    if ((rc = dx.alloc(8 * (size_t)std::max(M.ncols, 1)))) return rc;
+    if ((rc = dy.alloc(8 * (size_t)std::max(M.nrows, 1)))) return rc;
+    HIPC(hipMemcpy(dx.p, x, 8 * (size_t)M.ncols, hipMemcpyHostToDevice));
+    px = dx.as<double>();
+    py = dy.as<double>();
+  }
+  HIPC(launch_spmv(M, px, chr::EpiStore<double>{py}, nullptr, nullptr));
+  if (ptr_kind == CHR_PTR_HOST) HIPC(hipMemcpy(y, py, 8 * (size_t)M.nrows, hipMemcpyDeviceToHost));
+  HIPC(hipDeviceSynchronize());
+  return CHR_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// PCG_CJ
+// ---------------------------------------------------------------------------------------
+namespace {
+
+template <int L>
+hipError_t launch_pcg_init(const DevCsr &A, unsigned grid, const double *b, const double *x,
+                           double *r, double *part, chr::PcgState *st, double *resvec,
+                           hipStream_t s) {
+  hipLaunchKernelGGL((chr::pcg_init_kernel<L>), dim3(grid), dim3(chr::kBlock), 0, s, A.nrows,
+                     A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(), b, x, r, part, st, resvec);
+  return hipGetLastError();
+}
+
+template <int L>
+hipError_t launch_pcg_p(const DevCsr &A, unsigned grid, const double *z, const double *pold,
+                        double *pnew, double *ap, const double *r, double *part,
+                        chr::PcgState *st, hipStream_t s) {
+  hipLaunchKernelGGL((chr::pcg_p_spmv_kernel<L>), dim3(grid), dim3(chr::kBlock), 0, s, A.nrows,
+                     A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(), z, pold, pnew, ap, r,
+                     part, st);
+  return hipGetLastError();
+}
+
+#define CHR_DISPATCH_LANES(lanes, fn, ...)      \
+  switch (lanes) {                              \
+    case 2: return fn<2>(__VA_ARGS__);          \
+    case 4: return fn<4>(__VA_ARGS__);          \
+    case 8: return fn<8>(__VA_ARGS__);          \
+    case 16: return fn<16>(__VA_ARGS__);        \
+    case 32: return fn<32>(__VA_ARGS__);        \
+    default: return fn<64>(__VA_ARGS__);        \
+  }
+
+hipError_t pcg_init(const DevCsr &A, unsigned grid, const double *b, const double *x, double *r,
+                    double *part, chr::PcgState *st, double *resvec, hipStream_t s) {
+  CHR_DISPATCH_LANES(A.lanes, launch_pcg_init, A, grid, b, x, r, part, st, resvec, s)
+}
+
+hipError_t pcg_p(const DevCsr &A, unsigned grid, const double *z, const double *pold, double *pnew,
+                 double *ap, const double *r, double *part, chr::PcgState *st, hipStream_t s) {
+  CHR_DISPATCH_LANES(A.lanes, launch_pcg_p, A, grid, z, pold, pnew, ap, r, part, st, s)
+}
+
+// One PCG_CJ iteration (PCG_CJ.m:88-120), p read from pold and written to pnew.
+hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity) {
+  auto *st = w.state.as<chr::PcgState>();
+  const int *stop = &st->stop;
+  double *pold = parity ? w.p1.as<double>() : w.p0.as<double>();
+  double *pnew = parity ? w.p0.as<double>() : w.p1.as<double>();
+  hipError_t e;
+  if ((e = enqueue_vcycle(h, w.r.as<double>(), w.z.as<double>(), stop))) return e;
+  hipLaunchKernelGGL(chr::pcg_beta_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
+                     (const double *)w.z.as<double>(), (const double *)w.ap.as<double>(),
+                     w.part.as<double>(), st);
+  if ((e = hipGetLastError())) return e;
+  if ((e = pcg_p(A, w.nblk, w.z.as<double>(), pold, pnew, w.ap.as<double>(), w.r.as<double>(),
+                 w.part.as<double>(), st, h.stream)))
+    return e;
+  hipLaunchKernelGGL(chr::pcg_update_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
+                     (const double *)pnew, (const double *)w.ap.as<double>(), w.x.as<double>(),
+                     w.r.as<double>(), w.part.as<double>(), st, w.resvec.as<double>());
+  if ((e = hipGetLastError())) return e;
+  return hipMemcpyAsync(&w.pinned[parity], st, sizeof(chr::PcgState), hipMemcpyDeviceToHost,
+                        h.stream);
+}
+
+int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
+  if (!h.pcg) h.pcg = std::make_unique<PcgWork>();
+  PcgWork &w = *h.pcg;
+  const int32_t n = A.nrows;
+  int rc;
+  if (w.n != n) {
+    w.n = n;
+    const size_t vb = 8 * (size_t)n;
+    if ((rc = w.x.alloc(vb)) || (rc = w.r.alloc(vb)) || (rc = w.z.alloc(vb)) ||
+        (rc = w.p0.alloc(vb)) || (rc = w.p1.alloc(vb)) || (rc = w.ap.alloc(vb)))
+      return rc;
+    // Fixed grid: enough blocks to fill 256 CUs several times, small enough for the
+    // single-block finalisation of the partial sums.
+    w.nblk = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, ((int64_t)n + 1023) / 1024));
+    if ((rc = w.part.alloc(sizeof(double) * 6 * w.nblk))) return rc;
+    if ((rc = w.state.alloc(sizeof(chr::PcgState)))) return rc;
+    if (!w.pinned) HIPC(hipHostMalloc((void **)&w.pinned, 2 * sizeof(chr::PcgState), hipHostMallocDefault));
+    for (auto &e : w.ev)
+      if (!e) HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &g : w.exec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    w.graph_A = nullptr;
+  }
+  if (w.itmax_cap < itmax + 1) {
+    w.itmax_cap = itmax + 1;
+    if ((rc = w.resvec.alloc(sizeof(double) * w.itmax_cap))) return rc;
+  }
+  return CHR_OK;
+}
+
+int pcg_graphs(chr_hier &h, const chr_mat *Akey, const DevCsr &A) {
+  PcgWork &w = *h.pcg;
+  if (w.exec[0] && w.graph_A == Akey) return CHR_OK;
+  for (int parity = 0; parity < 2; ++parity) {
+    if (w.exec[parity]) {
+      (void)hipGraphExecDestroy(w.exec[parity]);
+      w.exec[parity] = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPC(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_pcg_iter(h, A, w, parity);
+    hipError_t e2 = hipStreamEndCapture(h.stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(CHR_EHIP, std::string("PCG graph capture: ") +
+                                hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&w.exec[parity], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(CHR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  w.graph_A = Akey;
+  return CHR_OK;
+}
+
+}  // namespace
+
+extern "C" int chr_pcg_solve(chr_hier *h, chr_mat *Aop, const double *b, double *x,
+                             int32_t itmax, double tol, int32_t ptr_kind, int32_t *iter,
+                             double *resvec, int32_t *flag) {
+  if (!h || !b || !x) return fail(CHR_EARG, "chr_pcg_solve: null argument");
+  if (itmax < 0) return fail(CHR_EARG, "chr_pcg_solve: itmax < 0");
+  const chr_mat *Akey = Aop ? Aop : &h->outer;
+  const DevCsr &A = Akey->A;
+  if (A.nrows != h->lev[0].n || A.ncols != A.nrows)
+    return fail(CHR_EARG, "chr_pcg_solve: operator size differs from the hierarchy");
+  int rc;
+  if ((rc = pcg_prepare(*h, A, itmax))) return rc;
+  if ((rc = pcg_graphs(*h, Akey, A))) return rc;
+  PcgWork &w = *h->pcg;
+  const int32_t n = A.nrows;
+  const size_t vb = 8 * (size_t)n;
+  const hipMemcpyKind in_kind = ptr_kind == CHR_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  // Inputs; p and Ap start at zero so that beta * p_old is defined on the first iteration.
+  const double *db = b;
+  if (ptr_kind == CHR_PTR_HOST) {
+    HIPC(hipMemcpyAsync(h->io_b.p, b, vb, hipMemcpyHostToDevice, h->stream));
+    db = h->io_b.as<double>();
+  }
+  HIPC(hipMemcpyAsync(w.x.p, x, vb, in_kind, h->stream));
+  HIPC(hipMemsetAsync(w.p0.p, 0, vb, h->stream));
+  HIPC(hipMemsetAsync(w.p1.p, 0, vb, h->stream));
+  HIPC(hipMemsetAsync(w.ap.p, 0, vb, h->stream));
+  chr::PcgState st0{};
+  st0.tol = tol;
+  st0.itmax = itmax;
+  HIPC(hipMemcpyAsync(w.state.p, &st0, sizeof(st0), hipMemcpyHostToDevice, h->stream));
+  HIPC(pcg_init(A, w.nblk, db, w.x.as<double>(), w.r.as<double>(), w.part.as<double>(),
+                w.state.as<chr::PcgState>(), w.resvec.as<double>(), h->stream));
+  HIPC(hipMemcpyAsync(&w.pinned[1], w.state.p, sizeof(chr::PcgState), hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  chr::PcgState fin = w.pinned[1];
+  if (!fin.stop) {
+    // Iterations run back to back; the host waits for iteration it-1 while iteration it is
+    // queued, and stops enqueueing once the device reports convergence.  Kernels of an
+    // iteration that starts after convergence exit at once.
+    for (int32_t it = 1; it <= itmax; ++it) {
+      const int parity = (it - 1) & 1;
+      HIPC(hipGraphLaunch(w.exec[parity], h->stream));
+      HIPC(hipEventRecord(w.ev[parity], h->stream));
+      if (it >= 2) {
+        HIPC(hipEventSynchronize(w.ev[parity ^ 1]));
+        if (w.pinned[parity ^ 1].stop) break;
+      }
+    }
+    HIPC(hipStreamSynchronize(h->stream));
+    fin = w.pinned[0].iter >= w.pinned[1].iter ? w.pinned[0] : w.pinned[1];
+  }
+  if (fin.bnorm == 0.0) {
+    HIPC(hipMemsetAsync(w.x.p, 0, vb, h->stream));
+  }
+  const hipMemcpyKind out_kind = ptr_kind == CHR_PTR_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  HIPC(hipMemcpyAsync(x, w.x.p, vb, out_kind, h->stream));
+  if (resvec) HIPC(hipMemcpyAsync(resvec, w.resvec.p, sizeof(double) * (fin.iter + 1), hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  if (iter) *iter = fin.iter;
+  if (flag) *flag = fin.resiter > tol ? 1 : 0;
+  if (fin.breakdown) return fail(CHR_EBREAKDOWN, "PCG breakdown: p^T A p is zero or not finite");
+  return CHR_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Timing helpers
+// ---------------------------------------------------------------------------------------
+extern "C" int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms) {
+  if (!h || reps < 1 || !ms) return fail(CHR_EARG, "chr_time_vcycle: arguments");
+  const int32_t n = h->lev[0].n;
+  std::vector<double> ones(n, 1.0);
+  HIPC(hipMemcpy(h->io_b.p, ones.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), nullptr);
+  hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+  if (e || e2) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(CHR_EHIP, "V-cycle capture failed");
+  }
+  HIPC(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(g);
+  hipEvent_t a, b;
+  HIPC(hipEventCreate(&a));
+  HIPC(hipEventCreate(&b));
+  HIPC(hipGraphLaunch(ex, h->stream));
+  HIPC(hipEventRecord(a, h->stream));
+  for (int i = 0; i < reps; ++i) HIPC(hipGraphLaunch(ex, h->stream));
+  HIPC(hipEventRecord(b, h->stream));
+  HIPC(hipEventSynchronize(b));
+  float t = 0.f;
+  HIPC(hipEventElapsedTime(&t, a, b));
+  *ms = t / reps;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipGraphExecDestroy(ex);
+  return CHR_OK;
+}
+
+extern "C" int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t reps, double *ms,
+                             double *bytes) {
+  if (!h || level < 0 || level + 1 >= (int32_t)h->lev.size() || reps < 1 || !ms)
+    return fail(CHR_EARG, "chr_time_spmv: arguments");
+  auto &L = h->lev[level];
+  const DevCsr *M = nullptr;
+  switch (which) {
+    case 'A': M = &L.A; break;
+    case 'G': M = &L.G; break;
+    case 'T': M = &L.GT; break;
+    case 'P': M = &L.P; break;
+    case 'R': M = &L.PT; break;
+    default: return fail(CHR_EARG, "chr_time_spmv: which must be A, G, T, P or R");
+  }
+  // x: a level vector long enough for M's columns; y: another one.
+  const bool f32 = h->fp32();
+  const double s = f32 ? 4.0 : 8.0;
+  DevBuf xb, yb;
+  int rc;
+  if ((rc = xb.alloc((size_t)s * std::max(M->ncols, 1))) || (rc = yb.alloc((size_t)s * std::max(M->nrows, 1))))
+    return rc;
+  HIPC(hipMemset(xb.p, 0, xb.bytes));
+  hipEvent_t a, b;
+  HIPC(hipEventCreate(&a));
+  HIPC(hipEventCreate(&b));
+  auto run = [&]() -> hipError_t {
+    if (f32) return launch_spmv(*M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, nullptr, h->stream);
+    return launch_spmv(*M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, nullptr, h->stream);
+  };
+  HIPC(run());
+  HIPC(hipEventRecord(a, h->stream));
+  for (int i = 0; i < reps; ++i) HIPC(run());
+  HIPC(hipEventRecord(b, h->stream));
+  HIPC(hipEventSynchronize(b));
+  float t = 0.f;
+  HIPC(hipEventElapsedTime(&t, a, b));
+  *ms = t / reps;
+  if (bytes) *bytes = M->stream_bytes() + s * (M->ncols + M->nrows);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return CHR_OK;
+}

@@ -1,0 +1,1086 @@
+// Host-side producer of the AMG hierarchy (CPU, C++17 + OpenMP).
+//
+// This is the classical + adaptive-FSAI setup that ChronosLab runs before its solve phase
+// (sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:68-351), restated from the reference's
+// behaviour.  The GPU solve phase consumes its output through chronos_amg.h.
+#include "chronos_setup.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+int nthreads_or_default(int nt) { return nt > 0 ? nt : omp_get_max_threads(); }
+
+}  // namespace
+
+struct chs_mat {
+  int32_t nrows = 0, ncols = 0;
+  std::vector<int32_t> rp, ci;
+  std::vector<double> v;
+  int64_t nnz() const { return (int64_t)ci.size(); }
+};
+
+struct chs_hier {
+  struct Level {
+    chs_mat A, P, G;
+    double omega = 1.0, lambda = 0.0;
+  };
+  std::vector<Level> lev;
+  double t[5] = {0, 0, 0, 0, 0};
+};
+
+// ----------------------------------------------------------------------------------------
+// RNG: SplitMix64's output function applied to a Weyl sequence, addressed by counter, so
+// the same draw can be produced by any consumer (CPU generators, the oracle, tests).
+// ----------------------------------------------------------------------------------------
+static inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+static const uint64_t kGamma = 0x9e3779b97f4a7c15ULL;
+
+extern "C" uint64_t chs_rng_u64(uint64_t seed, uint64_t stream, uint64_t k) {
+  uint64_t base = mix64(seed ^ (stream * 0xd1b54a32d192ed03ULL));
+  return mix64(base + (k + 1) * kGamma);
+}
+
+static inline double u01(uint64_t seed, uint64_t stream, uint64_t k) {
+  return (double)(chs_rng_u64(seed, stream, k) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+extern "C" void chs_rng_uniform(uint64_t seed, uint64_t stream, int64_t offset, int64_t n,
+                                double lo, double hi, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) out[i] = lo + (hi - lo) * u01(seed, stream, offset + i);
+}
+
+extern "C" void chs_rng_normal(uint64_t seed, uint64_t stream, int64_t offset, int64_t n,
+                               double *out) {
+  const double two_pi = 6.283185307179586476925286766559;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t g = offset + i;
+    int64_t pair = g >> 1;
+    double a = u01(seed, stream, 2 * pair), b = u01(seed, stream, 2 * pair + 1);
+    double rad = std::sqrt(-2.0 * std::log1p(-a));  // 1-a in (0,1]
+    out[i] = (g & 1) ? rad * std::sin(two_pi * b) : rad * std::cos(two_pi * b);
+  }
+}
+
+extern "C" void chs_params_default(chs_params *p) {
+  p->nlev_max = 10;
+  p->max_coarse = 1000;
+  p->soc_tau = 0.25;
+  p->nstep = 15;
+  p->step_size = 1;
+  p->fsai_eps = 1e-3;
+  p->nupre = 1;
+  p->nupost = 1;
+  p->smoother = 0;
+  p->nthreads = 0;
+  p->seed = 20240607ULL;
+  p->verbose = 0;
+}
+
+extern "C" const char *chs_last_error(void) { return g_err.c_str(); }
+
+// ----------------------------------------------------------------------------------------
+// Opaque matrix
+// ----------------------------------------------------------------------------------------
+static int check_csr(int32_t nr, int32_t nc, const int32_t *rp, const int32_t *ci) {
+  if (nr < 0 || nc < 0 || !rp) return fail(CHS_EARG, "csr: bad dimensions or null rowptr");
+  if (rp[0] != 0) return fail(CHS_EARG, "csr: rowptr[0] != 0");
+  for (int32_t i = 0; i < nr; ++i) {
+    if (rp[i + 1] < rp[i]) return fail(CHS_EARG, "csr: rowptr not monotone");
+    for (int32_t e = rp[i]; e < rp[i + 1]; ++e) {
+      if (ci[e] < 0 || ci[e] >= nc) return fail(CHS_EARG, "csr: column index out of range");
+      if (e > rp[i] && ci[e] <= ci[e - 1])
+        return fail(CHS_EARG, "csr: columns not strictly increasing within a row");
+    }
+  }
+  return CHS_OK;
+}
+
+extern "C" int chs_mat_from_csr(int32_t nrows, int32_t ncols, const int32_t *rowptr,
+                                const int32_t *col, const double *val, chs_mat **out) {
+  if (!out) return fail(CHS_EARG, "null out");
+  int rc = check_csr(nrows, ncols, rowptr, col);
+  if (rc) return rc;
+  try {
+    auto *m = new chs_mat;
+    m->nrows = nrows;
+    m->ncols = ncols;
+    m->rp.assign(rowptr, rowptr + nrows + 1);
+    m->ci.assign(col, col + rowptr[nrows]);
+    m->v.assign(val, val + rowptr[nrows]);
+    *out = m;
+  } catch (std::bad_alloc &) {
+    return fail(CHS_EALLOC, "allocation failed");
+  }
+  return CHS_OK;
+}
+
+extern "C" int chs_mat_dims(const chs_mat *m, int32_t *nrows, int32_t *ncols, int64_t *nnz) {
+  if (!m) return fail(CHS_EARG, "null matrix");
+  if (nrows) *nrows = m->nrows;
+  if (ncols) *ncols = m->ncols;
+  if (nnz) *nnz = m->nnz();
+  return CHS_OK;
+}
+
+extern "C" int chs_mat_copy(const chs_mat *m, int32_t *rowptr, int32_t *col, double *val) {
+  if (!m) return fail(CHS_EARG, "null matrix");
+  if (rowptr) std::memcpy(rowptr, m->rp.data(), sizeof(int32_t) * (m->nrows + 1));
+  if (col) std::memcpy(col, m->ci.data(), sizeof(int32_t) * m->nnz());
+  if (val) std::memcpy(val, m->v.data(), sizeof(double) * m->nnz());
+  return CHS_OK;
+}
+
+extern "C" void chs_mat_free(chs_mat *m) { delete m; }
+
+// ----------------------------------------------------------------------------------------
+// Generators
+// ----------------------------------------------------------------------------------------
+extern "C" int chs_gen_poisson7(int32_t N, chs_mat **out) {
+  if (N <= 0 || (int64_t)N * N * N > (1LL << 30)) return fail(CHS_EARG, "bad N");
+  auto *m = new (std::nothrow) chs_mat;
+  if (!m) return fail(CHS_EALLOC, "alloc");
+  const int64_t n = (int64_t)N * N * N;
+  m->nrows = m->ncols = (int32_t)n;
+  m->rp.resize(n + 1);
+  // Row lengths: 1 + number of in-grid neighbours.
+  m->rp[0] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int x = i % N, y = (i / N) % N, z = (int)(i / ((int64_t)N * N));
+    int deg = (x > 0) + (x < N - 1) + (y > 0) + (y < N - 1) + (z > 0) + (z < N - 1);
+    m->rp[i + 1] = m->rp[i] + 1 + deg;
+  }
+  m->ci.resize(m->rp[n]);
+  m->v.resize(m->rp[n]);
+  const int64_t NN = (int64_t)N * N;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int x = i % N, y = (i / N) % N, z = (int)(i / NN);
+    int32_t e = m->rp[i];
+    auto put = [&](int64_t j, double a) { m->ci[e] = (int32_t)j; m->v[e] = a; ++e; };
+    if (z > 0) put(i - NN, -1.0);
+    if (y > 0) put(i - N, -1.0);
+    if (x > 0) put(i - 1, -1.0);
+    put(i, 6.0);
+    if (x < N - 1) put(i + 1, -1.0);
+    if (y < N - 1) put(i + N, -1.0);
+    if (z < N - 1) put(i + NN, -1.0);
+  }
+  *out = m;
+  return CHS_OK;
+}
+
+extern "C" void chs_gen_lnk(int32_t N, double sigma, uint64_t seed, double *lnk) {
+  const int64_t n = (int64_t)N * N * N;
+  chs_rng_normal(seed, 1, 0, n, lnk);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) lnk[i] *= sigma;
+}
+
+extern "C" int chs_gen_diffusion(int32_t N, const double *k, chs_mat **out) {
+  if (N <= 0 || !k || (int64_t)N * N * N > (1LL << 30)) return fail(CHS_EARG, "bad N or k");
+  auto *m = new (std::nothrow) chs_mat;
+  if (!m) return fail(CHS_EALLOC, "alloc");
+  const int64_t n = (int64_t)N * N * N, NN = (int64_t)N * N;
+  m->nrows = m->ncols = (int32_t)n;
+  m->rp.resize(n + 1);
+  m->rp[0] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int x = i % N, y = (i / N) % N, z = (int)(i / NN);
+    int deg = (x > 0) + (x < N - 1) + (y > 0) + (y < N - 1) + (z > 0) + (z < N - 1);
+    m->rp[i + 1] = m->rp[i] + 1 + deg;
+  }
+  m->ci.resize(m->rp[n]);
+  m->v.resize(m->rp[n]);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int x = i % N, y = (i / N) % N, z = (int)(i / NN);
+    const double ki = k[i];
+    auto T = [&](int64_t j) { return 2.0 * ki * k[j] / (ki + k[j]); };
+    double diag = 0.0;
+    int32_t e = m->rp[i], ediag = -1;
+    auto put = [&](int64_t j) {
+      double t = T(j);
+      m->ci[e] = (int32_t)j;
+      m->v[e] = -t;
+      diag += t;
+      ++e;
+    };
+    if (z > 0) put(i - NN);
+    if (y > 0) put(i - N);
+    if (x > 0) put(i - 1);
+    ediag = e++;
+    if (x < N - 1) put(i + 1);
+    if (y < N - 1) put(i + N);
+    if (z < N - 1) put(i + NN);
+    if (x == 0) diag += 2.0 * ki;      // Dirichlet half cell on the x = 0 face
+    if (x == N - 1) diag += 2.0 * ki;  // Dirichlet half cell on the x = 1 face
+    m->ci[ediag] = (int32_t)i;
+    m->v[ediag] = diag;
+  }
+  *out = m;
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Sparse helpers
+// ----------------------------------------------------------------------------------------
+static void transpose_into(const chs_mat &M, chs_mat &T) {
+  T.nrows = M.ncols;
+  T.ncols = M.nrows;
+  T.rp.assign((size_t)T.nrows + 1, 0);
+  for (int32_t c : M.ci) T.rp[c + 1]++;
+  for (int32_t i = 0; i < T.nrows; ++i) T.rp[i + 1] += T.rp[i];
+  T.ci.resize(M.ci.size());
+  T.v.resize(M.v.size());
+  std::vector<int32_t> pos(T.rp.begin(), T.rp.end() - 1);
+  // Row order of M is ascending, so columns of T come out sorted.
+  for (int32_t i = 0; i < M.nrows; ++i)
+    for (int32_t e = M.rp[i]; e < M.rp[i + 1]; ++e) {
+      int32_t p = pos[M.ci[e]]++;
+      T.ci[p] = i;
+      T.v[p] = M.v[e];
+    }
+}
+
+static void spmv(const chs_mat &M, const double *x, double *y) {
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < M.nrows; ++i) {
+    double s = 0.0;
+    for (int32_t e = M.rp[i]; e < M.rp[i + 1]; ++e) s += M.v[e] * x[M.ci[e]];
+    y[i] = s;
+  }
+}
+
+// Gustavson SpGEMM C = A*B, rows in parallel, exact zeros dropped, columns sorted.
+static void spgemm(const chs_mat &A, const chs_mat &B, chs_mat &C, int nthreads) {
+  C.nrows = A.nrows;
+  C.ncols = B.ncols;
+  C.rp.assign((size_t)A.nrows + 1, 0);
+  std::vector<std::vector<int32_t>> rc(A.nrows);
+  std::vector<std::vector<double>> rv(A.nrows);
+#pragma omp parallel num_threads(nthreads)
+  {
+    std::vector<int32_t> mark(B.ncols, -1);
+    std::vector<double> acc(B.ncols, 0.0);
+    std::vector<int32_t> cols;
+#pragma omp for schedule(dynamic, 256)
+    for (int32_t i = 0; i < A.nrows; ++i) {
+      cols.clear();
+      for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+        const int32_t k = A.ci[e];
+        const double a = A.v[e];
+        for (int32_t f = B.rp[k]; f < B.rp[k + 1]; ++f) {
+          const int32_t j = B.ci[f];
+          if (mark[j] != i) {
+            mark[j] = i;
+            acc[j] = a * B.v[f];
+            cols.push_back(j);
+          } else {
+            acc[j] += a * B.v[f];
+          }
+        }
+      }
+      std::sort(cols.begin(), cols.end());
+      auto &oc = rc[i];
+      auto &ov = rv[i];
+      oc.reserve(cols.size());
+      ov.reserve(cols.size());
+      for (int32_t j : cols)
+        if (acc[j] != 0.0) {
+          oc.push_back(j);
+          ov.push_back(acc[j]);
+        }
+      C.rp[i + 1] = (int32_t)oc.size();
+    }
+  }
+  for (int32_t i = 0; i < A.nrows; ++i) C.rp[i + 1] += C.rp[i];
+  C.ci.resize(C.rp[A.nrows]);
+  C.v.resize(C.rp[A.nrows]);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int32_t i = 0; i < A.nrows; ++i) {
+    std::copy(rc[i].begin(), rc[i].end(), C.ci.begin() + C.rp[i]);
+    std::copy(rv[i].begin(), rv[i].end(), C.v.begin() + C.rp[i]);
+  }
+}
+
+extern "C" int chs_transpose(const chs_mat *M, chs_mat **MT) {
+  if (!M || !MT) return fail(CHS_EARG, "null");
+  auto *t = new (std::nothrow) chs_mat;
+  if (!t) return fail(CHS_EALLOC, "alloc");
+  transpose_into(*M, *t);
+  *MT = t;
+  return CHS_OK;
+}
+
+extern "C" int chs_galerkin(const chs_mat *A, const chs_mat *P, int nthreads, chs_mat **Ac) {
+  if (!A || !P || !Ac || A->ncols != P->nrows) return fail(CHS_EARG, "galerkin: shapes");
+  nthreads = nthreads_or_default(nthreads);
+  chs_mat AP, PT;
+  spgemm(*A, *P, AP, nthreads);
+  transpose_into(*P, PT);
+  auto *c = new (std::nothrow) chs_mat;
+  if (!c) return fail(CHS_EALLOC, "alloc");
+  spgemm(PT, AP, *c, nthreads);
+  *Ac = c;
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Classical strength of connection (cpt_SoC_cla.m:100-152).
+// The measure of entry (i,j) is a_ij / max(m_i, m_j), with m_c the minimum of column c of
+// the off-diagonal part including its implicit zeros, replaced by 1 where it is zero; if no
+// column has a negative off-diagonal entry the absolute-value variant is used instead.
+// ----------------------------------------------------------------------------------------
+extern "C" int chs_soc_cla(const chs_mat *Ap, double tau, int8_t *strong) {
+  if (!Ap || !strong) return fail(CHS_EARG, "null");
+  const chs_mat &A = *Ap;
+  const int32_t n = A.nrows;
+  std::vector<double> mincol(A.ncols, 0.0), maxabs(A.ncols, 0.0);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+      const int32_t j = A.ci[e];
+      if (j == i) continue;
+      mincol[j] = std::min(mincol[j], A.v[e]);
+      maxabs[j] = std::max(maxabs[j], std::fabs(A.v[e]));
+    }
+  // The off-diagonal part always holds an implicit zero on the diagonal position, so the
+  // column minimum (and maximum magnitude) includes 0: hence the 0 start values above.
+  bool any_neg = false;
+  for (int32_t j = 0; j < A.ncols; ++j) any_neg |= (mincol[j] < 0.0);
+  if (any_neg) {
+    for (auto &m : mincol)
+      if (m == 0.0) m = 1.0;
+  } else {
+    for (auto &m : maxabs)
+      if (m == 0.0) m = 1.0;
+  }
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+      const int32_t j = A.ci[e];
+      double s = any_neg ? A.v[e] / std::max(mincol[i], mincol[j])
+                         : std::fabs(A.v[e]) / std::max(maxabs[i], maxabs[j]);
+      strong[e] = (j == i) ? 1 : (s > tau ? 1 : -1);
+    }
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Dirichlet marking (mark_isolated_nodes.m) + sequential PMIS (cpt_PMIS.m:174-266).
+// S is the strong off-diagonal pattern.  A node's weight is the number of nodes it
+// strongly influences (column count of S) plus its tie-break draw.  Free nodes are visited
+// in list order; a node becomes coarse when no free strong neighbour (in S + S^T) has a
+// larger weight, and then the nodes it strongly influences become fine.  Removal swaps the
+// last list entry into the visited slot, which is then visited next.
+// ----------------------------------------------------------------------------------------
+extern "C" int chs_pmis(const chs_mat *Ap, const int8_t *strong, const double *rand01,
+                        int32_t *fcnode, int32_t *nc_out) {
+  if (!Ap || !strong || !rand01 || !fcnode) return fail(CHS_EARG, "null");
+  const chs_mat &A = *Ap;
+  const int32_t n = A.nrows;
+  if (A.ncols != n) return fail(CHS_EARG, "pmis: square matrix required");
+  // Column counts of A (Dirichlet marking) and of S (influence weights).
+  std::vector<int32_t> colcnt(n, 0), wcnt(n, 0);
+  chs_mat S;  // strong off-diagonal pattern, rows
+  S.nrows = S.ncols = n;
+  S.rp.assign(n + 1, 0);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+      colcnt[A.ci[e]]++;
+      if (A.ci[e] != i && strong[e] > 0) S.rp[i + 1]++;
+    }
+  for (int32_t i = 0; i < n; ++i) S.rp[i + 1] += S.rp[i];
+  S.ci.resize(S.rp[n]);
+  S.v.assign(S.rp[n], 1.0);
+  for (int32_t i = 0, p = 0; i < n; ++i)
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e)
+      if (A.ci[e] != i && strong[e] > 0) {
+        S.ci[p++] = A.ci[e];
+        wcnt[A.ci[e]]++;
+      }
+  chs_mat ST;
+  transpose_into(S, ST);  // row j of ST = nodes i with S(i,j) != 0 (column j of S)
+  // Symmetrised neighbourhood S + S^T, per node, merged from the sorted rows.
+  std::vector<int32_t> symrp(n + 1, 0);
+  std::vector<int32_t> symci;
+  symci.reserve(2 * S.ci.size());
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t a = S.rp[i], ae = S.rp[i + 1], b = ST.rp[i], be = ST.rp[i + 1];
+    while (a < ae || b < be) {
+      int32_t ca = a < ae ? S.ci[a] : INT32_MAX, cb = b < be ? ST.ci[b] : INT32_MAX;
+      int32_t c = std::min(ca, cb);
+      symci.push_back(c);
+      if (ca == c) ++a;
+      if (cb == c) ++b;
+    }
+    symrp[i + 1] = (int32_t)symci.size();
+  }
+  std::vector<double> w(n), wfree(n);
+  for (int32_t i = 0; i < n; ++i) {
+    fcnode[i] = (colcnt[i] == 1) ? -1 : 0;  // isolated (Dirichlet) nodes are fine
+    w[i] = (double)wcnt[i] + rand01[i];
+    wfree[i] = fcnode[i] == -1 ? -1.0 : w[i];
+  }
+  std::vector<int32_t> list;
+  list.reserve(n);
+  for (int32_t i = 0; i < n; ++i)
+    if (fcnode[i] == 0) list.push_back(i);
+  int64_t len = (int64_t)list.size();
+  int64_t ind = 0;
+  while (len > 0) {
+    const int32_t inod = list[ind];
+    if (fcnode[inod] != 0) {
+      list[ind] = list[len - 1];
+      --len;
+    } else {
+      bool is_coarse = true;
+      const double wi = w[inod];
+      for (int32_t e = symrp[inod]; e < symrp[inod + 1]; ++e)
+        if (wfree[symci[e]] > wi) {
+          is_coarse = false;
+          break;
+        }
+      if (!is_coarse) {
+        ++ind;
+      } else {
+        for (int32_t e = ST.rp[inod]; e < ST.rp[inod + 1]; ++e) {
+          fcnode[ST.ci[e]] = -1;
+          wfree[ST.ci[e]] = -1.0;
+        }
+        fcnode[inod] = 1;
+        wfree[inod] = -1.0;
+        list[ind] = list[len - 1];
+        --len;
+      }
+    }
+    if (ind >= len) ind = 0;
+  }
+  int32_t nc = 0;
+  for (int32_t i = 0; i < n; ++i) fcnode[i] = fcnode[i] > 0 ? nc++ : -1;
+  if (nc_out) *nc_out = nc;
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Classical interpolation (ProlStripe_Classical.cpp:44-160).  Coarse rows are e_c.  A fine
+// row i interpolates from its strong coarse neighbours C_i with
+//   w_ij = -(a_ij + sum_{k in FS_i, s_k != 0} a_ik * min(0,a_kj) / s_k) / d_i,
+//   s_k  = sum_{l in C_i} min(0, a_kl),
+//   d_i  = a_ii + sum_{weak j} a_ij + sum_{k in FS_i, |s_k| <= eps*a_ii} a_ik,
+// FS_i being the strong fine neighbours.  Accumulation orders follow the row orders of A
+// so that the result is reproducible bit for bit.
+// ----------------------------------------------------------------------------------------
+extern "C" int chs_prol_classical(const chs_mat *Ap, const int8_t *strong,
+                                  const int32_t *fcnode, int32_t nc, int nthreads,
+                                  chs_mat **Pout) {
+  if (!Ap || !strong || !fcnode || !Pout) return fail(CHS_EARG, "null");
+  const chs_mat &A = *Ap;
+  const int32_t n = A.nrows;
+  nthreads = nthreads_or_default(nthreads);
+  const double eps = std::numeric_limits<double>::epsilon();
+  std::vector<std::vector<int32_t>> rc(n);
+  std::vector<std::vector<double>> rv(n);
+  int bad = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : bad)
+  {
+    std::vector<int32_t> slot(n, 0);  // 1-based position of a strong C neighbour, 0 if none
+    std::vector<int32_t> cc, fs, pos;
+    std::vector<double> cv, fv, akj, w;
+#pragma omp for schedule(dynamic, 1024)
+    for (int32_t i = 0; i < n; ++i) {
+      if (fcnode[i] >= 0) {
+        rc[i].push_back(fcnode[i]);
+        rv[i].push_back(1.0);
+        continue;
+      }
+      cc.clear(); cv.clear(); fs.clear(); fv.clear();
+      double aii = 0.0, denom = 0.0;
+      bool has_diag = false;
+      for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+        const int32_t j = A.ci[e];
+        if (j == i) {
+          aii = A.v[e];
+          denom += aii;
+          has_diag = true;
+        } else if (strong[e] > 0) {
+          if (fcnode[j] >= 0) {
+            cc.push_back(j);
+            cv.push_back(A.v[e]);
+            slot[j] = (int32_t)cc.size();
+          } else {
+            fs.push_back(j);
+            fv.push_back(A.v[e]);
+          }
+        } else {
+          denom += A.v[e];
+        }
+      }
+      if (!has_diag) bad++;
+      w.assign(cv.begin(), cv.end());
+      for (size_t k = 0; k < fs.size(); ++k) {
+        const int32_t kn = fs[k];
+        const double aik = fv[k];
+        double ext = 0.0;
+        akj.clear();
+        pos.clear();
+        for (int32_t f = A.rp[kn]; f < A.rp[kn + 1]; ++f) {
+          const int32_t l = A.ci[f];
+          if (slot[l] > 0) {
+            const double a = std::min(0.0, A.v[f]);
+            ext += a;
+            akj.push_back(a);
+            pos.push_back(slot[l] - 1);
+          }
+        }
+        if (std::fabs(ext) > eps * aii) {
+          for (size_t q = 0; q < akj.size(); ++q) w[pos[q]] += aik * akj[q] / ext;
+        } else {
+          denom += aik;
+        }
+      }
+      // Map to coarse numbering; coarse indices increase with fine indices and cc is in
+      // ascending fine order, so the row is already sorted by coarse column.
+      auto &oc = rc[i];
+      auto &ov = rv[i];
+      oc.resize(cc.size());
+      ov.resize(cc.size());
+      for (size_t q = 0; q < cc.size(); ++q) {
+        slot[cc[q]] = 0;
+        oc[q] = fcnode[cc[q]];
+        ov[q] = -w[q] / denom;
+      }
+    }
+  }
+  if (bad) return fail(CHS_EARG, "classical prolongation: fine row without a diagonal entry");
+  auto *P = new (std::nothrow) chs_mat;
+  if (!P) return fail(CHS_EALLOC, "alloc");
+  P->nrows = n;
+  P->ncols = nc;
+  P->rp.assign(n + 1, 0);
+  for (int32_t i = 0; i < n; ++i) P->rp[i + 1] = P->rp[i] + (int32_t)rc[i].size();
+  P->ci.resize(P->rp[n]);
+  P->v.resize(P->rp[n]);
+  for (int32_t i = 0; i < n; ++i) {
+    std::copy(rc[i].begin(), rc[i].end(), P->ci.begin() + P->rp[i]);
+    std::copy(rv[i].begin(), rv[i].end(), P->v.begin() + P->rp[i]);
+  }
+  *Pout = P;
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Adaptive symmetric FSAI (cpt_afsai_coef.cpp:316-554, compute_local_fsai.cpp:6-116).
+//
+// Row i grows a pattern P_i of columns j < i.  At every step the candidates are the
+// columns j < i, j not in P_i, of row i of A and of the rows of A indexed by P_i; the
+// candidate gradient is g_j = a_ij + sum_{k in P_i} x_k a_kj (x = current solution).  The
+// step_size candidates of largest |g_j| join P_i (selection by the first-element-pivot
+// partial partition below, which fixes how ties are broken), then A_PP x = -a_Pi is solved
+// by Cholesky.  Growth stops after nstep steps, when the pattern does not grow, or when the
+// Kaporin quantity a_Pi^T A_PP^-1 a_Pi changes by less than eps relative.  The row is
+// s*[x; 1] with s = 1/sqrt(a_ii - a_Pi^T A_PP^-1 a_Pi), diagonal last.
+// ----------------------------------------------------------------------------------------
+namespace {
+
+// Partial partition: on exit the first ncut entries of (val, idx) hold the ncut largest
+// |val|.  Lomuto partition around the first element of the active range with a strict '>'
+// comparison (cpt_afsai_coef.cpp:40-89 is the procedure whose tie behaviour this keeps).
+void select_largest(int n, int ncut, double *val, int32_t *idx) {
+  int first = 0, last = n - 1;
+  if (ncut < 1 || ncut > n) return;
+  const int target = ncut - 1;
+  for (;;) {
+    int mid = first;
+    const double pivot = std::fabs(val[first]);
+    for (int j = first + 1; j <= last; ++j)
+      if (std::fabs(val[j]) > pivot) {
+        ++mid;
+        std::swap(val[mid], val[j]);
+        std::swap(idx[mid], idx[j]);
+      }
+    std::swap(val[mid], val[first]);
+    std::swap(idx[mid], idx[first]);
+    if (mid == target) return;
+    if (mid > target)
+      last = mid - 1;
+    else
+      first = mid + 1;
+  }
+}
+
+// In-place lower Cholesky of the leading m x m block of K (column-major, leading dim ld).
+bool chol_lower(int m, double *K, int ld) {
+  for (int j = 0; j < m; ++j) {
+    double d = K[j + j * ld];
+    for (int k = 0; k < j; ++k) d -= K[j + k * ld] * K[j + k * ld];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    K[j + j * ld] = d;
+    for (int i = j + 1; i < m; ++i) {
+      double s = K[i + j * ld];
+      for (int k = 0; k < j; ++k) s -= K[i + k * ld] * K[j + k * ld];
+      K[i + j * ld] = s / d;
+    }
+  }
+  return true;
+}
+
+void chol_solve(int m, const double *L, int ld, double *b) {
+  for (int i = 0; i < m; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= L[i + k * ld] * b[k];
+    b[i] = s / L[i + i * ld];
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int k = i + 1; k < m; ++k) s -= L[k + i * ld] * b[k];
+    b[i] = s / L[i + i * ld];
+  }
+}
+
+struct FsaiRowWork {
+  std::vector<int32_t> mark;  // 0 untouched, -1 in pattern, >0 candidate slot (1-based)
+  std::vector<int32_t> pat, cand;
+  std::vector<double> grad, K, w, wsav;
+  FsaiRowWork(int32_t n, int mmax)
+      : mark(n, 0), pat(mmax), cand(), grad(), K((size_t)mmax * mmax), w(mmax + 1, 0.0),
+        wsav(mmax + 1, 0.0) {}
+};
+
+// Gathers K = A_PP (lower part, column-major) and w = -a_iP.  Returns false when row i has
+// no entry left of the diagonal (the right-hand side is null).
+bool gather_system(const chs_mat &A, int32_t i, int m, const int32_t *pat, double *K, int ld,
+                   double *w) {
+  for (int c = 0; c < m; ++c) {
+    const int32_t r = pat[c];
+    int32_t e = A.rp[r], ee = A.rp[r + 1];
+    for (int q = c; q < m; ++q) {
+      const int32_t want = pat[q];
+      while (e < ee && A.ci[e] < want) ++e;
+      K[q + c * ld] = (e < ee && A.ci[e] == want) ? A.v[e] : 0.0;
+    }
+  }
+  int32_t e = A.rp[i], ee = A.rp[i + 1];
+  if (e >= ee || A.ci[e] >= i) return false;
+  for (int q = 0; q < m; ++q) {
+    const int32_t want = pat[q];
+    while (e < ee && A.ci[e] < want) ++e;
+    w[q] = (e < ee && A.ci[e] == want) ? -A.v[e] : 0.0;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int chs_afsai(const chs_mat *Ap, int32_t nstep, int32_t step_size, double eps,
+                         int nthreads, chs_mat **Gout, int32_t *is_posdef) {
+  if (!Ap || !Gout || nstep < 0 || step_size < 1) return fail(CHS_EARG, "afsai: arguments");
+  const chs_mat &A = *Ap;
+  const int32_t n = A.nrows;
+  nthreads = nthreads_or_default(nthreads);
+  const int mmax = std::max(1, nstep * step_size);
+  const int kmax = mmax + 1;
+  std::vector<int32_t> rowlen(n);
+  std::vector<int32_t> gcol((size_t)n * kmax);
+  std::vector<double> gval((size_t)n * kmax);
+  int posdef_all = 1;
+  const int chunk = std::max(1, n / (20 * nthreads));
+#pragma omp parallel num_threads(nthreads) reduction(& : posdef_all)
+  {
+    FsaiRowWork W(n, mmax);
+    int posdef = 1;
+#pragma omp for schedule(dynamic, chunk)
+    for (int32_t i = 0; i < n; ++i) {
+      int m = 0;
+      double dk_old = 0.0;
+      bool refine = nstep >= 1;
+      int istep = 0;
+      while (refine) {
+        ++istep;
+        const int m_old = m;
+        // Candidate gradient.
+        W.cand.clear();
+        W.grad.clear();
+        for (int32_t e = A.rp[i]; e < A.rp[i + 1] && A.ci[e] < i; ++e) {
+          const int32_t j = A.ci[e];
+          if (W.mark[j] >= 0) {
+            W.cand.push_back(j);
+            W.grad.push_back(A.v[e]);
+            W.mark[j] = (int32_t)W.cand.size();
+          }
+        }
+        for (int k = 0; k < m; ++k) {
+          const int32_t r = W.pat[k];
+          const double xk = W.w[k];
+          for (int32_t e = A.rp[r]; e < A.rp[r + 1]; ++e) {
+            const int32_t j = A.ci[e];
+            if (j >= i) continue;
+            const int32_t s = W.mark[j];
+            if (s == 0) {
+              W.cand.push_back(j);
+              W.grad.push_back(xk * A.v[e]);
+              W.mark[j] = (int32_t)W.cand.size();
+            } else if (s > 0) {
+              W.grad[s - 1] += xk * A.v[e];
+            }
+          }
+        }
+        const int ncand = (int)W.cand.size();
+        const int ncut = std::min<int>(step_size, ncand);
+        select_largest(ncand, ncut, W.grad.data(), W.cand.data());
+        for (int q = 0; q < ncut; ++q) W.mark[W.cand[q]] = -1;
+        for (int q = ncut; q < ncand; ++q) W.mark[W.cand[q]] = 0;
+        for (int q = 0; q < ncut; ++q) W.pat[m + q] = W.cand[q];
+        m += ncut;
+        std::sort(W.pat.begin(), W.pat.begin() + m);
+        if (m > m_old) {
+          const bool has_rhs = gather_system(A, i, m, W.pat.data(), W.K.data(), mmax, W.w.data());
+          if (has_rhs && m > 0) {
+            if (!chol_lower(m, W.K.data(), mmax)) {
+              posdef = 0;
+            } else {
+              std::copy(W.w.begin(), W.w.end(), W.wsav.begin());
+              chol_solve(m, W.K.data(), mmax, W.w.data());
+            }
+          }
+          double dk_new = 0.0;
+          for (int q = 0; q < m; ++q) dk_new += W.w[q] * W.wsav[q];
+          if (istep == nstep) {
+            refine = false;
+          } else {
+            refine = (std::fabs(dk_new - dk_old) >= eps * dk_old) && (dk_new != 0.0);
+            dk_old = dk_new;
+          }
+        } else {
+          refine = false;
+        }
+      }
+      // Scaling factor from the diagonal entry.
+      double aii = 0.0;
+      bool found = false;
+      for (int32_t e = A.rp[i]; e < A.rp[i + 1] && A.ci[e] <= i; ++e)
+        if (A.ci[e] == i) {
+          aii = A.v[e];
+          found = true;
+          break;
+        }
+      double sc = 0.0;
+      if (!found) {
+        posdef = 0;
+      } else {
+        double d = 0.0;
+        for (int q = 0; q < m; ++q) d += W.wsav[q] * W.w[q];
+        sc = aii - d;
+      }
+      if (sc < 0.0) {
+        posdef = 0;
+        if (m > 0 && found) {
+          gather_system(A, i, m, W.pat.data(), W.K.data(), mmax, W.w.data());
+          if (chol_lower(m, W.K.data(), mmax)) {
+            std::copy(W.w.begin(), W.w.end(), W.wsav.begin());
+            chol_solve(m, W.K.data(), mmax, W.w.data());
+            double d = 0.0;
+            for (int q = 0; q < m; ++q) d += W.wsav[q] * W.w[q];
+            sc = aii - d;
+          }
+        }
+      }
+      if (sc <= 0.0) {
+        posdef = 0;
+        sc = 1.0;
+      }
+      sc = 1.0 / std::sqrt(sc);
+      int32_t *gc = &gcol[(size_t)i * kmax];
+      double *gv = &gval[(size_t)i * kmax];
+      for (int q = 0; q < m; ++q) {
+        gc[q] = W.pat[q];
+        gv[q] = sc * W.w[q];
+        W.mark[W.pat[q]] = 0;
+      }
+      gc[m] = i;
+      gv[m] = sc;
+      rowlen[i] = m + 1;
+    }
+    posdef_all &= posdef;
+  }
+  auto *G = new (std::nothrow) chs_mat;
+  if (!G) return fail(CHS_EALLOC, "alloc");
+  G->nrows = G->ncols = n;
+  G->rp.assign(n + 1, 0);
+  for (int32_t i = 0; i < n; ++i) G->rp[i + 1] = G->rp[i] + rowlen[i];
+  G->ci.resize(G->rp[n]);
+  G->v.resize(G->rp[n]);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int32_t i = 0; i < n; ++i) {
+    std::copy(&gcol[(size_t)i * kmax], &gcol[(size_t)i * kmax] + rowlen[i], G->ci.begin() + G->rp[i]);
+    std::copy(&gval[(size_t)i * kmax], &gval[(size_t)i * kmax] + rowlen[i], G->v.begin() + G->rp[i]);
+  }
+  if (is_posdef) *is_posdef = posdef_all;
+  *Gout = G;
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Largest eigenvalue by Lanczos with full reorthogonalisation.  The tridiagonal's largest
+// eigenvalue is found by Sturm-sequence bisection.
+// ----------------------------------------------------------------------------------------
+namespace {
+
+int sturm_count(int k, const double *a, const double *b, double x) {
+  int c = 0;
+  double q = a[0] - x;
+  if (q < 0) ++c;
+  for (int i = 1; i < k; ++i) {
+    double qq = q == 0.0 ? 1e-300 : q;
+    q = a[i] - x - b[i - 1] * b[i - 1] / qq;
+    if (q < 0) ++c;
+  }
+  return c;  // number of eigenvalues < x
+}
+
+double tridiag_max_eig(int k, const double *a, const double *b) {
+  double lo = a[0], hi = a[0];
+  for (int i = 0; i < k; ++i) {
+    double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i < k - 1 ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  for (int it = 0; it < 200; ++it) {
+    double mid = 0.5 * (lo + hi);
+    if (mid == lo || mid == hi) break;
+    if (sturm_count(k, a, b, mid) >= k) hi = mid; else lo = mid;
+  }
+  return hi;
+}
+
+template <class Op>
+double lanczos_max(int32_t n, Op op, uint64_t seed, int maxit, double tol) {
+  maxit = std::max(2, std::min<int>(maxit, n));
+  std::vector<std::vector<double>> V;
+  std::vector<double> a, b, v(n), w(n);
+  chs_rng_uniform(seed, 7, 0, n, -1.0, 1.0, v.data());
+  double nv = 0.0;
+  for (double t : v) nv += t * t;
+  nv = std::sqrt(nv);
+  for (double &t : v) t /= nv;
+  double lam = 0.0, lam_old = 0.0;
+  for (int j = 0; j < maxit; ++j) {
+    V.push_back(v);
+    op(v.data(), w.data());
+    double alpha = 0.0;
+#pragma omp parallel for reduction(+ : alpha)
+    for (int32_t i = 0; i < n; ++i) alpha += w[i] * v[i];
+    a.push_back(alpha);
+    // Full reorthogonalisation (twice is enough).
+    for (int pass = 0; pass < 2; ++pass)
+      for (auto &u : V) {
+        double d = 0.0;
+#pragma omp parallel for reduction(+ : d)
+        for (int32_t i = 0; i < n; ++i) d += w[i] * u[i];
+#pragma omp parallel for
+        for (int32_t i = 0; i < n; ++i) w[i] -= d * u[i];
+      }
+    double beta = 0.0;
+#pragma omp parallel for reduction(+ : beta)
+    for (int32_t i = 0; i < n; ++i) beta += w[i] * w[i];
+    beta = std::sqrt(beta);
+    lam = tridiag_max_eig((int)a.size(), a.data(), b.data());
+    if (j > 3 && std::fabs(lam - lam_old) <= tol * std::fabs(lam)) break;
+    lam_old = lam;
+    if (beta <= 1e-14 * std::fabs(lam)) break;
+    b.push_back(beta);
+#pragma omp parallel for
+    for (int32_t i = 0; i < n; ++i) v[i] = w[i] / beta;
+  }
+  return lam;
+}
+
+}  // namespace
+
+extern "C" int chs_fsai_omega(const chs_mat *Ap, const chs_mat *Gp, uint64_t seed,
+                              int nthreads, double *omega, double *lambda) {
+  if (!Ap || !Gp) return fail(CHS_EARG, "null");
+  nthreads = nthreads_or_default(nthreads);
+  chs_mat GT;
+  transpose_into(*Gp, GT);
+  const int32_t n = Ap->nrows;
+  std::vector<double> t1(n), t2(n);
+  auto op = [&](const double *x, double *y) {
+    spmv(GT, x, t1.data());
+    spmv(*Ap, t1.data(), t2.data());
+    spmv(*Gp, t2.data(), y);
+  };
+  double lam = lanczos_max(n, op, seed, 40, 1e-7);
+  if (lambda) *lambda = lam;
+  if (omega) *omega = std::min(1.0, 1.9 / lam);
+  return CHS_OK;
+}
+
+extern "C" int chs_bc_fix(const chs_mat *Ap, uint64_t seed, chs_mat **Afix, int32_t *nfixed) {
+  if (!Ap || !Afix) return fail(CHS_EARG, "null");
+  auto *m = new (std::nothrow) chs_mat(*Ap);
+  if (!m) return fail(CHS_EALLOC, "alloc");
+  int32_t cnt = 0;
+  std::vector<int32_t> where;
+  for (int32_t i = 0; i < m->nrows; ++i)
+    for (int32_t e = m->rp[i]; e < m->rp[i + 1]; ++e)
+      if (m->ci[e] == i && m->v[e] == 1.0) where.push_back(e);
+  if (!where.empty()) {
+    auto op = [&](const double *x, double *y) { spmv(*Ap, x, y); };
+    double lmax = lanczos_max(m->nrows, op, seed, 40, 1e-6);
+    for (int32_t e : where) m->v[e] = lmax / 10.0;
+    cnt = (int32_t)where.size();
+  }
+  if (nfixed) *nfixed = cnt;
+  *Afix = m;
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Jacobi smoother factor: G = diag(1/sqrt(a_ii)) (smoother.m:112-132).
+// ----------------------------------------------------------------------------------------
+static int jacobi_factor(const chs_mat &A, chs_mat &G) {
+  G.nrows = G.ncols = A.nrows;
+  G.rp.resize(A.nrows + 1);
+  G.ci.resize(A.nrows);
+  G.v.resize(A.nrows);
+  for (int32_t i = 0; i <= A.nrows; ++i) G.rp[i] = i;
+  for (int32_t i = 0; i < A.nrows; ++i) {
+    double d = 0.0;
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e)
+      if (A.ci[e] == i) d = A.v[e];
+    if (!(d > 0.0)) return fail(CHS_ENOTSPD, "jacobi: non-positive diagonal");
+    G.ci[i] = i;
+    G.v[i] = 1.0 / std::sqrt(d);
+  }
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// The recursion of cpt_aspAMG.m:68-351 for the classical + afsai_sym / jacobi path.
+// ----------------------------------------------------------------------------------------
+extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out) {
+  if (!A0 || !out) return fail(CHS_EARG, "null");
+  chs_params p;
+  if (pp) p = *pp; else chs_params_default(&p);
+  const int nt = nthreads_or_default(p.nthreads);
+  auto h = std::make_unique<chs_hier>();
+  const double t_start = now_s();
+  chs_mat A = *A0;
+  for (int level = 1;; ++level) {
+    chs_hier::Level L;
+    const int32_t n = A.nrows;
+    if (n <= p.max_coarse || level >= p.nlev_max) {
+      L.A = std::move(A);
+      h->lev.push_back(std::move(L));
+      break;
+    }
+    double t0 = now_s();
+    if (p.smoother == 1) {
+      int rc = jacobi_factor(A, L.G);
+      if (rc) return rc;
+    } else {
+      chs_mat *G = nullptr;
+      int32_t pd = 1;
+      int rc = chs_afsai(&A, p.nstep, p.step_size, p.fsai_eps, nt, &G, &pd);
+      if (rc) return rc;
+      L.G = std::move(*G);
+      delete G;
+      if (!pd) return fail(CHS_ENOTSPD, "afsai_sym: matrix deemed indefinite at level " +
+                                            std::to_string(level));
+    }
+    chs_fsai_omega(&A, &L.G, p.seed + level, nt, &L.omega, &L.lambda);
+    double t1 = now_s();
+    std::vector<int8_t> strong(A.nnz());
+    chs_soc_cla(&A, p.soc_tau, strong.data());
+    std::vector<double> r01(n);
+    chs_rng_uniform(p.seed, 100 + level, 0, n, 0.0, 1.0, r01.data());
+    std::vector<int32_t> fc(n);
+    int32_t nc = 0;
+    chs_pmis(&A, strong.data(), r01.data(), fc.data(), &nc);
+    double t2 = now_s();
+    if (nc <= 0 || nc >= n) return fail(CHS_ECONSIST, "coarsening produced no reduction at level " +
+                                                         std::to_string(level));
+    chs_mat *P = nullptr;
+    int rc = chs_prol_classical(&A, strong.data(), fc.data(), nc, nt, &P);
+    if (rc) return rc;
+    double t3 = now_s();
+    chs_mat *Ac = nullptr;
+    rc = chs_galerkin(&A, P, nt, &Ac);
+    if (rc) { delete P; return rc; }
+    double t4 = now_s();
+    h->t[0] += t1 - t0;
+    h->t[1] += t2 - t1;
+    h->t[2] += t3 - t2;
+    h->t[3] += t4 - t3;
+    if (p.verbose)
+      std::fprintf(stderr, "level %d: n=%d nnz=%lld nnzG=%lld nc=%d nnzP=%lld omega=%.6f "
+                   "(smoother %.2fs coarsen %.2fs prol %.2fs galerkin %.2fs)\n",
+                   level, n, (long long)A.nnz(), (long long)L.G.nnz(), nc,
+                   (long long)P->nnz(), L.omega, t1 - t0, t2 - t1, t3 - t2, t4 - t3);
+    L.A = std::move(A);
+    L.P = std::move(*P);
+    delete P;
+    A = std::move(*Ac);
+    delete Ac;
+    h->lev.push_back(std::move(L));
+  }
+  h->t[4] = now_s() - t_start;
+  *out = h.release();
+  return CHS_OK;
+}
+
+extern "C" int chs_hier_nlev(const chs_hier *h) { return h ? (int)h->lev.size() : 0; }
+
+extern "C" const chs_mat *chs_hier_mat(const chs_hier *h, int32_t level, char which) {
+  if (!h || level < 0 || level >= (int32_t)h->lev.size()) return nullptr;
+  const auto &L = h->lev[level];
+  switch (which) {
+    case 'A': return &L.A;
+    case 'P': return level + 1 < (int32_t)h->lev.size() ? &L.P : nullptr;
+    case 'G': return level + 1 < (int32_t)h->lev.size() ? &L.G : nullptr;
+    default: return nullptr;
+  }
+}
+
+extern "C" double chs_hier_omega(const chs_hier *h, int32_t level) {
+  if (!h || level < 0 || level >= (int32_t)h->lev.size()) return 0.0;
+  return h->lev[level].omega;
+}
+
+extern "C" double chs_hier_setup_seconds(const chs_hier *h, int32_t phase) {
+  if (!h || phase < 0 || phase > 4) return 0.0;
+  return h->t[phase];
+}
+
+extern "C" void chs_hier_free(chs_hier *h) { delete h; }

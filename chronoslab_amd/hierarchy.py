@@ -1,0 +1,301 @@
+"""Host-side hierarchy producer and problem generators (ctypes over libchronos_setup.so).
+
+The setup mirrors ChronosLab's `cpt_aspAMG` recursion for the classical + afsai_sym path
+(sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:68-351): classical strength of connection,
+PMIS, classical interpolation, adaptive FSAI smoother with omega = min(1, 1.9/lambda_max),
+Galerkin coarse operators, and a last level of size <= maxCoarseSZ or at level nLevMax.
+
+Matrices cross this module as `Csr` (0-based int32 row pointers and columns, float64 values,
+columns sorted).  `scipy.sparse` is used only for convenience conversions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+
+import numpy as np
+
+from ._lib import load_setup_lib
+
+_I32P = C.POINTER(C.c_int32)
+_F64P = C.POINTER(C.c_double)
+
+
+@dataclasses.dataclass
+class Csr:
+    nrows: int
+    ncols: int
+    rowptr: np.ndarray  # int32 [nrows+1]
+    col: np.ndarray  # int32 [nnz]
+    val: np.ndarray  # float64 [nnz]
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.shape[0])
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+
+        return sp.csr_matrix((self.val, self.col, self.rowptr), shape=(self.nrows, self.ncols))
+
+    @staticmethod
+    def from_scipy(m) -> "Csr":
+        m = m.tocsr()
+        m.sort_indices()
+        m.eliminate_zeros()
+        return Csr(m.shape[0], m.shape[1], m.indptr.astype(np.int32), m.indices.astype(np.int32),
+                   m.data.astype(np.float64))
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        return self.to_scipy() @ x
+
+
+@dataclasses.dataclass
+class Level:
+    A: Csr
+    P: Csr | None = None
+    G: Csr | None = None
+    omega: float = 1.0
+    nupre: int = 1
+    nupost: int = 1
+
+
+@dataclasses.dataclass
+class Hierarchy:
+    levels: list  # list[Level]; the last one carries only A (the coarsest operator)
+    setup_seconds: dict
+
+    @property
+    def nlev(self) -> int:
+        return len(self.levels)
+
+    @property
+    def coarse(self) -> Csr:
+        return self.levels[-1].A
+
+    def describe(self) -> str:
+        rows = []
+        for l, L in enumerate(self.levels):
+            g = L.G.nnz if L.G is not None else 0
+            p = L.P.nnz if L.P is not None else 0
+            rows.append(f"L{l}: n={L.A.nrows} nnzA={L.A.nnz} nnzG={g} nnzP={p} omega={L.omega:.6f}")
+        return "\n".join(rows)
+
+
+def _lib():
+    return load_setup_lib()
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = _lib().chs_last_error().decode()
+        raise RuntimeError(f"chronos setup error {rc}: {msg}")
+
+
+class _Mat:
+    """Owns a chs_mat* handle."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib().chs_mat_free(self.ptr)
+            self.ptr = None
+
+
+def _to_handle(m: Csr) -> _Mat:
+    lib = _lib()
+    out = C.c_void_p()
+    rp = np.ascontiguousarray(m.rowptr, dtype=np.int32)
+    ci = np.ascontiguousarray(m.col, dtype=np.int32)
+    v = np.ascontiguousarray(m.val, dtype=np.float64)
+    _check(lib.chs_mat_from_csr(m.nrows, m.ncols, rp.ctypes.data_as(_I32P), ci.ctypes.data_as(_I32P),
+                                v.ctypes.data_as(_F64P), C.byref(out)))
+    return _Mat(out)
+
+
+def _from_ptr(ptr) -> Csr:
+    lib = _lib()
+    nr, nc, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+    _check(lib.chs_mat_dims(ptr, C.byref(nr), C.byref(nc), C.byref(nnz)))
+    rp = np.empty(nr.value + 1, np.int32)
+    ci = np.empty(nnz.value, np.int32)
+    v = np.empty(nnz.value, np.float64)
+    _check(lib.chs_mat_copy(ptr, rp.ctypes.data_as(_I32P), ci.ctypes.data_as(_I32P), v.ctypes.data_as(_F64P)))
+    return Csr(nr.value, nc.value, rp, ci, v)
+
+
+def _take(ptr) -> Csr:
+    m = _from_ptr(ptr)
+    _lib().chs_mat_free(ptr)
+    return m
+
+
+# ---------------------------------------------------------------------------------------
+# RNG and generators
+# ---------------------------------------------------------------------------------------
+def rng_uniform(seed: int, stream: int, n: int, lo: float = 0.0, hi: float = 1.0, offset: int = 0) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    _lib().chs_rng_uniform(C.c_uint64(seed), C.c_uint64(stream), C.c_int64(offset), C.c_int64(n),
+                           C.c_double(lo), C.c_double(hi), out.ctypes.data_as(_F64P))
+    return out
+
+
+def rng_normal(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    _lib().chs_rng_normal(C.c_uint64(seed), C.c_uint64(stream), C.c_int64(offset), C.c_int64(n),
+                          out.ctypes.data_as(_F64P))
+    return out
+
+
+def poisson7(N: int) -> Csr:
+    out = C.c_void_p()
+    _check(_lib().chs_gen_poisson7(N, C.byref(out)))
+    return _take(out)
+
+
+def lnk_field(N: int, sigma: float, seed: int) -> np.ndarray:
+    out = np.empty(N ** 3, np.float64)
+    _lib().chs_gen_lnk(N, C.c_double(sigma), C.c_uint64(seed), out.ctypes.data_as(_F64P))
+    return out
+
+
+def diffusion(N: int, k: np.ndarray) -> Csr:
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    if k.shape != (N ** 3,):
+        raise ValueError("k must have N^3 entries")
+    out = C.c_void_p()
+    _check(_lib().chs_gen_diffusion(N, k.ctypes.data_as(_F64P), C.byref(out)))
+    return _take(out)
+
+
+def heterogeneous_diffusion(N: int, sigma: float, seed: int) -> tuple[Csr, np.ndarray]:
+    lnk = lnk_field(N, sigma, seed)
+    return diffusion(N, np.exp(lnk)), lnk
+
+
+# ---------------------------------------------------------------------------------------
+# Setup components
+# ---------------------------------------------------------------------------------------
+def soc_cla(A: Csr, tau: float) -> np.ndarray:
+    h = _to_handle(A)
+    out = np.empty(A.nnz, np.int8)
+    _check(_lib().chs_soc_cla(h.ptr, C.c_double(tau), out.ctypes.data_as(C.POINTER(C.c_int8))))
+    return out
+
+
+def pmis(A: Csr, strong: np.ndarray, rand01: np.ndarray) -> tuple[np.ndarray, int]:
+    h = _to_handle(A)
+    strong = np.ascontiguousarray(strong, np.int8)
+    rand01 = np.ascontiguousarray(rand01, np.float64)
+    fc = np.empty(A.nrows, np.int32)
+    nc = C.c_int32()
+    _check(_lib().chs_pmis(h.ptr, strong.ctypes.data_as(C.POINTER(C.c_int8)), rand01.ctypes.data_as(_F64P),
+                           fc.ctypes.data_as(_I32P), C.byref(nc)))
+    return fc, nc.value
+
+
+def prol_classical(A: Csr, strong: np.ndarray, fcnode: np.ndarray, nc: int, nthreads: int = 0) -> Csr:
+    h = _to_handle(A)
+    strong = np.ascontiguousarray(strong, np.int8)
+    fcnode = np.ascontiguousarray(fcnode, np.int32)
+    out = C.c_void_p()
+    _check(_lib().chs_prol_classical(h.ptr, strong.ctypes.data_as(C.POINTER(C.c_int8)),
+                                     fcnode.ctypes.data_as(_I32P), nc, nthreads, C.byref(out)))
+    return _take(out)
+
+
+def afsai(A: Csr, nstep: int = 15, step_size: int = 1, eps: float = 1e-3, nthreads: int = 0) -> tuple[Csr, bool]:
+    h = _to_handle(A)
+    out = C.c_void_p()
+    pd = C.c_int32()
+    _check(_lib().chs_afsai(h.ptr, nstep, step_size, C.c_double(eps), nthreads, C.byref(out), C.byref(pd)))
+    return _take(out), bool(pd.value)
+
+
+def fsai_omega(A: Csr, G: Csr, seed: int = 1, nthreads: int = 0) -> tuple[float, float]:
+    ha, hg = _to_handle(A), _to_handle(G)
+    om, lam = C.c_double(), C.c_double()
+    _check(_lib().chs_fsai_omega(ha.ptr, hg.ptr, C.c_uint64(seed), nthreads, C.byref(om), C.byref(lam)))
+    return om.value, lam.value
+
+
+def galerkin(A: Csr, P: Csr, nthreads: int = 0) -> Csr:
+    ha, hp = _to_handle(A), _to_handle(P)
+    out = C.c_void_p()
+    _check(_lib().chs_galerkin(ha.ptr, hp.ptr, nthreads, C.byref(out)))
+    return _take(out)
+
+
+def transpose(M: Csr) -> Csr:
+    h = _to_handle(M)
+    out = C.c_void_p()
+    _check(_lib().chs_transpose(h.ptr, C.byref(out)))
+    return _take(out)
+
+
+def bc_fix(A: Csr, seed: int = 1) -> tuple[Csr, int]:
+    """computeAMG.m:5-10: unit diagonal entries become lambda_max(A)/10."""
+    h = _to_handle(A)
+    out = C.c_void_p()
+    nf = C.c_int32()
+    _check(_lib().chs_bc_fix(h.ptr, C.c_uint64(seed), C.byref(out), C.byref(nf)))
+    return _take(out), nf.value
+
+
+@dataclasses.dataclass
+class SetupParams:
+    """Fields of parm_AMG / parm_COARSEN / parm_SMOOTH read by the classical + FSAI path."""
+
+    nlev_max: int = 10
+    max_coarse: int = 1000
+    soc_tau: float = 0.25
+    nstep: int = 15
+    step_size: int = 1
+    fsai_eps: float = 1e-3
+    nupre: int = 1
+    nupost: int = 1
+    smoother: str = "afsai_sym"
+    nthreads: int = 0
+    seed: int = 20240607
+    verbose: bool = False
+
+
+class _CParams(C.Structure):
+    _fields_ = [("nlev_max", C.c_int32), ("max_coarse", C.c_int32), ("soc_tau", C.c_double),
+                ("nstep", C.c_int32), ("step_size", C.c_int32), ("fsai_eps", C.c_double),
+                ("nupre", C.c_int32), ("nupost", C.c_int32), ("smoother", C.c_int32),
+                ("nthreads", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32)]
+
+
+def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
+    p = params or SetupParams()
+    if p.nupre != 1 or p.nupost != 1:
+        raise ValueError("the V-cycle supports nupre = nupost = 1 (the reference's parameter files)")
+    sm = {"afsai_sym": 0, "jacobi": 1}[p.smoother.lower()]
+    cp = _CParams(p.nlev_max, p.max_coarse, p.soc_tau, p.nstep, p.step_size, p.fsai_eps, p.nupre,
+                  p.nupost, sm, p.nthreads, p.seed, int(p.verbose))
+    lib = _lib()
+    ha = _to_handle(A)
+    hh = C.c_void_p()
+    _check(lib.chs_build(ha.ptr, C.byref(cp), C.byref(hh)))
+    try:
+        nlev = lib.chs_hier_nlev(hh)
+        levels = []
+        for l in range(nlev):
+            last = l == nlev - 1
+            Al = _from_ptr(lib.chs_hier_mat(hh, l, b"A"))
+            if last:
+                levels.append(Level(Al))
+            else:
+                levels.append(Level(Al, _from_ptr(lib.chs_hier_mat(hh, l, b"P")),
+                                    _from_ptr(lib.chs_hier_mat(hh, l, b"G")),
+                                    lib.chs_hier_omega(hh, l), p.nupre, p.nupost))
+        secs = {k: lib.chs_hier_setup_seconds(hh, i)
+                for i, k in enumerate(["smoother", "coarsen", "prolong", "galerkin", "total"])}
+    finally:
+        lib.chs_hier_free(hh)
+    # The finest level's A is the operator the caller passed (possibly BC-modified).
+    return Hierarchy(levels, secs)

@@ -1,0 +1,123 @@
+/*
+ * chronos_amg.h — C ABI of the MI355X (gfx950) AMG solve phase.
+ *
+ * The AMG hierarchy (A_l, P_l, FSAI factors G_l, coarsest operator) is uploaded once; HIP
+ * kernels then run the V-cycle and the Krylov loops.  The V-cycle runs in FP64 or in FP32
+ * (CHR_FP32_CYCLE: A_l, P_l, G_l and the cycle vectors in FP32, coarsest solve in FP64)
+ * inside an FP64 outer PCG.
+ *
+ * Conventions
+ *   - Matrices: 0-based CSR, int32 row pointers and columns, columns sorted, FP64 values,
+ *     host memory, borrowed for the duration of the call.
+ *   - Dense vectors/blocks: FP64, column-major n x k with leading dimension ld (MATLAB
+ *     layout).  `ptr_kind` says whether they are host (CHR_PTR_HOST) or device
+ *     (CHR_PTR_DEVICE) pointers; device buffers must be ready when the call is made.
+ *   - Every entry point returns 0 or a negative CHR_E* code; chr_last_error() returns a
+ *     thread-local message.  No exception crosses the ABI.  The caller owns every buffer it
+ *     passes; library objects are opaque and released by their *_destroy call.
+ *   - A handle is bound to one HIP stream created by the library and is not thread safe;
+ *     distinct handles may be used concurrently.  Calls return when their results are
+ *     complete.
+ *
+ * Reference interfaces replaced (ChronosLab, paths relative to the reference root):
+ *   chr_hier_create    the hierarchy struct list AMG_hrc built by
+ *                      sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:82-130,288-302,347
+ *                      (fields A, P, omega, nupre, nupost, smootherOp.left/right; L,S last)
+ *   chr_vcycle_apply   M = @(x) AMG_Vcycle(AMG_prec, A, x)  (sources/utils/computeAMG.m:49,
+ *                      sources/Preconditioner/AMG/aspAMG/AMG_Vcycle.m:1-54), called with an
+ *                      n x k block as in sources/LinearSolver/PCG_CJ.m:38 and
+ *                      sources/EigenSolver/lobpcg.m:216-218
+ *   chr_pcg_solve      [x,iter,resvec,flag] = PCG_CJ(prodM,prodA,b,x0,itmax,tol)
+ *                      (sources/LinearSolver/PCG_CJ.m:1-73)
+ *   chr_spmv           prodA = @(x) A*x  (sources/Drivers/driver_AMG/aspAMG_main.m:200)
+ */
+#ifndef CHRONOS_AMG_H
+#define CHRONOS_AMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CHR_OK = 0,
+  CHR_EARG = -1,      /* bad argument */
+  CHR_EALLOC = -2,    /* device or host allocation failure */
+  CHR_EHIP = -3,      /* HIP runtime error */
+  CHR_ENOTSPD = -4,   /* coarsest operator not positive definite */
+  CHR_EBREAKDOWN = -5 /* Krylov breakdown (zero or non-finite p^T A p) */
+};
+
+enum { CHR_FP64 = 0, CHR_FP32_CYCLE = 1 };
+enum { CHR_PTR_HOST = 0, CHR_PTR_DEVICE = 1 };
+
+typedef struct chr_csr {
+  int32_t nrows, ncols;
+  const int32_t *rowptr;
+  const int32_t *col;
+  const double *val;
+} chr_csr;
+
+typedef struct chr_level {
+  chr_csr A;       /* level operator; level 0: the operator the V-cycle uses */
+  chr_csr P;       /* prolongation n_l x n_{l+1}; unused on the last level */
+  chr_csr G;       /* FSAI factor (lower triangular, n_l x n_l); unused on the last level */
+  double omega;    /* smoother damping: z = omega * G^T (G r) */
+  int32_t nupre;   /* pre-smoothing sweeps */
+  int32_t nupost;  /* post-smoothing sweeps */
+} chr_level;
+
+typedef struct chr_hier chr_hier;
+typedef struct chr_mat chr_mat;
+
+/* Per-hierarchy figures used for bandwidth accounting. */
+typedef struct chr_hier_stats {
+  int32_t nlev;
+  int32_t precision;
+  int64_t n[16];              /* rows per level */
+  double vcycle_bytes;        /* algorithmic bytes of one V-cycle (k = 1) */
+  double level_bytes[16];     /* the same, per level (last entry: coarsest solve) */
+  double pcg_iter_bytes;      /* V-cycle + outer FP64 SpMV + 18 n_0 * 8 vector bytes */
+  double device_bytes;        /* device memory held by the hierarchy */
+} chr_hier_stats;
+
+const char *chr_last_error(void);
+
+/* Select the HIP device used by subsequent calls on this thread. */
+int chr_init(int32_t device);
+
+/* Upload a hierarchy of nlev levels (the last one carries only A).  precision is
+ * CHR_FP64 or CHR_FP32_CYCLE.  Transposes of P and G are formed on upload. */
+int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t precision, chr_hier **out);
+int chr_hier_destroy(chr_hier *h);
+int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st);
+
+/* x = M b for an n x k block (k <= 16). */
+int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t ldb, double *x,
+                     int64_t ldx, int32_t ptr_kind);
+
+/* Outer operator: a device copy of a CSR matrix in FP64. */
+int chr_mat_create(const chr_csr *A, chr_mat **out);
+int chr_mat_destroy(chr_mat *m);
+/* y = A x (FP64). */
+int chr_spmv(chr_mat *A, const double *x, double *y, int32_t ptr_kind);
+
+/* PCG_CJ: preconditioner = V-cycle of h; prodA = A (NULL: the level-0 operator of h in
+ * FP64).  x holds x0 on entry and the solution on exit.  resvec has room for itmax+1
+ * entries (may be NULL).  flag = 0 converged, 1 not converged (PCG_CJ.m:123-125). */
+int chr_pcg_solve(chr_hier *h, chr_mat *A, const double *b, double *x, int32_t itmax,
+                  double tol, int32_t ptr_kind, int32_t *iter, double *resvec, int32_t *flag);
+
+/* Timing helpers (HIP events on the handle's stream, device-resident data):
+ * mean milliseconds of one V-cycle (k = 1) over reps launches after one warm-up. */
+int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms);
+/* Mean milliseconds of one launch of a level SpMV: which = 'A', 'G', 'T' (G^T), 'P',
+ * 'R' (P^T).  bytes receives the algorithmic bytes of one launch. */
+int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t reps, double *ms,
+                  double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+"""GPU parity of the HIP solve phase against the CPU oracle on the same hierarchy."""
+import numpy as np
+import pytest
+
+import oracle
+from chronoslab_amd import hierarchy as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(kind, N, seed=3):
+    if kind == "poisson":
+        return H.poisson7(N)
+    return H.heterogeneous_diffusion(N, 2.0, seed)[0]
+
+
+@pytest.fixture(scope="module")
+def case(request):
+    return request.param
+
+
+@pytest.mark.parametrize("kind,N", [("poisson", 12), ("diffusion", 20), ("poisson", 32)])
+def test_vcycle_fp64_matches_oracle(gpu, kind, N):
+    A = _problem(kind, N)
+    hier = H.build(A)
+    ref = oracle.Hier(hier)
+    M = gpu.AMGPreconditioner(hier, "fp64")
+    b = H.rng_uniform(5, 2, A.nrows, -1, 1)
+    x = M(b)
+    xr = ref.vcycle(b)
+    rel = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    print(f"{kind}{N} fp64 V-cycle rel diff {rel:.3e}", flush=True)
+    assert rel <= 1e-12
+
+
+@pytest.mark.parametrize("kind,N", [("poisson", 12), ("diffusion", 20)])
+def test_vcycle_fp32_close_to_oracle(gpu, kind, N):
+    A = _problem(kind, N)
+    hier = H.build(A)
+    ref = oracle.Hier(hier)
+    M = gpu.AMGPreconditioner(hier, "fp32")
+    b = H.rng_uniform(5, 2, A.nrows, -1, 1)
+    rel = np.linalg.norm(M(b) - ref.vcycle(b)) / np.linalg.norm(ref.vcycle(b))
+    print(f"{kind}{N} fp32 V-cycle rel diff {rel:.3e}", flush=True)
+    assert rel <= 1e-4  # FP32 cycle: single-precision rounding through ~L levels
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("kind,N", [("poisson", 32), ("diffusion", 24)])
+def test_pcg_matches_oracle(gpu, prec, kind, N):
+    A = _problem(kind, N)
+    hier = H.build(A)
+    ref = oracle.Hier(hier)
+    b = H.rng_uniform(9, 2, A.nrows, -1, 1)
+    xr, itr, rvr, flr = ref.pcg(A, b, itmax=200, tol=1e-8)
+    M = gpu.AMGPreconditioner(hier, prec)
+    x, it, rv, fl = M.pcg(b, itmax=200, tol=1e-8)
+    err = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    tres = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+    print(f"{kind}{N} {prec}: it {it} oracle {itr} err {err:.3e} true-res {tres:.3e}", flush=True)
+    assert fl == 0 and abs(it - itr) <= 1
+    assert tres <= 1e-8
+    if prec == "fp64":
+        assert err <= 1e-10
+        np.testing.assert_allclose(rv[: min(it, itr) + 1], rvr[: min(it, itr) + 1], rtol=1e-7)
