@@ -1,0 +1,194 @@
+"""Benchmark: time-to-1e-8 of AMG-preconditioned PCG_CJ on one MI355X.
+
+Workload: 3D heterogeneous diffusion (porous flow), 7-point finite-volume stencil on a
+128^3 grid (n = 2,097,152, nnz = 14,581,760), ln k ~ N(0, 2^2) i.i.d., harmonic face
+transmissibilities, Dirichlet on the x = 0 and x = 1 faces, RHS U[-1,1].  Classical AMG
+(CLA strength 0.25, PMIS, classical interpolation) with the adaptive FSAI smoother
+(nstep 15, step 1, eps 1e-3), V(1,1), coarsest size <= 1000.
+
+A step is one PCG_CJ solve from x0 = 0 to relative residual 1e-8 with b already resident in
+HBM.  `value` is the mean time-to-tolerance of the FP32-cycle solve (FP32 V-cycle inside
+FP64 PCG); the FP64-cycle solve, the V-cycle bandwidth and the CPU oracle are reported
+beside it.  The hierarchy setup (host, OpenMP) is timed separately and is not in `value`.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_info():
+    model = platform.processor() or "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=128, help="grid edge (default 128)")
+    ap.add_argument("--sigma", type=float, default=2.0)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from chronoslab_amd import amg, hierarchy as H
+
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    amg.init(0)
+    N = args.N
+    t0 = time.perf_counter()
+    A, lnk = H.heterogeneous_diffusion(N, args.sigma, args.seed)
+    b_host = H.rng_uniform(args.seed, 2, A.nrows, -1.0, 1.0)
+    hier = H.build(A)
+    t_setup = time.perf_counter() - t0
+    log(f"[bench] {N}^3 n={A.nrows} nnz={A.nnz} setup {t_setup:.1f}s\n{hier.describe()}")
+
+    dev = torch.device("cuda:0")
+    b = torch.from_numpy(b_host).to(dev)
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+
+    def run_solves(M, steps, warmup):
+        for _ in range(warmup):
+            x.zero_()
+            torch.cuda.synchronize()
+            M.pcg_device(b.data_ptr(), x.data_ptr(), itmax=500, tol=args.tol)
+        times, its = [], []
+        for _ in range(steps):
+            x.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            it, rv, fl = M.pcg_device(b.data_ptr(), x.data_ptr(), itmax=500, tol=args.tol)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+            its.append(it)
+            if fl != 0:
+                raise RuntimeError("PCG did not converge")
+        xs = x.cpu().numpy()
+        true_res = float(np.linalg.norm(b_host - A.matvec(xs)) / np.linalg.norm(b_host))
+        return times, its, xs, true_res
+
+    res = {}
+    precs = ["fp32", "fp64"] if args.only == "both" else [args.only]
+    for prec in precs:
+        M = amg.AMGPreconditioner(hier, prec)
+        times, its, xs, tres = run_solves(M, args.steps, args.warmup)
+        st = M.stats()
+        vms = M.time_vcycle(20)
+        gms, gbytes = M.time_spmv(0, "G", 20)
+        tms, tbytes = M.time_spmv(0, "T", 20)
+        ams, abytes = M.time_spmv(0, "A", 20)
+        res[prec] = dict(times=times, its=its, x=xs, true_res=tres, stats=st, vcycle_ms=vms,
+                         g_ms=gms, g_bytes=gbytes, t_ms=tms, t_bytes=tbytes, a_ms=ams, a_bytes=abytes)
+        log(f"[bench] {prec}: solve {1e3 * np.mean(times):.3f} ms (min {1e3 * np.min(times):.3f}) iters {its[0]} "
+            f"true-res {tres:.2e} | V-cycle {vms:.3f} ms = {st['vcycle_bytes'] / vms / 1e6:.0f} GB/s "
+            f"({st['vcycle_bytes'] / 1e9:.3f} GB) | G0 {gms * 1e3:.1f} us {gbytes / gms / 1e6:.0f} GB/s | "
+            f"GT0 {tms * 1e3:.1f} us {tbytes / tms / 1e6:.0f} GB/s | A0 {ams * 1e3:.1f} us {abytes / ams / 1e6:.0f} GB/s")
+        del M
+
+    # CPU leg: the oracle (FP64 restatement of AMG_Vcycle + PCG_CJ) on the same hierarchy.
+    cpu = None
+    err_vs_oracle = {}
+    if not args.no_cpu:
+        import oracle
+
+        model, cores = _cpu_info()
+        oh = oracle.Hier(hier)
+        oracle.set_threads(cores)
+        oh.pcg(A, b_host, itmax=500, tol=args.tol)  # warm-up
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            xo, ito, rvo, flo = oh.pcg(A, b_host, itmax=500, tol=args.tol)
+            ts.append(time.perf_counter() - t)
+        oracle.set_threads(1)
+        t = time.perf_counter()
+        oh.pcg(A, b_host, itmax=500, tol=args.tol)
+        t1 = time.perf_counter() - t
+        oracle.set_threads(0)
+        for prec, r in res.items():
+            err_vs_oracle[prec] = float(np.linalg.norm(r["x"] - xo) / np.linalg.norm(xo))
+        cpu = {"value": float(np.median(ts)), "unit": "s", "cores": cores, "kind": "port",
+               "sample": (f"full oracle PCG_CJ solve of the same {N}^3 system to {args.tol:g} "
+                          f"({ito} iterations), median of 3 after a warm-up, OMP threads = {cores} "
+                          f"({model}); 1-thread solve {t1:.2f} s"),
+               "one_thread_s": t1, "iters": ito}
+        log(f"[bench] cpu oracle: {np.median(ts):.3f} s ({ito} it) on {cores} threads; 1 thread {t1:.2f} s")
+
+    main_prec = "fp32" if "fp32" in res else "fp64"
+    r = res[main_prec]
+    val = float(np.mean(r["times"]))
+    # Dominant kernel: the level-0 FSAI product (G and G^T carry the largest byte share).
+    dom_ms = r["g_ms"]
+    achieved = r["g_bytes"] / dom_ms / 1e6
+    out = {
+        "metric": "time-to-1e-8 residual (s) & V-cycle HBM GB/s, FP32-cycle vs FP64, 1 GPU",
+        "value": val,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": val * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle in f64 PCG" if main_prec == "fp32" else "f64",
+        "data": "synthetic (seeded counter-based RNG: ln k field and RHS)",
+        "config": {"workload": f"heterogeneous diffusion {N}^3 FV 7-pt, ln k~N(0,{args.sigma:g}^2), "
+                               f"AMG(CLA+PMIS+classical, aFSAI) PCG_CJ to {args.tol:g}",
+                   "n": A.nrows, "nnz": A.nnz, "levels": hier.nlev, "iterations": r["its"][0],
+                   "setup_s": t_setup},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "level-0 G SpMV (FSAI factor, FP32)" if main_prec == "fp32" else "level-0 G SpMV (FP64)",
+                     "bytes_per_launch": r["g_bytes"], "avg_launch_ms": dom_ms},
+        "cpu_baseline": cpu,
+        "vcycle": {p: {"ms": q["vcycle_ms"], "bytes": q["stats"]["vcycle_bytes"],
+                       "GBs": q["stats"]["vcycle_bytes"] / q["vcycle_ms"] / 1e6,
+                       "frac_hbm": q["stats"]["vcycle_bytes"] / q["vcycle_ms"] / 1e6 / HBM_PEAK_GBS}
+                   for p, q in res.items()},
+        "solve": {p: {"mean_s": float(np.mean(q["times"])), "min_s": float(np.min(q["times"])),
+                      "iterations": q["its"][0], "true_rel_residual": q["true_res"],
+                      "err_vs_fp64_oracle": err_vs_oracle.get(p)} for p, q in res.items()},
+    }
+    if "fp32" in res and "fp64" in res:
+        out["speedup_fp32_vs_fp64"] = float(np.mean(res["fp64"]["times"]) / np.mean(res["fp32"]["times"]))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

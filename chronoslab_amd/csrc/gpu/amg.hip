@@ -127,7 +127,8 @@ struct DevCsr {
   int64_t nnz = 0;
   bool fp32 = false;
   int lanes = 8;
-  DevBuf rp, ci, val;
+  int32_t nblk = 0;  // CSR-stream row blocks
+  DevBuf rp, ci, val, blk;
   int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
              bool as_fp32) {
     nrows = nr;
@@ -140,6 +141,25 @@ struct DevCsr {
     if ((rc = ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
     if ((rc = val.alloc((as_fp32 ? 4 : 8) * std::max<int64_t>(nnz, 1)))) return rc;
     HIPC(hipMemcpy(rp.p, hrp, sizeof(int32_t) * (nr + 1), hipMemcpyHostToDevice));
+    // Row blocks for the stream kernel: whole rows, <= kStreamCap nonzeros, <= kBlock rows.
+    std::vector<int32_t> br;
+    br.push_back(0);
+    int32_t rows = 0;
+    int64_t cnt = 0;
+    for (int32_t r = 0; r < nr; ++r) {
+      const int64_t len = hrp[r + 1] - hrp[r];
+      if (rows > 0 && (rows == chr::kBlock || cnt + len > chr::kStreamCap)) {
+        br.push_back(r);
+        rows = 0;
+        cnt = 0;
+      }
+      ++rows;
+      cnt += len;
+    }
+    if (nr > 0) br.push_back(nr);
+    nblk = (int32_t)br.size() - 1;
+    if ((rc = blk.alloc(sizeof(int32_t) * br.size()))) return rc;
+    HIPC(hipMemcpy(blk.p, br.data(), sizeof(int32_t) * br.size(), hipMemcpyHostToDevice));
     if (nnz) {
       HIPC(hipMemcpy(ci.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
       if (as_fp32) {
@@ -158,34 +178,16 @@ struct DevCsr {
   double stream_bytes() const {
     return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1);
   }
-  double device_bytes() const { return (double)(rp.bytes + ci.bytes + val.bytes); }
+  double device_bytes() const { return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes); }
 };
 
 template <typename TV, typename TX, typename Epi>
 hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
                          hipStream_t s) {
   if (M.nrows == 0) return hipSuccess;
-  const int64_t threads = (int64_t)M.nrows * M.lanes;
-  const unsigned grid = (unsigned)((threads + chr::kBlock - 1) / chr::kBlock);
-  const int *rp = M.rp.as<int>();
-  const int *ci = M.ci.as<int>();
-  const TV *v = M.val.as<TV>();
-#define CHR_LANES_CASE(L)                                                                  \
-  case L:                                                                                  \
-    hipLaunchKernelGGL((chr::spmv_kernel<L, TV, TX, Epi>), dim3(grid), dim3(chr::kBlock), 0, s, \
-                       M.nrows, rp, ci, v, x, epi, stop);                                  \
-    break;
-  switch (M.lanes) {
-    CHR_LANES_CASE(2)
-    CHR_LANES_CASE(4)
-    CHR_LANES_CASE(8)
-    CHR_LANES_CASE(16)
-    CHR_LANES_CASE(32)
-    CHR_LANES_CASE(64)
-    default:
-      return hipErrorInvalidValue;
-  }
-#undef CHR_LANES_CASE
+  hipLaunchKernelGGL((chr::spmv_stream_kernel<TV, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock), 0, s,
+                     M.nblk, M.blk.as<int>(), M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, epi,
+                     stop);
   return hipGetLastError();
 }
 
@@ -206,7 +208,7 @@ struct chr_mat {
 
 struct PcgWork {
   int32_t n = 0, itmax_cap = 0;
-  DevBuf x, r, z, p0, p1, ap, part, state, resvec;
+  DevBuf x, r, z, p0, p1, ap, part, part_p, state, resvec;
   unsigned nblk = 0;
   chr::PcgState *pinned = nullptr;  // two slots
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -565,16 +567,6 @@ hipError_t launch_pcg_init(const DevCsr &A, unsigned grid, const double *b, cons
   return hipGetLastError();
 }
 
-template <int L>
-hipError_t launch_pcg_p(const DevCsr &A, unsigned grid, const double *z, const double *pold,
-                        double *pnew, double *ap, const double *r, double *part,
-                        chr::PcgState *st, hipStream_t s) {
-  hipLaunchKernelGGL((chr::pcg_p_spmv_kernel<L>), dim3(grid), dim3(chr::kBlock), 0, s, A.nrows,
-                     A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(), z, pold, pnew, ap, r,
-                     part, st);
-  return hipGetLastError();
-}
-
 #define CHR_DISPATCH_LANES(lanes, fn, ...)      \
   switch (lanes) {                              \
     case 2: return fn<2>(__VA_ARGS__);          \
@@ -590,11 +582,6 @@ hipError_t pcg_init(const DevCsr &A, unsigned grid, const double *b, const doubl
   CHR_DISPATCH_LANES(A.lanes, launch_pcg_init, A, grid, b, x, r, part, st, resvec, s)
 }
 
-hipError_t pcg_p(const DevCsr &A, unsigned grid, const double *z, const double *pold, double *pnew,
-                 double *ap, const double *r, double *part, chr::PcgState *st, hipStream_t s) {
-  CHR_DISPATCH_LANES(A.lanes, launch_pcg_p, A, grid, z, pold, pnew, ap, r, part, st, s)
-}
-
 // One PCG_CJ iteration (PCG_CJ.m:88-120), p read from pold and written to pnew.
 hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity) {
   auto *st = w.state.as<chr::PcgState>();
@@ -607,9 +594,11 @@ hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity
                      (const double *)w.z.as<double>(), (const double *)w.ap.as<double>(),
                      w.part.as<double>(), st);
   if ((e = hipGetLastError())) return e;
-  if ((e = pcg_p(A, w.nblk, w.z.as<double>(), pold, pnew, w.ap.as<double>(), w.r.as<double>(),
-                 w.part.as<double>(), st, h.stream)))
-    return e;
+  hipLaunchKernelGGL(chr::pcg_p_stream_kernel, dim3(A.nblk), dim3(chr::kBlock), 0, h.stream, A.nblk,
+                     A.blk.as<int>(), A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(),
+                     (const double *)w.z.as<double>(), (const double *)pold, pnew, w.ap.as<double>(),
+                     (const double *)w.r.as<double>(), w.part_p.as<double>(), st);
+  if ((e = hipGetLastError())) return e;
   hipLaunchKernelGGL(chr::pcg_update_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
                      (const double *)pnew, (const double *)w.ap.as<double>(), w.x.as<double>(),
                      w.r.as<double>(), w.part.as<double>(), st, w.resvec.as<double>());
@@ -633,6 +622,7 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
     // single-block finalisation of the partial sums.
     w.nblk = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, ((int64_t)n + 1023) / 1024));
     if ((rc = w.part.alloc(sizeof(double) * 6 * w.nblk))) return rc;
+    if ((rc = w.part_p.alloc(sizeof(double) * 2 * std::max(A.nblk, 1)))) return rc;
     if ((rc = w.state.alloc(sizeof(chr::PcgState)))) return rc;
     if (!w.pinned) HIPC(hipHostMalloc((void **)&w.pinned, 2 * sizeof(chr::PcgState), hipHostMallocDefault));
     for (auto &e : w.ev)

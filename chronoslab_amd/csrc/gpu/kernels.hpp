@@ -1,10 +1,8 @@
 // Device kernels of the AMG solve phase (gfx950, wave64).
 //
-// CSR SpMV: a group of LANES consecutive lanes owns one row (LANES in {2,...,64}, chosen
-// per matrix from its mean row length), strides over the row's entries, and reduces with
-// cross-lane shuffles inside the group.  The result is handed to an epilogue functor, which
-// fuses the vector operation that follows the product in the V-cycle (scaling by omega,
-// residual, correction) so that no intermediate vector makes an extra trip through HBM.
+// Every product hands its row sums to an epilogue functor, which fuses the vector operation
+// that follows the product in the V-cycle (scaling by omega, residual, correction) so that
+// no intermediate vector makes an extra trip through HBM.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -74,22 +72,98 @@ struct Acc<float> {
   using type = float;
 };
 
-template <int LANES, typename TV, typename TX, typename Epi>
-__global__ __launch_bounds__(kBlock) void spmv_kernel(int nrows, const int *__restrict__ rp,
-                                                      const int *__restrict__ ci,
-                                                      const TV *__restrict__ val,
-                                                      const TX *__restrict__ x, Epi epi,
-                                                      const int *__restrict__ stop) {
+// ---------------------------------------------------------------------------------------
+// CSR-stream SpMV.  The rows are cut on the host into blocks of whole consecutive rows
+// holding at most kStreamCap nonzeros and at most kBlock rows (a row longer than kStreamCap
+// is a block of its own).  A workgroup streams its block's values and column indices with
+// coalesced loads, kStreamPer independent loads in flight per thread, stores the products
+// in LDS, and then each thread sums one row in column order (deterministic) and hands the
+// sum to the epilogue.  No padding and no dependence on the row-length distribution.
+// ---------------------------------------------------------------------------------------
+constexpr int kStreamCap = 2048;
+constexpr int kStreamPer = kStreamCap / kBlock;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8).
+// Map b to a logical block so that each XCD owns one contiguous range of rows and the
+// gathers of neighbouring rows hit that XCD's L2.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb >> 3, rem = nb & 7;
+  const int x = b & 7, j = b >> 3;
+  return x * per + min(x, rem) + j;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum_t(T v, T *lds) {
+  v = group_sum<kWave>(v);
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  T s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / kWave; ++i) s += lds[i];
+  __syncthreads();
+  return s;
+}
+
+// Gathers kStreamPer products of the block's nonzeros into prod[]; `colval` maps a column
+// index to the multiplied operand.
+template <typename TA, typename TV, typename Xf>
+__device__ __forceinline__ void stream_products(int e0, int e1, const int *__restrict__ ci,
+                                                const TV *__restrict__ val, Xf colval, TA *prod) {
+  int c[kStreamPer];
+  TV v[kStreamPer];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kStreamPer; ++k) {
+    const int e = e0 + k * kBlock + tid;
+    const bool in = e < e1;
+    c[k] = in ? ci[e] : 0;
+    v[k] = in ? val[e] : TV(0);
+  }
+#pragma unroll
+  for (int k = 0; k < kStreamPer; ++k) {
+    const int e = e0 + k * kBlock + tid;
+    if (e < e1) prod[k * kBlock + tid] = (TA)v[k] * colval(c[k]);
+  }
+}
+
+template <typename TA, typename TX>
+struct ColLoad {
+  const TX *__restrict__ x;
+  __device__ __forceinline__ TA operator()(int j) const { return (TA)x[j]; }
+};
+
+template <typename TV, typename TX, typename Epi>
+__global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
+                                                             const int *__restrict__ blk_rows,
+                                                             const int *__restrict__ rp,
+                                                             const int *__restrict__ ci,
+                                                             const TV *__restrict__ val,
+                                                             const TX *__restrict__ x, Epi epi,
+                                                             const int *__restrict__ stop) {
   if (stop && *stop) return;
   using TA = typename Acc<TV>::type;
-  const int row = (blockIdx.x * kBlock + threadIdx.x) / LANES;
-  const int lane = threadIdx.x % LANES;
-  if (row >= nrows) return;  // whole row groups leave together
-  const int e0 = rp[row], e1 = rp[row + 1];
-  TA acc = 0;
-  for (int e = e0 + lane; e < e1; e += LANES) acc += (TA)val[e] * (TA)x[ci[e]];
-  acc = group_sum<LANES>(acc);
-  if (lane == 0) epi(row, (double)acc);
+  __shared__ TA prod[kStreamCap];
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int r0 = blk_rows[b], r1 = blk_rows[b + 1];
+  const int e0 = rp[r0], e1 = rp[r1];
+  const int tid = threadIdx.x;
+  if (e1 - e0 > kStreamCap) {  // one long row: strided sum + block reduction
+    TA s = 0;
+    for (int e = e0 + tid; e < e1; e += kBlock) s += (TA)val[e] * (TA)x[ci[e]];
+    s = block_sum_t<TA>(s, prod);
+    if (tid == 0) epi(r0, (double)s);
+    return;
+  }
+  stream_products<TA>(e0, e1, ci, val, ColLoad<TA, TX>{x}, prod);
+  __syncthreads();
+  const int r = r0 + tid;
+  if (r < r1) {
+    const int a = rp[r] - e0, z = rp[r + 1] - e0;
+    TA s = 0;
+    for (int q = a; q < z; ++q) s += prod[q];
+    epi(r, (double)s);
+  }
 }
 
 // Coarsest solve x = Ainv b with the FP64 dense inverse (row-major n x n): one wave per
@@ -227,46 +301,67 @@ __global__ __launch_bounds__(kBlock) void pcg_beta_kernel(int n, const double *_
 }
 
 // p_new = z + beta p_old (formed on the fly at every gathered column), Ap = A p_new,
-// partials of p.Ap and p.r ; last block: ptap, alpha = p.r / ptap.
-template <int LANES>
-__global__ __launch_bounds__(kBlock) void pcg_p_spmv_kernel(
-    int n, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
-    const double *__restrict__ z, const double *__restrict__ p_old, double *__restrict__ p_new,
-    double *__restrict__ ap, const double *__restrict__ r, double *__restrict__ part,
-    PcgState *st) {
+// partials of p.Ap and p.r over the row blocks of A (as in spmv_stream_kernel); last block:
+// ptap, alpha = p.r / ptap.
+struct PColVal {
+  const double *__restrict__ z;
+  const double *__restrict__ p;
+  double beta;
+  __device__ __forceinline__ double operator()(int j) const { return z[j] + beta * p[j]; }
+};
+
+__global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
+    int nblk, const int *__restrict__ blk_rows, const int *__restrict__ rp,
+    const int *__restrict__ ci, const double *__restrict__ val, const double *__restrict__ z,
+    const double *__restrict__ p_old, double *__restrict__ p_new, double *__restrict__ ap,
+    const double *__restrict__ r, double *__restrict__ part, PcgState *st) {
   if (st->stop) return;
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double prod[kStreamCap];
   __shared__ int flag;
   const double beta = st->beta;
-  const int groups_per_block = kBlock / LANES;
-  const int lane = threadIdx.x % LANES;
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int r0 = blk_rows[b], r1 = blk_rows[b + 1];
+  const int e0 = rp[r0], e1 = rp[r1];
+  const int tid = threadIdx.x;
+  PColVal cv{z, p_old, beta};
   double pap = 0.0, prr = 0.0;
-  for (int row = blockIdx.x * groups_per_block + threadIdx.x / LANES; row < n;
-       row += gridDim.x * groups_per_block) {
-    double acc = 0.0;
-    for (int e = rp[row] + lane; e < rp[row + 1]; e += LANES) {
-      const int j = ci[e];
-      acc += val[e] * (z[j] + beta * p_old[j]);
+  if (e1 - e0 > kStreamCap) {
+    double s = 0.0;
+    for (int e = e0 + tid; e < e1; e += kBlock) s += val[e] * cv(ci[e]);
+    s = block_sum_t<double>(s, prod);
+    if (tid == 0) {
+      const double pi = cv(r0);
+      p_new[r0] = pi;
+      ap[r0] = s;
+      pap = pi * s;
+      prr = pi * r[r0];
     }
-    acc = group_sum<LANES>(acc);
-    if (lane == 0) {
-      const double pi = z[row] + beta * p_old[row];
+  } else {
+    stream_products<double>(e0, e1, ci, val, cv, prod);
+    __syncthreads();
+    const int row = r0 + tid;
+    if (row < r1) {
+      const int a = rp[row] - e0, q1 = rp[row + 1] - e0;
+      double s = 0.0;
+      for (int q = a; q < q1; ++q) s += prod[q];
+      const double pi = cv(row);
       p_new[row] = pi;
-      ap[row] = acc;
-      pap += pi * acc;
-      prr += pi * r[row];
+      ap[row] = s;
+      pap = pi * s;
+      prr = pi * r[row];
     }
+    __syncthreads();
   }
-  pap = block_sum(pap, lds);
-  prr = block_sum(prr, lds);
-  if (threadIdx.x == 0) {
-    part[3 * gridDim.x + blockIdx.x] = pap;
-    part[4 * gridDim.x + blockIdx.x] = prr;
+  pap = block_sum_t<double>(pap, prod);
+  prr = block_sum_t<double>(prr, prod);
+  if (tid == 0) {
+    part[b] = pap;
+    part[nblk + b] = prr;
   }
   if (last_block(&st->ticket[2], gridDim.x, &flag)) {
-    double a = sum_partials(part + 3 * gridDim.x, gridDim.x, lds);
-    double c = sum_partials(part + 4 * gridDim.x, gridDim.x, lds);
-    if (threadIdx.x == 0) {
+    double a = sum_partials(part, nblk, prod);
+    double c = sum_partials(part + nblk, nblk, prod);
+    if (tid == 0) {
       st->ptap = a;
       st->pr = c;
       st->alpha = c / a;

@@ -1,0 +1,183 @@
+"""Pins of the CPU oracle (oracle/): closed forms, invariants and library routines that a
+dropped term, a wrong sign or a transposed operand in the restatement would break."""
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.sparse.linalg as spla
+
+import oracle
+from chronoslab_amd import hierarchy as H
+
+
+@pytest.fixture(scope="module")
+def small():
+    A = H.heterogeneous_diffusion(14, 2.0, 6)[0]
+    h = H.build(A, H.SetupParams(max_coarse=150))
+    assert h.nlev >= 3
+    return A, h, oracle.Hier(h)
+
+
+def test_vcycle_linear(small):
+    A, h, oh = small
+    u = H.rng_uniform(1, 2, A.nrows, -1, 1)
+    v = H.rng_uniform(2, 2, A.nrows, -1, 1)
+    lhs = oh.vcycle(2.5 * u - 0.75 * v)
+    rhs = 2.5 * oh.vcycle(u) - 0.75 * oh.vcycle(v)
+    assert np.linalg.norm(lhs - rhs) <= 1e-13 * np.linalg.norm(rhs)
+
+
+def test_vcycle_symmetric(small):
+    A, h, oh = small
+    u = H.rng_uniform(3, 2, A.nrows, -1, 1)
+    v = H.rng_uniform(4, 2, A.nrows, -1, 1)
+    a, b = u @ oh.vcycle(v), v @ oh.vcycle(u)
+    assert abs(a - b) <= 1e-12 * abs(a)
+
+
+def test_vcycle_block_equals_columns(small):
+    A, h, oh = small
+    B = np.asfortranarray(np.stack([H.rng_uniform(s, 2, A.nrows) for s in range(3)], axis=1))
+    X = oh.vcycle(B)
+    for c in range(3):
+        np.testing.assert_array_equal(X[:, c], oh.vcycle(B[:, c].copy()))
+
+
+def test_two_level_closed_form():
+    A = H.poisson7(8)
+    h = H.build(A, H.SetupParams(max_coarse=300))
+    assert h.nlev == 2
+    oh = oracle.Hier(h)
+    L = h.levels[0]
+    Ad, Pd, Gd = (m.to_scipy().toarray() for m in (L.A, L.P, L.G))
+    Ac = h.levels[1].A.to_scipy().toarray()
+    S = L.omega * Gd.T @ Gd
+    n = A.nrows
+    I = np.eye(n)
+    # V(1,1) with zero initial guess: x = S b, x += P Ac^-1 P^T (b - A x), x += S (b - A x).
+    M = S
+    M = M + Pd @ np.linalg.solve(Ac, Pd.T) @ (I - Ad @ S)
+    M = M + S @ (I - Ad @ M)
+    b = H.rng_uniform(5, 2, n, -1, 1)
+    np.testing.assert_allclose(oh.vcycle(b), M @ b, rtol=1e-10, atol=1e-12)
+
+
+def test_single_level_is_exact_solve():
+    A = H.heterogeneous_diffusion(6, 2.0, 3)[0]
+    h = H.build(A)  # n = 216 <= maxCoarseSZ
+    assert h.nlev == 1
+    oh = oracle.Hier(h)
+    b = H.rng_uniform(1, 2, A.nrows, -1, 1)
+    x = oh.vcycle(b)
+    assert np.linalg.norm(A.matvec(x) - b) <= 1e-12 * np.linalg.norm(b)
+    x2, it, rv, fl = oh.pcg(A, b, itmax=10, tol=1e-8)
+    assert it == 1 and fl == 0
+
+
+def test_pcg_matches_textbook_cg(small):
+    A, h, oh = small
+    b = H.rng_uniform(7, 2, A.nrows, -1, 1)
+    x, it, rv, fl = oh.pcg(A, b, itmax=200, tol=1e-10)
+    assert fl == 0
+    Ms = spla.LinearOperator(A.to_scipy().shape, matvec=lambda v: oh.vcycle(np.ascontiguousarray(v)))
+    hist = []
+    spla.cg(A.to_scipy(), b, M=Ms, rtol=1e-10, maxiter=200,
+            callback=lambda xk: hist.append(np.linalg.norm(b - A.matvec(xk)) / np.linalg.norm(b)))
+    m = min(len(hist), it, 8)
+    np.testing.assert_allclose(rv[1:m + 1], hist[:m], rtol=1e-5)
+    assert abs(len(hist) - it) <= 1
+    assert np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) <= 1.01e-10
+
+
+def test_pcg_edge_cases(small):
+    A, h, oh = small
+    z = np.zeros(A.nrows)
+    x, it, rv, fl = oh.pcg(A, z, x0=np.ones(A.nrows), itmax=50)
+    assert it == 0 and fl == 0 and not x.any() and rv[0] == 0
+    b = H.rng_uniform(8, 2, A.nrows, -1, 1)
+    x, it, rv, fl = oh.pcg(A, b, itmax=1, tol=1e-8)
+    assert it == 1 and fl == 1 and len(rv) == 2
+    x, it, rv, fl = oh.pcg(A, b, itmax=0, tol=1e-8)
+    assert it == 0 and fl == 1
+    # A nonzero x0 equal to the solution converges at once.
+    xs = spla.spsolve(A.to_scipy().tocsc(), b)
+    x, it, rv, fl = oh.pcg(A, b, x0=xs, itmax=5, tol=1e-8)
+    assert it <= 1 and rv[0] < 1e-10
+
+
+def test_sygv_matches_scipy():
+    rng = np.random.default_rng(0)
+    for m in (3, 17, 48):
+        X = rng.standard_normal((m, m))
+        Am = X + X.T
+        Y = rng.standard_normal((m, m))
+        Bm = Y @ Y.T + m * np.eye(m)
+        w, V = oracle.sygv(Am, Bm)
+        wr = scipy.linalg.eigh(Am, Bm, eigvals_only=True)
+        np.testing.assert_allclose(w, wr, rtol=1e-12, atol=1e-12 * abs(wr).max())
+        np.testing.assert_allclose(V.T @ Bm @ V, np.eye(m), atol=1e-11)
+        np.testing.assert_allclose(Am @ V, Bm @ V * w, atol=1e-10 * abs(wr).max())
+
+
+def _poisson_eigs(N, k):
+    j = np.arange(1, N + 1)
+    l1 = 2 - 2 * np.cos(j * np.pi / (N + 1))
+    return np.sort((l1[:, None, None] + l1[None, :, None] + l1[None, None, :]).ravel())[:k]
+
+
+def test_lobpcg_closed_form_with_amg():
+    N, k = 16, 8
+    A = H.poisson7(N)
+    h = H.build(A)
+    oh = oracle.Hier(h)
+    X0 = np.asfortranarray(np.stack([H.rng_uniform(1, 10 + c, A.nrows) for c in range(k)], axis=1))
+    niter, lam, X, rh, lh, ierr = oracle.lobpcg(A, oh, X0, itmax=200, tol=1e-8)
+    assert ierr == 0
+    np.testing.assert_allclose(lam, _poisson_eigs(N, k), rtol=1e-8)
+    R = A.to_scipy() @ X - X * lam
+    assert np.linalg.norm(R, axis=0).max() <= 1e-6
+    np.testing.assert_allclose(X.T @ X, np.eye(k), atol=1e-10)
+
+
+def test_lobpcg_unpreconditioned_and_largest():
+    A = H.heterogeneous_diffusion(5, 1.0, 4)[0]
+    Ad = A.to_scipy().toarray()
+    ev = np.linalg.eigvalsh(Ad)
+    X0 = np.asfortranarray(np.stack([H.rng_uniform(2, 20 + c, A.nrows) for c in range(4)], axis=1))
+    niter, lam, X, rh, lh, ierr = oracle.lobpcg(A, None, X0, itmax=400, tol=1e-9)
+    np.testing.assert_allclose(lam, ev[:4], rtol=1e-7)
+    niter, lam, X, rh, lh, ierr = oracle.lobpcg(A, None, X0, largest=True, itmax=400, tol=1e-9)
+    # The final Rayleigh-Ritz (lobpcg.m:437) returns eig's ascending order in both modes.
+    np.testing.assert_allclose(lam, np.sort(ev[::-1][:4]), rtol=1e-7)
+
+
+def test_lobpcg_rank_deficient_start():
+    A = H.poisson7(6)
+    X0 = np.asfortranarray(np.ones((A.nrows, 3)))
+    niter, lam, X, rh, lh, ierr = oracle.lobpcg(A, None, X0, itmax=20, tol=1e-8)
+    assert ierr == 1
+
+
+def test_fv_grad_central_differences():
+    N = 5
+    lnk = H.lnk_field(N, 1.0, 9)
+    lnk_obs = H.lnk_field(N, 1.0, 10)
+    b = H.rng_uniform(3, 2, N ** 3, -1, 1)
+
+    def solve(l):
+        return spla.spsolve(H.diffusion(N, np.exp(l)).to_scipy().tocsc(), b)
+
+    x_obs = solve(lnk_obs)
+
+    def loss(l):
+        d = solve(l) - x_obs
+        return 0.5 * d @ d
+
+    x = solve(lnk)
+    lam = spla.spsolve(H.diffusion(N, np.exp(lnk)).to_scipy().tocsc(), x - x_obs)
+    g = oracle.fv_grad(N, np.exp(lnk), x, lam)
+    h = 1e-6
+    for c in (0, 7, 31, 62, 124):
+        e = np.zeros(N ** 3)
+        e[c] = h
+        fd = (loss(lnk + e) - loss(lnk - e)) / (2 * h)
+        assert g[c] == pytest.approx(fd, rel=1e-5, abs=1e-12)
