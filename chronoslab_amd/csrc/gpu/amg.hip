@@ -142,24 +142,25 @@ struct DevCsr {
     if ((rc = val.alloc((as_fp32 ? 4 : 8) * std::max<int64_t>(nnz, 1)))) return rc;
     HIPC(hipMemcpy(rp.p, hrp, sizeof(int32_t) * (nr + 1), hipMemcpyHostToDevice));
     // Row blocks for the stream kernel: whole rows, <= kStreamCap nonzeros, <= kBlock rows.
-    std::vector<int32_t> br;
-    br.push_back(0);
+    // Stored as (first row, first nonzero) pairs, nblk + 1 of them.
+    std::vector<int2> br;
+    br.push_back(make_int2(0, 0));
     int32_t rows = 0;
     int64_t cnt = 0;
     for (int32_t r = 0; r < nr; ++r) {
       const int64_t len = hrp[r + 1] - hrp[r];
       if (rows > 0 && (rows == chr::kBlock || cnt + len > chr::kStreamCap)) {
-        br.push_back(r);
+        br.push_back(make_int2(r, hrp[r]));
         rows = 0;
         cnt = 0;
       }
       ++rows;
       cnt += len;
     }
-    if (nr > 0) br.push_back(nr);
+    if (nr > 0) br.push_back(make_int2(nr, hrp[nr]));
     nblk = (int32_t)br.size() - 1;
-    if ((rc = blk.alloc(sizeof(int32_t) * br.size()))) return rc;
-    HIPC(hipMemcpy(blk.p, br.data(), sizeof(int32_t) * br.size(), hipMemcpyHostToDevice));
+    if ((rc = blk.alloc(sizeof(int2) * br.size()))) return rc;
+    HIPC(hipMemcpy(blk.p, br.data(), sizeof(int2) * br.size(), hipMemcpyHostToDevice));
     if (nnz) {
       HIPC(hipMemcpy(ci.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
       if (as_fp32) {
@@ -186,7 +187,7 @@ hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
                          hipStream_t s) {
   if (M.nrows == 0) return hipSuccess;
   hipLaunchKernelGGL((chr::spmv_stream_kernel<TV, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock), 0, s,
-                     M.nblk, M.blk.as<int>(), M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, epi,
+                     M.nblk, M.blk.as<int2>(), M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, epi,
                      stop);
   return hipGetLastError();
 }
@@ -560,10 +561,9 @@ namespace {
 
 template <int L>
 hipError_t launch_pcg_init(const DevCsr &A, unsigned grid, const double *b, const double *x,
-                           double *r, double *part, chr::PcgState *st, double *resvec,
-                           hipStream_t s) {
+                           double *r, double *part, hipStream_t s) {
   hipLaunchKernelGGL((chr::pcg_init_kernel<L>), dim3(grid), dim3(chr::kBlock), 0, s, A.nrows,
-                     A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(), b, x, r, part, st, resvec);
+                     A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(), b, x, r, part);
   return hipGetLastError();
 }
 
@@ -578,8 +578,15 @@ hipError_t launch_pcg_init(const DevCsr &A, unsigned grid, const double *b, cons
   }
 
 hipError_t pcg_init(const DevCsr &A, unsigned grid, const double *b, const double *x, double *r,
-                    double *part, chr::PcgState *st, double *resvec, hipStream_t s) {
-  CHR_DISPATCH_LANES(A.lanes, launch_pcg_init, A, grid, b, x, r, part, st, resvec, s)
+                    double *part, hipStream_t s) {
+  CHR_DISPATCH_LANES(A.lanes, launch_pcg_init, A, grid, b, x, r, part, s)
+}
+
+hipError_t pcg_finalize(int kind, int count, double *part, chr::PcgState *st, double *resvec,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(chr::pcg_finalize_kernel, dim3(1), dim3(chr::kFinBlock), 0, s, kind, count,
+                     (const double *)part, st, resvec);
+  return hipGetLastError();
 }
 
 // One PCG_CJ iteration (PCG_CJ.m:88-120), p read from pold and written to pnew.
@@ -590,19 +597,23 @@ hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity
   double *pnew = parity ? w.p0.as<double>() : w.p1.as<double>();
   hipError_t e;
   if ((e = enqueue_vcycle(h, w.r.as<double>(), w.z.as<double>(), stop))) return e;
-  hipLaunchKernelGGL(chr::pcg_beta_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
-                     (const double *)w.z.as<double>(), (const double *)w.ap.as<double>(),
-                     w.part.as<double>(), st);
+  double *part = w.part.as<double>(), *resvec = w.resvec.as<double>();
+  hipLaunchKernelGGL(chr::pcg_zap_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
+                     (const double *)w.z.as<double>(), (const double *)w.ap.as<double>(), part,
+                     (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
+  if ((e = pcg_finalize(chr::kFinBeta, (int)w.nblk, part, st, resvec, h.stream))) return e;
   hipLaunchKernelGGL(chr::pcg_p_stream_kernel, dim3(A.nblk), dim3(chr::kBlock), 0, h.stream, A.nblk,
-                     A.blk.as<int>(), A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(),
+                     A.blk.as<int2>(), A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(),
                      (const double *)w.z.as<double>(), (const double *)pold, pnew, w.ap.as<double>(),
-                     (const double *)w.r.as<double>(), w.part_p.as<double>(), st);
+                     (const double *)w.r.as<double>(), w.part_p.as<double>(), (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
+  if ((e = pcg_finalize(chr::kFinAlpha, A.nblk, w.part_p.as<double>(), st, resvec, h.stream))) return e;
   hipLaunchKernelGGL(chr::pcg_update_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
                      (const double *)pnew, (const double *)w.ap.as<double>(), w.x.as<double>(),
-                     w.r.as<double>(), w.part.as<double>(), st, w.resvec.as<double>());
+                     w.r.as<double>(), part, (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
+  if ((e = pcg_finalize(chr::kFinUpdate, (int)w.nblk, part, st, resvec, h.stream))) return e;
   return hipMemcpyAsync(&w.pinned[parity], st, sizeof(chr::PcgState), hipMemcpyDeviceToHost,
                         h.stream);
 }
@@ -621,7 +632,7 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
     // Fixed grid: enough blocks to fill 256 CUs several times, small enough for the
     // single-block finalisation of the partial sums.
     w.nblk = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, ((int64_t)n + 1023) / 1024));
-    if ((rc = w.part.alloc(sizeof(double) * 6 * w.nblk))) return rc;
+    if ((rc = w.part.alloc(sizeof(double) * 2 * w.nblk))) return rc;
     if ((rc = w.part_p.alloc(sizeof(double) * 2 * std::max(A.nblk, 1)))) return rc;
     if ((rc = w.state.alloc(sizeof(chr::PcgState)))) return rc;
     if (!w.pinned) HIPC(hipHostMalloc((void **)&w.pinned, 2 * sizeof(chr::PcgState), hipHostMallocDefault));
@@ -698,8 +709,9 @@ extern "C" int chr_pcg_solve(chr_hier *h, chr_mat *Aop, const double *b, double 
   st0.tol = tol;
   st0.itmax = itmax;
   HIPC(hipMemcpyAsync(w.state.p, &st0, sizeof(st0), hipMemcpyHostToDevice, h->stream));
-  HIPC(pcg_init(A, w.nblk, db, w.x.as<double>(), w.r.as<double>(), w.part.as<double>(),
-                w.state.as<chr::PcgState>(), w.resvec.as<double>(), h->stream));
+  HIPC(pcg_init(A, w.nblk, db, w.x.as<double>(), w.r.as<double>(), w.part.as<double>(), h->stream));
+  HIPC(pcg_finalize(chr::kFinInit, (int)w.nblk, w.part.as<double>(), w.state.as<chr::PcgState>(),
+                    w.resvec.as<double>(), h->stream));
   HIPC(hipMemcpyAsync(&w.pinned[1], w.state.p, sizeof(chr::PcgState), hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
   chr::PcgState fin = w.pinned[1];

@@ -35,31 +35,36 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Epilogues: called once per row by lane 0 of the row group with the row's product acc.
+// Epilogues: called once per row with the row's product acc.  pre(i) loads the row's own
+// operand (issued at kernel start, before the product is known); operator() finishes.
 // ---------------------------------------------------------------------------------------
 template <typename TO>
 struct EpiStore {  // out = acc
   TO *out;
-  __device__ void operator()(int i, double acc) const { out[i] = (TO)acc; }
+  __device__ __forceinline__ double pre(int) const { return 0.0; }
+  __device__ __forceinline__ void operator()(int i, double acc, double) const { out[i] = (TO)acc; }
 };
 template <typename TO>
 struct EpiScale {  // out = w * acc
   TO *out;
   double w;
-  __device__ void operator()(int i, double acc) const { out[i] = (TO)(w * acc); }
+  __device__ __forceinline__ double pre(int) const { return 0.0; }
+  __device__ __forceinline__ void operator()(int i, double acc, double) const { out[i] = (TO)(w * acc); }
 };
 template <typename TB, typename TO>
 struct EpiResidual {  // out = b - acc
   const TB *b;
   TO *out;
-  __device__ void operator()(int i, double acc) const { out[i] = (TO)((double)b[i] - acc); }
+  __device__ __forceinline__ double pre(int i) const { return (double)b[i]; }
+  __device__ __forceinline__ void operator()(int i, double acc, double bi) const { out[i] = (TO)(bi - acc); }
 };
 template <typename TX, typename TO>
-struct EpiAxpy {  // out = x + w * acc   (x and out may alias: same row, same lane)
+struct EpiAxpy {  // out = x + w * acc   (x and out may alias: same row, same thread)
   const TX *x;
   TO *out;
   double w;
-  __device__ void operator()(int i, double acc) const { out[i] = (TO)((double)x[i] + w * acc); }
+  __device__ __forceinline__ double pre(int i) const { return (double)x[i]; }
+  __device__ __forceinline__ void operator()(int i, double acc, double xi) const { out[i] = (TO)(xi + w * acc); }
 };
 
 // Accumulator type: float products for the FP32 cycle, double otherwise.
@@ -105,25 +110,29 @@ __device__ __forceinline__ T block_sum_t(T v, T *lds) {
   return s;
 }
 
-// Gathers kStreamPer products of the block's nonzeros into prod[]; `colval` maps a column
-// index to the multiplied operand.
+// Stages the products of the block's nonzeros [e0, e1) in prod[].  Indices are clamped to
+// the last nonzero so that every load is unconditional: all 2*kStreamPer index/value loads
+// issue back to back, then the kStreamPer gathers, then the LDS stores (masked).
 template <typename TA, typename TV, typename Xf>
 __device__ __forceinline__ void stream_products(int e0, int e1, const int *__restrict__ ci,
                                                 const TV *__restrict__ val, Xf colval, TA *prod) {
+  if (e1 <= e0) return;
   int c[kStreamPer];
   TV v[kStreamPer];
-  const int tid = threadIdx.x;
+  TA xv[kStreamPer];
+  const int tid = threadIdx.x, last = e1 - 1;
 #pragma unroll
   for (int k = 0; k < kStreamPer; ++k) {
-    const int e = e0 + k * kBlock + tid;
-    const bool in = e < e1;
-    c[k] = in ? ci[e] : 0;
-    v[k] = in ? val[e] : TV(0);
+    const int e = min(e0 + k * kBlock + tid, last);
+    c[k] = ci[e];
+    v[k] = val[e];
   }
 #pragma unroll
+  for (int k = 0; k < kStreamPer; ++k) xv[k] = colval(c[k]);
+#pragma unroll
   for (int k = 0; k < kStreamPer; ++k) {
-    const int e = e0 + k * kBlock + tid;
-    if (e < e1) prod[k * kBlock + tid] = (TA)v[k] * colval(c[k]);
+    const int o = k * kBlock + tid;
+    if (e0 + o < e1) prod[o] = (TA)v[k] * xv[k];
   }
 }
 
@@ -135,7 +144,7 @@ struct ColLoad {
 
 template <typename TV, typename TX, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
-                                                             const int *__restrict__ blk_rows,
+                                                             const int2 *__restrict__ blk,
                                                              const int *__restrict__ rp,
                                                              const int *__restrict__ ci,
                                                              const TV *__restrict__ val,
@@ -145,24 +154,27 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   using TA = typename Acc<TV>::type;
   __shared__ TA prod[kStreamCap];
   const int b = xcd_block(blockIdx.x, nblk);
-  const int r0 = blk_rows[b], r1 = blk_rows[b + 1];
-  const int e0 = rp[r0], e1 = rp[r1];
+  const int2 lo = blk[b], hi = blk[b + 1];  // (first row, first nonzero) of b and b+1
+  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y;
   const int tid = threadIdx.x;
   if (e1 - e0 > kStreamCap) {  // one long row: strided sum + block reduction
     TA s = 0;
     for (int e = e0 + tid; e < e1; e += kBlock) s += (TA)val[e] * (TA)x[ci[e]];
     s = block_sum_t<TA>(s, prod);
-    if (tid == 0) epi(r0, (double)s);
+    if (tid == 0) epi(r0, (double)s, epi.pre(r0));
     return;
   }
+  // Row-phase operands first, so their latency overlaps the product phase.
+  const int r = r0 + tid;
+  const int rc = min(r, r1 - 1);
+  const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
+  const double pv = epi.pre(rc);
   stream_products<TA>(e0, e1, ci, val, ColLoad<TA, TX>{x}, prod);
   __syncthreads();
-  const int r = r0 + tid;
   if (r < r1) {
-    const int a = rp[r] - e0, z = rp[r + 1] - e0;
     TA s = 0;
-    for (int q = a; q < z; ++q) s += prod[q];
-    epi(r, (double)s);
+    for (int q = ra; q < rz; ++q) s += prod[q];
+    epi(r, (double)s, pv);
   }
 }
 
@@ -194,37 +206,18 @@ __global__ __launch_bounds__(kBlock) void convert_kernel(int n, const TS *__rest
 
 // ---------------------------------------------------------------------------------------
 // PCG_CJ kernels (FP64).  Reductions are deterministic: every block of a fixed grid writes
-// its partial sum; the last block to finish (atomic ticket) adds the partials in block
-// order and computes the scalar update, then re-arms the ticket.
+// its partial sums with plain stores; a one-workgroup finalize kernel adds them in block
+// order and performs the scalar update (the kernel boundary makes the partials visible,
+// so no device-scope fence is needed per block).
 // ---------------------------------------------------------------------------------------
 struct PcgState {
   double bnorm, resiter, ptap, pr, zap, alpha, beta, tol;
   int iter, itmax, stop, breakdown;
-  unsigned int ticket[4];
 };
 
-__device__ __forceinline__ bool last_block(unsigned int *ticket, unsigned int nblocks,
-                                           int *flag_lds) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int t = atomicAdd(ticket, 1u);
-    *flag_lds = (t == nblocks - 1);
-  }
-  __syncthreads();
-  const bool last = *flag_lds != 0;
-  if (last) __threadfence();  // acquire: the other blocks' partials (other XCDs' L2s)
-  return last;
-}
+constexpr int kFinBlock = 1024;
 
-// Deterministic sum of `count` partials by the calling block (thread 0 holds the result).
-__device__ __forceinline__ double sum_partials(const double *part, int count, double *lds) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < count; i += kBlock) s += part[i];
-  return block_sum(s, lds);
-}
-
-// r = b - A x ; partials of b.b and r.r ; last block: bnorm, resvec[0], stop when b = 0.
+// r = b - A x ; partials of b.b and r.r.
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void pcg_init_kernel(int n, const int *__restrict__ rp,
                                                           const int *__restrict__ ci,
@@ -232,10 +225,8 @@ __global__ __launch_bounds__(kBlock) void pcg_init_kernel(int n, const int *__re
                                                           const double *__restrict__ b,
                                                           const double *__restrict__ x,
                                                           double *__restrict__ r,
-                                                          double *__restrict__ part,
-                                                          PcgState *st, double *resvec) {
+                                                          double *__restrict__ part) {
   __shared__ double lds[kBlock / kWave];
-  __shared__ int flag;
   const int groups_per_block = kBlock / LANES;
   const int lane = threadIdx.x % LANES;
   double bb = 0.0, rr = 0.0;
@@ -257,52 +248,23 @@ __global__ __launch_bounds__(kBlock) void pcg_init_kernel(int n, const int *__re
     part[blockIdx.x] = bb;
     part[gridDim.x + blockIdx.x] = rr;
   }
-  if (last_block(&st->ticket[0], gridDim.x, &flag)) {
-    double sbb = sum_partials(part, gridDim.x, lds);
-    double srr = sum_partials(part + gridDim.x, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      st->bnorm = sqrt(sbb);
-      st->iter = 0;
-      st->breakdown = 0;
-      if (sbb == 0.0) {
-        st->resiter = 0.0;
-        st->stop = 1;
-      } else {
-        st->resiter = sqrt(srr) / st->bnorm;
-        st->stop = st->itmax <= 0 ? 1 : 0;
-      }
-      resvec[0] = st->resiter;
-      st->ticket[0] = 0;
-    }
-  }
 }
 
-// zap = z . Ap_old ; last block: beta = -zap / ptap (iteration > 1).
-__global__ __launch_bounds__(kBlock) void pcg_beta_kernel(int n, const double *__restrict__ z,
-                                                          const double *__restrict__ ap,
-                                                          double *__restrict__ part,
-                                                          PcgState *st) {
-  if (st->stop) return;
+// Partials of z . Ap_old (iterations > 1).
+__global__ __launch_bounds__(kBlock) void pcg_zap_kernel(int n, const double *__restrict__ z,
+                                                         const double *__restrict__ ap,
+                                                         double *__restrict__ part,
+                                                         const PcgState *st) {
+  if (st->stop || st->iter == 0) return;
   __shared__ double lds[kBlock / kWave];
-  __shared__ int flag;
   double s = 0.0;
-  if (st->iter > 0)
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += z[i] * ap[i];
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += z[i] * ap[i];
   s = block_sum(s, lds);
-  if (threadIdx.x == 0) part[2 * gridDim.x + blockIdx.x] = s;
-  if (last_block(&st->ticket[1], gridDim.x, &flag)) {
-    double t = sum_partials(part + 2 * gridDim.x, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      st->zap = t;
-      st->beta = st->iter > 0 ? -t / st->ptap : 0.0;
-      st->ticket[1] = 0;
-    }
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// p_new = z + beta p_old (formed on the fly at every gathered column), Ap = A p_new,
-// partials of p.Ap and p.r over the row blocks of A (as in spmv_stream_kernel); last block:
-// ptap, alpha = p.r / ptap.
+// p_new = z + beta p_old (formed on the fly at every gathered column), Ap = A p_new, and
+// per-block partials of p.Ap and p.r over the row blocks of A (as in spmv_stream_kernel).
 struct PColVal {
   const double *__restrict__ z;
   const double *__restrict__ p;
@@ -311,17 +273,16 @@ struct PColVal {
 };
 
 __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
-    int nblk, const int *__restrict__ blk_rows, const int *__restrict__ rp,
-    const int *__restrict__ ci, const double *__restrict__ val, const double *__restrict__ z,
-    const double *__restrict__ p_old, double *__restrict__ p_new, double *__restrict__ ap,
-    const double *__restrict__ r, double *__restrict__ part, PcgState *st) {
+    int nblk, const int2 *__restrict__ blk, const int *__restrict__ rp, const int *__restrict__ ci,
+    const double *__restrict__ val, const double *__restrict__ z, const double *__restrict__ p_old,
+    double *__restrict__ p_new, double *__restrict__ ap, const double *__restrict__ r,
+    double *__restrict__ part, const PcgState *st) {
   if (st->stop) return;
   __shared__ double prod[kStreamCap];
-  __shared__ int flag;
   const double beta = st->beta;
   const int b = xcd_block(blockIdx.x, nblk);
-  const int r0 = blk_rows[b], r1 = blk_rows[b + 1];
-  const int e0 = rp[r0], e1 = rp[r1];
+  const int2 lo = blk[b], hi = blk[b + 1];
+  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y;
   const int tid = threadIdx.x;
   PColVal cv{z, p_old, beta};
   double pap = 0.0, prr = 0.0;
@@ -337,18 +298,19 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
       prr = pi * r[r0];
     }
   } else {
+    const int row = r0 + tid;
+    const int rc = min(row, r1 - 1);
+    const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
+    const double pi = cv(rc), ri = r[rc];
     stream_products<double>(e0, e1, ci, val, cv, prod);
     __syncthreads();
-    const int row = r0 + tid;
     if (row < r1) {
-      const int a = rp[row] - e0, q1 = rp[row + 1] - e0;
       double s = 0.0;
-      for (int q = a; q < q1; ++q) s += prod[q];
-      const double pi = cv(row);
+      for (int q = ra; q < rz; ++q) s += prod[q];
       p_new[row] = pi;
       ap[row] = s;
       pap = pi * s;
-      prr = pi * r[row];
+      prr = pi * ri;
     }
     __syncthreads();
   }
@@ -358,32 +320,17 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
     part[b] = pap;
     part[nblk + b] = prr;
   }
-  if (last_block(&st->ticket[2], gridDim.x, &flag)) {
-    double a = sum_partials(part, nblk, prod);
-    double c = sum_partials(part + nblk, nblk, prod);
-    if (tid == 0) {
-      st->ptap = a;
-      st->pr = c;
-      st->alpha = c / a;
-      if (!(a != 0.0) || !isfinite(a)) {
-        st->breakdown = 1;
-        st->stop = 1;
-      }
-      st->ticket[2] = 0;
-    }
-  }
 }
 
-// x += alpha p ; r -= alpha Ap ; partial r.r ; last block: resiter, resvec, stop test.
+// x += alpha p ; r -= alpha Ap ; partials of r.r.
 __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, const double *__restrict__ p,
                                                             const double *__restrict__ ap,
                                                             double *__restrict__ x,
                                                             double *__restrict__ r,
                                                             double *__restrict__ part,
-                                                            PcgState *st, double *resvec) {
+                                                            const PcgState *st) {
   if (st->stop) return;
   __shared__ double lds[kBlock / kWave];
-  __shared__ int flag;
   const double alpha = st->alpha;
   double rr = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -393,16 +340,73 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, const double 
     rr += ri * ri;
   }
   rr = block_sum(rr, lds);
-  if (threadIdx.x == 0) part[5 * gridDim.x + blockIdx.x] = rr;
-  if (last_block(&st->ticket[3], gridDim.x, &flag)) {
-    double s = sum_partials(part + 5 * gridDim.x, gridDim.x, lds);
-    if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// Deterministic sum of count partials by one workgroup of kFinBlock threads (thread 0).
+__device__ __forceinline__ double fin_sum(const double *part, int count, double *lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < count; i += kFinBlock) s += part[i];
+  s = group_sum<kWave>(s);
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  if (l == 0) lds[w] = s;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kFinBlock / kWave; ++i) t += lds[i];
+  __syncthreads();
+  return t;
+}
+
+enum { kFinInit = 0, kFinBeta = 1, kFinAlpha = 2, kFinUpdate = 3 };
+
+// Scalar updates of PCG_CJ.m after each reduction (one workgroup).
+__global__ __launch_bounds__(kFinBlock) void pcg_finalize_kernel(int kind, int count,
+                                                                 const double *__restrict__ part,
+                                                                 PcgState *st,
+                                                                 double *__restrict__ resvec) {
+  __shared__ double lds[kFinBlock / kWave];
+  if (kind != kFinInit && st->stop) return;
+  if (kind == kFinBeta && st->iter == 0) {
+    if (threadIdx.x == 0) st->beta = 0.0;  // first iteration: p = z (PCG_CJ.m:95-96)
+    return;
+  }
+  const double s0 = fin_sum(part, count, lds);
+  const double s1 = (kind == kFinInit || kind == kFinAlpha) ? fin_sum(part + count, count, lds) : 0.0;
+  if (threadIdx.x != 0) return;
+  switch (kind) {
+    case kFinInit:
+      st->bnorm = sqrt(s0);
+      st->iter = 0;
+      st->breakdown = 0;
+      if (s0 == 0.0) {
+        st->resiter = 0.0;
+        st->stop = 1;
+      } else {
+        st->resiter = sqrt(s1) / st->bnorm;
+        st->stop = st->itmax <= 0 ? 1 : 0;
+      }
+      resvec[0] = st->resiter;
+      break;
+    case kFinBeta:
+      st->zap = s0;
+      st->beta = -s0 / st->ptap;
+      break;
+    case kFinAlpha:
+      st->ptap = s0;
+      st->pr = s1;
+      st->alpha = s1 / s0;
+      if (!(s0 != 0.0) || !isfinite(s0)) {
+        st->breakdown = 1;
+        st->stop = 1;
+      }
+      break;
+    default: {
       const int it = st->iter + 1;
       st->iter = it;
-      st->resiter = sqrt(s) / st->bnorm;
+      st->resiter = sqrt(s0) / st->bnorm;
       resvec[it] = st->resiter;
       if (st->resiter < st->tol || it >= st->itmax) st->stop = 1;
-      st->ticket[3] = 0;
     }
   }
 }
