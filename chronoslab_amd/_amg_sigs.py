@@ -19,7 +19,7 @@ class ChrLevel(C.Structure):
 
 
 class ChrHierStats(C.Structure):
-    _fields_ = [("nlev", _I32), ("precision", _I32), ("n", _I64 * 16), ("vcycle_bytes", _F64),
+    _fields_ = [("nlev", _I32), ("precision", _I32), ("n", _I64 * 16), ("vcycle_bytes", _F64), ("vcycle_csr_bytes", _F64),
                 ("level_bytes", _F64 * 16), ("pcg_iter_bytes", _F64), ("device_bytes", _F64)]
 
 

@@ -106,7 +106,7 @@ class AMGPreconditioner:
         st = ChrHierStats()
         _check(_lib().chr_hier_get_stats(self.ptr, C.byref(st)))
         nl = st.nlev
-        return {"nlev": nl, "n": list(st.n[:nl]), "vcycle_bytes": st.vcycle_bytes,
+        return {"nlev": nl, "n": list(st.n[:nl]), "vcycle_bytes": st.vcycle_bytes, "vcycle_csr_bytes": st.vcycle_csr_bytes,
                 "level_bytes": list(st.level_bytes[:nl]), "pcg_iter_bytes": st.pcg_iter_bytes,
                 "device_bytes": st.device_bytes}
 

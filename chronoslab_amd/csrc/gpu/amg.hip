@@ -114,55 +114,70 @@ void transpose_host(const chr_csr &M, std::vector<int32_t> &rp, std::vector<int3
     }
 }
 
-int pick_lanes(int64_t nnz, int64_t nrows) {
-  const double avg = nrows ? (double)nnz / (double)nrows : 1.0;
-  int lanes = 2;
-  while (lanes < 64 && lanes < avg) lanes *= 2;
-  return lanes;
-}
-
 // Device CSR in FP32 or FP64.
 struct DevCsr {
   int32_t nrows = 0, ncols = 0;
   int64_t nnz = 0;
   bool fp32 = false;
-  int lanes = 8;
+  bool idx16 = false;  // uint16 column offsets from a per-block base
   int32_t nblk = 0;  // CSR-stream row blocks
   DevBuf rp, ci, val, blk;
+  DevBuf ci32;       // int32 columns (when the uint16 form does not apply)
   int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
              bool as_fp32) {
     nrows = nr;
     ncols = nc;
     nnz = hrp[nr];
     fp32 = as_fp32;
-    lanes = pick_lanes(nnz, nr);
     int rc;
     if ((rc = rp.alloc(sizeof(int32_t) * (nr + 1)))) return rc;
-    if ((rc = ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
     if ((rc = val.alloc((as_fp32 ? 4 : 8) * std::max<int64_t>(nnz, 1)))) return rc;
     HIPC(hipMemcpy(rp.p, hrp, sizeof(int32_t) * (nr + 1), hipMemcpyHostToDevice));
-    // Row blocks for the stream kernel: whole rows, <= kStreamCap nonzeros, <= kBlock rows.
-    // Stored as (first row, first nonzero) pairs, nblk + 1 of them.
-    std::vector<int2> br;
-    br.push_back(make_int2(0, 0));
+    // Row blocks for the stream kernel: whole rows, <= kStreamCap nonzeros, <= kBlock rows,
+    // stored as int4 {first row, first nonzero, base column, 0}, nblk + 1 of them.
+    std::vector<int4> br;
+    br.push_back(make_int4(0, 0, 0, 0));
     int32_t rows = 0;
     int64_t cnt = 0;
     for (int32_t r = 0; r < nr; ++r) {
       const int64_t len = hrp[r + 1] - hrp[r];
       if (rows > 0 && (rows == chr::kBlock || cnt + len > chr::kStreamCap)) {
-        br.push_back(make_int2(r, hrp[r]));
+        br.push_back(make_int4(r, hrp[r], 0, 0));
         rows = 0;
         cnt = 0;
       }
       ++rows;
       cnt += len;
     }
-    if (nr > 0) br.push_back(make_int2(nr, hrp[nr]));
+    if (nr > 0) br.push_back(make_int4(nr, hrp[nr], 0, 0));
     nblk = (int32_t)br.size() - 1;
-    if ((rc = blk.alloc(sizeof(int2) * br.size()))) return rc;
-    HIPC(hipMemcpy(blk.p, br.data(), sizeof(int2) * br.size(), hipMemcpyHostToDevice));
+    // Index compression: per-block base = smallest column; usable when every block spans
+    // fewer than 65536 columns.
+    idx16 = nnz > 0;
+    for (int32_t b = 0; b < nblk && idx16; ++b) {
+      int32_t lo = INT32_MAX, hi = -1;
+      for (int64_t e = br[b].y; e < br[b + 1].y; ++e) {
+        lo = std::min(lo, hci[e]);
+        hi = std::max(hi, hci[e]);
+      }
+      if (hi >= 0 && hi - lo > 65535) idx16 = false;
+      br[b].z = hi >= 0 ? lo : 0;
+    }
+    if (!idx16)
+      for (auto &q : br) q.z = 0;
+    if ((rc = blk.alloc(sizeof(int4) * br.size()))) return rc;
+    HIPC(hipMemcpy(blk.p, br.data(), sizeof(int4) * br.size(), hipMemcpyHostToDevice));
+    if (!idx16) {
+      if ((rc = ci32.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
+      if (nnz) HIPC(hipMemcpy(ci32.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+    } else {
+      std::vector<uint16_t> d(nnz);
+      for (int32_t b = 0; b < nblk; ++b)
+        for (int64_t e = br[b].y; e < br[b + 1].y; ++e) d[e] = (uint16_t)(hci[e] - br[b].z);
+      if ((rc = ci.alloc(sizeof(uint16_t) * nnz))) return rc;
+      HIPC(hipMemcpy(ci.p, d.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
+    }
     if (nnz) {
-      HIPC(hipMemcpy(ci.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
       if (as_fp32) {
         std::vector<float> f(hv, hv + nnz);
         HIPC(hipMemcpy(val.p, f.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
@@ -175,20 +190,29 @@ struct DevCsr {
   int upload(const chr_csr &m, bool as_fp32) {
     return upload(m.nrows, m.ncols, m.rowptr, m.col, m.val, as_fp32);
   }
-  // Algorithmic bytes of one product: matrix stream + row pointers.
+  // Bytes of one product as stored: values + column indices + row pointers + block table.
   double stream_bytes() const {
-    return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1);
+    return (double)nnz * ((fp32 ? 4 : 8) + (idx16 ? 2 : 4)) + 4.0 * (nrows + 1) + 16.0 * (nblk + 1);
   }
-  double device_bytes() const { return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes); }
+  // The same with 4-byte column indices and no block table (the reference's CSR format).
+  double csr_bytes() const { return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1); }
+  double device_bytes() const {
+    return (double)(rp.bytes + ci.bytes + ci32.bytes + val.bytes + blk.bytes);
+  }
 };
 
 template <typename TV, typename TX, typename Epi>
 hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
                          hipStream_t s) {
   if (M.nrows == 0) return hipSuccess;
-  hipLaunchKernelGGL((chr::spmv_stream_kernel<TV, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock), 0, s,
-                     M.nblk, M.blk.as<int2>(), M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, epi,
-                     stop);
+  if (M.idx16)
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<TV, uint16_t, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
+                       0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(), (const uint16_t *)M.ci.p,
+                       M.val.as<TV>(), x, epi, stop);
+  else
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<TV, int, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock), 0,
+                       s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(), M.ci32.as<int>(), M.val.as<TV>(), x,
+                       epi, stop);
   return hipGetLastError();
 }
 
@@ -366,29 +390,35 @@ void fill_stats(chr_hier &h) {
   st.nlev = (int32_t)h.lev.size();
   st.precision = h.precision;
   const double s = h.fp32() ? 4.0 : 8.0;
-  double total = 0.0, dev = 0.0;
+  double total = 0.0, total_csr = 0.0, dev = 0.0;
   for (size_t l = 0; l < h.lev.size(); ++l) {
     auto &L = h.lev[l];
     if (l < 16) st.n[l] = L.n;
-    double b;
+    double b, bc;
     if (l + 1 == h.lev.size()) {
-      b = 8.0 * (double)h.nc * h.nc + 2.0 * h.nc * 8.0;
+      b = bc = 8.0 * (double)h.nc * h.nc + 2.0 * h.nc * 8.0;
       dev += h.Ainv.bytes;
     } else {
+      // Per level: 2 A, 2 G, 2 G^T, P, P^T, 18 n_l + 2 n_{l+1} vector words (AMG_Vcycle.m).
       const double nc = h.lev[l + 1].n;
+      const double vec = 18.0 * L.n * s + 2.0 * nc * s;
       b = 2 * L.A.stream_bytes() + 2 * L.G.stream_bytes() + 2 * L.GT.stream_bytes() +
-          L.P.stream_bytes() + L.PT.stream_bytes() + 18.0 * L.n * s + 2.0 * nc * s;
+          L.P.stream_bytes() + L.PT.stream_bytes() + vec;
+      bc = 2 * L.A.csr_bytes() + 2 * L.G.csr_bytes() + 2 * L.GT.csr_bytes() + L.P.csr_bytes() +
+           L.PT.csr_bytes() + vec;
       dev += L.A.device_bytes() + L.G.device_bytes() + L.GT.device_bytes() + L.P.device_bytes() +
              L.PT.device_bytes() + L.b.bytes + L.x.bytes + L.r.bytes + L.y.bytes;
     }
     if (l < 16) st.level_bytes[l] = b;
     total += b;
+    total_csr += bc;
   }
   st.vcycle_bytes = total;
+  st.vcycle_csr_bytes = total_csr;
   dev += h.outer.A.device_bytes();
   st.device_bytes = dev;
   const auto &A0 = h.outer.A;
-  st.pcg_iter_bytes = total + ((double)A0.nnz * 12.0 + 4.0 * (A0.nrows + 1)) + 18.0 * A0.nrows * 8.0;
+  st.pcg_iter_bytes = total + A0.stream_bytes() + 18.0 * A0.nrows * 8.0;
 }
 
 int sync_stream(hipStream_t s) {
@@ -559,29 +589,6 @@ extern "C" int chr_spmv(chr_mat *A, const double *x, double *y, int32_t ptr_kind
 // ---------------------------------------------------------------------------------------
 namespace {
 
-template <int L>
-hipError_t launch_pcg_init(const DevCsr &A, unsigned grid, const double *b, const double *x,
-                           double *r, double *part, hipStream_t s) {
-  hipLaunchKernelGGL((chr::pcg_init_kernel<L>), dim3(grid), dim3(chr::kBlock), 0, s, A.nrows,
-                     A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(), b, x, r, part);
-  return hipGetLastError();
-}
-
-#define CHR_DISPATCH_LANES(lanes, fn, ...)      \
-  switch (lanes) {                              \
-    case 2: return fn<2>(__VA_ARGS__);          \
-    case 4: return fn<4>(__VA_ARGS__);          \
-    case 8: return fn<8>(__VA_ARGS__);          \
-    case 16: return fn<16>(__VA_ARGS__);        \
-    case 32: return fn<32>(__VA_ARGS__);        \
-    default: return fn<64>(__VA_ARGS__);        \
-  }
-
-hipError_t pcg_init(const DevCsr &A, unsigned grid, const double *b, const double *x, double *r,
-                    double *part, hipStream_t s) {
-  CHR_DISPATCH_LANES(A.lanes, launch_pcg_init, A, grid, b, x, r, part, s)
-}
-
 hipError_t pcg_finalize(int kind, int count, double *part, chr::PcgState *st, double *resvec,
                         hipStream_t s) {
   hipLaunchKernelGGL(chr::pcg_finalize_kernel, dim3(1), dim3(chr::kFinBlock), 0, s, kind, count,
@@ -603,10 +610,17 @@ hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity
                      (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
   if ((e = pcg_finalize(chr::kFinBeta, (int)w.nblk, part, st, resvec, h.stream))) return e;
-  hipLaunchKernelGGL(chr::pcg_p_stream_kernel, dim3(A.nblk), dim3(chr::kBlock), 0, h.stream, A.nblk,
-                     A.blk.as<int2>(), A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(),
-                     (const double *)w.z.as<double>(), (const double *)pold, pnew, w.ap.as<double>(),
-                     (const double *)w.r.as<double>(), w.part_p.as<double>(), (const chr::PcgState *)st);
+  if (A.idx16)
+    hipLaunchKernelGGL(chr::pcg_p_stream_kernel<uint16_t>, dim3(A.nblk), dim3(chr::kBlock), 0, h.stream,
+                       A.nblk, A.blk.as<int4>(), A.rp.as<int>(), (const uint16_t *)A.ci.p,
+                       A.val.as<double>(), (const double *)w.z.as<double>(), (const double *)pold, pnew,
+                       w.ap.as<double>(), (const double *)w.r.as<double>(), w.part_p.as<double>(),
+                       (const chr::PcgState *)st);
+  else
+    hipLaunchKernelGGL(chr::pcg_p_stream_kernel<int>, dim3(A.nblk), dim3(chr::kBlock), 0, h.stream,
+                       A.nblk, A.blk.as<int4>(), A.rp.as<int>(), A.ci32.as<int>(), A.val.as<double>(),
+                       (const double *)w.z.as<double>(), (const double *)pold, pnew, w.ap.as<double>(),
+                       (const double *)w.r.as<double>(), w.part_p.as<double>(), (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
   if ((e = pcg_finalize(chr::kFinAlpha, A.nblk, w.part_p.as<double>(), st, resvec, h.stream))) return e;
   hipLaunchKernelGGL(chr::pcg_update_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
@@ -709,7 +723,11 @@ extern "C" int chr_pcg_solve(chr_hier *h, chr_mat *Aop, const double *b, double 
   st0.tol = tol;
   st0.itmax = itmax;
   HIPC(hipMemcpyAsync(w.state.p, &st0, sizeof(st0), hipMemcpyHostToDevice, h->stream));
-  HIPC(pcg_init(A, w.nblk, db, w.x.as<double>(), w.r.as<double>(), w.part.as<double>(), h->stream));
+  HIPC(launch_spmv(A, w.x.as<double>(), chr::EpiResidual<double, double>{db, w.r.as<double>()}, nullptr,
+                   h->stream));
+  hipLaunchKernelGGL(chr::pcg_norms_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h->stream, n, db,
+                     (const double *)w.r.as<double>(), w.part.as<double>());
+  HIPC(hipGetLastError());
   HIPC(pcg_finalize(chr::kFinInit, (int)w.nblk, w.part.as<double>(), w.state.as<chr::PcgState>(),
                     w.resvec.as<double>(), h->stream));
   HIPC(hipMemcpyAsync(&w.pinned[1], w.state.p, sizeof(chr::PcgState), hipMemcpyDeviceToHost, h->stream));

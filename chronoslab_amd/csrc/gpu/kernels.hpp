@@ -78,6 +78,11 @@ struct Acc<float> {
 };
 
 // ---------------------------------------------------------------------------------------
+// Column indices are stored either as int32 or, when every row block's columns fit in a
+// 65536-wide window, as uint16 offsets from a per-block base column (6 instead of 8 bytes
+// per FP32 nonzero, 10 instead of 12 per FP64 nonzero).  The block table holds, per block,
+// int4 {first row, first nonzero, base column, 0}; entry nblk closes the last block.
+// ---------------------------------------------------------------------------------------
 // CSR-stream SpMV.  The rows are cut on the host into blocks of whole consecutive rows
 // holding at most kStreamCap nonzeros and at most kBlock rows (a row longer than kStreamCap
 // is a block of its own).  A workgroup streams its block's values and column indices with
@@ -113,8 +118,8 @@ __device__ __forceinline__ T block_sum_t(T v, T *lds) {
 // Stages the products of the block's nonzeros [e0, e1) in prod[].  Indices are clamped to
 // the last nonzero so that every load is unconditional: all 2*kStreamPer index/value loads
 // issue back to back, then the kStreamPer gathers, then the LDS stores (masked).
-template <typename TA, typename TV, typename Xf>
-__device__ __forceinline__ void stream_products(int e0, int e1, const int *__restrict__ ci,
+template <typename TA, typename TV, typename TI, typename Xf>
+__device__ __forceinline__ void stream_products(int e0, int e1, int base, const TI *__restrict__ ci,
                                                 const TV *__restrict__ val, Xf colval, TA *prod) {
   if (e1 <= e0) return;
   int c[kStreamPer];
@@ -124,7 +129,7 @@ __device__ __forceinline__ void stream_products(int e0, int e1, const int *__res
 #pragma unroll
   for (int k = 0; k < kStreamPer; ++k) {
     const int e = min(e0 + k * kBlock + tid, last);
-    c[k] = ci[e];
+    c[k] = base + (int)ci[e];
     v[k] = val[e];
   }
 #pragma unroll
@@ -142,11 +147,11 @@ struct ColLoad {
   __device__ __forceinline__ TA operator()(int j) const { return (TA)x[j]; }
 };
 
-template <typename TV, typename TX, typename Epi>
+template <typename TV, typename TI, typename TX, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
-                                                             const int2 *__restrict__ blk,
+                                                             const int4 *__restrict__ blk,
                                                              const int *__restrict__ rp,
-                                                             const int *__restrict__ ci,
+                                                             const TI *__restrict__ ci,
                                                              const TV *__restrict__ val,
                                                              const TX *__restrict__ x, Epi epi,
                                                              const int *__restrict__ stop) {
@@ -154,12 +159,12 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   using TA = typename Acc<TV>::type;
   __shared__ TA prod[kStreamCap];
   const int b = xcd_block(blockIdx.x, nblk);
-  const int2 lo = blk[b], hi = blk[b + 1];  // (first row, first nonzero) of b and b+1
-  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y;
+  const int4 lo = blk[b], hi = blk[b + 1];
+  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
   const int tid = threadIdx.x;
   if (e1 - e0 > kStreamCap) {  // one long row: strided sum + block reduction
     TA s = 0;
-    for (int e = e0 + tid; e < e1; e += kBlock) s += (TA)val[e] * (TA)x[ci[e]];
+    for (int e = e0 + tid; e < e1; e += kBlock) s += (TA)val[e] * (TA)x[base + (int)ci[e]];
     s = block_sum_t<TA>(s, prod);
     if (tid == 0) epi(r0, (double)s, epi.pre(r0));
     return;
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   const int rc = min(r, r1 - 1);
   const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
   const double pv = epi.pre(rc);
-  stream_products<TA>(e0, e1, ci, val, ColLoad<TA, TX>{x}, prod);
+  stream_products<TA>(e0, e1, base, ci, val, ColLoad<TA, TX>{x}, prod);
   __syncthreads();
   if (r < r1) {
     TA s = 0;
@@ -217,30 +222,15 @@ struct PcgState {
 
 constexpr int kFinBlock = 1024;
 
-// r = b - A x ; partials of b.b and r.r.
-template <int LANES>
-__global__ __launch_bounds__(kBlock) void pcg_init_kernel(int n, const int *__restrict__ rp,
-                                                          const int *__restrict__ ci,
-                                                          const double *__restrict__ val,
-                                                          const double *__restrict__ b,
-                                                          const double *__restrict__ x,
-                                                          double *__restrict__ r,
-                                                          double *__restrict__ part) {
+// Partials of b.b and r.r (r = b - A x is formed by the stream SpMV beforehand).
+__global__ __launch_bounds__(kBlock) void pcg_norms_kernel(int n, const double *__restrict__ b,
+                                                           const double *__restrict__ r,
+                                                           double *__restrict__ part) {
   __shared__ double lds[kBlock / kWave];
-  const int groups_per_block = kBlock / LANES;
-  const int lane = threadIdx.x % LANES;
   double bb = 0.0, rr = 0.0;
-  for (int row = blockIdx.x * groups_per_block + threadIdx.x / LANES; row < n;
-       row += gridDim.x * groups_per_block) {
-    double acc = 0.0;
-    for (int e = rp[row] + lane; e < rp[row + 1]; e += LANES) acc += val[e] * x[ci[e]];
-    acc = group_sum<LANES>(acc);
-    if (lane == 0) {
-      const double bi = b[row], ri = bi - acc;
-      r[row] = ri;
-      bb += bi * bi;
-      rr += ri * ri;
-    }
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    bb += b[i] * b[i];
+    rr += r[i] * r[i];
   }
   bb = block_sum(bb, lds);
   rr = block_sum(rr, lds);
@@ -272,8 +262,9 @@ struct PColVal {
   __device__ __forceinline__ double operator()(int j) const { return z[j] + beta * p[j]; }
 };
 
+template <typename TI>
 __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
-    int nblk, const int2 *__restrict__ blk, const int *__restrict__ rp, const int *__restrict__ ci,
+    int nblk, const int4 *__restrict__ blk, const int *__restrict__ rp, const TI *__restrict__ ci,
     const double *__restrict__ val, const double *__restrict__ z, const double *__restrict__ p_old,
     double *__restrict__ p_new, double *__restrict__ ap, const double *__restrict__ r,
     double *__restrict__ part, const PcgState *st) {
@@ -281,14 +272,14 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
   __shared__ double prod[kStreamCap];
   const double beta = st->beta;
   const int b = xcd_block(blockIdx.x, nblk);
-  const int2 lo = blk[b], hi = blk[b + 1];
-  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y;
+  const int4 lo = blk[b], hi = blk[b + 1];
+  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
   const int tid = threadIdx.x;
   PColVal cv{z, p_old, beta};
   double pap = 0.0, prr = 0.0;
   if (e1 - e0 > kStreamCap) {
     double s = 0.0;
-    for (int e = e0 + tid; e < e1; e += kBlock) s += val[e] * cv(ci[e]);
+    for (int e = e0 + tid; e < e1; e += kBlock) s += val[e] * cv(base + (int)ci[e]);
     s = block_sum_t<double>(s, prod);
     if (tid == 0) {
       const double pi = cv(r0);
@@ -302,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
     const int rc = min(row, r1 - 1);
     const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
     const double pi = cv(rc), ri = r[rc];
-    stream_products<double>(e0, e1, ci, val, cv, prod);
+    stream_products<double>(e0, e1, base, ci, val, cv, prod);
     __syncthreads();
     if (row < r1) {
       double s = 0.0;

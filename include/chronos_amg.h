@@ -76,7 +76,8 @@ typedef struct chr_hier_stats {
   int32_t nlev;
   int32_t precision;
   int64_t n[16];              /* rows per level */
-  double vcycle_bytes;        /* algorithmic bytes of one V-cycle (k = 1) */
+  double vcycle_bytes;        /* bytes one V-cycle (k = 1) streams in the stored format */
+  double vcycle_csr_bytes;    /* the same with plain CSR (4-byte column indices) */
   double level_bytes[16];     /* the same, per level (last entry: coarsest solve) */
   double pcg_iter_bytes;      /* V-cycle + outer FP64 SpMV + 18 n_0 * 8 vector bytes */
   double device_bytes;        /* device memory held by the hierarchy */
