@@ -107,6 +107,7 @@ def main() -> int:
         M = amg.AMGPreconditioner(hier, prec)
         times, its, xs, tres = run_solves(M, args.steps, args.warmup)
         st = M.stats()
+        log(f"[bench] {prec} layouts (PER, uint16): {M.layout()}")
         vms = M.time_vcycle(20)
         gms, gbytes = M.time_spmv(0, "G", 20)
         tms, tbytes = M.time_spmv(0, "T", 20)

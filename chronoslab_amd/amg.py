@@ -110,6 +110,18 @@ class AMGPreconditioner:
                 "level_bytes": list(st.level_bytes[:nl]), "pcg_iter_bytes": st.pcg_iter_bytes,
                 "device_bytes": st.device_bytes}
 
+    def layout(self) -> list:
+        """Per level, the stream-kernel layout chosen for A, G, G^T, P, P^T."""
+        out = []
+        for l in range(len(self.hier.levels) - 1):
+            row = {}
+            for w in "AGTPR":
+                per, i16, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+                _check(_lib().chr_hier_layout(self.ptr, l, w.encode(), C.byref(per), C.byref(i16), C.byref(nnz)))
+                row[w] = (per.value, bool(i16.value))
+            out.append(row)
+        return out
+
     def __call__(self, b: np.ndarray) -> np.ndarray:
         b = np.asfortranarray(b, dtype=np.float64)
         k = 1 if b.ndim == 1 else b.shape[1]

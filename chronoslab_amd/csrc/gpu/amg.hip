@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -114,17 +115,21 @@ void transpose_host(const chr_csr &M, std::vector<int32_t> &rp, std::vector<int3
     }
 }
 
-// Device CSR in FP32 or FP64.
+// Candidate nonzeros-per-thread of the stream kernel (block capacity PER * 256).
+constexpr int kPers[] = {4, 8, 16};
+
+// Device CSR in FP32 or FP64, laid out for the stream kernel.
 struct DevCsr {
   int32_t nrows = 0, ncols = 0;
   int64_t nnz = 0;
   bool fp32 = false;
   bool idx16 = false;  // uint16 column offsets from a per-block base
-  int32_t nblk = 0;  // CSR-stream row blocks
-  DevBuf rp, ci, val, blk;
-  DevBuf ci32;       // int32 columns (when the uint16 form does not apply)
+  int per = 8;         // nonzeros per thread (block capacity per * kBlock)
+  int32_t nblk = 0;    // row blocks
+  DevBuf rp, val, blk;
+  DevBuf ci;           // uint16 offsets when idx16, else int32 columns
   int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
-             bool as_fp32) {
+             bool as_fp32, int per_ = 8) {
     nrows = nr;
     ncols = nc;
     nnz = hrp[nr];
@@ -133,15 +138,29 @@ struct DevCsr {
     if ((rc = rp.alloc(sizeof(int32_t) * (nr + 1)))) return rc;
     if ((rc = val.alloc((as_fp32 ? 4 : 8) * std::max<int64_t>(nnz, 1)))) return rc;
     HIPC(hipMemcpy(rp.p, hrp, sizeof(int32_t) * (nr + 1), hipMemcpyHostToDevice));
-    // Row blocks for the stream kernel: whole rows, <= kStreamCap nonzeros, <= kBlock rows,
-    // stored as int4 {first row, first nonzero, base column, 0}, nblk + 1 of them.
+    if (nnz) {
+      if (as_fp32) {
+        std::vector<float> f(hv, hv + nnz);
+        HIPC(hipMemcpy(val.p, f.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+      } else {
+        HIPC(hipMemcpy(val.p, hv, sizeof(double) * nnz, hipMemcpyHostToDevice));
+      }
+    }
+    return layout(per_, hrp, hci);
+  }
+  // Row blocks for a given per: whole rows, <= per * kBlock nonzeros, <= kBlock rows, as
+  // int4 {first row, first nonzero, base column, 0}, nblk + 1 of them; column indices as
+  // uint16 offsets from the block's smallest column when every block spans < 65536 columns.
+  int layout(int per_, const int32_t *hrp, const int32_t *hci) {
+    per = per_;
+    const int64_t cap = (int64_t)per * chr::kBlock;
     std::vector<int4> br;
     br.push_back(make_int4(0, 0, 0, 0));
     int32_t rows = 0;
     int64_t cnt = 0;
-    for (int32_t r = 0; r < nr; ++r) {
+    for (int32_t r = 0; r < nrows; ++r) {
       const int64_t len = hrp[r + 1] - hrp[r];
-      if (rows > 0 && (rows == chr::kBlock || cnt + len > chr::kStreamCap)) {
+      if (rows > 0 && (rows == chr::kBlock || cnt + len > cap)) {
         br.push_back(make_int4(r, hrp[r], 0, 0));
         rows = 0;
         cnt = 0;
@@ -149,10 +168,8 @@ struct DevCsr {
       ++rows;
       cnt += len;
     }
-    if (nr > 0) br.push_back(make_int4(nr, hrp[nr], 0, 0));
+    if (nrows > 0) br.push_back(make_int4(nrows, hrp[nrows], 0, 0));
     nblk = (int32_t)br.size() - 1;
-    // Index compression: per-block base = smallest column; usable when every block spans
-    // fewer than 65536 columns.
     idx16 = nnz > 0;
     for (int32_t b = 0; b < nblk && idx16; ++b) {
       int32_t lo = INT32_MAX, hi = -1;
@@ -165,25 +182,18 @@ struct DevCsr {
     }
     if (!idx16)
       for (auto &q : br) q.z = 0;
+    int rc;
     if ((rc = blk.alloc(sizeof(int4) * br.size()))) return rc;
     HIPC(hipMemcpy(blk.p, br.data(), sizeof(int4) * br.size(), hipMemcpyHostToDevice));
-    if (!idx16) {
-      if ((rc = ci32.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
-      if (nnz) HIPC(hipMemcpy(ci32.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
-    } else {
+    if (idx16) {
       std::vector<uint16_t> d(nnz);
       for (int32_t b = 0; b < nblk; ++b)
         for (int64_t e = br[b].y; e < br[b + 1].y; ++e) d[e] = (uint16_t)(hci[e] - br[b].z);
       if ((rc = ci.alloc(sizeof(uint16_t) * nnz))) return rc;
       HIPC(hipMemcpy(ci.p, d.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
-    }
-    if (nnz) {
-      if (as_fp32) {
-        std::vector<float> f(hv, hv + nnz);
-        HIPC(hipMemcpy(val.p, f.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
-      } else {
-        HIPC(hipMemcpy(val.p, hv, sizeof(double) * nnz, hipMemcpyHostToDevice));
-      }
+    } else {
+      if ((rc = ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
+      if (nnz) HIPC(hipMemcpy(ci.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
     }
     return CHR_OK;
   }
@@ -196,30 +206,86 @@ struct DevCsr {
   }
   // The same with 4-byte column indices and no block table (the reference's CSR format).
   double csr_bytes() const { return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1); }
-  double device_bytes() const {
-    return (double)(rp.bytes + ci.bytes + ci32.bytes + val.bytes + blk.bytes);
-  }
+  double device_bytes() const { return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes); }
 };
+
+template <int PER, typename TV, typename TX, typename Epi>
+hipError_t launch_spmv_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  if (M.idx16)
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, uint16_t, TX, Epi>), dim3(M.nblk),
+                       dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
+                       M.ci.as<uint16_t>(), M.val.as<TV>(), x, epi, stop);
+  else
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, int, TX, Epi>), dim3(M.nblk),
+                       dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
+                       M.ci.as<int>(), M.val.as<TV>(), x, epi, stop);
+  return hipGetLastError();
+}
 
 template <typename TV, typename TX, typename Epi>
 hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
                          hipStream_t s) {
   if (M.nrows == 0) return hipSuccess;
-  if (M.idx16)
-    hipLaunchKernelGGL((chr::spmv_stream_kernel<TV, uint16_t, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
-                       0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(), (const uint16_t *)M.ci.p,
-                       M.val.as<TV>(), x, epi, stop);
-  else
-    hipLaunchKernelGGL((chr::spmv_stream_kernel<TV, int, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock), 0,
-                       s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(), M.ci32.as<int>(), M.val.as<TV>(), x,
-                       epi, stop);
-  return hipGetLastError();
+  switch (M.per) {
+    case 4: return launch_spmv_p<4, TV>(M, x, epi, stop, s);
+    case 16: return launch_spmv_p<16, TV>(M, x, epi, stop, s);
+    default: return launch_spmv_p<8, TV>(M, x, epi, stop, s);
+  }
 }
 
 template <typename TX, typename Epi>
 hipError_t launch_spmv(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
   return M.fp32 ? launch_spmv_t<float>(M, x, epi, stop, s)
                 : launch_spmv_t<double>(M, x, epi, stop, s);
+}
+
+// Autotuner: times the stream kernel on M for every candidate PER (HIP events, 2 warm-up
+// and 5 timed launches each) and keeps the fastest layout.  CHRONOS_STREAM_PER=4|8|16 in
+// the environment fixes PER instead.
+int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s, const int *zero) {
+  if (M.nnz == 0) return CHR_OK;
+  if (const char *env = std::getenv("CHRONOS_STREAM_PER")) {
+    const int forced = std::atoi(env);
+    for (int p : kPers)
+      if (p == forced) return p == M.per ? CHR_OK : M.layout(p, hrp, hci);
+  }
+  const size_t sz = M.fp32 ? 4 : 8;
+  DevBuf xb, yb;
+  int rc;
+  if ((rc = xb.alloc(sz * std::max(M.ncols, 1))) || (rc = yb.alloc(sz * std::max(M.nrows, 1)))) return rc;
+  HIPC(hipMemsetAsync(xb.p, 0, xb.bytes, s));
+  hipEvent_t ea, eb;
+  HIPC(hipEventCreate(&ea));
+  HIPC(hipEventCreate(&eb));
+  auto run = [&]() -> hipError_t {
+    if (M.fp32) return launch_spmv(M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, zero, s);
+    return launch_spmv(M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, zero, s);
+  };
+  float best = 1e30f;
+  int best_per = M.per;
+  for (int p : kPers) {
+    if (p != M.per && (rc = M.layout(p, hrp, hci))) break;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && !e; ++i) e = run();
+    if (!e) e = hipEventRecord(ea, s);
+    for (int i = 0; i < 5 && !e; ++i) e = run();
+    if (!e) e = hipEventRecord(eb, s);
+    if (!e) e = hipEventSynchronize(eb);
+    float ms = 0.f;
+    if (!e) e = hipEventElapsedTime(&ms, ea, eb);
+    if (e) {
+      rc = fail(CHR_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+      break;
+    }
+    if (ms < best) {
+      best = ms;
+      best_per = p;
+    }
+  }
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  if (rc) return rc;
+  return best_per == M.per ? CHR_OK : M.layout(best_per, hrp, hci);
 }
 
 }  // namespace
@@ -229,6 +295,7 @@ hipError_t launch_spmv(const DevCsr &M, const TX *x, Epi epi, const int *stop, h
 // ---------------------------------------------------------------------------------------
 struct chr_mat {
   DevCsr A;
+  DevBuf zero;  // an int 0: the stop flag of products outside a PCG solve
 };
 
 struct PcgWork {
@@ -262,6 +329,7 @@ struct chr_hier {
   DevBuf Ainv;  // FP64 dense inverse of the coarsest operator (row-major)
   DevBuf io_b, io_x;  // FP64 staging for host-pointer calls (n_0)
   chr_mat outer;     // level-0 operator in FP64 (default outer operator of PCG)
+  DevBuf zero;       // an int 0: the stop flag of V-cycles outside a PCG solve
   hipStream_t stream = nullptr;
   std::unique_ptr<PcgWork> pcg;
   chr_hier_stats stats{};
@@ -463,6 +531,10 @@ extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t pr
   auto h = std::make_unique<chr_hier>();
   h->precision = precision;
   HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  int rc0;
+  if ((rc0 = h->zero.alloc(sizeof(int)))) return rc0;
+  HIPC(hipMemset(h->zero.p, 0, sizeof(int)));
+  const int *zf = h->zero.as<int>();
   const bool f32 = precision == CHR_FP32_CYCLE;
   const size_t s = f32 ? 4 : 8;
   h->lev.resize(nlev);
@@ -472,15 +544,22 @@ extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t pr
     auto &L = h->lev[l];
     L.n = in.A.nrows;
     if (l + 1 < nlev) {
-      if ((rc = L.A.upload(in.A, f32))) return rc;
-      if ((rc = L.G.upload(in.G, f32))) return rc;
-      if ((rc = L.P.upload(in.P, f32))) return rc;
+      if ((rc = L.A.upload(in.A, f32)) || (rc = tune_layout(L.A, in.A.rowptr, in.A.col, h->stream, zf)))
+        return rc;
+      if ((rc = L.G.upload(in.G, f32)) || (rc = tune_layout(L.G, in.G.rowptr, in.G.col, h->stream, zf)))
+        return rc;
+      if ((rc = L.P.upload(in.P, f32)) || (rc = tune_layout(L.P, in.P.rowptr, in.P.col, h->stream, zf)))
+        return rc;
       std::vector<int32_t> trp, tci;
       std::vector<double> tv;
       transpose_host(in.G, trp, tci, tv);
-      if ((rc = L.GT.upload(in.G.ncols, in.G.nrows, trp.data(), tci.data(), tv.data(), f32))) return rc;
+      if ((rc = L.GT.upload(in.G.ncols, in.G.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
+          (rc = tune_layout(L.GT, trp.data(), tci.data(), h->stream, zf)))
+        return rc;
       transpose_host(in.P, trp, tci, tv);
-      if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, trp.data(), tci.data(), tv.data(), f32))) return rc;
+      if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
+          (rc = tune_layout(L.PT, trp.data(), tci.data(), h->stream, zf)))
+        return rc;
       L.omega = in.omega;
       L.nupre = in.nupre;
       L.nupost = in.nupost;
@@ -505,7 +584,11 @@ extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t pr
     if (l > 0 && (rc = L.b.alloc(s * L.n))) return rc;
   }
   // Level-0 operator in FP64 for the outer Krylov loop.
-  if ((rc = h->outer.A.upload(levels[0].A, false))) return rc;
+  if ((rc = h->outer.A.upload(levels[0].A, false)) ||
+      (rc = tune_layout(h->outer.A, levels[0].A.rowptr, levels[0].A.col, h->stream, zf)))
+    return rc;
+  if ((rc = h->outer.zero.alloc(sizeof(int)))) return rc;
+  HIPC(hipMemset(h->outer.zero.p, 0, sizeof(int)));
   const int32_t n0 = h->lev[0].n;
   if ((rc = h->io_b.alloc(8 * (size_t)n0))) return rc;
   if ((rc = h->io_x.alloc(8 * (size_t)n0))) return rc;
@@ -528,6 +611,19 @@ extern "C" int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st) {
   return CHR_OK;
 }
 
+extern "C" int chr_hier_layout(const chr_hier *h, int32_t level, char which, int32_t *per,
+                               int32_t *idx16, int64_t *nnz) {
+  if (!h || level < 0 || level + 1 >= (int32_t)h->lev.size()) return fail(CHR_EARG, "chr_hier_layout: level");
+  const auto &L = h->lev[level];
+  const DevCsr *M = which == 'A' ? &L.A : which == 'G' ? &L.G : which == 'T' ? &L.GT
+                  : which == 'P' ? &L.P : which == 'R' ? &L.PT : nullptr;
+  if (!M) return fail(CHR_EARG, "chr_hier_layout: which must be A, G, T, P or R");
+  if (per) *per = M->per;
+  if (idx16) *idx16 = M->idx16;
+  if (nnz) *nnz = M->nnz;
+  return CHR_OK;
+}
+
 extern "C" int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t ldb, double *x,
                                 int64_t ldx, int32_t ptr_kind) {
   if (!h || k < 0 || (k > 0 && (!b || !x))) return fail(CHR_EARG, "chr_vcycle_apply: arguments");
@@ -538,12 +634,12 @@ extern "C" int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t
     double *xc = x + (int64_t)c * ldx;
     if (ptr_kind == CHR_PTR_HOST) {
       HIPC(hipMemcpyAsync(h->io_b.p, bc, 8 * (size_t)n, hipMemcpyHostToDevice, h->stream));
-      HIPC(enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), nullptr));
+      HIPC(enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), h->zero.as<int>()));
       HIPC(hipMemcpyAsync(xc, h->io_x.p, 8 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
       int rc = sync_stream(h->stream);
       if (rc) return rc;
     } else {
-      HIPC(enqueue_vcycle(*h, bc, xc, nullptr));
+      HIPC(enqueue_vcycle(*h, bc, xc, h->zero.as<int>()));
     }
   }
   return sync_stream(h->stream);
@@ -554,7 +650,12 @@ extern "C" int chr_mat_create(const chr_csr *A, chr_mat **out) {
   int rc = check_csr(*A, "A");
   if (rc) return rc;
   auto m = std::make_unique<chr_mat>();
-  if ((rc = m->A.upload(*A, false))) return rc;
+  if ((rc = m->zero.alloc(sizeof(int)))) return rc;
+  HIPC(hipMemset(m->zero.p, 0, sizeof(int)));
+  if ((rc = m->A.upload(*A, false)) ||
+      (rc = tune_layout(m->A, A->rowptr, A->col, nullptr, m->zero.as<int>())))
+    return rc;
+  HIPC(hipDeviceSynchronize());
   *out = m.release();
   return CHR_OK;
 }
@@ -578,7 +679,7 @@ extern "C" int chr_spmv(chr_mat *A, const double *x, double *y, int32_t ptr_kind
     px = dx.as<double>();
     py = dy.as<double>();
   }
-  HIPC(launch_spmv(M, px, chr::EpiStore<double>{py}, nullptr, nullptr));
+  HIPC(launch_spmv(M, px, chr::EpiStore<double>{py}, A->zero.as<int>(), nullptr));
   if (ptr_kind == CHR_PTR_HOST) HIPC(hipMemcpy(y, py, 8 * (size_t)M.nrows, hipMemcpyDeviceToHost));
   HIPC(hipDeviceSynchronize());
   return CHR_OK;
@@ -596,6 +697,28 @@ hipError_t pcg_finalize(int kind, int count, double *part, chr::PcgState *st, do
   return hipGetLastError();
 }
 
+template <int PER, typename TI>
+hipError_t launch_pcg_p_t(const DevCsr &A, PcgWork &w, const double *pold, double *pnew,
+                          const chr::PcgState *st, hipStream_t s) {
+  hipLaunchKernelGGL((chr::pcg_p_stream_kernel<PER, TI>), dim3(A.nblk), dim3(chr::kBlock), 0, s,
+                     A.nblk, A.blk.as<int4>(), A.rp.as<int>(), A.ci.as<TI>(), A.val.as<double>(),
+                     (const double *)w.z.as<double>(), pold, pnew, w.ap.as<double>(),
+                     (const double *)w.r.as<double>(), w.part_p.as<double>(), st);
+  return hipGetLastError();
+}
+
+hipError_t launch_pcg_p(const DevCsr &A, PcgWork &w, const double *pold, double *pnew,
+                        const chr::PcgState *st, hipStream_t s) {
+  switch (A.per * 2 + (A.idx16 ? 1 : 0)) {
+    case 8: return launch_pcg_p_t<4, int>(A, w, pold, pnew, st, s);
+    case 9: return launch_pcg_p_t<4, uint16_t>(A, w, pold, pnew, st, s);
+    case 32: return launch_pcg_p_t<16, int>(A, w, pold, pnew, st, s);
+    case 33: return launch_pcg_p_t<16, uint16_t>(A, w, pold, pnew, st, s);
+    case 17: return launch_pcg_p_t<8, uint16_t>(A, w, pold, pnew, st, s);
+    default: return launch_pcg_p_t<8, int>(A, w, pold, pnew, st, s);
+  }
+}
+
 // One PCG_CJ iteration (PCG_CJ.m:88-120), p read from pold and written to pnew.
 hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity) {
   auto *st = w.state.as<chr::PcgState>();
@@ -610,18 +733,7 @@ hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity
                      (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
   if ((e = pcg_finalize(chr::kFinBeta, (int)w.nblk, part, st, resvec, h.stream))) return e;
-  if (A.idx16)
-    hipLaunchKernelGGL(chr::pcg_p_stream_kernel<uint16_t>, dim3(A.nblk), dim3(chr::kBlock), 0, h.stream,
-                       A.nblk, A.blk.as<int4>(), A.rp.as<int>(), (const uint16_t *)A.ci.p,
-                       A.val.as<double>(), (const double *)w.z.as<double>(), (const double *)pold, pnew,
-                       w.ap.as<double>(), (const double *)w.r.as<double>(), w.part_p.as<double>(),
-                       (const chr::PcgState *)st);
-  else
-    hipLaunchKernelGGL(chr::pcg_p_stream_kernel<int>, dim3(A.nblk), dim3(chr::kBlock), 0, h.stream,
-                       A.nblk, A.blk.as<int4>(), A.rp.as<int>(), A.ci32.as<int>(), A.val.as<double>(),
-                       (const double *)w.z.as<double>(), (const double *)pold, pnew, w.ap.as<double>(),
-                       (const double *)w.r.as<double>(), w.part_p.as<double>(), (const chr::PcgState *)st);
-  if ((e = hipGetLastError())) return e;
+  if ((e = launch_pcg_p(A, w, pold, pnew, st, h.stream))) return e;
   if ((e = pcg_finalize(chr::kFinAlpha, A.nblk, w.part_p.as<double>(), st, resvec, h.stream))) return e;
   hipLaunchKernelGGL(chr::pcg_update_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
                      (const double *)pnew, (const double *)w.ap.as<double>(), w.x.as<double>(),
@@ -723,8 +835,8 @@ extern "C" int chr_pcg_solve(chr_hier *h, chr_mat *Aop, const double *b, double 
   st0.tol = tol;
   st0.itmax = itmax;
   HIPC(hipMemcpyAsync(w.state.p, &st0, sizeof(st0), hipMemcpyHostToDevice, h->stream));
-  HIPC(launch_spmv(A, w.x.as<double>(), chr::EpiResidual<double, double>{db, w.r.as<double>()}, nullptr,
-                   h->stream));
+  HIPC(launch_spmv(A, w.x.as<double>(), chr::EpiResidual<double, double>{db, w.r.as<double>()},
+                   h->zero.as<int>(), h->stream));
   hipLaunchKernelGGL(chr::pcg_norms_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h->stream, n, db,
                      (const double *)w.r.as<double>(), w.part.as<double>());
   HIPC(hipGetLastError());
@@ -773,7 +885,7 @@ extern "C" int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms) {
   hipGraph_t g = nullptr;
   hipGraphExec_t ex = nullptr;
   HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), nullptr);
+  hipError_t e = enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), h->zero.as<int>());
   hipError_t e2 = hipStreamEndCapture(h->stream, &g);
   if (e || e2) {
     if (g) (void)hipGraphDestroy(g);
@@ -824,8 +936,9 @@ extern "C" int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t rep
   HIPC(hipEventCreate(&a));
   HIPC(hipEventCreate(&b));
   auto run = [&]() -> hipError_t {
-    if (f32) return launch_spmv(*M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, nullptr, h->stream);
-    return launch_spmv(*M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, nullptr, h->stream);
+    if (f32)
+      return launch_spmv(*M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, h->zero.as<int>(), h->stream);
+    return launch_spmv(*M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, h->zero.as<int>(), h->stream);
   };
   HIPC(run());
   HIPC(hipEventRecord(a, h->stream));

@@ -84,14 +84,14 @@ struct Acc<float> {
 // int4 {first row, first nonzero, base column, 0}; entry nblk closes the last block.
 // ---------------------------------------------------------------------------------------
 // CSR-stream SpMV.  The rows are cut on the host into blocks of whole consecutive rows
-// holding at most kStreamCap nonzeros and at most kBlock rows (a row longer than kStreamCap
-// is a block of its own).  A workgroup streams its block's values and column indices with
-// coalesced loads, kStreamPer independent loads in flight per thread, stores the products
+// holding at most PER * kBlock nonzeros and at most kBlock rows (a longer row is a block of
+// its own).  A workgroup streams its block's values and column indices with coalesced
+// loads, PER independent loads in flight per thread, stores the products
 // in LDS, and then each thread sums one row in column order (deterministic) and hands the
 // sum to the epilogue.  No padding and no dependence on the row-length distribution.
 // ---------------------------------------------------------------------------------------
-constexpr int kStreamCap = 2048;
-constexpr int kStreamPer = kStreamCap / kBlock;
+// PER = nonzeros staged per thread; the block capacity is PER * kBlock nonzeros.  PER is
+// chosen per matrix at upload by timing the candidates.
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8).
 // Map b to a logical block so that each XCD owns one contiguous range of rows and the
@@ -116,26 +116,26 @@ __device__ __forceinline__ T block_sum_t(T v, T *lds) {
 }
 
 // Stages the products of the block's nonzeros [e0, e1) in prod[].  Indices are clamped to
-// the last nonzero so that every load is unconditional: all 2*kStreamPer index/value loads
-// issue back to back, then the kStreamPer gathers, then the LDS stores (masked).
-template <typename TA, typename TV, typename TI, typename Xf>
+// the last nonzero so that every load is unconditional: all 2*PER index/value loads issue
+// back to back, then the PER gathers, then the LDS stores (masked).
+template <int PER, typename TA, typename TV, typename TI, typename Xf>
 __device__ __forceinline__ void stream_products(int e0, int e1, int base, const TI *__restrict__ ci,
                                                 const TV *__restrict__ val, Xf colval, TA *prod) {
   if (e1 <= e0) return;
-  int c[kStreamPer];
-  TV v[kStreamPer];
-  TA xv[kStreamPer];
+  int c[PER];
+  TV v[PER];
+  TA xv[PER];
   const int tid = threadIdx.x, last = e1 - 1;
 #pragma unroll
-  for (int k = 0; k < kStreamPer; ++k) {
+  for (int k = 0; k < PER; ++k) {
     const int e = min(e0 + k * kBlock + tid, last);
     c[k] = base + (int)ci[e];
     v[k] = val[e];
   }
 #pragma unroll
-  for (int k = 0; k < kStreamPer; ++k) xv[k] = colval(c[k]);
+  for (int k = 0; k < PER; ++k) xv[k] = colval(c[k]);
 #pragma unroll
-  for (int k = 0; k < kStreamPer; ++k) {
+  for (int k = 0; k < PER; ++k) {
     const int o = k * kBlock + tid;
     if (e0 + o < e1) prod[o] = (TA)v[k] * xv[k];
   }
@@ -147,7 +147,9 @@ struct ColLoad {
   __device__ __forceinline__ TA operator()(int j) const { return (TA)x[j]; }
 };
 
-template <typename TV, typename TI, typename TX, typename Epi>
+// stop: device flag (never null); the kernel does nothing once it is set.  It is read
+// together with the block table so that it adds no latency of its own.
+template <int PER, typename TV, typename TI, typename TX, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
                                                              const int4 *__restrict__ blk,
                                                              const int *__restrict__ rp,
@@ -155,14 +157,14 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
                                                              const TV *__restrict__ val,
                                                              const TX *__restrict__ x, Epi epi,
                                                              const int *__restrict__ stop) {
-  if (stop && *stop) return;
   using TA = typename Acc<TV>::type;
-  __shared__ TA prod[kStreamCap];
+  __shared__ TA prod[PER * kBlock];
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 lo = blk[b], hi = blk[b + 1];
+  if (*stop) return;
   const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
   const int tid = threadIdx.x;
-  if (e1 - e0 > kStreamCap) {  // one long row: strided sum + block reduction
+  if (e1 - e0 > PER * kBlock) {  // one long row: strided sum + block reduction
     TA s = 0;
     for (int e = e0 + tid; e < e1; e += kBlock) s += (TA)val[e] * (TA)x[base + (int)ci[e]];
     s = block_sum_t<TA>(s, prod);
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   const int rc = min(r, r1 - 1);
   const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
   const double pv = epi.pre(rc);
-  stream_products<TA>(e0, e1, base, ci, val, ColLoad<TA, TX>{x}, prod);
+  stream_products<PER, TA>(e0, e1, base, ci, val, ColLoad<TA, TX>{x}, prod);
   __syncthreads();
   if (r < r1) {
     TA s = 0;
@@ -262,22 +264,22 @@ struct PColVal {
   __device__ __forceinline__ double operator()(int j) const { return z[j] + beta * p[j]; }
 };
 
-template <typename TI>
+template <int PER, typename TI>
 __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
     int nblk, const int4 *__restrict__ blk, const int *__restrict__ rp, const TI *__restrict__ ci,
     const double *__restrict__ val, const double *__restrict__ z, const double *__restrict__ p_old,
     double *__restrict__ p_new, double *__restrict__ ap, const double *__restrict__ r,
     double *__restrict__ part, const PcgState *st) {
-  if (st->stop) return;
-  __shared__ double prod[kStreamCap];
-  const double beta = st->beta;
+  __shared__ double prod[PER * kBlock];
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 lo = blk[b], hi = blk[b + 1];
+  if (st->stop) return;
+  const double beta = st->beta;
   const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
   const int tid = threadIdx.x;
   PColVal cv{z, p_old, beta};
   double pap = 0.0, prr = 0.0;
-  if (e1 - e0 > kStreamCap) {
+  if (e1 - e0 > PER * kBlock) {
     double s = 0.0;
     for (int e = e0 + tid; e < e1; e += kBlock) s += val[e] * cv(base + (int)ci[e]);
     s = block_sum_t<double>(s, prod);
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
     const int rc = min(row, r1 - 1);
     const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
     const double pi = cv(rc), ri = r[rc];
-    stream_products<double>(e0, e1, base, ci, val, cv, prod);
+    stream_products<PER, double>(e0, e1, base, ci, val, cv, prod);
     __syncthreads();
     if (row < r1) {
       double s = 0.0;

@@ -94,6 +94,11 @@ int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t precision, ch
 int chr_hier_destroy(chr_hier *h);
 int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st);
 
+/* Layout the autotuner chose for one level matrix (which = 'A','G','T','P','R'): nonzeros
+ * per thread of the stream kernel, whether columns are uint16 block offsets, nonzeros. */
+int chr_hier_layout(const chr_hier *h, int32_t level, char which, int32_t *per, int32_t *idx16,
+                    int64_t *nnz);
+
 /* x = M b for an n x k block (k <= 16). */
 int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t ldb, double *x,
                      int64_t ldx, int32_t ptr_kind);
