@@ -25,6 +25,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 --pmc summary of the roofline kernel (measured HBM bytes per launch).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_G0_probe_heterodiff128_fp32.md")
+
+
+def _measured_traffic(n: int):
+    """HBM bytes per launch of the probe kernel, from the committed counter summary, when it
+    was measured on this workload size (None otherwise)."""
+    import re
+
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)\s+n=(\d+)", text, re.M)
+    if not m or int(m.group(2)) != n:
+        return None
+    return int(m.group(1))
 
 
 def _cpu_info():
@@ -54,7 +71,20 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run 8 device-to-device copies of 512 MiB (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calib:
+        import torch
+
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(8):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_each": src.numel() * 8, "copies": 8}))
+        return 0
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -173,8 +203,10 @@ def main() -> int:
                    "n": A.nrows, "nnz": A.nnz, "levels": hier.nlev, "iterations": r["its"][0],
                    "setup_s": t_setup},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "level-0 G SpMV (FSAI factor, FP32)" if main_prec == "fp32" else "level-0 G SpMV (FP64)",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": _measured_traffic(A.nrows) if main_prec == "fp32" else None,
+                     "kernel": ("level-0 G SpMV (FSAI factor, FP32), probe instance "
+                                "spmv_stream_kernel<..., EpiProbe<float>>"),
                      "bytes_per_launch": r["g_bytes"], "avg_launch_ms": dom_ms},
         "cpu_baseline": cpu,
         "vcycle": {p: {"ms": q["vcycle_ms"], "bytes": q["stats"]["vcycle_bytes"],

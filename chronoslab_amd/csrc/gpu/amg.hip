@@ -937,8 +937,8 @@ extern "C" int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t rep
   HIPC(hipEventCreate(&b));
   auto run = [&]() -> hipError_t {
     if (f32)
-      return launch_spmv(*M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, h->zero.as<int>(), h->stream);
-    return launch_spmv(*M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, h->zero.as<int>(), h->stream);
+      return launch_spmv(*M, xb.as<float>(), chr::EpiProbe<float>{{yb.as<float>()}}, h->zero.as<int>(), h->stream);
+    return launch_spmv(*M, xb.as<double>(), chr::EpiProbe<double>{{yb.as<double>()}}, h->zero.as<int>(), h->stream);
   };
   HIPC(run());
   HIPC(hipEventRecord(a, h->stream));
