@@ -21,6 +21,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.hpp"
@@ -261,7 +262,9 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s
     if (M.fp32) return launch_spmv(M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, zero, s);
     return launch_spmv(M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, zero, s);
   };
-  float best = 1e30f;
+  // The default (8) is kept unless another candidate is more than 3% faster, so that
+  // near-ties do not flip the layout from run to run.
+  float best = 1e30f, def_ms = 1e30f;
   int best_per = M.per;
   for (int p : kPers) {
     if (p != M.per && (rc = M.layout(p, hrp, hci))) break;
@@ -277,11 +280,13 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s
       rc = fail(CHR_EHIP, std::string("autotune: ") + hipGetErrorString(e));
       break;
     }
+    if (p == 8) def_ms = ms;
     if (ms < best) {
       best = ms;
       best_per = p;
     }
   }
+  if (best_per != 8 && best > 0.97f * def_ms) best_per = 8;
   (void)hipEventDestroy(ea);
   (void)hipEventDestroy(eb);
   if (rc) return rc;
@@ -935,10 +940,22 @@ extern "C" int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t rep
   hipEvent_t a, b;
   HIPC(hipEventCreate(&a));
   HIPC(hipEventCreate(&b));
-  auto run = [&]() -> hipError_t {
+  auto probe = [&](auto tag) -> hipError_t {
+    constexpr char W = decltype(tag)::value;
     if (f32)
-      return launch_spmv(*M, xb.as<float>(), chr::EpiProbe<float>{{yb.as<float>()}}, h->zero.as<int>(), h->stream);
-    return launch_spmv(*M, xb.as<double>(), chr::EpiProbe<double>{{yb.as<double>()}}, h->zero.as<int>(), h->stream);
+      return launch_spmv(*M, xb.as<float>(), chr::EpiProbe<float, W>{{yb.as<float>()}}, h->zero.as<int>(),
+                         h->stream);
+    return launch_spmv(*M, xb.as<double>(), chr::EpiProbe<double, W>{{yb.as<double>()}}, h->zero.as<int>(),
+                       h->stream);
+  };
+  auto run = [&]() -> hipError_t {
+    switch (which) {
+      case 'A': return probe(std::integral_constant<char, 'A'>{});
+      case 'G': return probe(std::integral_constant<char, 'G'>{});
+      case 'T': return probe(std::integral_constant<char, 'T'>{});
+      case 'P': return probe(std::integral_constant<char, 'P'>{});
+      default: return probe(std::integral_constant<char, 'R'>{});
+    }
   };
   HIPC(run());
   HIPC(hipEventRecord(a, h->stream));

@@ -46,7 +46,7 @@ struct EpiStore {  // out = acc
 };
 // Same as EpiStore; a distinct type so that the timing probe's kernel has a name of its own
 // in profiler output (its average duration then covers the probe's launches only).
-template <typename TO>
+template <typename TO, char W>
 struct EpiProbe : EpiStore<TO> {};
 template <typename TO>
 struct EpiScale {  // out = w * acc
