@@ -24,221 +24,11 @@
 #include <type_traits>
 #include <vector>
 
-#include "kernels.hpp"
+#include "runtime.hpp"
 
-namespace {
+namespace chrrt {
 
 thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIPC(expr)                                                                        \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess)                                                                 \
-      return fail(CHR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
-  } while (0)
-
-// Owned device allocation.
-struct DevBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
-    o.p = nullptr;
-    o.bytes = 0;
-  }
-  DevBuf &operator=(DevBuf &&o) noexcept {
-    if (this != &o) {
-      if (p) (void)hipFree(p);
-      p = o.p;
-      bytes = o.bytes;
-      o.p = nullptr;
-      o.bytes = 0;
-    }
-    return *this;
-  }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  int alloc(size_t b) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = b;
-    if (b == 0) return CHR_OK;
-    hipError_t e = hipMalloc(&p, b);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return fail(CHR_EALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
-    return CHR_OK;
-  }
-  template <typename T>
-  T *as() const {
-    return static_cast<T *>(p);
-  }
-};
-
-int check_csr(const chr_csr &m, const char *what) {
-  if (m.nrows < 0 || m.ncols < 0 || !m.rowptr || (m.rowptr[m.nrows] > 0 && (!m.col || !m.val)))
-    return fail(CHR_EARG, std::string(what) + ": null pointer or negative size");
-  if (m.rowptr[0] != 0) return fail(CHR_EARG, std::string(what) + ": rowptr[0] != 0");
-  for (int32_t i = 0; i < m.nrows; ++i) {
-    if (m.rowptr[i + 1] < m.rowptr[i])
-      return fail(CHR_EARG, std::string(what) + ": rowptr not monotone");
-    for (int32_t e = m.rowptr[i]; e < m.rowptr[i + 1]; ++e)
-      if (m.col[e] < 0 || m.col[e] >= m.ncols)
-        return fail(CHR_EARG, std::string(what) + ": column index out of range");
-  }
-  return CHR_OK;
-}
-
-// Host CSR transpose (columns of the result come out sorted).
-void transpose_host(const chr_csr &M, std::vector<int32_t> &rp, std::vector<int32_t> &ci,
-                    std::vector<double> &v) {
-  const int64_t nnz = M.rowptr[M.nrows];
-  rp.assign((size_t)M.ncols + 1, 0);
-  ci.resize(nnz);
-  v.resize(nnz);
-  for (int64_t e = 0; e < nnz; ++e) rp[M.col[e] + 1]++;
-  for (int32_t i = 0; i < M.ncols; ++i) rp[i + 1] += rp[i];
-  std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
-  for (int32_t i = 0; i < M.nrows; ++i)
-    for (int32_t e = M.rowptr[i]; e < M.rowptr[i + 1]; ++e) {
-      int32_t p = pos[M.col[e]]++;
-      ci[p] = i;
-      v[p] = M.val[e];
-    }
-}
-
-// Candidate nonzeros-per-thread of the stream kernel (block capacity PER * 256).
-constexpr int kPers[] = {4, 8, 16};
-
-// Device CSR in FP32 or FP64, laid out for the stream kernel.
-struct DevCsr {
-  int32_t nrows = 0, ncols = 0;
-  int64_t nnz = 0;
-  bool fp32 = false;
-  bool idx16 = false;  // uint16 column offsets from a per-block base
-  int per = 8;         // nonzeros per thread (block capacity per * kBlock)
-  int32_t nblk = 0;    // row blocks
-  DevBuf rp, val, blk;
-  DevBuf ci;           // uint16 offsets when idx16, else int32 columns
-  int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
-             bool as_fp32, int per_ = 8) {
-    nrows = nr;
-    ncols = nc;
-    nnz = hrp[nr];
-    fp32 = as_fp32;
-    int rc;
-    if ((rc = rp.alloc(sizeof(int32_t) * (nr + 1)))) return rc;
-    if ((rc = val.alloc((as_fp32 ? 4 : 8) * std::max<int64_t>(nnz, 1)))) return rc;
-    HIPC(hipMemcpy(rp.p, hrp, sizeof(int32_t) * (nr + 1), hipMemcpyHostToDevice));
-    if (nnz) {
-      if (as_fp32) {
-        std::vector<float> f(hv, hv + nnz);
-        HIPC(hipMemcpy(val.p, f.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
-      } else {
-        HIPC(hipMemcpy(val.p, hv, sizeof(double) * nnz, hipMemcpyHostToDevice));
-      }
-    }
-    return layout(per_, hrp, hci);
-  }
-  // Row blocks for a given per: whole rows, <= per * kBlock nonzeros, <= kBlock rows, as
-  // int4 {first row, first nonzero, base column, 0}, nblk + 1 of them; column indices as
-  // uint16 offsets from the block's smallest column when every block spans < 65536 columns.
-  int layout(int per_, const int32_t *hrp, const int32_t *hci) {
-    per = per_;
-    const int64_t cap = (int64_t)per * chr::kBlock;
-    std::vector<int4> br;
-    br.push_back(make_int4(0, 0, 0, 0));
-    int32_t rows = 0;
-    int64_t cnt = 0;
-    for (int32_t r = 0; r < nrows; ++r) {
-      const int64_t len = hrp[r + 1] - hrp[r];
-      if (rows > 0 && (rows == chr::kBlock || cnt + len > cap)) {
-        br.push_back(make_int4(r, hrp[r], 0, 0));
-        rows = 0;
-        cnt = 0;
-      }
-      ++rows;
-      cnt += len;
-    }
-    if (nrows > 0) br.push_back(make_int4(nrows, hrp[nrows], 0, 0));
-    nblk = (int32_t)br.size() - 1;
-    idx16 = nnz > 0;
-    for (int32_t b = 0; b < nblk && idx16; ++b) {
-      int32_t lo = INT32_MAX, hi = -1;
-      for (int64_t e = br[b].y; e < br[b + 1].y; ++e) {
-        lo = std::min(lo, hci[e]);
-        hi = std::max(hi, hci[e]);
-      }
-      if (hi >= 0 && hi - lo > 65535) idx16 = false;
-      br[b].z = hi >= 0 ? lo : 0;
-    }
-    if (!idx16)
-      for (auto &q : br) q.z = 0;
-    int rc;
-    if ((rc = blk.alloc(sizeof(int4) * br.size()))) return rc;
-    HIPC(hipMemcpy(blk.p, br.data(), sizeof(int4) * br.size(), hipMemcpyHostToDevice));
-    if (idx16) {
-      std::vector<uint16_t> d(nnz);
-      for (int32_t b = 0; b < nblk; ++b)
-        for (int64_t e = br[b].y; e < br[b + 1].y; ++e) d[e] = (uint16_t)(hci[e] - br[b].z);
-      if ((rc = ci.alloc(sizeof(uint16_t) * nnz))) return rc;
-      HIPC(hipMemcpy(ci.p, d.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
-    } else {
-      if ((rc = ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
-      if (nnz) HIPC(hipMemcpy(ci.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
-    }
-    return CHR_OK;
-  }
-  int upload(const chr_csr &m, bool as_fp32) {
-    return upload(m.nrows, m.ncols, m.rowptr, m.col, m.val, as_fp32);
-  }
-  // Bytes of one product as stored: values + column indices + row pointers + block table.
-  double stream_bytes() const {
-    return (double)nnz * ((fp32 ? 4 : 8) + (idx16 ? 2 : 4)) + 4.0 * (nrows + 1) + 16.0 * (nblk + 1);
-  }
-  // The same with 4-byte column indices and no block table (the reference's CSR format).
-  double csr_bytes() const { return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1); }
-  double device_bytes() const { return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes); }
-};
-
-template <int PER, typename TV, typename TX, typename Epi>
-hipError_t launch_spmv_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
-  if (M.idx16)
-    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, uint16_t, TX, Epi>), dim3(M.nblk),
-                       dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
-                       M.ci.as<uint16_t>(), M.val.as<TV>(), x, epi, stop);
-  else
-    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, int, TX, Epi>), dim3(M.nblk),
-                       dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
-                       M.ci.as<int>(), M.val.as<TV>(), x, epi, stop);
-  return hipGetLastError();
-}
-
-template <typename TV, typename TX, typename Epi>
-hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
-                         hipStream_t s) {
-  if (M.nrows == 0) return hipSuccess;
-  switch (M.per) {
-    case 4: return launch_spmv_p<4, TV>(M, x, epi, stop, s);
-    case 16: return launch_spmv_p<16, TV>(M, x, epi, stop, s);
-    default: return launch_spmv_p<8, TV>(M, x, epi, stop, s);
-  }
-}
-
-template <typename TX, typename Epi>
-hipError_t launch_spmv(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
-  return M.fp32 ? launch_spmv_t<float>(M, x, epi, stop, s)
-                : launch_spmv_t<double>(M, x, epi, stop, s);
-}
 
 // Autotuner: times the stream kernel on M for every candidate PER (HIP events, 2 warm-up
 // and 5 timed launches each) and keeps the fastest layout.  CHRONOS_STREAM_PER=4|8|16 in
@@ -293,59 +83,6 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s
   return best_per == M.per ? CHR_OK : M.layout(best_per, hrp, hci);
 }
 
-}  // namespace
-
-// ---------------------------------------------------------------------------------------
-// Opaque objects
-// ---------------------------------------------------------------------------------------
-struct chr_mat {
-  DevCsr A;
-  DevBuf zero;  // an int 0: the stop flag of products outside a PCG solve
-};
-
-struct PcgWork {
-  int32_t n = 0, itmax_cap = 0;
-  DevBuf x, r, z, p0, p1, ap, part, part_p, state, resvec;
-  unsigned nblk = 0;
-  chr::PcgState *pinned = nullptr;  // two slots
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  const chr_mat *graph_A = nullptr;  // outer operator the cached graphs were built for
-  ~PcgWork() {
-    for (auto &g : exec)
-      if (g) (void)hipGraphExecDestroy(g);
-    for (auto &e : ev)
-      if (e) (void)hipEventDestroy(e);
-    if (pinned) (void)hipHostFree(pinned);
-  }
-};
-
-struct chr_hier {
-  struct Level {
-    int32_t n = 0;
-    DevCsr A, G, GT, P, PT;
-    double omega = 1.0;
-    int32_t nupre = 1, nupost = 1;
-    DevBuf b, x, r, y;  // cycle vectors (b unused on level 0)
-  };
-  int32_t precision = CHR_FP64;
-  std::vector<Level> lev;
-  int32_t nc = 0;
-  DevBuf Ainv;  // FP64 dense inverse of the coarsest operator (row-major)
-  DevBuf io_b, io_x;  // FP64 staging for host-pointer calls (n_0)
-  chr_mat outer;     // level-0 operator in FP64 (default outer operator of PCG)
-  DevBuf zero;       // an int 0: the stop flag of V-cycles outside a PCG solve
-  hipStream_t stream = nullptr;
-  std::unique_ptr<PcgWork> pcg;
-  chr_hier_stats stats{};
-  ~chr_hier() {
-    pcg.reset();
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-  bool fp32() const { return precision == CHR_FP32_CYCLE; }
-};
-
-namespace {
 
 // Dense SPD inverse via Cholesky (FP64, host).  Returns false if not positive definite.
 bool spd_inverse(int32_t n, std::vector<double> &a) {
@@ -382,79 +119,43 @@ bool spd_inverse(int32_t n, std::vector<double> &a) {
   return true;
 }
 
-// ---------------------------------------------------------------------------------------
-// V-cycle enqueue.  T is the cycle type (float or double); level 0 reads b0 and writes
-// out0 in FP64.
-// ---------------------------------------------------------------------------------------
-template <typename T>
-struct Cycle {
-  chr_hier &h;
-  const int *stop;
-  hipStream_t s;
-
-  hipError_t coarse(const void *b, void *x, bool level0) {
-    const int n = h.nc;
-    const unsigned grid = (unsigned)(((int64_t)n * chr::kWave + chr::kBlock - 1) / chr::kBlock);
-    if (level0) {
-      hipLaunchKernelGGL((chr::coarse_gemv_kernel<double>), dim3(grid), dim3(chr::kBlock), 0, s, n,
-                         h.Ainv.as<double>(), (const double *)b, (double *)x, stop);
-    } else {
-      hipLaunchKernelGGL((chr::coarse_gemv_kernel<T>), dim3(grid), dim3(chr::kBlock), 0, s, n,
-                         h.Ainv.as<double>(), (const T *)b, (T *)x, stop);
-    }
-    return hipGetLastError();
-  }
-
-  // x <- x + omega G^T (G (b - A x)) sweep, with the output written to out (may be x).
-  template <typename TB, typename TO>
-  hipError_t post_sweep(chr_hier::Level &L, const TB *b, TO *out) {
-    hipError_t e;
-    T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
-    if ((e = launch_spmv(L.A, x, chr::EpiResidual<TB, T>{b, r}, stop, s))) return e;
-    if ((e = launch_spmv(L.G, r, chr::EpiStore<T>{y}, stop, s))) return e;
-    return launch_spmv(L.GT, y, chr::EpiAxpy<T, TO>{x, out, L.omega}, stop, s);
-  }
-
-  template <typename TB, typename TO>
-  hipError_t level(int l, const TB *b, TO *out) {
-    chr_hier::Level &L = h.lev[l];
-    const bool last = (l == (int)h.lev.size() - 1);
-    if (last) return coarse(b, out, l == 0);
-    hipError_t e;
-    T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
-    // Pre-smoothing from a zero initial guess, then nupre-1 further sweeps.
-    if ((e = launch_spmv(L.G, b, chr::EpiStore<T>{y}, stop, s))) return e;
-    if ((e = launch_spmv(L.GT, y, chr::EpiScale<T>{x, L.omega}, stop, s))) return e;
-    for (int k = 2; k <= L.nupre; ++k)
-      if ((e = post_sweep(L, b, x))) return e;
-    // Residual and restriction.
-    if ((e = launch_spmv(L.A, x, chr::EpiResidual<TB, T>{b, r}, stop, s))) return e;
-    chr_hier::Level &C = h.lev[l + 1];
-    T *bc = C.b.as<T>(), *xc = C.x.as<T>();
-    if ((e = launch_spmv(L.PT, r, chr::EpiStore<T>{bc}, stop, s))) return e;
-    // Coarse correction.
-    if ((e = level<T, T>(l + 1, bc, xc))) return e;
-    if ((e = launch_spmv(L.P, xc, chr::EpiAxpy<T, T>{x, x, 1.0}, stop, s))) return e;
-    // Post-smoothing; the last sweep writes the level's output.
-    if (L.nupost == 0) {
-      const unsigned grid = (unsigned)((L.n + chr::kBlock - 1) / chr::kBlock);
-      hipLaunchKernelGGL((chr::convert_kernel<T, TO>), dim3(grid), dim3(chr::kBlock), 0, s, L.n,
-                         (const T *)x, out);
-      return hipGetLastError();
-    }
-    for (int k = 1; k < L.nupost; ++k)
-      if ((e = post_sweep(L, b, x))) return e;
-    return post_sweep(L, b, out);
-  }
-};
-
-hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop) {
+hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop, int k) {
   if (h.fp32()) {
-    Cycle<float> c{h, stop, h.stream};
+    Cycle<float> c{h, stop, h.stream, k};
     return c.level<double, double>(0, b, out);
   }
-  Cycle<double> c{h, stop, h.stream};
+  Cycle<double> c{h, stop, h.stream, k};
   return c.level<double, double>(0, b, out);
+}
+
+int ensure_block_capacity(chr_hier &h, int k) {
+  if (k <= h.kcap) return CHR_OK;
+  HIPC(hipStreamSynchronize(h.stream));
+  if (h.pcg) {  // the captured PCG graphs point at the buffers about to be replaced
+    for (auto &g : h.pcg->exec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    h.pcg->graph_A = nullptr;
+  }
+  const size_t s = h.fp32() ? 4 : 8;
+  int rc;
+  for (size_t l = 0; l < h.lev.size(); ++l) {
+    auto &L = h.lev[l];
+    const bool last = l + 1 == h.lev.size();
+    if (!last) {
+      if ((rc = L.x.alloc(s * L.n * k)) || (rc = L.r.alloc(s * L.n * k)) || (rc = L.y.alloc(s * L.n * k)))
+        return rc;
+    } else if ((rc = L.x.alloc((l == 0 ? 8 : s) * L.n * k))) {
+      return rc;
+    }
+    if (l > 0 && (rc = L.b.alloc(s * L.n * k))) return rc;
+  }
+  const int32_t n0 = h.lev[0].n;
+  if ((rc = h.io_b.alloc(8 * (size_t)n0 * k)) || (rc = h.io_x.alloc(8 * (size_t)n0 * k))) return rc;
+  h.kcap = k;
+  return CHR_OK;
 }
 
 void fill_stats(chr_hier &h) {
@@ -499,7 +200,9 @@ int sync_stream(hipStream_t s) {
   return CHR_OK;
 }
 
-}  // namespace
+}  // namespace chrrt
+
+using namespace chrrt;
 
 // ---------------------------------------------------------------------------------------
 // C ABI
@@ -631,21 +334,50 @@ extern "C" int chr_hier_layout(const chr_hier *h, int32_t level, char which, int
 
 extern "C" int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t ldb, double *x,
                                 int64_t ldx, int32_t ptr_kind) {
-  if (!h || k < 0 || (k > 0 && (!b || !x))) return fail(CHR_EARG, "chr_vcycle_apply: arguments");
+  if (!h || k < 0 || k > kMaxBlockCols || (k > 0 && (!b || !x)))
+    return fail(CHR_EARG, "chr_vcycle_apply: need 0 <= k <= 16 and non-null blocks");
   const int32_t n = h->lev[0].n;
   if (k > 0 && (ldb < n || ldx < n)) return fail(CHR_EARG, "chr_vcycle_apply: leading dimension < n");
-  for (int32_t c = 0; c < k; ++c) {
-    const double *bc = b + (int64_t)c * ldb;
-    double *xc = x + (int64_t)c * ldx;
+  if (k == 0) return CHR_OK;
+  int rc;
+  if ((rc = ensure_block_capacity(*h, k))) return rc;
+  const int *zf = h->zero.as<int>();
+  if (k == 1) {
     if (ptr_kind == CHR_PTR_HOST) {
-      HIPC(hipMemcpyAsync(h->io_b.p, bc, 8 * (size_t)n, hipMemcpyHostToDevice, h->stream));
-      HIPC(enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), h->zero.as<int>()));
-      HIPC(hipMemcpyAsync(xc, h->io_x.p, 8 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
-      int rc = sync_stream(h->stream);
-      if (rc) return rc;
+      HIPC(hipMemcpyAsync(h->io_b.p, b, 8 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+      HIPC(enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), zf));
+      HIPC(hipMemcpyAsync(x, h->io_x.p, 8 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
     } else {
-      HIPC(enqueue_vcycle(*h, bc, xc, h->zero.as<int>()));
+      HIPC(enqueue_vcycle(*h, b, x, zf));
     }
+    return sync_stream(h->stream);
+  }
+  // Blocks: column-major (ld) at the boundary, row-major n x k inside the cycle.
+  DevBuf cm;
+  const double *bsrc = b;
+  int64_t lds = ldb;
+  if (ptr_kind == CHR_PTR_HOST) {
+    if ((rc = cm.alloc(8 * (size_t)n * k))) return rc;
+    HIPC(hipMemcpy2DAsync(cm.p, 8 * (size_t)n, b, 8 * (size_t)ldb, 8 * (size_t)n, k,
+                          hipMemcpyHostToDevice, h->stream));
+    bsrc = cm.as<double>();
+    lds = n;
+  }
+  const unsigned grid = (unsigned)(((int64_t)n * k + chr::kBlock - 1) / chr::kBlock);
+  hipLaunchKernelGGL(chr::cm_to_rm_kernel, dim3(grid), dim3(chr::kBlock), 0, h->stream, n, k, bsrc, lds,
+                     h->io_b.as<double>());
+  HIPC(hipGetLastError());
+  HIPC(enqueue_vcycle(*h, h->io_b.as<double>(), h->io_x.as<double>(), zf, k));
+  if (ptr_kind == CHR_PTR_HOST) {
+    hipLaunchKernelGGL(chr::rm_to_cm_kernel, dim3(grid), dim3(chr::kBlock), 0, h->stream, n, k,
+                       (const double *)h->io_x.as<double>(), cm.as<double>(), (int64_t)n);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpy2DAsync(x, 8 * (size_t)ldx, cm.p, 8 * (size_t)n, 8 * (size_t)n, k,
+                          hipMemcpyDeviceToHost, h->stream));
+  } else {
+    hipLaunchKernelGGL(chr::rm_to_cm_kernel, dim3(grid), dim3(chr::kBlock), 0, h->stream, n, k,
+                       (const double *)h->io_x.as<double>(), x, ldx);
+    HIPC(hipGetLastError());
   }
   return sync_stream(h->stream);
 }

@@ -189,22 +189,146 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Block product Y = M X for row-major n x k blocks (element (i, c) at i*k + c, k <= KL):
+// KL lanes per row, one per column; each lane sums its row in nonzero order with the
+// product rounded before the add, so that column c is bit for bit the single-vector stream
+// kernel's result on column c.  Epilogues see flat indices i*k + c.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+
+template <int KL, typename TV, typename TI, typename TX, typename Epi>
+__global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__restrict__ blk,
+                                                      const int *__restrict__ rp,
+                                                      const TI *__restrict__ ci,
+                                                      const TV *__restrict__ val,
+                                                      const TX *__restrict__ x, int k, Epi epi,
+                                                      const int *__restrict__ stop) {
+  using TA = typename Acc<TV>::type;
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 lo = blk[b], hi = blk[b + 1];
+  if (*stop) return;
+  const int r0 = lo.x, r1 = hi.x, base = lo.z;
+  const int c = threadIdx.x % KL;
+  if (c >= k) return;  // no cross-lane operations below
+  for (int r = r0 + threadIdx.x / KL; r < r1; r += kBlock / KL) {
+    const int i = r * k + c;
+    const double pv = epi.pre(i);
+    const int e1 = rp[r + 1];
+    TA s = 0;
+    for (int e = rp[r]; e < e1; ++e) s = s + mul_rn((TA)val[e], (TA)x[(base + (int)ci[e]) * k + c]);
+    epi(i, (double)s, pv);
+  }
+}
+
 // Coarsest solve x = Ainv b with the FP64 dense inverse (row-major n x n): one wave per
-// row, FP64 arithmetic whatever the cycle precision.
+// row, FP64 arithmetic whatever the cycle precision.  Column blockIdx.y of a row-major
+// n x k block (stride k); k = 1 for a single vector.
 template <typename TX>
 __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(int n, const double *__restrict__ Ainv,
                                                              const TX *__restrict__ b,
                                                              TX *__restrict__ x,
-                                                             const int *__restrict__ stop) {
+                                                             const int *__restrict__ stop, int k) {
   if (stop && *stop) return;
   const int row = (blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int lane = threadIdx.x % kWave;
+  const int c = blockIdx.y;
   if (row >= n) return;
   const double *a = Ainv + (size_t)row * n;
   double acc = 0.0;
-  for (int j = lane; j < n; j += kWave) acc += a[j] * (double)b[j];
+  for (int j = lane; j < n; j += kWave) acc += a[j] * (double)b[(size_t)j * k + c];
   acc = group_sum<kWave>(acc);
-  if (lane == 0) x[row] = (TX)acc;
+  if (lane == 0) x[(size_t)row * k + c] = (TX)acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// Dense block helpers (FP64, row-major blocks with leading dimension = column count).
+// ---------------------------------------------------------------------------------------
+// dst (row-major n x k) <- src (column-major, leading dimension ld)
+static __global__ __launch_bounds__(kBlock) void cm_to_rm_kernel(int n, int k, const double *__restrict__ src,
+                                                          int64_t ld, double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * k) return;
+  const int64_t i = t / k;
+  const int c = (int)(t % k);
+  dst[t] = src[c * ld + i];
+}
+// dst (column-major, leading dimension ld) <- src (row-major n x k)
+static __global__ __launch_bounds__(kBlock) void rm_to_cm_kernel(int n, int k, const double *__restrict__ src,
+                                                          double *__restrict__ dst, int64_t ld) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * k) return;
+  const int64_t i = t % n;  // consecutive threads walk down a column of dst
+  const int c = (int)(t / n);
+  dst[c * ld + i] = src[i * k + c];
+}
+
+// Per-block partial Gram products: part[g][p*b + q] = sum over the block's rows of
+// X[i*a + p] * Y[i*b + q] (a*b <= kBlock).  Rows are split in fixed chunks, so the partials
+// (and the in-order reduction that follows) do not depend on timing.
+static __global__ __launch_bounds__(kBlock) void gram_partial_kernel(int n, int a, const double *__restrict__ X,
+                                                              int b, const double *__restrict__ Y,
+                                                              int rows_per_block,
+                                                              double *__restrict__ part) {
+  const int t = threadIdx.x;
+  const int ab = a * b;
+  if (t >= ab) return;
+  const int p = t / b, q = t % b;
+  const int i0 = blockIdx.x * rows_per_block, i1 = min(n, i0 + rows_per_block);
+  double s = 0.0;
+  for (int i = i0; i < i1; ++i) s += X[(size_t)i * a + p] * Y[(size_t)i * b + q];
+  part[(size_t)blockIdx.x * ab + t] = s;
+}
+static __global__ __launch_bounds__(kBlock) void gram_reduce_kernel(int nparts, int ab,
+                                                             const double *__restrict__ part,
+                                                             double *__restrict__ out) {
+  const int t = threadIdx.x;
+  if (t >= ab) return;
+  double s = 0.0;
+  for (int g = 0; g < nparts; ++g) s += part[(size_t)g * ab + t];
+  out[t] = s;  // row-major a x b: out[p*b + q]
+}
+
+// Z (n x b) = beta * Z + X (n x a) * Cm, Cm column-major a x b.
+static __global__ __launch_bounds__(kBlock) void block_gemm_kernel(int n, int a, const double *__restrict__ X,
+                                                            int b, const double *__restrict__ Cm,
+                                                            double beta, double *__restrict__ Z) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * b) return;
+  const int64_t i = t / b;
+  const int q = (int)(t % b);
+  double s = 0.0;
+  for (int p = 0; p < a; ++p) s += X[i * a + p] * Cm[p + (size_t)q * a];
+  Z[t] = (beta != 0.0 ? beta * Z[t] : 0.0) + s;
+}
+
+// R = AX - X diag(lam)   (n x k)
+static __global__ __launch_bounds__(kBlock) void block_resid_kernel(int n, int k, const double *__restrict__ AX,
+                                                             const double *__restrict__ X,
+                                                             const double *__restrict__ lam,
+                                                             double *__restrict__ R) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * k) return;
+  R[t] = AX[t] - X[t] * lam[t % k];
+}
+
+// Y (n x m) = columns idx[0..m) of X (n x k).
+static __global__ __launch_bounds__(kBlock) void block_gather_kernel(int n, int k, const double *__restrict__ X,
+                                                              int m, const int *__restrict__ idx,
+                                                              double *__restrict__ Y) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * m) return;
+  const int64_t i = t / m;
+  Y[t] = X[i * k + idx[t % m]];
+}
+
+// Z = X + Y (elementwise, n*k entries)
+static __global__ __launch_bounds__(kBlock) void block_add_kernel(int64_t nk, const double *__restrict__ X,
+                                                           const double *__restrict__ Y,
+                                                           double *__restrict__ Z) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t < nk) Z[t] = X[t] + Y[t];
 }
 
 // Strided block column copy with type conversion: dst[i] = src[i] (one column).
@@ -229,7 +353,7 @@ struct PcgState {
 constexpr int kFinBlock = 1024;
 
 // Partials of b.b and r.r (r = b - A x is formed by the stream SpMV beforehand).
-__global__ __launch_bounds__(kBlock) void pcg_norms_kernel(int n, const double *__restrict__ b,
+static __global__ __launch_bounds__(kBlock) void pcg_norms_kernel(int n, const double *__restrict__ b,
                                                            const double *__restrict__ r,
                                                            double *__restrict__ part) {
   __shared__ double lds[kBlock / kWave];
@@ -247,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void pcg_norms_kernel(int n, const double *
 }
 
 // Partials of z . Ap_old (iterations > 1).
-__global__ __launch_bounds__(kBlock) void pcg_zap_kernel(int n, const double *__restrict__ z,
+static __global__ __launch_bounds__(kBlock) void pcg_zap_kernel(int n, const double *__restrict__ z,
                                                          const double *__restrict__ ap,
                                                          double *__restrict__ part,
                                                          const PcgState *st) {
@@ -320,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
 }
 
 // x += alpha p ; r -= alpha Ap ; partials of r.r.
-__global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, const double *__restrict__ p,
+static __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, const double *__restrict__ p,
                                                             const double *__restrict__ ap,
                                                             double *__restrict__ x,
                                                             double *__restrict__ r,
@@ -358,7 +482,7 @@ __device__ __forceinline__ double fin_sum(const double *part, int count, double 
 enum { kFinInit = 0, kFinBeta = 1, kFinAlpha = 2, kFinUpdate = 3 };
 
 // Scalar updates of PCG_CJ.m after each reduction (one workgroup).
-__global__ __launch_bounds__(kFinBlock) void pcg_finalize_kernel(int kind, int count,
+static __global__ __launch_bounds__(kFinBlock) void pcg_finalize_kernel(int kind, int count,
                                                                  const double *__restrict__ part,
                                                                  PcgState *st,
                                                                  double *__restrict__ resvec) {

@@ -1,0 +1,409 @@
+// Shared host runtime of the HIP solve phase: device buffers, the device CSR layout,
+// kernel launchers, the opaque handle types and the V-cycle sequence.
+#pragma once
+
+#include "chronos_amg.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace chrrt {
+
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(CHR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+// Owned device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t b) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = b;
+    if (b == 0) return CHR_OK;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(CHR_EALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return CHR_OK;
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+inline int check_csr(const chr_csr &m, const char *what) {
+  if (m.nrows < 0 || m.ncols < 0 || !m.rowptr || (m.rowptr[m.nrows] > 0 && (!m.col || !m.val)))
+    return fail(CHR_EARG, std::string(what) + ": null pointer or negative size");
+  if (m.rowptr[0] != 0) return fail(CHR_EARG, std::string(what) + ": rowptr[0] != 0");
+  for (int32_t i = 0; i < m.nrows; ++i) {
+    if (m.rowptr[i + 1] < m.rowptr[i])
+      return fail(CHR_EARG, std::string(what) + ": rowptr not monotone");
+    for (int32_t e = m.rowptr[i]; e < m.rowptr[i + 1]; ++e)
+      if (m.col[e] < 0 || m.col[e] >= m.ncols)
+        return fail(CHR_EARG, std::string(what) + ": column index out of range");
+  }
+  return CHR_OK;
+}
+
+// Host CSR transpose (columns of the result come out sorted).
+inline void transpose_host(const chr_csr &M, std::vector<int32_t> &rp, std::vector<int32_t> &ci,
+                    std::vector<double> &v) {
+  const int64_t nnz = M.rowptr[M.nrows];
+  rp.assign((size_t)M.ncols + 1, 0);
+  ci.resize(nnz);
+  v.resize(nnz);
+  for (int64_t e = 0; e < nnz; ++e) rp[M.col[e] + 1]++;
+  for (int32_t i = 0; i < M.ncols; ++i) rp[i + 1] += rp[i];
+  std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
+  for (int32_t i = 0; i < M.nrows; ++i)
+    for (int32_t e = M.rowptr[i]; e < M.rowptr[i + 1]; ++e) {
+      int32_t p = pos[M.col[e]]++;
+      ci[p] = i;
+      v[p] = M.val[e];
+    }
+}
+
+// Candidate nonzeros-per-thread of the stream kernel (block capacity PER * 256).
+constexpr int kPers[] = {4, 8, 16};
+
+// Device CSR in FP32 or FP64, laid out for the stream kernel.
+struct DevCsr {
+  int32_t nrows = 0, ncols = 0;
+  int64_t nnz = 0;
+  bool fp32 = false;
+  bool idx16 = false;  // uint16 column offsets from a per-block base
+  int per = 8;         // nonzeros per thread (block capacity per * kBlock)
+  int32_t nblk = 0;    // row blocks
+  DevBuf rp, val, blk;
+  DevBuf ci;           // uint16 offsets when idx16, else int32 columns
+  int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
+             bool as_fp32, int per_ = 8) {
+    nrows = nr;
+    ncols = nc;
+    nnz = hrp[nr];
+    fp32 = as_fp32;
+    int rc;
+    if ((rc = rp.alloc(sizeof(int32_t) * (nr + 1)))) return rc;
+    if ((rc = val.alloc((as_fp32 ? 4 : 8) * std::max<int64_t>(nnz, 1)))) return rc;
+    HIPC(hipMemcpy(rp.p, hrp, sizeof(int32_t) * (nr + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+      if (as_fp32) {
+        std::vector<float> f(hv, hv + nnz);
+        HIPC(hipMemcpy(val.p, f.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+      } else {
+        HIPC(hipMemcpy(val.p, hv, sizeof(double) * nnz, hipMemcpyHostToDevice));
+      }
+    }
+    return layout(per_, hrp, hci);
+  }
+  // Row blocks for a given per: whole rows, <= per * kBlock nonzeros, <= kBlock rows, as
+  // int4 {first row, first nonzero, base column, 0}, nblk + 1 of them; column indices as
+  // uint16 offsets from the block's smallest column when every block spans < 65536 columns.
+  int layout(int per_, const int32_t *hrp, const int32_t *hci) {
+    per = per_;
+    const int64_t cap = (int64_t)per * chr::kBlock;
+    std::vector<int4> br;
+    br.push_back(make_int4(0, 0, 0, 0));
+    int32_t rows = 0;
+    int64_t cnt = 0;
+    for (int32_t r = 0; r < nrows; ++r) {
+      const int64_t len = hrp[r + 1] - hrp[r];
+      if (rows > 0 && (rows == chr::kBlock || cnt + len > cap)) {
+        br.push_back(make_int4(r, hrp[r], 0, 0));
+        rows = 0;
+        cnt = 0;
+      }
+      ++rows;
+      cnt += len;
+    }
+    if (nrows > 0) br.push_back(make_int4(nrows, hrp[nrows], 0, 0));
+    nblk = (int32_t)br.size() - 1;
+    idx16 = nnz > 0;
+    for (int32_t b = 0; b < nblk && idx16; ++b) {
+      int32_t lo = INT32_MAX, hi = -1;
+      for (int64_t e = br[b].y; e < br[b + 1].y; ++e) {
+        lo = std::min(lo, hci[e]);
+        hi = std::max(hi, hci[e]);
+      }
+      if (hi >= 0 && hi - lo > 65535) idx16 = false;
+      br[b].z = hi >= 0 ? lo : 0;
+    }
+    if (!idx16)
+      for (auto &q : br) q.z = 0;
+    int rc;
+    if ((rc = blk.alloc(sizeof(int4) * br.size()))) return rc;
+    HIPC(hipMemcpy(blk.p, br.data(), sizeof(int4) * br.size(), hipMemcpyHostToDevice));
+    if (idx16) {
+      std::vector<uint16_t> d(nnz);
+      for (int32_t b = 0; b < nblk; ++b)
+        for (int64_t e = br[b].y; e < br[b + 1].y; ++e) d[e] = (uint16_t)(hci[e] - br[b].z);
+      if ((rc = ci.alloc(sizeof(uint16_t) * nnz))) return rc;
+      HIPC(hipMemcpy(ci.p, d.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
+    } else {
+      if ((rc = ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
+      if (nnz) HIPC(hipMemcpy(ci.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+    }
+    return CHR_OK;
+  }
+  int upload(const chr_csr &m, bool as_fp32) {
+    return upload(m.nrows, m.ncols, m.rowptr, m.col, m.val, as_fp32);
+  }
+  // Bytes of one product as stored: values + column indices + row pointers + block table.
+  double stream_bytes() const {
+    return (double)nnz * ((fp32 ? 4 : 8) + (idx16 ? 2 : 4)) + 4.0 * (nrows + 1) + 16.0 * (nblk + 1);
+  }
+  // The same with 4-byte column indices and no block table (the reference's CSR format).
+  double csr_bytes() const { return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1); }
+  double device_bytes() const { return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes); }
+};
+
+template <int PER, typename TV, typename TX, typename Epi>
+hipError_t launch_spmv_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  if (M.idx16)
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, uint16_t, TX, Epi>), dim3(M.nblk),
+                       dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
+                       M.ci.as<uint16_t>(), M.val.as<TV>(), x, epi, stop);
+  else
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, int, TX, Epi>), dim3(M.nblk),
+                       dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
+                       M.ci.as<int>(), M.val.as<TV>(), x, epi, stop);
+  return hipGetLastError();
+}
+
+template <typename TV, typename TX, typename Epi>
+hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
+                         hipStream_t s) {
+  if (M.nrows == 0) return hipSuccess;
+  switch (M.per) {
+    case 4: return launch_spmv_p<4, TV>(M, x, epi, stop, s);
+    case 16: return launch_spmv_p<16, TV>(M, x, epi, stop, s);
+    default: return launch_spmv_p<8, TV>(M, x, epi, stop, s);
+  }
+}
+
+template <typename TX, typename Epi>
+hipError_t launch_spmv(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  return M.fp32 ? launch_spmv_t<float>(M, x, epi, stop, s)
+                : launch_spmv_t<double>(M, x, epi, stop, s);
+}
+
+template <int KL, typename TV, typename TX, typename Epi>
+hipError_t launch_spmm_kl(const DevCsr &M, const TX *x, int k, Epi epi, const int *stop,
+                          hipStream_t s) {
+  if (M.idx16)
+    hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, uint16_t, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
+                       0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(), M.ci.as<uint16_t>(),
+                       M.val.as<TV>(), x, k, epi, stop);
+  else
+    hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, int, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock), 0, s,
+                       M.nblk, M.blk.as<int4>(), M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, k,
+                       epi, stop);
+  return hipGetLastError();
+}
+
+template <typename TV, typename TX, typename Epi>
+hipError_t launch_spmm_t(const DevCsr &M, const TX *x, int k, Epi epi, const int *stop,
+                         hipStream_t s) {
+  if (k <= 2) return launch_spmm_kl<2, TV>(M, x, k, epi, stop, s);
+  if (k <= 4) return launch_spmm_kl<4, TV>(M, x, k, epi, stop, s);
+  if (k <= 8) return launch_spmm_kl<8, TV>(M, x, k, epi, stop, s);
+  return launch_spmm_kl<16, TV>(M, x, k, epi, stop, s);
+}
+
+// Y = M X on row-major n x k blocks (k = 1: the stream kernel; 2 <= k <= 16: the block
+// kernel, whose columns equal the stream kernel's results bit for bit).
+template <typename TX, typename Epi>
+hipError_t launch_op(const DevCsr &M, const TX *x, int k, Epi epi, const int *stop, hipStream_t s) {
+  if (k == 1) return launch_spmv(M, x, epi, stop, s);
+  if (M.nrows == 0) return hipSuccess;
+  return M.fp32 ? launch_spmm_t<float>(M, x, k, epi, stop, s)
+                : launch_spmm_t<double>(M, x, k, epi, stop, s);
+}
+
+constexpr int kMaxBlockCols = 16;
+
+
+int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s, const int *zero);
+
+}  // namespace chrrt
+
+using chrrt::DevBuf;
+using chrrt::DevCsr;
+
+// ---------------------------------------------------------------------------------------
+// Opaque objects
+// ---------------------------------------------------------------------------------------
+struct chr_mat {
+  DevCsr A;
+  DevBuf zero;  // an int 0: the stop flag of products outside a PCG solve
+};
+
+struct PcgWork {
+  int32_t n = 0, itmax_cap = 0;
+  DevBuf x, r, z, p0, p1, ap, part, part_p, state, resvec;
+  unsigned nblk = 0;
+  chr::PcgState *pinned = nullptr;  // two slots
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  const chr_mat *graph_A = nullptr;  // outer operator the cached graphs were built for
+  ~PcgWork() {
+    for (auto &g : exec)
+      if (g) (void)hipGraphExecDestroy(g);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (pinned) (void)hipHostFree(pinned);
+  }
+};
+
+struct chr_hier {
+  struct Level {
+    int32_t n = 0;
+    DevCsr A, G, GT, P, PT;
+    double omega = 1.0;
+    int32_t nupre = 1, nupost = 1;
+    DevBuf b, x, r, y;  // cycle vectors, n x kcap (b unused on level 0)
+  };
+  int32_t precision = CHR_FP64;
+  std::vector<Level> lev;
+  int32_t nc = 0;
+  DevBuf Ainv;  // FP64 dense inverse of the coarsest operator (row-major)
+  int32_t kcap = 1;   // columns the cycle vectors are allocated for
+  DevBuf io_b, io_x;  // FP64 staging for host-pointer and block calls (n_0 x kcap)
+  chr_mat outer;     // level-0 operator in FP64 (default outer operator of PCG)
+  DevBuf zero;       // an int 0: the stop flag of V-cycles outside a PCG solve
+  hipStream_t stream = nullptr;
+  std::unique_ptr<PcgWork> pcg;
+  chr_hier_stats stats{};
+  ~chr_hier() {
+    pcg.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  bool fp32() const { return precision == CHR_FP32_CYCLE; }
+};
+
+namespace chrrt {
+
+// ---------------------------------------------------------------------------------------
+// V-cycle enqueue.  T is the cycle type (float or double); level 0 reads b0 and writes
+// out0 in FP64.
+// ---------------------------------------------------------------------------------------
+// The cycle runs on row-major n_l x k blocks (k = 1: single vectors).
+template <typename T>
+struct Cycle {
+  chr_hier &h;
+  const int *stop;
+  hipStream_t s;
+  int k = 1;
+
+  template <typename TX, typename Epi>
+  hipError_t op(const DevCsr &M, const TX *x, Epi epi) {
+    return launch_op(M, x, k, epi, stop, s);
+  }
+
+  hipError_t coarse(const void *b, void *x, bool level0) {
+    const int n = h.nc;
+    const dim3 grid((unsigned)(((int64_t)n * chr::kWave + chr::kBlock - 1) / chr::kBlock), (unsigned)k);
+    if (level0) {
+      hipLaunchKernelGGL((chr::coarse_gemv_kernel<double>), grid, dim3(chr::kBlock), 0, s, n,
+                         h.Ainv.as<double>(), (const double *)b, (double *)x, stop, k);
+    } else {
+      hipLaunchKernelGGL((chr::coarse_gemv_kernel<T>), grid, dim3(chr::kBlock), 0, s, n,
+                         h.Ainv.as<double>(), (const T *)b, (T *)x, stop, k);
+    }
+    return hipGetLastError();
+  }
+
+  // x <- x + omega G^T (G (b - A x)) sweep, with the output written to out (may be x).
+  template <typename TB, typename TO>
+  hipError_t post_sweep(chr_hier::Level &L, const TB *b, TO *out) {
+    hipError_t e;
+    T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
+    if ((e = op(L.A, x, chr::EpiResidual<TB, T>{b, r}))) return e;
+    if ((e = op(L.G, r, chr::EpiStore<T>{y}))) return e;
+    return op(L.GT, y, chr::EpiAxpy<T, TO>{x, out, L.omega});
+  }
+
+  template <typename TB, typename TO>
+  hipError_t level(int l, const TB *b, TO *out) {
+    chr_hier::Level &L = h.lev[l];
+    const bool last = (l == (int)h.lev.size() - 1);
+    if (last) return coarse(b, out, l == 0);
+    hipError_t e;
+    T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
+    // Pre-smoothing from a zero initial guess, then nupre-1 further sweeps.
+    if ((e = op(L.G, b, chr::EpiStore<T>{y}))) return e;
+    if ((e = op(L.GT, y, chr::EpiScale<T>{x, L.omega}))) return e;
+    for (int k = 2; k <= L.nupre; ++k)
+      if ((e = post_sweep(L, b, x))) return e;
+    // Residual and restriction.
+    if ((e = op(L.A, x, chr::EpiResidual<TB, T>{b, r}))) return e;
+    chr_hier::Level &C = h.lev[l + 1];
+    T *bc = C.b.as<T>(), *xc = C.x.as<T>();
+    if ((e = op(L.PT, r, chr::EpiStore<T>{bc}))) return e;
+    // Coarse correction.
+    if ((e = level<T, T>(l + 1, bc, xc))) return e;
+    if ((e = op(L.P, xc, chr::EpiAxpy<T, T>{x, x, 1.0}))) return e;
+    // Post-smoothing; the last sweep writes the level's output.
+    if (L.nupost == 0) {
+      const unsigned grid = (unsigned)(((int64_t)L.n * k + chr::kBlock - 1) / chr::kBlock);
+      hipLaunchKernelGGL((chr::convert_kernel<T, TO>), dim3(grid), dim3(chr::kBlock), 0, s, L.n * k,
+                         (const T *)x, out);
+      return hipGetLastError();
+    }
+    for (int k = 1; k < L.nupost; ++k)
+      if ((e = post_sweep(L, b, x))) return e;
+    return post_sweep(L, b, out);
+  }
+};
+
+// One V-cycle on FP64 row-major n_0 x k blocks (k <= kcap).
+hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop, int k = 1);
+// Grows the cycle vectors to k columns (synchronises the handle's stream).
+int ensure_block_capacity(chr_hier &h, int k);
+
+}  // namespace chrrt

@@ -115,6 +115,19 @@ int chr_spmv(chr_mat *A, const double *x, double *y, int32_t ptr_kind);
 int chr_pcg_solve(chr_hier *h, chr_mat *A, const double *b, double *x, int32_t itmax,
                   double tol, int32_t ptr_kind, int32_t *iter, double *resvec, int32_t *flag);
 
+/* lobpcg (sources/EigenSolver/lobpcg.m:21-467, B_orthonormalize.m:1-25) with prodB = [] and
+ * no constraint block, as sources/Drivers/driver_Eigen/EigDriver.m:202-226 calls it:
+ * prodA = A (NULL: the level-0 operator of h), prodM = V-cycle of h (h NULL: no
+ * preconditioner, A required).  X: n x neig column-major (ldx), X0 on entry and the
+ * eigenvectors on exit (neig <= 16).  lambda[neig]; resnorm_hist and lambda_hist, when not
+ * NULL, are (itmax+1) x neig column-major histories.  ierr: 0 converged, -1 not converged
+ * (X = best iterate), 1 X0 or a Rayleigh-Ritz step rank deficient, 2 B-orthonormalisation
+ * of the residual block failed (lobpcg.m:86-89,242-246,401-414). */
+int chr_lobpcg(chr_hier *h, chr_mat *A, int32_t n, int32_t neig, double *X, int64_t ldx,
+               int32_t largest, int32_t reslambda_check, int32_t itmax, double tol,
+               int32_t restart_control, int32_t ptr_kind, int32_t *niter, double *lambda,
+               double *resnorm_hist, double *lambda_hist, int32_t *ierr);
+
 /* Timing helpers (HIP events on the handle's stream, device-resident data):
  * mean milliseconds of one V-cycle (k = 1) over reps launches after one warm-up. */
 int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms);

@@ -1,0 +1,73 @@
+"""LOBPCG on the GPU against the oracle (same hierarchy, same X0) and against the closed-form
+spectrum of the Dirichlet 7-point Laplacian."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from chronoslab_amd import hierarchy as H
+
+pytestmark = pytest.mark.gpu
+
+
+def poisson_eigs(N, k):
+    j = np.arange(1, N + 1)
+    l1 = 2 - 2 * np.cos(j * np.pi / (N + 1))
+    return np.sort((l1[:, None, None] + l1[None, :, None] + l1[None, None, :]).ravel())[:k]
+
+
+def x0(n, k, seed=1):
+    return np.asfortranarray(np.stack([H.rng_uniform(seed, 10 + c, n) for c in range(k)], axis=1))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lobpcg_matches_oracle_and_closed_form(gpu, prec):
+    N, k = 16, 8
+    A = H.poisson7(N)
+    h = H.build(A)
+    M = gpu.AMGPreconditioner(h, prec)
+    X0 = x0(A.nrows, k)
+    it, lam, X, rh, lh, ierr = gpu.lobpcg(None, M, X0, itmax=200, tol=1e-8)
+    ito, lamo, Xo, rho, lho, ierro = oracle.lobpcg(A, oracle.Hier(h), X0, itmax=200, tol=1e-8)
+    print(f"{prec}: gpu it {it} ierr {ierr}; oracle it {ito} ierr {ierro}", flush=True)
+    assert ierr == 0 and ierro == 0
+    assert abs(it - ito) <= 1
+    np.testing.assert_allclose(lam, poisson_eigs(N, k), rtol=1e-8)
+    np.testing.assert_allclose(lam, lamo, rtol=1e-10)
+    R = A.to_scipy() @ X - X * lam
+    assert np.linalg.norm(R, axis=0).max() <= 1e-6
+
+
+def test_lobpcg_unpreconditioned_largest(gpu):
+    A = H.heterogeneous_diffusion(6, 1.0, 4)[0]
+    ev = np.linalg.eigvalsh(A.to_scipy().toarray())
+    X0 = x0(A.nrows, 4, 2)
+    it, lam, X, rh, lh, ierr = gpu.lobpcg(A, None, X0, largest=True, itmax=400, tol=1e-9)
+    ito, lamo, Xo, rho, lho, ierro = oracle.lobpcg(A, None, X0, largest=True, itmax=400, tol=1e-9)
+    np.testing.assert_allclose(lam, np.sort(ev[::-1][:4]), rtol=1e-7)
+    np.testing.assert_allclose(lam, lamo, rtol=1e-10)
+    assert abs(it - ito) <= 1
+
+
+def test_lobpcg_rank_deficient_start(gpu):
+    A = H.poisson7(6)
+    it, lam, X, rh, lh, ierr = gpu.lobpcg(A, None, np.ones((A.nrows, 3)), itmax=20, tol=1e-8)
+    assert ierr == 1
+
+
+def test_lobpcg_96cube_16_pairs(gpu):
+    """The LOBPCG configuration: 96^3 Poisson, 16 smallest pairs, AMG preconditioner,
+    FP32 V-cycle, FP64 Rayleigh-Ritz; eigenvalue relative error <= 1e-8 vs closed form."""
+    N, k = 96, 16
+    A = H.poisson7(N)
+    h = H.build(A)
+    M = gpu.AMGPreconditioner(h, "fp32")
+    X0 = x0(A.nrows, k, 7)
+    t = time.perf_counter()
+    it, lam, X, rh, lh, ierr = gpu.lobpcg(None, M, X0, itmax=300, tol=1e-8)
+    dt = time.perf_counter() - t
+    err = np.abs(lam - poisson_eigs(N, k)) / poisson_eigs(N, k)
+    print(f"96^3 LOBPCG: {it} iterations, {dt:.2f} s, ierr {ierr}, max eig rel err {err.max():.2e}", flush=True)
+    assert ierr == 0
+    assert err.max() <= 1e-8
