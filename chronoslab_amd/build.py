@@ -22,6 +22,9 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 ops stay off in every device compilation
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no FMA contraction: every product is rounded before it is added, so the block and
+    # single-vector kernels (and the FP64 oracle) round identically
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-function",
 ]
 HOST_FLAGS = ["-O3", "-march=x86-64-v2", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-Wall"]
