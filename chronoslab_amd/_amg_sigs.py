@@ -38,6 +38,7 @@ EXPORTS = {
     "chr_pcg_solve": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _F64, _I32, _I32P, _F64P, _I32P]),
     "chr_lobpcg": (C.c_int, [_VP, _VP, _I32, _I32, _VP, _I64, _I32, _I32, _I32, _F64, _I32, _I32, _I32P, _F64P,
                              _F64P, _F64P, _I32P]),
+    "chr_fv_grad": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _I32]),
     "chr_time_vcycle": (C.c_int, [_VP, _I32, _F64P]),
     "chr_time_spmv": (C.c_int, [_VP, _I32, C.c_char, _I32, _F64P, _F64P]),
 }

@@ -612,6 +612,43 @@ extern "C" int chr_pcg_solve(chr_hier *h, chr_mat *Aop, const double *b, double 
 }
 
 // ---------------------------------------------------------------------------------------
+// Adjoint gradient (finite-volume diffusion)
+// ---------------------------------------------------------------------------------------
+extern "C" int chr_fv_grad(chr_hier *h, chr_mat *Aop, const double *k, const double *x,
+                           const double *lam, double *g, int32_t ptr_kind) {
+  if ((!h && !Aop) || !k || !x || !lam || !g) return fail(CHR_EARG, "chr_fv_grad: null argument");
+  const DevCsr &A = Aop ? Aop->A : h->outer.A;
+  if (A.fp32) return fail(CHR_EARG, "chr_fv_grad: needs the FP64 operator");
+  const int32_t n = A.nrows;
+  hipStream_t s = h ? h->stream : nullptr;
+  DevBuf dk, dx, dl, dg;
+  const double *pk = k, *px = x, *pl = lam;
+  double *pg = g;
+  int rc;
+  if (ptr_kind == CHR_PTR_HOST) {
+    const size_t vb = 8 * (size_t)n;
+    if ((rc = dk.alloc(vb)) || (rc = dx.alloc(vb)) || (rc = dl.alloc(vb)) || (rc = dg.alloc(vb))) return rc;
+    HIPC(hipMemcpyAsync(dk.p, k, vb, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dx.p, x, vb, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dl.p, lam, vb, hipMemcpyHostToDevice, s));
+    pk = dk.as<double>();
+    px = dx.as<double>();
+    pl = dl.as<double>();
+    pg = dg.as<double>();
+  }
+  if (A.idx16)
+    hipLaunchKernelGGL(chr::fv_grad_kernel<uint16_t>, dim3(A.nblk), dim3(chr::kBlock), 0, s, A.nblk,
+                       A.blk.as<int4>(), A.rp.as<int>(), A.ci.as<uint16_t>(), A.val.as<double>(), pk, px, pl, pg);
+  else
+    hipLaunchKernelGGL(chr::fv_grad_kernel<int>, dim3(A.nblk), dim3(chr::kBlock), 0, s, A.nblk,
+                       A.blk.as<int4>(), A.rp.as<int>(), A.ci.as<int>(), A.val.as<double>(), pk, px, pl, pg);
+  HIPC(hipGetLastError());
+  if (ptr_kind == CHR_PTR_HOST) HIPC(hipMemcpyAsync(g, pg, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  return CHR_OK;
+}
+
+// ---------------------------------------------------------------------------------------
 // Timing helpers
 // ---------------------------------------------------------------------------------------
 extern "C" int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms) {

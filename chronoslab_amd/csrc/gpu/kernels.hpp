@@ -532,4 +532,39 @@ static __global__ __launch_bounds__(kFinBlock) void pcg_finalize_kernel(int kind
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Adjoint gradient for the finite-volume diffusion operator with harmonic transmissibilities
+// T_ij = 2 k_i k_j / (k_i + k_j) = -a_ij and Dirichlet terms proportional to k_i (their sum
+// over the row is the row sum of A).  With A lam = dL/dx:
+//   dL/d ln k_i = -( sum_{j != i} T_ij k_j/(k_i+k_j) (lam_i - lam_j)(x_i - x_j) + rowsum_i lam_i x_i )
+// One thread per row over A's pattern (row blocks of the stream layout).
+// ---------------------------------------------------------------------------------------
+template <typename TI>
+__global__ __launch_bounds__(kBlock) void fv_grad_kernel(int nblk, const int4 *__restrict__ blk,
+                                                         const int *__restrict__ rp,
+                                                         const TI *__restrict__ ci,
+                                                         const double *__restrict__ val,
+                                                         const double *__restrict__ kk,
+                                                         const double *__restrict__ x,
+                                                         const double *__restrict__ lam,
+                                                         double *__restrict__ g) {
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 lo = blk[b], hi = blk[b + 1];
+  const int base = lo.z;
+  for (int r = lo.x + threadIdx.x; r < hi.x; r += kBlock) {
+    const double ki = kk[r], xi = x[r], li = lam[r];
+    double s = 0.0, rowsum = 0.0;
+    for (int e = rp[r]; e < rp[r + 1]; ++e) {
+      const int j = base + (int)ci[e];
+      const double a = val[e];
+      rowsum += a;
+      if (j != r) {
+        const double kj = kk[j];
+        s += (-a) * kj / (ki + kj) * (li - lam[j]) * (xi - x[j]);
+      }
+    }
+    g[r] = -(s + rowsum * li * xi);
+  }
+}
+
 }  // namespace chr

@@ -128,6 +128,15 @@ int chr_lobpcg(chr_hier *h, chr_mat *A, int32_t n, int32_t neig, double *X, int6
                int32_t restart_control, int32_t ptr_kind, int32_t *niter, double *lambda,
                double *resnorm_hist, double *lambda_hist, int32_t *ierr);
 
+/* Adjoint gradient of a scalar loss for the cell-centred finite-volume diffusion operator
+ * (harmonic face transmissibilities T_ij = 2 k_i k_j/(k_i+k_j) = -A_ij, Dirichlet terms
+ * proportional to k_i): given the forward field x (A x = b) and the adjoint field lam
+ * (A^T lam = A lam = dL/dx, e.g. from chr_pcg_solve on the same hierarchy), writes
+ * g = dL/d(ln k) on A's sparsity pattern in one fused kernel.  A: NULL = the level-0
+ * operator of h.  No reference counterpart (the differentiation row of this build). */
+int chr_fv_grad(chr_hier *h, chr_mat *A, const double *k, const double *x, const double *lam,
+                double *g, int32_t ptr_kind);
+
 /* Timing helpers (HIP events on the handle's stream, device-resident data):
  * mean milliseconds of one V-cycle (k = 1) over reps launches after one warm-up. */
 int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms);
