@@ -73,12 +73,21 @@ def load_setup_lib():
 
 
 def load_amg_lib():
-    """The HIP solve-phase library.  Raises when it is not built."""
+    """The HIP solve-phase library.  Raises when it is not built.
+
+    PyTorch wheels for ROCm carry their own HIP/HSA runtime (torch/lib/libamdhip64.so, same
+    soname as the system one).  Two runtimes in one process cannot both enumerate the GPU,
+    so torch, when installed, is imported first: the library then binds to the runtime torch
+    already loaded, and torch tensors and this library share one device context."""
     global _amg
     with _lock:
         if _amg is None:
             if not os.path.exists(AMG_LIB):
                 raise RuntimeError(f"{AMG_LIB} is not built: run chronoslab_amd.build.build_all()")
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             from . import _amg_sigs
 
             lib = C.CDLL(AMG_LIB)

@@ -30,6 +30,11 @@ namespace chrrt {
 
 thread_local std::string g_err;
 
+uint64_t next_uid() {
+  static std::atomic<uint64_t> counter{0};
+  return ++counter;
+}
+
 // Autotuner: times the stream kernel on M for every candidate PER (HIP events, 2 warm-up
 // and 5 timed launches each) and keeps the fastest layout.  CHRONOS_STREAM_PER=4|8|16 in
 // the environment fixes PER instead.
@@ -131,14 +136,7 @@ hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *
 int ensure_block_capacity(chr_hier &h, int k) {
   if (k <= h.kcap) return CHR_OK;
   HIPC(hipStreamSynchronize(h.stream));
-  if (h.pcg) {  // the captured PCG graphs point at the buffers about to be replaced
-    for (auto &g : h.pcg->exec)
-      if (g) {
-        (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
-    h.pcg->graph_A = nullptr;
-  }
+  if (h.pcg) h.pcg->drop_graphs();  // they point at the buffers about to be replaced
   const size_t s = h.fp32() ? 4 : 8;
   int rc;
   for (size_t l = 0; l < h.lev.size(); ++l) {
@@ -487,6 +485,7 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
   const int32_t n = A.nrows;
   int rc;
   if (w.n != n) {
+    w.drop_graphs();
     w.n = n;
     const size_t vb = 8 * (size_t)n;
     if ((rc = w.x.alloc(vb)) || (rc = w.r.alloc(vb)) || (rc = w.z.alloc(vb)) ||
@@ -496,19 +495,21 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
     // single-block finalisation of the partial sums.
     w.nblk = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, ((int64_t)n + 1023) / 1024));
     if ((rc = w.part.alloc(sizeof(double) * 2 * w.nblk))) return rc;
-    if ((rc = w.part_p.alloc(sizeof(double) * 2 * std::max(A.nblk, 1)))) return rc;
     if ((rc = w.state.alloc(sizeof(chr::PcgState)))) return rc;
     if (!w.pinned) HIPC(hipHostMalloc((void **)&w.pinned, 2 * sizeof(chr::PcgState), hipHostMallocDefault));
     for (auto &e : w.ev)
       if (!e) HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    for (auto &g : w.exec)
-      if (g) {
-        (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
-    w.graph_A = nullptr;
+  }
+  // p-SpMV partials: two per row block of the outer operator (its layout may differ from
+  // one operator to the next even at equal n).
+  const size_t need = 2 * (size_t)std::max(A.nblk, 1);
+  if (w.part_p_cap < need) {
+    w.drop_graphs();
+    if ((rc = w.part_p.alloc(sizeof(double) * need))) return rc;
+    w.part_p_cap = need;
   }
   if (w.itmax_cap < itmax + 1) {
+    w.drop_graphs();
     w.itmax_cap = itmax + 1;
     if ((rc = w.resvec.alloc(sizeof(double) * w.itmax_cap))) return rc;
   }
@@ -517,7 +518,8 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
 
 int pcg_graphs(chr_hier &h, const chr_mat *Akey, const DevCsr &A) {
   PcgWork &w = *h.pcg;
-  if (w.exec[0] && w.graph_A == Akey) return CHR_OK;
+  if (w.exec[0] && w.exec[1] && w.graph_uid == Akey->uid) return CHR_OK;
+  w.drop_graphs();
   for (int parity = 0; parity < 2; ++parity) {
     if (w.exec[parity]) {
       (void)hipGraphExecDestroy(w.exec[parity]);
@@ -536,7 +538,7 @@ int pcg_graphs(chr_hier &h, const chr_mat *Akey, const DevCsr &A) {
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(CHR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
-  w.graph_A = Akey;
+  w.graph_uid = Akey->uid;
   return CHR_OK;
 }
 

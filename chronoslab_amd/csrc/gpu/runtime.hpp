@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -270,6 +271,8 @@ constexpr int kMaxBlockCols = 16;
 
 int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s, const int *zero);
 
+uint64_t next_uid();
+
 }  // namespace chrrt
 
 using chrrt::DevBuf;
@@ -279,6 +282,9 @@ using chrrt::DevCsr;
 // Opaque objects
 // ---------------------------------------------------------------------------------------
 struct chr_mat {
+  // Unique per object (never reused): keys the PCG graphs captured with this operator, so a
+  // new matrix that happens to get a freed one's address never reuses its graphs.
+  uint64_t uid = chrrt::next_uid();
   DevCsr A;
   DevBuf zero;  // an int 0: the stop flag of products outside a PCG solve
 };
@@ -290,7 +296,16 @@ struct PcgWork {
   chr::PcgState *pinned = nullptr;  // two slots
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
-  const chr_mat *graph_A = nullptr;  // outer operator the cached graphs were built for
+  uint64_t graph_uid = 0;  // uid of the outer operator the cached graphs were built for
+  size_t part_p_cap = 0;   // doubles in part_p
+  void drop_graphs() {     // the graphs hold buffer addresses: drop them before any realloc
+    for (auto &g : exec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    graph_uid = 0;
+  }
   ~PcgWork() {
     for (auto &g : exec)
       if (g) (void)hipGraphExecDestroy(g);

@@ -63,3 +63,22 @@ def test_pcg_matches_oracle(gpu, prec, kind, N):
     if prec == "fp64":
         assert err <= 1e-10
         np.testing.assert_allclose(rv[: min(it, itr) + 1], rvr[: min(it, itr) + 1], rtol=1e-7)
+
+
+def test_pcg_fresh_outer_operators_and_growing_itmax(gpu):
+    """Repeated solves with a new outer operator object each time (the graphs captured for a
+    freed operator must not be reused even if the new one reuses its address, and the
+    partial-sum buffer must fit each operator's row blocks), then a larger itmax."""
+    from chronoslab_amd import amg
+
+    A = _problem("diffusion", 16, 7)
+    hier = H.build(A)
+    M = gpu.AMGPreconditioner(hier, "fp64")
+    b = H.rng_uniform(3, 2, A.nrows, -1, 1)
+    x_ref, it_ref, _, _ = M.pcg(b, itmax=50, tol=1e-10)
+    for trial in range(4):
+        x, it, rv, fl = M.pcg(b, itmax=50, tol=1e-10, A=amg.DeviceMatrix(A))
+        assert fl == 0 and it == it_ref
+        np.testing.assert_allclose(x, x_ref, rtol=0, atol=1e-13 * np.abs(x_ref).max())
+    x, it, rv, fl = M.pcg(b, itmax=400, tol=1e-10)
+    assert it == it_ref and len(rv) == it + 1
