@@ -39,6 +39,11 @@ EXPORTS = {
     "chr_lobpcg": (C.c_int, [_VP, _VP, _I32, _I32, _VP, _I64, _I32, _I32, _I32, _F64, _I32, _I32, _I32P, _F64P,
                              _F64P, _F64P, _I32P]),
     "chr_fv_grad": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _I32]),
+    "chr_batch_create": (C.c_int, [_I32, _I32P, C.POINTER(C.POINTER(ChrLevel)), _I32, C.POINTER(_VP)]),
+    "chr_batch_destroy": (C.c_int, [_VP]),
+    "chr_batch_hier": (_VP, [_VP]),
+    "chr_batch_vcycle": (C.c_int, [_VP, _I32, _VP, _VP, _I32]),
+    "chr_batch_pcg": (C.c_int, [_VP, _I32, _VP, _VP, _I32, _F64, _I32, _I32P, _I32P, _F64P]),
     "chr_time_vcycle": (C.c_int, [_VP, _I32, _F64P]),
     "chr_time_spmv": (C.c_int, [_VP, _I32, C.c_char, _I32, _F64P, _F64P]),
 }

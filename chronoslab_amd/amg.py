@@ -166,6 +166,65 @@ class AMGPreconditioner:
         return ms.value, by.value
 
 
+class BatchSolver:
+    """Many independent systems (hierarchies of uneven depth) merged on the device; one
+    V-cycle / PCG_CJ call advances every system and every right-hand-side column."""
+
+    def __init__(self, hiers: list, precision="fp32"):
+        self._keep = []
+        self.nsys = len(hiers)
+        arrays = []
+        for hh in hiers:
+            arr = (ChrLevel * hh.nlev)()
+            for l, L in enumerate(hh.levels):
+                arr[l] = ChrLevel(_csr(L.A, self._keep), _csr(L.P, self._keep), _csr(L.G, self._keep), L.omega,
+                                  L.nupre, L.nupost)
+            arrays.append(arr)
+        self._arrays = arrays
+        ptrs = (C.POINTER(ChrLevel) * self.nsys)(*[C.cast(a, C.POINTER(ChrLevel)) for a in arrays])
+        nlev = np.array([hh.nlev for hh in hiers], np.int32)
+        self.sizes = [hh.levels[0].A.nrows for hh in hiers]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n = int(self.offsets[-1])
+        self.ptr = C.c_void_p()
+        _check(_lib().chr_batch_create(self.nsys, nlev.ctypes.data_as(_I32P), ptrs, _precision(precision),
+                                       C.byref(self.ptr)), "chr_batch_create")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib().chr_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def vcycle(self, B: np.ndarray) -> np.ndarray:
+        B = np.asfortranarray(B, np.float64)
+        k = 1 if B.ndim == 1 else B.shape[1]
+        X = np.empty_like(B, order="F")
+        _check(_lib().chr_batch_vcycle(self.ptr, k, B.ctypes.data, X.ctypes.data, PTR_HOST), "chr_batch_vcycle")
+        return X
+
+    def pcg(self, B: np.ndarray, X0=None, itmax: int = 500, tol: float = 1e-8):
+        B = np.asfortranarray(B, np.float64)
+        k = 1 if B.ndim == 1 else B.shape[1]
+        X = np.zeros_like(B, order="F") if X0 is None else np.asfortranarray(X0, np.float64).copy(order="F")
+        Q = self.nsys * k
+        iters = np.zeros(Q, np.int32)
+        flags = np.zeros(Q, np.int32)
+        resvec = np.zeros((itmax + 1) * Q)
+        _check(_lib().chr_batch_pcg(self.ptr, k, B.ctypes.data, X.ctypes.data, itmax, tol, PTR_HOST,
+                                    iters.ctypes.data_as(_I32P), flags.ctypes.data_as(_I32P),
+                                    resvec.ctypes.data_as(_F64P)), "chr_batch_pcg")
+        return X, iters.reshape(self.nsys, k), flags.reshape(self.nsys, k), resvec.reshape(Q, itmax + 1)
+
+    def pcg_device(self, k: int, b_ptr: int, x_ptr: int, itmax: int = 500, tol: float = 1e-8):
+        Q = self.nsys * k
+        iters = np.zeros(Q, np.int32)
+        flags = np.zeros(Q, np.int32)
+        _check(_lib().chr_batch_pcg(self.ptr, k, b_ptr, x_ptr, itmax, tol, PTR_DEVICE,
+                                    iters.ctypes.data_as(_I32P), flags.ctypes.data_as(_I32P), None),
+               "chr_batch_pcg")
+        return iters.reshape(self.nsys, k), flags.reshape(self.nsys, k)
+
+
 def lobpcg(prodA, prodM: AMGPreconditioner | None, X0: np.ndarray, largest: bool = False,
            reslambda_check: bool = False, itmax: int = 100, tol: float = 1e-8, restart_control: int = 7):
     """[niter, lambda, X, resnorm_vec, lambda_vec, ierr] = lobpcg(prodA, [], prodM, [], X0, ...)

@@ -141,13 +141,8 @@ int ensure_block_capacity(chr_hier &h, int k) {
   int rc;
   for (size_t l = 0; l < h.lev.size(); ++l) {
     auto &L = h.lev[l];
-    const bool last = l + 1 == h.lev.size();
-    if (!last) {
-      if ((rc = L.x.alloc(s * L.n * k)) || (rc = L.r.alloc(s * L.n * k)) || (rc = L.y.alloc(s * L.n * k)))
-        return rc;
-    } else if ((rc = L.x.alloc((l == 0 ? 8 : s) * L.n * k))) {
+    if ((rc = L.x.alloc(s * L.n * k)) || (rc = L.r.alloc(s * L.n * k)) || (rc = L.y.alloc(s * L.n * k)))
       return rc;
-    }
     if (l > 0 && (rc = L.b.alloc(s * L.n * k))) return rc;
   }
   const int32_t n0 = h.lev[0].n;
@@ -166,18 +161,18 @@ void fill_stats(chr_hier &h) {
   for (size_t l = 0; l < h.lev.size(); ++l) {
     auto &L = h.lev[l];
     if (l < 16) st.n[l] = L.n;
-    double b, bc;
-    if (l + 1 == h.lev.size()) {
-      b = bc = 8.0 * (double)h.nc * h.nc + 2.0 * h.nc * 8.0;
-      dev += h.Ainv.bytes;
-    } else {
+    // Direct solves: the FP64 inverses once + input and output vectors.
+    const double nd_direct = (double)(L.n - L.nd);
+    double b = 8.0 * (double)L.coarse_m2 + 2.0 * nd_direct * 8.0, bc = b;
+    dev += 8.0 * (double)L.coarse_m2;
+    if (L.nd > 0) {
       // Per level: 2 A, 2 G, 2 G^T, P, P^T, 18 n_l + 2 n_{l+1} vector words (AMG_Vcycle.m).
       const double nc = h.lev[l + 1].n;
-      const double vec = 18.0 * L.n * s + 2.0 * nc * s;
-      b = 2 * L.A.stream_bytes() + 2 * L.G.stream_bytes() + 2 * L.GT.stream_bytes() +
-          L.P.stream_bytes() + L.PT.stream_bytes() + vec;
-      bc = 2 * L.A.csr_bytes() + 2 * L.G.csr_bytes() + 2 * L.GT.csr_bytes() + L.P.csr_bytes() +
-           L.PT.csr_bytes() + vec;
+      const double vec = 18.0 * L.nd * s + 2.0 * nc * s;
+      b += 2 * L.A.stream_bytes() + 2 * L.G.stream_bytes() + 2 * L.GT.stream_bytes() +
+           L.P.stream_bytes() + L.PT.stream_bytes() + vec;
+      bc += 2 * L.A.csr_bytes() + 2 * L.G.csr_bytes() + 2 * L.GT.csr_bytes() + L.P.csr_bytes() +
+            L.PT.csr_bytes() + vec;
       dev += L.A.device_bytes() + L.G.device_bytes() + L.GT.device_bytes() + L.P.device_bytes() +
              L.PT.device_bytes() + L.b.bytes + L.x.bytes + L.r.bytes + L.y.bytes;
     }
@@ -212,44 +207,31 @@ extern "C" int chr_init(int32_t device) {
   return CHR_OK;
 }
 
-extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t precision,
-                               chr_hier **out) {
-  if (nlev < 1 || nlev > 16 || !levels || !out)
-    return fail(CHR_EARG, "chr_hier_create: need 1..16 levels and non-null pointers");
-  if (precision != CHR_FP64 && precision != CHR_FP32_CYCLE)
-    return fail(CHR_EARG, "chr_hier_create: unknown precision");
-  // Shape checks before any device work.
-  for (int32_t l = 0; l < nlev; ++l) {
-    int rc;
-    const chr_level &L = levels[l];
-    if ((rc = check_csr(L.A, "A"))) return rc;
-    if (L.A.nrows != L.A.ncols) return fail(CHR_EARG, "level operator must be square");
-    if (l + 1 < nlev) {
-      if ((rc = check_csr(L.P, "P"))) return rc;
-      if ((rc = check_csr(L.G, "G"))) return rc;
-      if (L.P.nrows != L.A.nrows || L.P.ncols != levels[l + 1].A.nrows)
-        return fail(CHR_EARG, "P shape does not match consecutive level sizes");
-      if (L.G.nrows != L.A.nrows || L.G.ncols != L.A.nrows)
-        return fail(CHR_EARG, "G shape does not match the level size");
-      if (L.nupre < 1 || L.nupost < 0) return fail(CHR_EARG, "nupre must be >= 1, nupost >= 0");
-    }
-  }
+namespace chrrt {
+
+// Builds a hierarchy on the device from per-level host descriptions (see chr_hier::Level).
+int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out) {
   auto h = std::make_unique<chr_hier>();
   h->precision = precision;
   HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  int rc0;
-  if ((rc0 = h->zero.alloc(sizeof(int)))) return rc0;
+  int rc;
+  if ((rc = h->zero.alloc(sizeof(int)))) return rc;
   HIPC(hipMemset(h->zero.p, 0, sizeof(int)));
   const int *zf = h->zero.as<int>();
   const bool f32 = precision == CHR_FP32_CYCLE;
   const size_t s = f32 ? 4 : 8;
+  const int32_t nlev = (int32_t)lv.size();
   h->lev.resize(nlev);
-  int rc;
+  std::vector<double> inv_all;  // packed inverses of every direct block
   for (int32_t l = 0; l < nlev; ++l) {
-    const chr_level &in = levels[l];
+    const LevelIn &in = lv[l];
     auto &L = h->lev[l];
-    L.n = in.A.nrows;
-    if (l + 1 < nlev) {
+    L.n = in.n;
+    L.nd = in.nd;
+    L.omega = in.omega;
+    L.nupre = in.nupre;
+    L.nupost = in.nupost;
+    if (in.nd > 0) {
       if ((rc = L.A.upload(in.A, f32)) || (rc = tune_layout(L.A, in.A.rowptr, in.A.col, h->stream, zf)))
         return rc;
       if ((rc = L.G.upload(in.G, f32)) || (rc = tune_layout(L.G, in.G.rowptr, in.G.col, h->stream, zf)))
@@ -266,32 +248,44 @@ extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t pr
       if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
           (rc = tune_layout(L.PT, trp.data(), tci.data(), h->stream, zf)))
         return rc;
-      L.omega = in.omega;
-      L.nupre = in.nupre;
-      L.nupost = in.nupost;
-      if ((rc = L.x.alloc(s * L.n))) return rc;
-      if ((rc = L.r.alloc(s * L.n))) return rc;
-      if ((rc = L.y.alloc(s * L.n))) return rc;
-    } else {
-      // Coarsest level: FP64 dense inverse.
-      const int32_t n = in.A.nrows;
-      if ((int64_t)n * n > (int64_t)1 << 26)
-        return fail(CHR_EARG, "coarsest level too large for a dense inverse");
-      std::vector<double> a((size_t)n * n, 0.0);
-      for (int32_t i = 0; i < n; ++i)
-        for (int32_t e = in.A.rowptr[i]; e < in.A.rowptr[i + 1]; ++e)
-          a[(size_t)i * n + in.A.col[e]] = in.A.val[e];
-      if (!spd_inverse(n, a)) return fail(CHR_ENOTSPD, "coarsest operator is not positive definite");
-      h->nc = n;
-      if ((rc = h->Ainv.alloc(sizeof(double) * a.size()))) return rc;
-      HIPC(hipMemcpy(h->Ainv.p, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice));
-      if ((rc = L.x.alloc((l == 0 ? 8 : s) * n))) return rc;
+      if (!in.omega_rows.empty()) {
+        if ((rc = L.omega_rows.alloc(sizeof(double) * in.nd))) return rc;
+        HIPC(hipMemcpy(L.omega_rows.p, in.omega_rows.data(), sizeof(double) * in.nd, hipMemcpyHostToDevice));
+      }
     }
+    // Direct blocks: FP64 dense inverses packed back to back, one cmap entry per row.
+    if (!in.direct.empty()) {
+      std::vector<int4> cmap((size_t)(in.n - in.nd));
+      for (const auto &d : in.direct) {
+        const int32_t m = d.A.nrows;
+        if ((int64_t)m * m > (int64_t)1 << 26)
+          return fail(CHR_EARG, "coarsest level too large for a dense inverse");
+        if (d.start < in.nd || d.start + m > in.n) return fail(CHR_EARG, "direct block outside the level");
+        std::vector<double> a((size_t)m * m, 0.0);
+        for (int32_t i = 0; i < m; ++i)
+          for (int32_t e = d.A.rowptr[i]; e < d.A.rowptr[i + 1]; ++e) a[(size_t)i * m + d.A.col[e]] = d.A.val[e];
+        if (!spd_inverse(m, a)) return fail(CHR_ENOTSPD, "coarsest operator is not positive definite");
+        const int64_t off = (int64_t)inv_all.size();
+        if (off > INT32_MAX) return fail(CHR_EARG, "direct blocks too large");
+        inv_all.insert(inv_all.end(), a.begin(), a.end());
+        for (int32_t i = 0; i < m; ++i) cmap[d.start + i - in.nd] = make_int4(d.start, m, (int32_t)off, 0);
+        L.ncoarse_blocks++;
+        L.coarse_m2 += (int64_t)m * m;
+        h->nc += m;
+      }
+      if ((rc = L.cmap.alloc(sizeof(int4) * cmap.size()))) return rc;
+      HIPC(hipMemcpy(L.cmap.p, cmap.data(), sizeof(int4) * cmap.size(), hipMemcpyHostToDevice));
+    }
+    if ((rc = L.x.alloc(s * L.n)) || (rc = L.r.alloc(s * L.n)) || (rc = L.y.alloc(s * L.n))) return rc;
     if (l > 0 && (rc = L.b.alloc(s * L.n))) return rc;
   }
+  if (!inv_all.empty()) {
+    if ((rc = h->Ainv.alloc(sizeof(double) * inv_all.size()))) return rc;
+    HIPC(hipMemcpy(h->Ainv.p, inv_all.data(), sizeof(double) * inv_all.size(), hipMemcpyHostToDevice));
+  }
   // Level-0 operator in FP64 for the outer Krylov loop.
-  if ((rc = h->outer.A.upload(levels[0].A, false)) ||
-      (rc = tune_layout(h->outer.A, levels[0].A.rowptr, levels[0].A.col, h->stream, zf)))
+  if ((rc = h->outer.A.upload(outer, false)) ||
+      (rc = tune_layout(h->outer.A, outer.rowptr, outer.col, h->stream, zf)))
     return rc;
   if ((rc = h->outer.zero.alloc(sizeof(int)))) return rc;
   HIPC(hipMemset(h->outer.zero.p, 0, sizeof(int)));
@@ -301,6 +295,45 @@ extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t pr
   fill_stats(*h);
   *out = h.release();
   return CHR_OK;
+}
+
+}  // namespace chrrt
+
+extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t precision,
+                               chr_hier **out) {
+  if (nlev < 1 || nlev > 16 || !levels || !out)
+    return fail(CHR_EARG, "chr_hier_create: need 1..16 levels and non-null pointers");
+  if (precision != CHR_FP64 && precision != CHR_FP32_CYCLE)
+    return fail(CHR_EARG, "chr_hier_create: unknown precision");
+  // Shape checks before any device work.
+  std::vector<LevelIn> lv(nlev);
+  for (int32_t l = 0; l < nlev; ++l) {
+    int rc;
+    const chr_level &L = levels[l];
+    if ((rc = check_csr(L.A, "A"))) return rc;
+    if (L.A.nrows != L.A.ncols) return fail(CHR_EARG, "level operator must be square");
+    lv[l].n = L.A.nrows;
+    if (l + 1 < nlev) {
+      if ((rc = check_csr(L.P, "P"))) return rc;
+      if ((rc = check_csr(L.G, "G"))) return rc;
+      if (L.P.nrows != L.A.nrows || L.P.ncols != levels[l + 1].A.nrows)
+        return fail(CHR_EARG, "P shape does not match consecutive level sizes");
+      if (L.G.nrows != L.A.nrows || L.G.ncols != L.A.nrows)
+        return fail(CHR_EARG, "G shape does not match the level size");
+      if (L.nupre < 1 || L.nupost < 0) return fail(CHR_EARG, "nupre must be >= 1, nupost >= 0");
+      lv[l].nd = L.A.nrows;
+      lv[l].A = L.A;
+      lv[l].G = L.G;
+      lv[l].P = L.P;
+      lv[l].omega = L.omega;
+      lv[l].nupre = L.nupre;
+      lv[l].nupost = L.nupost;
+    } else {
+      lv[l].nd = 0;
+      lv[l].direct.push_back({0, L.A});
+    }
+  }
+  return build_hier(lv, precision, levels[0].A, out);
 }
 
 extern "C" int chr_hier_destroy(chr_hier *h) {
@@ -319,7 +352,8 @@ extern "C" int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st) {
 
 extern "C" int chr_hier_layout(const chr_hier *h, int32_t level, char which, int32_t *per,
                                int32_t *idx16, int64_t *nnz) {
-  if (!h || level < 0 || level + 1 >= (int32_t)h->lev.size()) return fail(CHR_EARG, "chr_hier_layout: level");
+  if (!h || level < 0 || level >= (int32_t)h->lev.size() || h->lev[level].nd == 0)
+    return fail(CHR_EARG, "chr_hier_layout: level without smoother/transfers");
   const auto &L = h->lev[level];
   const DevCsr *M = which == 'A' ? &L.A : which == 'G' ? &L.G : which == 'T' ? &L.GT
                   : which == 'P' ? &L.P : which == 'R' ? &L.PT : nullptr;
@@ -688,7 +722,7 @@ extern "C" int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms) {
 
 extern "C" int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t reps, double *ms,
                              double *bytes) {
-  if (!h || level < 0 || level + 1 >= (int32_t)h->lev.size() || reps < 1 || !ms)
+  if (!h || level < 0 || level >= (int32_t)h->lev.size() || h->lev[level].nd == 0 || reps < 1 || !ms)
     return fail(CHR_EARG, "chr_time_spmv: arguments");
   auto &L = h->lev[level];
   const DevCsr *M = nullptr;

@@ -48,12 +48,16 @@ struct EpiStore {  // out = acc
 // in profiler output (its average duration then covers the probe's launches only).
 template <typename TO, char W>
 struct EpiProbe : EpiStore<TO> {};
+// wv (optional): a per-row weight (the smoother damping of the row's system in a batched
+// hierarchy); row = flat index / k.
 template <typename TO>
 struct EpiScale {  // out = w * acc
   TO *out;
   double w;
-  __device__ __forceinline__ double pre(int) const { return 0.0; }
-  __device__ __forceinline__ void operator()(int i, double acc, double) const { out[i] = (TO)(w * acc); }
+  const double *wv = nullptr;
+  int k = 1;
+  __device__ __forceinline__ double pre(int i) const { return wv ? wv[i / k] : w; }
+  __device__ __forceinline__ void operator()(int i, double acc, double ww) const { out[i] = (TO)(ww * acc); }
 };
 template <typename TB, typename TO>
 struct EpiResidual {  // out = b - acc
@@ -67,8 +71,12 @@ struct EpiAxpy {  // out = x + w * acc   (x and out may alias: same row, same th
   const TX *x;
   TO *out;
   double w;
+  const double *wv = nullptr;
+  int k = 1;
   __device__ __forceinline__ double pre(int i) const { return (double)x[i]; }
-  __device__ __forceinline__ void operator()(int i, double acc, double xi) const { out[i] = (TO)(xi + w * acc); }
+  __device__ __forceinline__ void operator()(int i, double acc, double xi) const {
+    out[i] = (TO)(xi + (wv ? wv[i / k] : w) * acc);
+  }
 };
 
 // Accumulator type: float products for the FP32 cycle, double otherwise.
@@ -222,22 +230,28 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__re
   }
 }
 
-// Coarsest solve x = Ainv b with the FP64 dense inverse (row-major n x n): one wave per
-// row, FP64 arithmetic whatever the cycle precision.  Column blockIdx.y of a row-major
-// n x k block (stride k); k = 1 for a single vector.
+// Direct solves on a level's coarse rows [nd, nd + nrows): each belongs to a dense block
+// (a system whose coarsest level this is) with its FP64 inverse (row-major m x m) packed in
+// Ainv.  cmap[t] = {first row of the block, m, offset of the block's inverse, 0} for coarse
+// row t.  One wave per row, FP64 arithmetic whatever the cycle precision; column blockIdx.y
+// of a row-major n x k block.
 template <typename TX>
-__global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(int n, const double *__restrict__ Ainv,
+__global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(int nrows, int nd,
+                                                             const int4 *__restrict__ cmap,
+                                                             const double *__restrict__ Ainv,
                                                              const TX *__restrict__ b,
                                                              TX *__restrict__ x,
                                                              const int *__restrict__ stop, int k) {
   if (stop && *stop) return;
-  const int row = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int t = (blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int lane = threadIdx.x % kWave;
   const int c = blockIdx.y;
-  if (row >= n) return;
-  const double *a = Ainv + (size_t)row * n;
+  if (t >= nrows) return;
+  const int4 info = cmap[t];
+  const int row = nd + t, m = info.y, r0 = info.x;
+  const double *a = Ainv + (size_t)info.z + (size_t)(row - r0) * m;
   double acc = 0.0;
-  for (int j = lane; j < n; j += kWave) acc += a[j] * (double)b[(size_t)j * k + c];
+  for (int j = lane; j < m; j += kWave) acc += a[j] * (double)b[(size_t)(r0 + j) * k + c];
   acc = group_sum<kWave>(acc);
   if (lane == 0) x[(size_t)row * k + c] = (TX)acc;
 }

@@ -316,17 +316,26 @@ struct PcgWork {
 };
 
 struct chr_hier {
+  // Level l holds n rows.  Rows [0, nd) go through smoothing and the transfers (A, G, G^T
+  // have nd rows, P is nd x n_{l+1}); rows [nd, n) are the coarsest levels of the systems
+  // that end here and are solved directly with dense FP64 inverses.  A single system has
+  // nd = n on every level but the last, where nd = 0.  A batch of systems merged into one
+  // block-diagonal hierarchy has systems ending at different levels.
   struct Level {
-    int32_t n = 0;
+    int32_t n = 0, nd = 0;
     DevCsr A, G, GT, P, PT;
     double omega = 1.0;
+    DevBuf omega_rows;  // per-row damping (batched hierarchies with differing omegas)
     int32_t nupre = 1, nupost = 1;
     DevBuf b, x, r, y;  // cycle vectors, n x kcap (b unused on level 0)
+    DevBuf cmap;        // int4 per direct row: {block first row, m, inverse offset, 0}
+    int32_t ncoarse_blocks = 0;
+    int64_t coarse_m2 = 0;  // sum of m^2 over the level's direct blocks
   };
   int32_t precision = CHR_FP64;
   std::vector<Level> lev;
-  int32_t nc = 0;
-  DevBuf Ainv;  // FP64 dense inverse of the coarsest operator (row-major)
+  int32_t nc = 0;  // rows solved directly, over all levels
+  DevBuf Ainv;     // packed FP64 dense inverses of the direct blocks (row-major each)
   int32_t kcap = 1;   // columns the cycle vectors are allocated for
   DevBuf io_b, io_x;  // FP64 staging for host-pointer and block calls (n_0 x kcap)
   chr_mat outer;     // level-0 operator in FP64 (default outer operator of PCG)
@@ -360,16 +369,15 @@ struct Cycle {
     return launch_op(M, x, k, epi, stop, s);
   }
 
-  hipError_t coarse(const void *b, void *x, bool level0) {
-    const int n = h.nc;
-    const dim3 grid((unsigned)(((int64_t)n * chr::kWave + chr::kBlock - 1) / chr::kBlock), (unsigned)k);
-    if (level0) {
-      hipLaunchKernelGGL((chr::coarse_gemv_kernel<double>), grid, dim3(chr::kBlock), 0, s, n,
-                         h.Ainv.as<double>(), (const double *)b, (double *)x, stop, k);
-    } else {
-      hipLaunchKernelGGL((chr::coarse_gemv_kernel<T>), grid, dim3(chr::kBlock), 0, s, n,
-                         h.Ainv.as<double>(), (const T *)b, (T *)x, stop, k);
-    }
+  // Direct solves of rows [nd, n) of level L.
+  template <typename TB, typename TO>
+  hipError_t coarse(const chr_hier::Level &L, const TB *b, TO *x) {
+    const int nr = L.n - L.nd;
+    if (nr <= 0) return hipSuccess;
+    static_assert(std::is_same<TB, TO>::value, "direct solve reads and writes one type");
+    const dim3 grid((unsigned)(((int64_t)nr * chr::kWave + chr::kBlock - 1) / chr::kBlock), (unsigned)k);
+    hipLaunchKernelGGL((chr::coarse_gemv_kernel<TO>), grid, dim3(chr::kBlock), 0, s, nr, L.nd,
+                       L.cmap.as<int4>(), h.Ainv.as<double>(), b, x, stop, k);
     return hipGetLastError();
   }
 
@@ -380,20 +388,28 @@ struct Cycle {
     T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
     if ((e = op(L.A, x, chr::EpiResidual<TB, T>{b, r}))) return e;
     if ((e = op(L.G, r, chr::EpiStore<T>{y}))) return e;
-    return op(L.GT, y, chr::EpiAxpy<T, TO>{x, out, L.omega});
+    return op(L.GT, y, chr::EpiAxpy<T, TO>{x, out, L.omega, L.omega_rows.as<double>(), k});
   }
 
   template <typename TB, typename TO>
   hipError_t level(int l, const TB *b, TO *out) {
     chr_hier::Level &L = h.lev[l];
-    const bool last = (l == (int)h.lev.size() - 1);
-    if (last) return coarse(b, out, l == 0);
     hipError_t e;
+    // Systems ending on this level: direct solves of rows [nd, n).
+    if (L.nd < L.n) {
+      if constexpr (std::is_same<TB, TO>::value) {
+        if ((e = coarse(L, b, out))) return e;
+      } else {
+        // Level 0 with TB != TO never happens: level-0 input and output are both FP64.
+        return hipErrorInvalidValue;
+      }
+    }
+    if (L.nd == 0) return hipSuccess;
     T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
     // Pre-smoothing from a zero initial guess, then nupre-1 further sweeps.
     if ((e = op(L.G, b, chr::EpiStore<T>{y}))) return e;
-    if ((e = op(L.GT, y, chr::EpiScale<T>{x, L.omega}))) return e;
-    for (int k = 2; k <= L.nupre; ++k)
+    if ((e = op(L.GT, y, chr::EpiScale<T>{x, L.omega, L.omega_rows.as<double>(), k}))) return e;
+    for (int q = 2; q <= L.nupre; ++q)
       if ((e = post_sweep(L, b, x))) return e;
     // Residual and restriction.
     if ((e = op(L.A, x, chr::EpiResidual<TB, T>{b, r}))) return e;
@@ -405,16 +421,31 @@ struct Cycle {
     if ((e = op(L.P, xc, chr::EpiAxpy<T, T>{x, x, 1.0}))) return e;
     // Post-smoothing; the last sweep writes the level's output.
     if (L.nupost == 0) {
-      const unsigned grid = (unsigned)(((int64_t)L.n * k + chr::kBlock - 1) / chr::kBlock);
-      hipLaunchKernelGGL((chr::convert_kernel<T, TO>), dim3(grid), dim3(chr::kBlock), 0, s, L.n * k,
+      const unsigned grid = (unsigned)(((int64_t)L.nd * k + chr::kBlock - 1) / chr::kBlock);
+      hipLaunchKernelGGL((chr::convert_kernel<T, TO>), dim3(grid), dim3(chr::kBlock), 0, s, L.nd * k,
                          (const T *)x, out);
       return hipGetLastError();
     }
-    for (int k = 1; k < L.nupost; ++k)
+    for (int q = 1; q < L.nupost; ++q)
       if ((e = post_sweep(L, b, x))) return e;
     return post_sweep(L, b, out);
   }
 };
+
+// Host description of one level for build_hier (views into caller-owned CSR arrays).
+struct LevelIn {
+  int32_t n = 0, nd = 0;
+  chr_csr A{}, G{}, P{};            // nd-row matrices (used when nd > 0)
+  double omega = 1.0;
+  std::vector<double> omega_rows;   // per-row damping of rows [0, nd), or empty
+  int32_t nupre = 1, nupost = 1;
+  struct Direct {
+    int32_t start;  // first row of the block on this level
+    chr_csr A;      // the block's operator (square, dense-invertible)
+  };
+  std::vector<Direct> direct;       // rows [nd, n), in order
+};
+int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out);
 
 // One V-cycle on FP64 row-major n_0 x k blocks (k <= kcap).
 hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop, int k = 1);

@@ -137,6 +137,28 @@ int chr_lobpcg(chr_hier *h, chr_mat *A, int32_t n, int32_t neig, double *X, int6
 int chr_fv_grad(chr_hier *h, chr_mat *A, const double *k, const double *x, const double *lam,
                 double *g, int32_t ptr_kind);
 
+/* Batched solves: nsys independent systems, each described as for chr_hier_create
+ * (levels[s][0..nlev[s]-1]), merged into one block-diagonal hierarchy on the device.  The
+ * merged V-cycle applied to a system's rows equals that system's own V-cycle bit for bit.
+ * No reference counterpart (the reference solves one system per call of PCG_CJ). */
+typedef struct chr_batch chr_batch;
+int chr_batch_create(int32_t nsys, const int32_t *nlev, const chr_level *const *levels,
+                     int32_t precision, chr_batch **out);
+int chr_batch_destroy(chr_batch *bt);
+/* The merged hierarchy (owned by the batch; e.g. for chr_hier_get_stats). */
+chr_hier *chr_batch_hier(chr_batch *bt);
+/* x = M b for every system: b and x hold the systems' blocks stacked in system order
+ * (sum of n_s rows) x k, column-major with leading dimension = total rows (k <= 16). */
+int chr_batch_vcycle(chr_batch *bt, int32_t k, const double *b, double *x, int32_t ptr_kind);
+/* PCG_CJ (sources/LinearSolver/PCG_CJ.m:1-73) for every system and every column of its
+ * right-hand-side block, all advancing in one captured graph per iteration; each
+ * (system, column) keeps its own scalars and stops at its own tolerance.  b, x as for
+ * chr_batch_vcycle (x holds x0).  iters and flags (nsys*k, index s*k + c) receive the
+ * iteration count and flag (0 converged, 1 not converged, -1 breakdown); resvec, when not
+ * NULL, (itmax+1) x (nsys*k) column-major.  At most 1024 solves per call. */
+int chr_batch_pcg(chr_batch *bt, int32_t k, const double *b, double *x, int32_t itmax,
+                  double tol, int32_t ptr_kind, int32_t *iters, int32_t *flags, double *resvec);
+
 /* Timing helpers (HIP events on the handle's stream, device-resident data):
  * mean milliseconds of one V-cycle (k = 1) over reps launches after one warm-up. */
 int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms);
