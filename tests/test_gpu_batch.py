@@ -54,7 +54,7 @@ def test_batch_pcg_matches_single_solves(gpu):
             assert abs(iters[s, c] - it1) <= 1
             assert np.linalg.norm(X[o0:o1, c] - x1) <= 1e-10 * np.linalg.norm(x1)
             np.testing.assert_allclose(resv[s * k + c, : min(it1, iters[s, c]) + 1], rv1[: min(it1, iters[s, c]) + 1],
-                                       rtol=1e-6)
+                                       rtol=1e-6, atol=1e-12)
 
 
 def test_batch_zero_rhs_column_and_itmax(gpu):
