@@ -30,7 +30,7 @@ EXPORTS = {
     "chr_hier_create": (C.c_int, [_I32, C.POINTER(ChrLevel), _I32, C.POINTER(_VP)]),
     "chr_hier_destroy": (C.c_int, [_VP]),
     "chr_hier_get_stats": (C.c_int, [_VP, C.POINTER(ChrHierStats)]),
-    "chr_hier_layout": (C.c_int, [_VP, _I32, C.c_char, _I32P, _I32P, C.POINTER(_I64)]),
+    "chr_hier_layout": (C.c_int, [_VP, _I32, C.c_char, _I32P, _I32P, C.POINTER(_I64), C.POINTER(C.c_double)]),
     "chr_vcycle_apply": (C.c_int, [_VP, _I32, _VP, _I64, _VP, _I64, _I32]),
     "chr_mat_create": (C.c_int, [C.POINTER(ChrCsr), C.POINTER(_VP)]),
     "chr_mat_destroy": (C.c_int, [_VP]),

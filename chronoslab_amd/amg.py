@@ -110,15 +110,18 @@ class AMGPreconditioner:
                 "level_bytes": list(st.level_bytes[:nl]), "pcg_iter_bytes": st.pcg_iter_bytes,
                 "device_bytes": st.device_bytes}
 
-    def layout(self) -> list:
-        """Per level, the stream-kernel layout chosen for A, G, G^T, P, P^T."""
+    def layout(self, stored_bytes: bool = False) -> list:
+        """Per level, the layout chosen for the single-vector products of A, G, G^T, P, P^T:
+        ("sell", code bits) for the sliced-ELL kernel, (PER, code bits) for the stream kernel,
+        with the bytes one product reads of the stored matrix appended when stored_bytes."""
         out = []
         for l in range(len(self.hier.levels) - 1):
             row = {}
             for w in "AGTPR":
-                per, i16, nnz = C.c_int32(), C.c_int32(), C.c_int64()
-                _check(_lib().chr_hier_layout(self.ptr, l, w.encode(), C.byref(per), C.byref(i16), C.byref(nnz)))
-                row[w] = (per.value, bool(i16.value))
+                per, bits, nnz, sb = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
+                _check(_lib().chr_hier_layout(self.ptr, l, w.encode(), C.byref(per), C.byref(bits),
+                                              C.byref(nnz), C.byref(sb)))
+                row[w] = ("sell" if per.value == 0 else per.value, bits.value) + ((sb.value,) if stored_bytes else ())
             out.append(row)
         return out
 

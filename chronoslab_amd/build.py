@@ -43,7 +43,7 @@ def _run(cmd: list[str]) -> None:
 def build_setup() -> str:
     out = os.path.join(HERE, "libchronos_setup.so")
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "host", "*.cpp")))
-    _run(["g++", *HOST_FLAGS, f"-I{INC}", "-o", out + ".tmp", *srcs])
+    _run(["g++", *HOST_FLAGS, f"-I{INC}", f"-I{os.path.join(HERE, 'csrc', 'common')}", "-o", out + ".tmp", *srcs])
     os.replace(out + ".tmp", out)
     return out
 
@@ -51,7 +51,8 @@ def build_setup() -> str:
 def build_amg() -> str:
     out = os.path.join(HERE, "libchronos_amg.so")
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "gpu", "*.hip")))
-    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{os.path.join(HERE, 'csrc', 'gpu')}", "-o", out + ".tmp", *srcs])
+    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{os.path.join(HERE, 'csrc', 'gpu')}",
+          f"-I{os.path.join(HERE, 'csrc', 'common')}", "-o", out + ".tmp", *srcs])
     os.replace(out + ".tmp", out)
     return out
 

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "runtime.hpp"
+#include "sell_layout.hpp"
 
 namespace chrrt {
 
@@ -35,16 +36,68 @@ uint64_t next_uid() {
   return ++counter;
 }
 
-// Autotuner: times the stream kernel on M for every candidate PER (HIP events, 2 warm-up
-// and 5 timed launches each) and keeps the fastest layout.  CHRONOS_STREAM_PER=4|8|16 in
-// the environment fixes PER instead.
-int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s, const int *zero) {
-  if (M.nnz == 0) return CHR_OK;
-  if (const char *env = std::getenv("CHRONOS_STREAM_PER")) {
-    const int forced = std::atoi(env);
-    for (int p : kPers)
-      if (p == forced) return p == M.per ? CHR_OK : M.layout(p, hrp, hci);
+int DevSell::build(int32_t n, int32_t nc, const int32_t *hrp, const int32_t *hci,
+                   const double *hv, bool fp32) {
+  release();
+  elems = 0;
+  nrows = n;
+  chrsell::Layout L;
+  if (!chrsell::build(n, nc, hrp, hci, hv, fp32 ? 4 : 8, L)) return CHR_OK;
+  const size_t vb = fp32 ? 4 : 8, cbytes = (size_t)L.cb / 8;
+  int rc;
+  if ((rc = sp.alloc(sizeof(int32_t) * L.sp.size())) || (rc = val.alloc(vb * (size_t)L.elems)) ||
+      (rc = code.alloc(cbytes * (size_t)L.elems)))
+    return rc;
+  HIPC(hipMemcpy(sp.p, L.sp.data(), sizeof(int32_t) * L.sp.size(), hipMemcpyHostToDevice));
+  if (fp32) {
+    std::vector<float> f(L.val.begin(), L.val.end());
+    HIPC(hipMemcpy(val.p, f.data(), vb * (size_t)L.elems, hipMemcpyHostToDevice));
+  } else {
+    HIPC(hipMemcpy(val.p, L.val.data(), vb * (size_t)L.elems, hipMemcpyHostToDevice));
   }
+  const void *cp = L.cb == 8    ? (const void *)L.c8.data()
+                   : L.cb == 16 ? (const void *)L.c16.data()
+                                : (const void *)L.c32.data();
+  HIPC(hipMemcpy(code.p, cp, cbytes * (size_t)L.elems, hipMemcpyHostToDevice));
+  if (L.sorted) {
+    if ((rc = perm.alloc(sizeof(uint16_t) * L.perm.size()))) return rc;
+    HIPC(hipMemcpy(perm.p, L.perm.data(), sizeof(uint16_t) * L.perm.size(), hipMemcpyHostToDevice));
+  }
+  if (L.cb == 8) {
+    if ((rc = dict.alloc(sizeof(int32_t) * L.dict.size()))) return rc;
+    HIPC(hipMemcpy(dict.p, L.dict.data(), sizeof(int32_t) * L.dict.size(), hipMemcpyHostToDevice));
+  }
+  cb = L.cb;
+  nslices = L.nslices;
+  wshift = L.wshift;
+  dmin = L.dmin;
+  elems = L.elems;
+  sorted = L.sorted;
+  return CHR_OK;
+}
+
+// Autotuner for single-vector products, in two steps:
+//  1. the CSR-stream layout: PER is timed over the candidates (HIP events, 2 warm-up and 5
+//     timed launches each); PER 8 is kept unless another is more than 3% faster, so that
+//     near-ties do not flip from run to run.  tune_per = false keeps PER 8: the outer
+//     operator of PCG, whose fused dot products are summed over the stream row blocks, so
+//     that the solver's rounding never depends on a timing;
+//  2. the sliced-ELL layout, built when the matrix admits it and kept if more than 3%
+//     faster than the stream layout of step 1.
+// Every layout sums a row in the same order, so the choice never changes a product.
+// Environment: CHRONOS_STREAM_PER=4|8|16 fixes PER (tune_per only); CHRONOS_SELL=0
+// excludes and CHRONOS_SELL=1 forces the sliced-ELL layout where the matrix admits it.
+int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
+                const int *zero, bool tune_per) {
+  M.use_sell = false;
+  M.sell.release();
+  if (M.nnz == 0) return CHR_OK;
+  const char *env_sell = std::getenv("CHRONOS_SELL");
+  const int sell_mode = env_sell ? std::atoi(env_sell) : -1;  // -1 timed, 0 off, 1 forced
+  int fixed_per = tune_per ? 0 : 8;
+  if (const char *env = std::getenv("CHRONOS_STREAM_PER"); env && tune_per)
+    for (int p : kPers)
+      if (p == std::atoi(env)) fixed_per = p;
   const size_t sz = M.fp32 ? 4 : 8;
   DevBuf xb, yb;
   int rc;
@@ -53,39 +106,67 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s
   hipEvent_t ea, eb;
   HIPC(hipEventCreate(&ea));
   HIPC(hipEventCreate(&eb));
-  auto run = [&]() -> hipError_t {
-    if (M.fp32) return launch_spmv(M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, zero, s);
-    return launch_spmv(M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, zero, s);
-  };
-  // The default (8) is kept unless another candidate is more than 3% faster, so that
-  // near-ties do not flip the layout from run to run.
-  float best = 1e30f, def_ms = 1e30f;
-  int best_per = M.per;
-  for (int p : kPers) {
-    if (p != M.per && (rc = M.layout(p, hrp, hci))) break;
+  auto time_it = [&](float &ms) -> int {
+    auto run = [&]() -> hipError_t {
+      if (M.fp32) return launch_spmv(M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, zero, s);
+      return launch_spmv(M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, zero, s);
+    };
     hipError_t e = hipSuccess;
     for (int i = 0; i < 2 && !e; ++i) e = run();
     if (!e) e = hipEventRecord(ea, s);
     for (int i = 0; i < 5 && !e; ++i) e = run();
     if (!e) e = hipEventRecord(eb, s);
     if (!e) e = hipEventSynchronize(eb);
-    float ms = 0.f;
     if (!e) e = hipEventElapsedTime(&ms, ea, eb);
-    if (e) {
-      rc = fail(CHR_EHIP, std::string("autotune: ") + hipGetErrorString(e));
-      break;
+    if (e) return fail(CHR_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+    return CHR_OK;
+  };
+  // Step 1: stream layout.
+  float stream_ms = -1.f;
+  if (fixed_per) {
+    if (fixed_per != M.per) rc = M.layout(fixed_per, hrp, hci);
+  } else {
+    float best = 1e30f, def_ms = 1e30f;
+    int best_per = 8;
+    for (int p : kPers) {
+      if (p != M.per && (rc = M.layout(p, hrp, hci))) break;
+      float ms = 0.f;
+      if ((rc = time_it(ms))) break;
+      if (p == 8) def_ms = ms;
+      if (ms < best) {
+        best = ms;
+        best_per = p;
+      }
     }
-    if (p == 8) def_ms = ms;
-    if (ms < best) {
-      best = ms;
-      best_per = p;
+    if (!rc) {
+      if (best > 0.97f * def_ms) best_per = 8;
+      stream_ms = best_per == 8 ? def_ms : best;
+      if (best_per != M.per) rc = M.layout(best_per, hrp, hci);
     }
   }
-  if (best_per != 8 && best > 0.97f * def_ms) best_per = 8;
+  // Step 2: sliced ELL.
+  if (!rc && sell_mode != 0 && !(rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32)) && M.sell.cb) {
+    if (sell_mode != 1) {
+      float ms = 0.f;
+      if (stream_ms < 0.f) rc = time_it(stream_ms);
+      if (!rc) {
+        M.use_sell = true;
+        rc = time_it(ms);
+      }
+      if (!rc && !(ms < 0.97f * stream_ms)) {
+        M.use_sell = false;
+        M.sell.release();
+      }
+    }
+    M.use_sell = M.sell.cb != 0;
+  }
   (void)hipEventDestroy(ea);
   (void)hipEventDestroy(eb);
-  if (rc) return rc;
-  return best_per == M.per ? CHR_OK : M.layout(best_per, hrp, hci);
+  if (rc) {
+    M.use_sell = false;
+    M.sell.release();
+  }
+  return rc;
 }
 
 
@@ -176,7 +257,7 @@ void fill_stats(chr_hier &h) {
       dev += L.A.device_bytes() + L.G.device_bytes() + L.GT.device_bytes() + L.P.device_bytes() +
              L.PT.device_bytes() + L.b.bytes + L.x.bytes + L.r.bytes + L.y.bytes;
     }
-    if (l < 16) st.level_bytes[l] = b;
+    if (l < 16) st.level_bytes[l] = bc;
     total += b;
     total_csr += bc;
   }
@@ -185,7 +266,7 @@ void fill_stats(chr_hier &h) {
   dev += h.outer.A.device_bytes();
   st.device_bytes = dev;
   const auto &A0 = h.outer.A;
-  st.pcg_iter_bytes = total + A0.stream_bytes() + 18.0 * A0.nrows * 8.0;
+  st.pcg_iter_bytes = total_csr + A0.csr_bytes() + 18.0 * A0.nrows * 8.0;
 }
 
 int sync_stream(hipStream_t s) {
@@ -232,21 +313,21 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
     L.nupre = in.nupre;
     L.nupost = in.nupost;
     if (in.nd > 0) {
-      if ((rc = L.A.upload(in.A, f32)) || (rc = tune_layout(L.A, in.A.rowptr, in.A.col, h->stream, zf)))
+      if ((rc = L.A.upload(in.A, f32)) || (rc = tune_layout(L.A, in.A.rowptr, in.A.col, in.A.val, h->stream, zf)))
         return rc;
-      if ((rc = L.G.upload(in.G, f32)) || (rc = tune_layout(L.G, in.G.rowptr, in.G.col, h->stream, zf)))
+      if ((rc = L.G.upload(in.G, f32)) || (rc = tune_layout(L.G, in.G.rowptr, in.G.col, in.G.val, h->stream, zf)))
         return rc;
-      if ((rc = L.P.upload(in.P, f32)) || (rc = tune_layout(L.P, in.P.rowptr, in.P.col, h->stream, zf)))
+      if ((rc = L.P.upload(in.P, f32)) || (rc = tune_layout(L.P, in.P.rowptr, in.P.col, in.P.val, h->stream, zf)))
         return rc;
       std::vector<int32_t> trp, tci;
       std::vector<double> tv;
       transpose_host(in.G, trp, tci, tv);
       if ((rc = L.GT.upload(in.G.ncols, in.G.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
-          (rc = tune_layout(L.GT, trp.data(), tci.data(), h->stream, zf)))
+          (rc = tune_layout(L.GT, trp.data(), tci.data(), tv.data(), h->stream, zf)))
         return rc;
       transpose_host(in.P, trp, tci, tv);
       if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
-          (rc = tune_layout(L.PT, trp.data(), tci.data(), h->stream, zf)))
+          (rc = tune_layout(L.PT, trp.data(), tci.data(), tv.data(), h->stream, zf)))
         return rc;
       if (!in.omega_rows.empty()) {
         if ((rc = L.omega_rows.alloc(sizeof(double) * in.nd))) return rc;
@@ -285,7 +366,7 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
   }
   // Level-0 operator in FP64 for the outer Krylov loop.
   if ((rc = h->outer.A.upload(outer, false)) ||
-      (rc = tune_layout(h->outer.A, outer.rowptr, outer.col, h->stream, zf)))
+      (rc = tune_layout(h->outer.A, outer.rowptr, outer.col, outer.val, h->stream, zf, false)))
     return rc;
   if ((rc = h->outer.zero.alloc(sizeof(int)))) return rc;
   HIPC(hipMemset(h->outer.zero.p, 0, sizeof(int)));
@@ -351,16 +432,17 @@ extern "C" int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st) {
 }
 
 extern "C" int chr_hier_layout(const chr_hier *h, int32_t level, char which, int32_t *per,
-                               int32_t *idx16, int64_t *nnz) {
+                               int32_t *code_bits, int64_t *nnz, double *stored_bytes) {
   if (!h || level < 0 || level >= (int32_t)h->lev.size() || h->lev[level].nd == 0)
     return fail(CHR_EARG, "chr_hier_layout: level without smoother/transfers");
   const auto &L = h->lev[level];
   const DevCsr *M = which == 'A' ? &L.A : which == 'G' ? &L.G : which == 'T' ? &L.GT
                   : which == 'P' ? &L.P : which == 'R' ? &L.PT : nullptr;
   if (!M) return fail(CHR_EARG, "chr_hier_layout: which must be A, G, T, P or R");
-  if (per) *per = M->per;
-  if (idx16) *idx16 = M->idx16;
+  if (per) *per = M->use_sell ? 0 : M->per;
+  if (code_bits) *code_bits = M->use_sell ? M->sell.cb : (M->idx16 ? 16 : 32);
   if (nnz) *nnz = M->nnz;
+  if (stored_bytes) *stored_bytes = M->stream_bytes();
   return CHR_OK;
 }
 
@@ -422,7 +504,7 @@ extern "C" int chr_mat_create(const chr_csr *A, chr_mat **out) {
   if ((rc = m->zero.alloc(sizeof(int)))) return rc;
   HIPC(hipMemset(m->zero.p, 0, sizeof(int)));
   if ((rc = m->A.upload(*A, false)) ||
-      (rc = tune_layout(m->A, A->rowptr, A->col, nullptr, m->zero.as<int>())))
+      (rc = tune_layout(m->A, A->rowptr, A->col, A->val, nullptr, m->zero.as<int>(), false)))
     return rc;
   HIPC(hipDeviceSynchronize());
   *out = m.release();
@@ -770,7 +852,7 @@ extern "C" int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t rep
   float t = 0.f;
   HIPC(hipEventElapsedTime(&t, a, b));
   *ms = t / reps;
-  if (bytes) *bytes = M->stream_bytes() + s * (M->ncols + M->nrows);
+  if (bytes) *bytes = M->csr_bytes() + s * (M->ncols + M->nrows);
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   return CHR_OK;

@@ -79,6 +79,11 @@ struct EpiAxpy {  // out = x + w * acc   (x and out may alias: same row, same th
   }
 };
 
+// Products rounded before they are added (the build also disables contraction), so that
+// every layout and the block kernel sum a row to the same bits.
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+
 // Accumulator type: float products for the FP32 cycle, double otherwise.
 template <typename TV>
 struct Acc {
@@ -198,13 +203,110 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
 }
 
 // ---------------------------------------------------------------------------------------
+// Sliced-ELL SpMV for square matrices with short rows (the smoother factors G, G^T and
+// the fine-level operators).  Rows are taken in slices of 64 slots, one wave per slice and
+// one row per lane; slice s stores its elements column by column, element (k, lane) at
+// sp[s] + 64 k + lane, so every load of a wave is one contiguous 64-element segment and
+// the row sum stays in a register (no LDS staging, no row pointers).  A slice is as long
+// as its longest row; shorter rows are padded with a PAD code.  Within windows of 2^wshift
+// slots the rows may be ordered by decreasing length to cut padding: perm then holds the
+// row of each slot as an offset from the window's first row (0xFFFF: empty slot).
+//
+// Column codes, as row-relative offsets d = row - col:
+//   CB = 8:  an index into the window's dictionary of at most 255 offsets (dict[w*256 +
+//            code]; code 255 = PAD, whose dictionary entry is 0);
+//   CB = 16: d - dmin as uint16 (0xFFFF = PAD);
+//   CB = 32: the column itself (-1 = PAD).
+// Each lane sums its row in column order with the product rounded before the add, which
+// is the stream kernel's order and rounding: both layouts give the same bits.
+// ---------------------------------------------------------------------------------------
+template <int CB>
+struct SellCode;
+template <>
+struct SellCode<8> {
+  using type = uint8_t;
+  static constexpr uint32_t pad = 0xFF;
+};
+template <>
+struct SellCode<16> {
+  using type = uint16_t;
+  static constexpr uint32_t pad = 0xFFFF;
+};
+template <>
+struct SellCode<32> {
+  using type = int32_t;
+  static constexpr uint32_t pad = 0xFFFFFFFFu;
+};
+
+constexpr int kSellUnroll = 8;
+
+template <int CB, typename TV, typename TX, typename Epi>
+__global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
+    int nslices, int nrows, int wshift, const int *__restrict__ sp,
+    const uint16_t *__restrict__ perm, const TV *__restrict__ val,
+    const typename SellCode<CB>::type *__restrict__ code, const int *__restrict__ dict, int dmin,
+    const TX *__restrict__ x, Epi epi, const int *__restrict__ stop) {
+  using TA = typename Acc<TV>::type;
+  using CT = typename SellCode<CB>::type;
+  constexpr int kSlices = kBlock / kWave;
+  __shared__ int ld[CB == 8 ? 256 : 1];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  if (*stop) return;
+  const int s = b * kSlices + threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if constexpr (CB == 8) {
+    // The block's slices lie in one window (windows hold a multiple of kBlock slots).
+    ld[threadIdx.x] = dict[(((b * kSlices) * kWave) >> wshift) * 256 + threadIdx.x];
+    __syncthreads();
+  }
+  if (s >= nslices) return;
+  const int slot = s * kWave + lane;
+  int row = slot;
+  bool valid = slot < nrows;
+  if (perm) {
+    const int q = perm[slot];
+    valid = q != 0xFFFF;
+    row = ((slot >> wshift) << wshift) + q;
+  }
+  const int rc = valid ? row : 0;  // PAD elements read x[rc] and drop the product
+  const double pv = epi.pre(rc);
+  const int base = sp[s] + lane, len = (sp[s + 1] - sp[s]) / kWave;
+  auto col_of = [&](CT c) -> int {
+    if constexpr (CB == 8) return rc - ld[c];
+    else if constexpr (CB == 16) return rc - (c == (CT)SellCode<16>::pad ? 0 : (int)c + dmin);
+    else return c < 0 ? rc : (int)c;
+  };
+  // Chunks of kSellUnroll elements; the last chunk clamps its loads to the row's last
+  // element and drops the surplus products, so every load of a chunk issues at once.
+  TA acc = 0;
+  const int nch = (len + kSellUnroll - 1) / kSellUnroll;
+  for (int ch = 0; ch < nch; ++ch) {
+    TV v[kSellUnroll];
+    CT c[kSellUnroll];
+    TA xv[kSellUnroll];
+#pragma unroll
+    for (int u = 0; u < kSellUnroll; ++u) {
+      const int e = base + min(ch * kSellUnroll + u, len - 1) * kWave;
+      v[u] = val[e];
+      c[u] = code[e];
+    }
+#pragma unroll
+    for (int u = 0; u < kSellUnroll; ++u) xv[u] = (TA)x[col_of(c[u])];
+#pragma unroll
+    for (int u = 0; u < kSellUnroll; ++u) {
+      const TA p = mul_rn((TA)v[u], xv[u]);
+      const bool live = ch * kSellUnroll + u < len && ((uint32_t)c[u] & SellCode<CB>::pad) != SellCode<CB>::pad;
+      acc = live ? acc + p : acc;
+    }
+  }
+  if (valid) epi(row, (double)acc, pv);
+}
+
+// ---------------------------------------------------------------------------------------
 // Block product Y = M X for row-major n x k blocks (element (i, c) at i*k + c, k <= KL):
 // KL lanes per row, one per column; each lane sums its row in nonzero order with the
 // product rounded before the add, so that column c is bit for bit the single-vector stream
 // kernel's result on column c.  Epilogues see flat indices i*k + c.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
 
 template <int KL, typename TV, typename TI, typename TX, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__restrict__ blk,

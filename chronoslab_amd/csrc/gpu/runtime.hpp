@@ -112,6 +112,33 @@ inline void transpose_host(const chr_csr &M, std::vector<int32_t> &rp, std::vect
 // Candidate nonzeros-per-thread of the stream kernel (block capacity PER * 256).
 constexpr int kPers[] = {4, 8, 16};
 
+// Sliced-ELL copy of a square matrix for single-vector products (spmv_sell_kernel): 64-row
+// slices stored column by column, rows optionally ordered by decreasing length inside
+// windows of 2^wshift rows, column codes as described at the kernel.
+struct DevSell {
+  int cb = 0;  // code bits (8, 16, 32); 0 = not built
+  int32_t nrows = 0, nslices = 0, wshift = 14, dmin = 0;
+  int64_t elems = 0;  // stored elements, padding included
+  bool sorted = false;
+  DevBuf sp, perm, val, code, dict;
+  // Builds the layout from host CSR (chrsell::build) and uploads it; leaves cb = 0 (and
+  // returns CHR_OK) when the matrix is not a candidate.
+  int build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *hci, const double *hv,
+            bool fp32);
+  void release() {
+    sp = DevBuf();
+    perm = DevBuf();
+    val = DevBuf();
+    code = DevBuf();
+    dict = DevBuf();
+    cb = 0;
+  }
+  double bytes(bool fp32) const {
+    return (double)elems * ((fp32 ? 4 : 8) + cb / 8) + 4.0 * (nslices + 1) +
+           (double)(perm.bytes + dict.bytes);
+  }
+};
+
 // Device CSR in FP32 or FP64, laid out for the stream kernel.
 struct DevCsr {
   int32_t nrows = 0, ncols = 0;
@@ -122,6 +149,8 @@ struct DevCsr {
   int32_t nblk = 0;    // row blocks
   DevBuf rp, val, blk;
   DevBuf ci;           // uint16 offsets when idx16, else int32 columns
+  DevSell sell;        // sliced-ELL copy, used for single-vector products when use_sell
+  bool use_sell = false;
   int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
              bool as_fp32, int per_ = 8) {
     nrows = nr;
@@ -194,14 +223,39 @@ struct DevCsr {
   int upload(const chr_csr &m, bool as_fp32) {
     return upload(m.nrows, m.ncols, m.rowptr, m.col, m.val, as_fp32);
   }
-  // Bytes of one product as stored: values + column indices + row pointers + block table.
+  // Bytes of one single-vector product as stored: values + column codes + row pointers and
+  // block table (stream layout) or slice offsets, permutation and dictionaries (sliced ELL).
   double stream_bytes() const {
+    if (use_sell) return sell.bytes(fp32);
     return (double)nnz * ((fp32 ? 4 : 8) + (idx16 ? 2 : 4)) + 4.0 * (nrows + 1) + 16.0 * (nblk + 1);
   }
   // The same with 4-byte column indices and no block table (the reference's CSR format).
   double csr_bytes() const { return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1); }
-  double device_bytes() const { return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes); }
+  double device_bytes() const {
+    return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes + sell.sp.bytes + sell.perm.bytes +
+                    sell.val.bytes + sell.code.bytes + sell.dict.bytes);
+  }
 };
+
+template <int CB, typename TV, typename TX, typename Epi>
+hipError_t launch_sell_cb(const DevSell &S, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  const unsigned grid = (unsigned)((S.nslices + chr::kBlock / chr::kWave - 1) / (chr::kBlock / chr::kWave));
+  hipLaunchKernelGGL((chr::spmv_sell_kernel<CB, TV, TX, Epi>), dim3(grid), dim3(chr::kBlock), 0, s,
+                     S.nslices, S.nrows, S.wshift, S.sp.as<int>(),
+                     S.sorted ? S.perm.as<uint16_t>() : nullptr, S.val.as<TV>(),
+                     S.code.as<typename chr::SellCode<CB>::type>(), S.dict.as<int>(), S.dmin, x, epi,
+                     stop);
+  return hipGetLastError();
+}
+
+template <typename TV, typename TX, typename Epi>
+hipError_t launch_sell(const DevSell &S, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  switch (S.cb) {
+    case 8: return launch_sell_cb<8, TV>(S, x, epi, stop, s);
+    case 16: return launch_sell_cb<16, TV>(S, x, epi, stop, s);
+    default: return launch_sell_cb<32, TV>(S, x, epi, stop, s);
+  }
+}
 
 template <int PER, typename TV, typename TX, typename Epi>
 hipError_t launch_spmv_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
@@ -220,6 +274,7 @@ template <typename TV, typename TX, typename Epi>
 hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
                          hipStream_t s) {
   if (M.nrows == 0) return hipSuccess;
+  if (M.use_sell) return launch_sell<TV>(M.sell, x, epi, stop, s);
   switch (M.per) {
     case 4: return launch_spmv_p<4, TV>(M, x, epi, stop, s);
     case 16: return launch_spmv_p<16, TV>(M, x, epi, stop, s);
@@ -269,7 +324,8 @@ hipError_t launch_op(const DevCsr &M, const TX *x, int k, Epi epi, const int *st
 constexpr int kMaxBlockCols = 16;
 
 
-int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, hipStream_t s, const int *zero);
+int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
+                const int *zero, bool tune_per = true);
 
 uint64_t next_uid();
 

@@ -1,0 +1,108 @@
+// Host replay of the GPU library's sliced-ELL product (spmv_sell_kernel): builds the layout
+// with the same builder (sell_layout.hpp) and walks it with the kernel's own index
+// arithmetic, workgroup by workgroup and lane by lane, checking every load against the
+// buffers the kernel would read.  It lets the layout and the kernel's addressing be
+// verified without a GPU; the product it returns has the kernel's summation order and
+// rounding (FP32 or FP64 products and sums).
+#include <stdint.h>
+
+#include <cmath>
+#include <vector>
+
+#include "chronos_setup.h"
+#include "sell_layout.hpp"
+
+namespace {
+
+constexpr int kBlock = 256, kUnroll = 8;
+
+template <typename TA>
+int replay(const chrsell::Layout &L, const double *x, double *y) {
+  const int32_t n = L.nrows;
+  const int64_t nslots = (int64_t)L.nslices * chrsell::kSlice;
+  const int nblocks = (L.nslices + kBlock / chrsell::kSlice - 1) / (kBlock / chrsell::kSlice);
+  const int64_t ndict = (int64_t)L.dict.size();
+  std::vector<int> written((size_t)n, 0);
+  for (int b = 0; b < nblocks; ++b) {
+    const int64_t wi = ((int64_t)b * kBlock) >> L.wshift;  // the block's dictionary window
+    if (L.cb == 8 && (wi + 1) * 256 > ndict) return -10;
+    for (int t = 0; t < kBlock; ++t) {
+      const int64_t s = (int64_t)b * (kBlock / chrsell::kSlice) + t / chrsell::kSlice;
+      const int lane = t % chrsell::kSlice;
+      if (s >= L.nslices) continue;
+      const int64_t slot = s * chrsell::kSlice + lane;
+      int64_t row = slot;
+      bool valid = slot < n;
+      if (L.sorted) {
+        if (slot >= (int64_t)L.perm.size()) return -11;
+        const int q = L.perm[(size_t)slot];
+        valid = q != 0xFFFF;
+        row = ((slot >> L.wshift) << L.wshift) + q;
+      }
+      if (valid && (row < 0 || row >= n)) return -12;
+      const int64_t rc = valid ? row : 0;
+      const int64_t d0 = L.sp[(size_t)s], d1 = L.sp[(size_t)s + 1];
+      if ((d1 - d0) % chrsell::kSlice) return -13;
+      const int64_t base = d0 + lane, len = (d1 - d0) / chrsell::kSlice;
+      TA acc = 0;
+      const int64_t nch = (len + kUnroll - 1) / kUnroll;
+      for (int64_t ch = 0; ch < nch; ++ch)
+        for (int u = 0; u < kUnroll; ++u) {
+          const int64_t k = std::min<int64_t>(ch * kUnroll + u, len - 1);
+          const int64_t e = base + k * chrsell::kSlice;
+          if (e < 0 || e >= L.elems) return -14;
+          bool pad;
+          int64_t col;
+          if (L.cb == 8) {
+            const uint8_t c = L.c8[(size_t)e];
+            pad = c == 0xFF;
+            col = rc - L.dict[(size_t)(wi * 256 + c)];
+          } else if (L.cb == 16) {
+            const uint16_t c = L.c16[(size_t)e];
+            pad = c == 0xFFFF;
+            col = rc - (pad ? 0 : (int64_t)c + L.dmin);
+          } else {
+            const int32_t c = L.c32[(size_t)e];
+            pad = c < 0;
+            col = pad ? rc : c;
+          }
+          if (col < 0 || col >= n) return -15;
+          const TA p = (TA)L.val[(size_t)e] * (TA)x[col];
+          if (ch * kUnroll + u < len && !pad) acc = acc + p;
+        }
+      if (valid) {
+        y[row] = (double)acc;
+        written[(size_t)row]++;
+      }
+    }
+  }
+  for (int32_t r = 0; r < n; ++r)
+    if (written[(size_t)r] != 1) return -16;
+  (void)nslots;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int chs_sell_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                              const double *val, int32_t fp32, const double *x, double *y,
+                              int32_t *info, int64_t *elems) {
+  chrsell::Layout L;
+  if (!chrsell::build(n, ncols, rowptr, col, val, fp32 ? 4 : 8, L)) return 1;
+  if (info) {
+    info[0] = L.cb;
+    info[1] = L.sorted ? 1 : 0;
+    info[2] = L.wshift;
+    info[3] = L.nslices;
+  }
+  if (elems) *elems = L.elems;
+  if (fp32) {
+    // The kernel reads FP32 values and an FP32 vector.
+    chrsell::Layout &M = L;
+    for (auto &v : M.val) v = (double)(float)v;
+    std::vector<double> xf((size_t)n);
+    for (int32_t i = 0; i < n; ++i) xf[(size_t)i] = (double)(float)x[i];
+    return replay<float>(M, xf.data(), y);
+  }
+  return replay<double>(L, x, y);
+}

@@ -76,10 +76,11 @@ typedef struct chr_hier_stats {
   int32_t nlev;
   int32_t precision;
   int64_t n[16];              /* rows per level */
-  double vcycle_bytes;        /* bytes one V-cycle (k = 1) streams in the stored format */
-  double vcycle_csr_bytes;    /* the same with plain CSR (4-byte column indices) */
-  double level_bytes[16];     /* the same, per level (last entry: coarsest solve) */
-  double pcg_iter_bytes;      /* V-cycle + outer FP64 SpMV + 18 n_0 * 8 vector bytes */
+  double vcycle_bytes;        /* bytes one V-cycle (k = 1) streams in the stored layouts */
+  double vcycle_csr_bytes;    /* algorithmic bytes of one V-cycle: the same products with
+                                 plain CSR (4-byte column indices, row pointers) */
+  double level_bytes[16];     /* algorithmic bytes per level (a level's direct solves included) */
+  double pcg_iter_bytes;      /* algorithmic: V-cycle + outer FP64 CSR product + 18 n_0 * 8 */
   double device_bytes;        /* device memory held by the hierarchy */
 } chr_hier_stats;
 
@@ -94,10 +95,14 @@ int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t precision, ch
 int chr_hier_destroy(chr_hier *h);
 int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st);
 
-/* Layout the autotuner chose for one level matrix (which = 'A','G','T','P','R'): nonzeros
- * per thread of the stream kernel, whether columns are uint16 block offsets, nonzeros. */
-int chr_hier_layout(const chr_hier *h, int32_t level, char which, int32_t *per, int32_t *idx16,
-                    int64_t *nnz);
+/* Layout the autotuner chose for the single-vector products of one level matrix (which =
+ * 'A','G','T','P','R'): per = nonzeros per thread of the CSR-stream kernel, or 0 for the
+ * sliced-ELL kernel; code_bits = bits per stored column code (stream: 16 for block
+ * offsets, 32 for columns; sliced ELL: 8 for dictionary codes, 16 for row-relative
+ * offsets, 32 for columns); nnz = nonzeros; stored_bytes = bytes one product reads of
+ * the matrix in that layout.  Any output may be NULL. */
+int chr_hier_layout(const chr_hier *h, int32_t level, char which, int32_t *per,
+                    int32_t *code_bits, int64_t *nnz, double *stored_bytes);
 
 /* x = M b for an n x k block (k <= 16). */
 int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t ldb, double *x,
@@ -163,7 +168,9 @@ int chr_batch_pcg(chr_batch *bt, int32_t k, const double *b, double *x, int32_t 
  * mean milliseconds of one V-cycle (k = 1) over reps launches after one warm-up. */
 int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms);
 /* Mean milliseconds of one launch of a level SpMV: which = 'A', 'G', 'T' (G^T), 'P',
- * 'R' (P^T).  bytes receives the algorithmic bytes of one launch. */
+ * 'R' (P^T).  bytes receives the algorithmic bytes of one launch: the matrix as plain CSR
+ * (4-byte column indices and row pointers), the input vector read once and the output
+ * written once. */
 int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t reps, double *ms,
                   double *bytes);
 

@@ -119,6 +119,17 @@ int chs_bc_fix(const chs_mat *A, uint64_t seed, chs_mat **Afix, int32_t *nfixed)
 /* Builds levels l = 0..nlev-1.  Level l < nlev-1 holds A_l, P_l, G_l, omega_l; the last
  * level holds only A_c (its factorisation is done by the consumer). */
 int chs_build(const chs_mat *A, const chs_params *p, chs_hier **out);
+
+/* Host replay of the GPU library's sliced-ELL single-vector product (no reference
+ * counterpart): builds the layout the GPU library would upload for this CSR matrix and
+ * walks it with the kernel's index arithmetic, checking every access against the layout's
+ * buffers; y = A x with the kernel's summation order and rounding (fp32 != 0: FP32 values,
+ * vector, products and sums).  info[4] = {code bits, sorted, log2 window, slices}; elems =
+ * stored elements.  Returns 0, 1 when the matrix is not a sliced-ELL candidate, or a
+ * negative code naming the first out-of-range access. */
+int chs_sell_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                   const double *val, int32_t fp32, const double *x, double *y, int32_t *info,
+                   int64_t *elems);
 int chs_hier_nlev(const chs_hier *h);
 /* which: 'A', 'P' or 'G'.  Returned matrix is borrowed (owned by the hierarchy). */
 const chs_mat *chs_hier_mat(const chs_hier *h, int32_t level, char which);
