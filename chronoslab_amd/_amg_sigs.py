@@ -44,6 +44,9 @@ EXPORTS = {
     "chr_batch_hier": (_VP, [_VP]),
     "chr_batch_vcycle": (C.c_int, [_VP, _I32, _VP, _VP, _I32]),
     "chr_batch_pcg": (C.c_int, [_VP, _I32, _VP, _VP, _I32, _F64, _I32, _I32P, _I32P, _F64P]),
+    "chr_afsai": (C.c_int, [C.POINTER(ChrCsr), _I32, _I32, _F64, _I32P, _I32P, _F64P, C.POINTER(_I64), _I32P]),
+    "chr_setup_afsai": (C.c_int, [_VP, _I32, _I32P, _I32P, _F64P, _I32, _I32, _F64, _I32P, _I32P, _F64P,
+                                  C.POINTER(_I64), _I32P]),
     "chr_time_vcycle": (C.c_int, [_VP, _I32, _F64P]),
     "chr_time_spmv": (C.c_int, [_VP, _I32, C.c_char, _I32, _F64P, _F64P]),
 }

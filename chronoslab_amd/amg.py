@@ -60,6 +60,27 @@ def init(device: int = 0) -> None:
     _check(_lib().chr_init(device), "chr_init")
 
 
+def afsai(A: H.Csr, nstep: int = 15, step_size: int = 1, eps: float = 1e-3) -> tuple[H.Csr, bool]:
+    """Adaptive symmetric FSAI factor G of A on the GPU (chr_afsai): one batched launch over
+    all rows.  Mirrors the reference's SYM_aFSAI_compute gateway
+    (sources/Preconditioner/AMG/smoother/MEX_SYM_AFSAI/SYM_aFSAI_compute.cpp:49-149) and
+    returns (G, is_posdef); G equals hierarchy.afsai's host factor bit for bit."""
+    keep = []
+    c = _csr(A, keep)
+    n = A.nrows
+    kmax = max(1, nstep * step_size) + 1
+    rp = np.empty(n + 1, np.int32)
+    ci = np.empty(max(1, n * kmax), np.int32)
+    v = np.empty(max(1, n * kmax), np.float64)
+    nnz = C.c_int64()
+    pd = C.c_int32()
+    _check(_lib().chr_afsai(C.byref(c), nstep, step_size, eps, rp.ctypes.data_as(_I32P),
+                            ci.ctypes.data_as(_I32P), v.ctypes.data_as(_F64P), C.byref(nnz), C.byref(pd)),
+           "chr_afsai")
+    k = nnz.value
+    return H.Csr(n, n, rp, ci[:k].copy(), v[:k].copy()), bool(pd.value)
+
+
 class DeviceMatrix:
     """A CSR operator resident on the GPU in FP64 (the outer prodA)."""
 

@@ -104,6 +104,8 @@ extern "C" void chs_params_default(chs_params *p) {
   p->nthreads = 0;
   p->seed = 20240607ULL;
   p->verbose = 0;
+  p->afsai_fn = nullptr;
+  p->afsai_ctx = nullptr;
 }
 
 extern "C" const char *chs_last_error(void) { return g_err.c_str(); }
@@ -1009,6 +1011,24 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     if (p.smoother == 1) {
       int rc = jacobi_factor(A, L.G);
       if (rc) return rc;
+    } else if (p.afsai_fn) {
+      if (p.nstep < 0 || p.step_size < 1) return fail(CHS_EARG, "afsai: arguments");
+      const int64_t kmax = (int64_t)std::max(1, p.nstep * p.step_size) + 1;
+      int32_t pd = 1;
+      int64_t nnz_g = 0;
+      L.G.nrows = L.G.ncols = n;
+      L.G.rp.assign((size_t)n + 1, 0);
+      L.G.ci.resize((size_t)n * kmax);
+      L.G.v.resize((size_t)n * kmax);
+      int rc = p.afsai_fn(p.afsai_ctx, n, A.rp.data(), A.ci.data(), A.v.data(), p.nstep, p.step_size,
+                          p.fsai_eps, L.G.rp.data(), L.G.ci.data(), L.G.v.data(), &nnz_g, &pd);
+      if (rc) return fail(rc, std::string("afsai (external): error ") + std::to_string(rc));
+      L.G.ci.resize((size_t)nnz_g);
+      L.G.v.resize((size_t)nnz_g);
+      L.G.ci.shrink_to_fit();
+      L.G.v.shrink_to_fit();
+      if (!pd) return fail(CHS_ENOTSPD, "afsai_sym: matrix deemed indefinite at level " +
+                                            std::to_string(level));
     } else {
       chs_mat *G = nullptr;
       int32_t pd = 1;

@@ -261,13 +261,17 @@ class SetupParams:
     nthreads: int = 0
     seed: int = 20240607
     verbose: bool = False
+    # Compute the aFSAI factors with the GPU kernel (chr_setup_afsai of libchronos_amg)
+    # instead of the host restatement; both give the same factor bit for bit.
+    gpu_afsai: bool = False
 
 
 class _CParams(C.Structure):
     _fields_ = [("nlev_max", C.c_int32), ("max_coarse", C.c_int32), ("soc_tau", C.c_double),
                 ("nstep", C.c_int32), ("step_size", C.c_int32), ("fsai_eps", C.c_double),
                 ("nupre", C.c_int32), ("nupost", C.c_int32), ("smoother", C.c_int32),
-                ("nthreads", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32)]
+                ("nthreads", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32),
+                ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p)]
 
 
 def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
@@ -275,8 +279,13 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
     if p.nupre != 1 or p.nupost != 1:
         raise ValueError("the V-cycle supports nupre = nupost = 1 (the reference's parameter files)")
     sm = {"afsai_sym": 0, "jacobi": 1}[p.smoother.lower()]
+    fn = None
+    if p.gpu_afsai:
+        from ._lib import load_amg_lib
+
+        fn = C.cast(load_amg_lib().chr_setup_afsai, C.c_void_p).value
     cp = _CParams(p.nlev_max, p.max_coarse, p.soc_tau, p.nstep, p.step_size, p.fsai_eps, p.nupre,
-                  p.nupost, sm, p.nthreads, p.seed, int(p.verbose))
+                  p.nupost, sm, p.nthreads, p.seed, int(p.verbose), fn, None)
     lib = _lib()
     ha = _to_handle(A)
     hh = C.c_void_p()

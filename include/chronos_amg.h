@@ -164,13 +164,38 @@ int chr_batch_vcycle(chr_batch *bt, int32_t k, const double *b, double *x, int32
 int chr_batch_pcg(chr_batch *bt, int32_t k, const double *b, double *x, int32_t itmax,
                   double tol, int32_t ptr_kind, int32_t *iters, int32_t *flags, double *resvec);
 
+/* Adaptive symmetric FSAI factor of A computed on the GPU, every row's pattern growth and
+ * dense solves in one batched launch (one wave per row).  Replaces the reference's native
+ * setup kernel [nterm_G, iat_G, ja_G, coef_G, is_posdef] = SYM_aFSAI_compute(...)
+ * (sources/Preconditioner/AMG/smoother/MEX_SYM_AFSAI/SYM_aFSAI_compute.cpp:49-149, cores
+ * compute_local_fsai.cpp:6-116 and cpt_afsai_coef.cpp:316-554, called from afsai_cpp.m:50),
+ * with that gateway's caller-allocated outputs: rowptr_G[n+1], and col_G / val_G of
+ * n * (nstep*step_size + 1) entries, of which the first *nnz_G are written (rows in
+ * ascending column order, diagonal last).  A: host CSR, square, 0-based, columns sorted.
+ * nstep * step_size <= 63.  tau = 0 (no dropping), as afsai_cpp.m:46 sets it.  is_posdef
+ * = 0 when a pattern block or a row's Schur complement was not positive (the reference's
+ * is_posdef = false).  The factor equals chs_afsai's (chronos_setup.h) bit for bit.
+ * Synchronous; uses a stream of its own on the current device. */
+int chr_afsai(const chr_csr *A, int32_t nstep, int32_t step_size, double eps,
+              int32_t *rowptr_G, int32_t *col_G, double *val_G, int64_t *nnz_G,
+              int32_t *is_posdef);
+/* The same with the signature of chs_afsai_fn (chronos_setup.h), so that the host setup
+ * chs_build can take its aFSAI factors from the GPU: set chs_params.afsai_fn to it (ctx
+ * unused).  Replaces the same reference gateway as chr_afsai. */
+int chr_setup_afsai(void *ctx, int32_t n, const int32_t *rowptr, const int32_t *col,
+                    const double *val, int32_t nstep, int32_t step_size, double eps,
+                    int32_t *rowptr_G, int32_t *col_G, double *val_G, int64_t *nnz_G,
+                    int32_t *is_posdef);
+
 /* Timing helpers (HIP events on the handle's stream, device-resident data):
  * mean milliseconds of one V-cycle (k = 1) over reps launches after one warm-up. */
 int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms);
 /* Mean milliseconds of one launch of a level SpMV: which = 'A', 'G', 'T' (G^T), 'P',
- * 'R' (P^T).  bytes receives the algorithmic bytes of one launch: the matrix as plain CSR
- * (4-byte column indices and row pointers), the input vector read once and the output
- * written once. */
+ * 'R' (P^T), each launch timed on its own after a 512 MiB device copy has flushed the
+ * 256 MB Infinity Cache (as in a V-cycle, where the other products evict the matrix).
+ * bytes receives the algorithmic bytes of one launch: the matrix as plain CSR (4-byte
+ * column indices and row pointers), the input vector read once and the output written
+ * once. */
 int chr_time_spmv(chr_hier *h, int32_t level, char which, int32_t reps, double *ms,
                   double *bytes);
 

@@ -44,6 +44,15 @@ typedef struct chs_mat chs_mat;
 /* Opaque hierarchy owned by the library. */
 typedef struct chs_hier chs_hier;
 
+/* Replacement of the aFSAI factor computation inside chs_build (e.g. the GPU kernel of
+ * libchronos_amg, chr_setup_afsai in chronos_amg.h).  Contract of chr_afsai: A in CSR,
+ * outputs rowptr_G[n+1] and col_G / val_G of n * (nstep*step_size + 1) entries, nnz_G the
+ * entries written; returns 0 or a negative code. */
+typedef int (*chs_afsai_fn)(void *ctx, int32_t n, const int32_t *rowptr, const int32_t *col,
+                            const double *val, int32_t nstep, int32_t step_size, double eps,
+                            int32_t *rowptr_G, int32_t *col_G, double *val_G, int64_t *nnz_G,
+                            int32_t *is_posdef);
+
 /* Setup parameters: the fields of the reference's parm_AMG / parm_COARSEN / parm_SMOOTH
  * files that the classical + aFSAI path reads (Tests/parm_*, examples/AMG/parm_*). */
 typedef struct chs_params {
@@ -59,6 +68,8 @@ typedef struct chs_params {
   int32_t nthreads;     /* OpenMP threads (<=0: runtime default) */
   uint64_t seed;        /* PMIS tie-break stream and Lanczos start vector */
   int32_t verbose;
+  chs_afsai_fn afsai_fn; /* NULL: chs_afsai on the host */
+  void *afsai_ctx;       /* passed to afsai_fn */
 } chs_params;
 
 void chs_params_default(chs_params *p);
