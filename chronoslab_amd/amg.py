@@ -81,6 +81,17 @@ def afsai(A: H.Csr, nstep: int = 15, step_size: int = 1, eps: float = 1e-3) -> t
     return H.Csr(n, n, rp, ci[:k].copy(), v[:k].copy()), bool(pd.value)
 
 
+def fsai_omega(A: H.Csr, G: H.Csr, seed: int = 1) -> tuple[float, float]:
+    """omega = min(1, 1.9 / lambda_max(G A G^T)) by Lanczos on the GPU (chr_fsai_omega);
+    returns (omega, lambda_max).  The smoother damping of
+    sources/Preconditioner/AMG/smoother/smoother.m:72-81."""
+    keep = []
+    ca, cg = _csr(A, keep), _csr(G, keep)
+    om, lam = C.c_double(), C.c_double()
+    _check(_lib().chr_fsai_omega(C.byref(ca), C.byref(cg), seed, C.byref(om), C.byref(lam)), "chr_fsai_omega")
+    return om.value, lam.value
+
+
 class DeviceMatrix:
     """A CSR operator resident on the GPU in FP64 (the outer prodA)."""
 

@@ -18,6 +18,8 @@
 
 #include <omp.h>
 
+#include "rng.hpp"
+
 namespace {
 
 thread_local std::string g_err;
@@ -53,24 +55,13 @@ struct chs_hier {
 };
 
 // ----------------------------------------------------------------------------------------
-// RNG: SplitMix64's output function applied to a Weyl sequence, addressed by counter, so
-// the same draw can be produced by any consumer (CPU generators, the oracle, tests).
+// RNG (rng.hpp): SplitMix64's output function over a counter-addressed Weyl sequence.
 // ----------------------------------------------------------------------------------------
-static inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-static const uint64_t kGamma = 0x9e3779b97f4a7c15ULL;
-
 extern "C" uint64_t chs_rng_u64(uint64_t seed, uint64_t stream, uint64_t k) {
-  uint64_t base = mix64(seed ^ (stream * 0xd1b54a32d192ed03ULL));
-  return mix64(base + (k + 1) * kGamma);
+  return chrng::u64(seed, stream, k);
 }
 
-static inline double u01(uint64_t seed, uint64_t stream, uint64_t k) {
-  return (double)(chs_rng_u64(seed, stream, k) >> 11) * (1.0 / 9007199254740992.0);
-}
+static inline double u01(uint64_t seed, uint64_t stream, uint64_t k) { return chrng::u01(seed, stream, k); }
 
 extern "C" void chs_rng_uniform(uint64_t seed, uint64_t stream, int64_t offset, int64_t n,
                                 double lo, double hi, double *out) {
@@ -106,6 +97,8 @@ extern "C" void chs_params_default(chs_params *p) {
   p->verbose = 0;
   p->afsai_fn = nullptr;
   p->afsai_ctx = nullptr;
+  p->omega_fn = nullptr;
+  p->omega_ctx = nullptr;
 }
 
 extern "C" const char *chs_last_error(void) { return g_err.c_str(); }
@@ -1039,7 +1032,13 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
       if (!pd) return fail(CHS_ENOTSPD, "afsai_sym: matrix deemed indefinite at level " +
                                             std::to_string(level));
     }
-    chs_fsai_omega(&A, &L.G, p.seed + level, nt, &L.omega, &L.lambda);
+    if (p.omega_fn) {
+      int rc = p.omega_fn(p.omega_ctx, n, A.rp.data(), A.ci.data(), A.v.data(), L.G.rp.data(),
+                          L.G.ci.data(), L.G.v.data(), p.seed + level, &L.omega, &L.lambda);
+      if (rc) return fail(rc, std::string("omega (external): error ") + std::to_string(rc));
+    } else {
+      chs_fsai_omega(&A, &L.G, p.seed + level, nt, &L.omega, &L.lambda);
+    }
     double t1 = now_s();
     std::vector<int8_t> strong(A.nnz());
     chs_soc_cla(&A, p.soc_tau, strong.data());

@@ -264,6 +264,8 @@ class SetupParams:
     # Compute the aFSAI factors with the GPU kernel (chr_setup_afsai of libchronos_amg)
     # instead of the host restatement; both give the same factor bit for bit.
     gpu_afsai: bool = False
+    # Compute the smoother damping's lambda_max by Lanczos on the GPU (chr_setup_omega).
+    gpu_omega: bool = False
 
 
 class _CParams(C.Structure):
@@ -271,7 +273,8 @@ class _CParams(C.Structure):
                 ("nstep", C.c_int32), ("step_size", C.c_int32), ("fsai_eps", C.c_double),
                 ("nupre", C.c_int32), ("nupost", C.c_int32), ("smoother", C.c_int32),
                 ("nthreads", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32),
-                ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p)]
+                ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p), ("omega_fn", C.c_void_p),
+                ("omega_ctx", C.c_void_p)]
 
 
 def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
@@ -279,13 +282,15 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
     if p.nupre != 1 or p.nupost != 1:
         raise ValueError("the V-cycle supports nupre = nupost = 1 (the reference's parameter files)")
     sm = {"afsai_sym": 0, "jacobi": 1}[p.smoother.lower()]
-    fn = None
-    if p.gpu_afsai:
+    fn = ofn = None
+    if p.gpu_afsai or p.gpu_omega:
         from ._lib import load_amg_lib
 
-        fn = C.cast(load_amg_lib().chr_setup_afsai, C.c_void_p).value
+        g = load_amg_lib()
+        fn = C.cast(g.chr_setup_afsai, C.c_void_p).value if p.gpu_afsai else None
+        ofn = C.cast(g.chr_setup_omega, C.c_void_p).value if p.gpu_omega else None
     cp = _CParams(p.nlev_max, p.max_coarse, p.soc_tau, p.nstep, p.step_size, p.fsai_eps, p.nupre,
-                  p.nupost, sm, p.nthreads, p.seed, int(p.verbose), fn, None)
+                  p.nupost, sm, p.nthreads, p.seed, int(p.verbose), fn, None, ofn, None)
     lib = _lib()
     ha = _to_handle(A)
     hh = C.c_void_p()

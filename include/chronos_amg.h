@@ -187,6 +187,18 @@ int chr_setup_afsai(void *ctx, int32_t n, const int32_t *rowptr, const int32_t *
                     int32_t *rowptr_G, int32_t *col_G, double *val_G, int64_t *nnz_G,
                     int32_t *is_posdef);
 
+/* Smoother damping omega = min(1, 1.9 / lambda_max(G A G^T)) with lambda_max by Lanczos
+ * (40 steps at most, full reorthogonalisation, stop at 1e-7 relative change) on the GPU.
+ * Replaces the eigs call of sources/Preconditioner/AMG/smoother/smoother.m:72-81; the
+ * start vector is U[-1,1) from the shared counter RNG (seed, stream 7) as in chs_fsai_omega
+ * (chronos_setup.h), whose lambda it matches to rounding. */
+int chr_fsai_omega(const chr_csr *A, const chr_csr *G, uint64_t seed, double *omega,
+                   double *lambda);
+/* The same with the signature of chs_omega_fn (chronos_setup.h), for chs_params.omega_fn. */
+int chr_setup_omega(void *ctx, int32_t n, const int32_t *rowptr_A, const int32_t *col_A,
+                    const double *val_A, const int32_t *rowptr_G, const int32_t *col_G,
+                    const double *val_G, uint64_t seed, double *omega, double *lambda);
+
 /* Timing helpers (HIP events on the handle's stream, device-resident data):
  * mean milliseconds of one V-cycle (k = 1) over reps launches after one warm-up. */
 int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms);

@@ -53,6 +53,12 @@ typedef int (*chs_afsai_fn)(void *ctx, int32_t n, const int32_t *rowptr, const i
                             int32_t *rowptr_G, int32_t *col_G, double *val_G, int64_t *nnz_G,
                             int32_t *is_posdef);
 
+/* Replacement of the smoother damping computation inside chs_build (e.g. chr_setup_omega
+ * of libchronos_amg): omega = min(1, 1.9 / lambda_max(G A G^T)) for the level's A and G. */
+typedef int (*chs_omega_fn)(void *ctx, int32_t n, const int32_t *rowptr_A, const int32_t *col_A,
+                            const double *val_A, const int32_t *rowptr_G, const int32_t *col_G,
+                            const double *val_G, uint64_t seed, double *omega, double *lambda);
+
 /* Setup parameters: the fields of the reference's parm_AMG / parm_COARSEN / parm_SMOOTH
  * files that the classical + aFSAI path reads (Tests/parm_*, examples/AMG/parm_*). */
 typedef struct chs_params {
@@ -70,6 +76,8 @@ typedef struct chs_params {
   int32_t verbose;
   chs_afsai_fn afsai_fn; /* NULL: chs_afsai on the host */
   void *afsai_ctx;       /* passed to afsai_fn */
+  chs_omega_fn omega_fn; /* NULL: chs_fsai_omega on the host */
+  void *omega_ctx;       /* passed to omega_fn */
 } chs_params;
 
 void chs_params_default(chs_params *p);

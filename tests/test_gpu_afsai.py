@@ -106,3 +106,56 @@ def test_hierarchy_with_gpu_afsai_equals_host(gpu):
             np.testing.assert_array_equal(x.col, y.col)
             np.testing.assert_array_equal(x.val, y.val)
         assert a.omega == b.omega
+
+
+# ------------------------------------------------------------------ smoother damping omega
+def _jacobi_factor(A):
+    d = A.to_scipy().diagonal()
+    return H.Csr.from_scipy(sp.diags(1.0 / np.sqrt(d)).tocsr())
+
+
+def test_omega_closed_form(gpu):
+    """G = D^-1/2 on the 7-point Poisson matrix: lambda_max(G A G^T) = 1 + cos(pi/(N+1))."""
+    from chronoslab_amd import amg
+
+    N = 8
+    A = H.poisson7(N)
+    G = _jacobi_factor(A)
+    exact = 1.0 + np.cos(np.pi / (N + 1))
+    om_d, lam_d = amg.fsai_omega(A, G, seed=3)
+    om_h, lam_h = H.fsai_omega(A, G, seed=3)
+    print(f"lambda gpu {lam_d!r} host {lam_h!r} exact {exact!r}", flush=True)
+    assert abs(lam_d - exact) <= 1e-6 * exact
+    assert abs(lam_h - exact) <= 1e-6 * exact
+    assert om_d == min(1.0, 1.9 / lam_d)
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_omega_matches_host(gpu, level):
+    from chronoslab_amd import amg
+
+    A = _diffusion_levels()[level]
+    G, _ = H.afsai(A)
+    om_d, lam_d = amg.fsai_omega(A, G, seed=5)
+    om_h, lam_h = H.fsai_omega(A, G, seed=5)
+    print(f"level {level}: lambda gpu {lam_d!r} host {lam_h!r}", flush=True)
+    # Lanczos stops at a 1e-7 relative change: the two runs differ in rounding only, which
+    # can move the stopping step by one.
+    assert abs(lam_d - lam_h) <= 1e-6 * lam_h
+    assert abs(om_d - om_h) <= 1e-6 * om_h
+    # Dense check of the largest eigenvalue of G A G^T (small level).
+    if A.nrows <= 4000:
+        M = G.to_scipy() @ A.to_scipy() @ G.to_scipy().T
+        ev = np.linalg.eigvalsh(M.toarray())[-1]
+        assert abs(lam_d - ev) <= 1e-5 * ev
+
+
+def test_hierarchy_with_gpu_omega(gpu):
+    A, _ = H.heterogeneous_diffusion(18, 2.0, 5)
+    h_host = H.build(A)
+    h_gpu = H.build(A, H.SetupParams(gpu_afsai=True, gpu_omega=True))
+    assert h_host.nlev == h_gpu.nlev
+    for a, b in zip(h_host.levels, h_gpu.levels):
+        assert abs(a.omega - b.omega) <= 1e-6 * a.omega
+        if a.G is not None:
+            np.testing.assert_array_equal(a.G.val, b.G.val)
