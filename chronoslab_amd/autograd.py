@@ -4,7 +4,8 @@ the C ABI).
     x = FVDiffusionSolve.apply(lnk, b, N)          # A(exp(lnk)) x = b, AMG-PCG on the GPU
     loss = 0.5 * ((x - x_obs) ** 2).sum(); loss.backward()
 
-forward: host setup of the AMG hierarchy for A(k) (classical + aFSAI), upload, PCG_CJ with
+forward: setup of the AMG hierarchy for A(k) (classical coarsening on the host, aFSAI factors
+and smoother damping on the GPU), upload, PCG_CJ with
 the chosen V-cycle precision.  backward: the adjoint A^T lam = dL/dx is solved with the same
 hierarchy (A is SPD, so A^T = A), and one fused kernel (chr_fv_grad) forms dL/d(ln k) on A's
 sparsity pattern; dL/db = lam.  The reference has no counterpart.
@@ -34,6 +35,8 @@ class FVDiffusionSolve(torch.autograd.Function):
                 params: H.SetupParams | None = None):
         k_host = np.exp(lnk.detach().double().cpu().numpy())
         A = H.diffusion(N, k_host)
+        if params is None:  # aFSAI factors and omega on the GPU
+            params = H.SetupParams(gpu_afsai=True, gpu_omega=True)
         M = amg.AMGPreconditioner(H.build(A, params), precision)
         bb = b.detach().double().contiguous()
         x = torch.zeros_like(bb)

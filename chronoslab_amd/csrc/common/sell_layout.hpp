@@ -5,9 +5,10 @@
 // Layout:
 //   * slots are taken in slices of 64 (one wave, one row per lane); slice s stores element
 //     (k, lane) at sp[s] + 64 k + lane, k < len_s = the longest row of the slice;
-//   * windows of 2^wshift slots (256 <= 2^wshift <= 16384); when `sorted`, the rows of a
-//     window sit in its slots by decreasing length (stable) and perm[slot] is the row's
-//     offset from the window's first row (0xFFFF: empty slot); otherwise slot = row;
+//   * dictionary windows of 2^wshift slots (256 <= 2^wshift <= 16384); when `sorted`, the
+//     rows of each sort window of 2^sshift slots (256 <= 2^sshift <= 2^wshift) sit in its
+//     slots by decreasing length (stable) and perm[slot] is the row's offset from the sort
+//     window's first row (0xFFFF: empty slot); otherwise slot = row;
 //   * column codes are row-relative offsets d = row - col: cb = 8, an index into the
 //     window's dictionary dict[w*256 + code] of at most 255 offsets (code 255 = padding,
 //     dictionary entry 0); cb = 16, d - dmin (0xFFFF = padding); cb = 32, the column
@@ -27,7 +28,8 @@ constexpr int kMaxRow = 256;
 
 struct Layout {
   int cb = 0;  // 0: the matrix is not a candidate
-  int32_t nrows = 0, nslices = 0, wshift = 8, dmin = 0;
+  int32_t nrows = 0, nslices = 0, wshift = 8, sshift = 8, dmin = 0;
+  int unroll = 8;  // kernel chunk: 16 when some slice is longer than 8 elements
   int64_t elems = 0;
   bool sorted = false;
   std::vector<int32_t> sp, dict;
@@ -42,7 +44,7 @@ struct Layout {
 // Returns false (L.cb = 0) when the matrix is not square, empty, has a row longer than
 // kMaxRow, or would be padded by more than 50%.
 inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci, const double *v,
-                  int value_bytes, Layout &L) {
+                  int value_bytes, Layout &L, int sort_shift = -1) {
   L = Layout();
   L.nrows = n;
   if (n <= 0 || ncols != n || rp[n] == 0) return false;
@@ -57,17 +59,24 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
   const int64_t nslots = (int64_t)nslices * kSlice;
   auto rlen = [&](int32_t r) { return r < 0 ? 0 : rp[r + 1] - rp[r]; };
 
-  // Candidate slot -> row maps: identity, and rows by decreasing length per window.
-  std::vector<int32_t> ident((size_t)nslots, -1), bylen((size_t)nslots, -1);
+  // Candidate slot -> row maps: identity, and rows by decreasing length inside sort windows
+  // of 2^ss slots (ss = 8, 10, 12, ..., ws).  Small windows keep a wave's rows (and so its
+  // gathers and stores) close together; large ones cut the padding.
+  std::vector<int32_t> ident((size_t)nslots, -1);
   for (int32_t r = 0; r < n; ++r) ident[(size_t)r] = r;
-  for (int64_t w0 = 0; w0 < n; w0 += W) {
-    const int64_t w1 = std::min<int64_t>(w0 + W, n);
+  auto sorted_map = [&](int ss) {
+    std::vector<int32_t> m((size_t)nslots, -1);
+    const int64_t SW = (int64_t)1 << ss;
     std::vector<int32_t> rows;
-    rows.reserve((size_t)(w1 - w0));
-    for (int64_t r = w0; r < w1; ++r) rows.push_back((int32_t)r);
-    std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return rlen(a) > rlen(b); });
-    std::copy(rows.begin(), rows.end(), bylen.begin() + w0);
-  }
+    for (int64_t w0 = 0; w0 < n; w0 += SW) {
+      const int64_t w1 = std::min<int64_t>(w0 + SW, n);
+      rows.clear();
+      for (int64_t r = w0; r < w1; ++r) rows.push_back((int32_t)r);
+      std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return rlen(a) > rlen(b); });
+      std::copy(rows.begin(), rows.end(), m.begin() + w0);
+    }
+    return m;
+  };
   auto padded = [&](const std::vector<int32_t> &m) {
     int64_t tot = 0;
     for (int64_t s = 0; s < nslices; ++s) {
@@ -77,7 +86,18 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
     }
     return tot;
   };
-  const int64_t e_ident = padded(ident), e_sorted = padded(bylen);
+  // Sort windows considered: all of them for the builder's own choice (sort_shift < 0),
+  // else only the one asked for (0: none).
+  std::vector<int> sshifts;
+  if (sort_shift < 0) {
+    for (int ss = 8; ss < ws; ss += 2) sshifts.push_back(ss);
+    sshifts.push_back(ws);
+  } else if (sort_shift > 0) {
+    sshifts.push_back(std::max(8, std::min(sort_shift, ws)));
+  }
+  const int64_t e_ident = padded(ident);
+  std::vector<int64_t> e_sorted_at;
+  for (int ss : sshifts) e_sorted_at.push_back(padded(sorted_map(ss)));
 
   // Code width: per-window dictionaries of <= 255 offsets, else 16-bit offsets, else columns.
   int64_t dlo = INT64_MAX, dhi = INT64_MIN;
@@ -120,25 +140,37 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
   }
   if (cb != 8) L.dict.clear();
   const int cbytes = cb / 8;
-  const bool sorted = (double)e_sorted * (value_bytes + cbytes) + 2.0 * nslots <
-                      (double)e_ident * (value_bytes + cbytes);
-  const int64_t elems = sorted ? e_sorted : e_ident;
+  // Layout choice: the bytes of each candidate (padded elements, plus the row permutation
+  // when sorted); the smallest sort window (or none) within 3% of the fewest bytes wins.
+  // sort_shift >= 0 forces a window (0: unsorted), e.g. for measurements.
+  std::vector<double> cost;
+  cost.push_back((double)e_ident * (value_bytes + cbytes));
+  for (int64_t e : e_sorted_at) cost.push_back((double)e * (value_bytes + cbytes) + 2.0 * nslots);
+  const double best = *std::min_element(cost.begin(), cost.end());
+  int pick = 0;
+  while (cost[(size_t)pick] > 1.03 * best) ++pick;
+  if (sort_shift == 0) pick = 0;
+  else if (sort_shift > 0) pick = 1;
+  const bool sorted = pick > 0;
+  const int sshift = sorted ? sshifts[(size_t)pick - 1] : ws;
+  const int64_t elems = sorted ? e_sorted_at[(size_t)pick - 1] : e_ident;
   if (elems > (int64_t)INT32_MAX - kSlice || elems > nnz + nnz / 2) {
     L = Layout();
     L.nrows = n;
     return false;
   }
-  const std::vector<int32_t> &map = sorted ? bylen : ident;
+  const std::vector<int32_t> map = sorted ? sorted_map(sshift) : ident;
   L.cb = cb;
   L.nslices = nslices;
   L.wshift = ws;
+  L.sshift = sshift;
   L.dmin = cb == 16 ? (int32_t)dlo : 0;
   L.elems = elems;
   L.sorted = sorted;
   if (sorted) {
     L.perm.assign((size_t)nslots, 0xFFFF);
     for (int64_t s = 0; s < nslots; ++s)
-      if (map[(size_t)s] >= 0) L.perm[(size_t)s] = (uint16_t)(map[(size_t)s] - ((s >> ws) << ws));
+      if (map[(size_t)s] >= 0) L.perm[(size_t)s] = (uint16_t)(map[(size_t)s] - ((s >> sshift) << sshift));
   }
   L.sp.assign((size_t)nslices + 1, 0);
   L.val.assign((size_t)elems, 0.0);
@@ -169,6 +201,7 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
       }
     }
     pos += (int64_t)mx * kSlice;
+    if (mx > 8) L.unroll = 16;
   }
   L.sp[(size_t)nslices] = (int32_t)pos;
   return true;

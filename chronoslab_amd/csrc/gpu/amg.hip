@@ -37,12 +37,12 @@ uint64_t next_uid() {
 }
 
 int DevSell::build(int32_t n, int32_t nc, const int32_t *hrp, const int32_t *hci,
-                   const double *hv, bool fp32) {
+                   const double *hv, bool fp32, int sort_shift) {
   release();
   elems = 0;
   nrows = n;
   chrsell::Layout L;
-  if (!chrsell::build(n, nc, hrp, hci, hv, fp32 ? 4 : 8, L)) return CHR_OK;
+  if (!chrsell::build(n, nc, hrp, hci, hv, fp32 ? 4 : 8, L, sort_shift)) return CHR_OK;
   const size_t vb = fp32 ? 4 : 8, cbytes = (size_t)L.cb / 8;
   int rc;
   if ((rc = sp.alloc(sizeof(int32_t) * L.sp.size())) || (rc = val.alloc(vb * (size_t)L.elems)) ||
@@ -70,42 +70,55 @@ int DevSell::build(int32_t n, int32_t nc, const int32_t *hrp, const int32_t *hci
   cb = L.cb;
   nslices = L.nslices;
   wshift = L.wshift;
+  sshift = L.sshift;
+  unroll = L.unroll;
   dmin = L.dmin;
   elems = L.elems;
   sorted = L.sorted;
   return CHR_OK;
 }
 
-// Autotuner for single-vector products, in two steps:
-//  1. the CSR-stream layout: PER is timed over the candidates (HIP events, 2 warm-up and 5
-//     timed launches each); PER 8 is kept unless another is more than 3% faster, so that
-//     near-ties do not flip from run to run.  tune_per = false keeps PER 8: the outer
-//     operator of PCG, whose fused dot products are summed over the stream row blocks, so
-//     that the solver's rounding never depends on a timing;
-//  2. the sliced-ELL layout, built when the matrix admits it and kept if more than 3%
-//     faster than the stream layout of step 1.
-// Every layout sums a row in the same order, so the choice never changes a product.
+// Autotuner for single-vector products.  Candidates:
+//  1. the CSR-stream layout with PER in {4, 8, 16} nonzeros per thread (tune_per = false
+//     keeps PER 8: the outer operator of PCG, whose fused dot products are summed over the
+//     stream row blocks, so that the solver's rounding never depends on a timing);
+//  2. the sliced-ELL layout, when the matrix admits it, unsorted, sorted in 256-row
+//     windows, and with the builder's own choice of sort window.
+// Each candidate is timed with HIP events, 2 warm-up and 5 timed launches; a matrix of 32 MB
+// or more is timed cache-cold (a 512 MiB device copy before every timed launch, as the
+// other products of a V-cycle evict it), smaller ones back to back.  The fastest candidate
+// wins, but PER 8 is kept unless another is more than 3% faster, so that near-ties do not
+// flip from run to run.  Every layout sums a row in the same order, so the choice never
+// changes a product.
 // Environment: CHRONOS_STREAM_PER=4|8|16 fixes PER (tune_per only); CHRONOS_SELL=0
-// excludes and CHRONOS_SELL=1 forces the sliced-ELL layout where the matrix admits it.
+// excludes and CHRONOS_SELL=1 forces the sliced-ELL layout where the matrix admits it;
+// CHRONOS_SELL_SORT=0|8|10|12|14 fixes its sort window (0: unsorted).
 int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
-                const int *zero, bool tune_per) {
+                const int *zero, bool tune_per, const DevBuf *flush) {
   M.use_sell = false;
   M.sell.release();
   if (M.nnz == 0) return CHR_OK;
   const char *env_sell = std::getenv("CHRONOS_SELL");
   const int sell_mode = env_sell ? std::atoi(env_sell) : -1;  // -1 timed, 0 off, 1 forced
+  const char *env_sort = std::getenv("CHRONOS_SELL_SORT");
   int fixed_per = tune_per ? 0 : 8;
   if (const char *env = std::getenv("CHRONOS_STREAM_PER"); env && tune_per)
     for (int p : kPers)
       if (p == std::atoi(env)) fixed_per = p;
   const size_t sz = M.fp32 ? 4 : 8;
+  const bool cold = flush && flush->p && M.csr_bytes() >= 32e6;
   DevBuf xb, yb;
   int rc;
   if ((rc = xb.alloc(sz * std::max(M.ncols, 1))) || (rc = yb.alloc(sz * std::max(M.nrows, 1)))) return rc;
   HIPC(hipMemsetAsync(xb.p, 0, xb.bytes, s));
-  hipEvent_t ea, eb;
-  HIPC(hipEventCreate(&ea));
-  HIPC(hipEventCreate(&eb));
+  hipEvent_t ev[12];
+  for (auto &e : ev) HIPC(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t *e;
+    ~EvGuard() {
+      for (int i = 0; i < 12; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } evg{ev};
   auto time_it = [&](float &ms) -> int {
     auto run = [&]() -> hipError_t {
       if (M.fp32) return launch_spmv(M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, zero, s);
@@ -113,11 +126,28 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     };
     hipError_t e = hipSuccess;
     for (int i = 0; i < 2 && !e; ++i) e = run();
-    if (!e) e = hipEventRecord(ea, s);
-    for (int i = 0; i < 5 && !e; ++i) e = run();
-    if (!e) e = hipEventRecord(eb, s);
-    if (!e) e = hipEventSynchronize(eb);
-    if (!e) e = hipEventElapsedTime(&ms, ea, eb);
+    if (cold) {
+      const size_t half = flush->bytes / 2;
+      for (int i = 0; i < 5 && !e; ++i) {
+        e = hipMemcpyAsync(flush->as<char>() + half, flush->p, half, hipMemcpyDeviceToDevice, s);
+        if (!e) e = hipEventRecord(ev[2 * i], s);
+        if (!e) e = run();
+        if (!e) e = hipEventRecord(ev[2 * i + 1], s);
+      }
+      if (!e) e = hipEventSynchronize(ev[9]);
+      ms = 0.f;
+      for (int i = 0; i < 5 && !e; ++i) {
+        float t = 0.f;
+        e = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
+        ms += t;
+      }
+    } else {
+      if (!e) e = hipEventRecord(ev[10], s);
+      for (int i = 0; i < 5 && !e; ++i) e = run();
+      if (!e) e = hipEventRecord(ev[11], s);
+      if (!e) e = hipEventSynchronize(ev[11]);
+      if (!e) e = hipEventElapsedTime(&ms, ev[10], ev[11]);
+    }
     if (e) return fail(CHR_EHIP, std::string("autotune: ") + hipGetErrorString(e));
     return CHR_OK;
   };
@@ -145,23 +175,39 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     }
   }
   // Step 2: sliced ELL.
-  if (!rc && sell_mode != 0 && !(rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32)) && M.sell.cb) {
-    if (sell_mode != 1) {
+  if (!rc && sell_mode != 0) {
+    std::vector<int> sorts;
+    if (env_sort) sorts.push_back(std::atoi(env_sort));
+    else sorts = {-1, 0, 8};
+    if (sell_mode == 1 && !env_sort) sorts = {-1};
+    if (stream_ms < 0.f && sell_mode != 1) rc = time_it(stream_ms);
+    float best_ms = sell_mode == 1 ? 1e30f : 0.97f * stream_ms;
+    int best_sort = -2, last_sort = -2;  // -2: stream layout
+    std::vector<std::pair<int, int>> seen;  // (sorted, sshift) of the layouts built so far
+    for (size_t c = 0; c < sorts.size() && !rc; ++c) {
+      if ((rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32, sorts[c]))) break;
+      if (!M.sell.cb) break;  // not a candidate
+      last_sort = sorts[c];
+      const std::pair<int, int> key{M.sell.sorted ? 1 : 0, M.sell.sorted ? M.sell.sshift : 0};
+      if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+      seen.push_back(key);
       float ms = 0.f;
-      if (stream_ms < 0.f) rc = time_it(stream_ms);
-      if (!rc) {
-        M.use_sell = true;
-        rc = time_it(ms);
-      }
-      if (!rc && !(ms < 0.97f * stream_ms)) {
-        M.use_sell = false;
-        M.sell.release();
+      M.use_sell = true;
+      rc = time_it(ms);
+      M.use_sell = false;
+      if (!rc && (ms < best_ms || sell_mode == 1)) {
+        best_ms = ms;
+        best_sort = sorts[c];
+        if (sell_mode == 1) break;
       }
     }
-    M.use_sell = M.sell.cb != 0;
+    if (!rc && best_sort != -2) {
+      if (best_sort != last_sort) rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32, best_sort);
+      M.use_sell = !rc && M.sell.cb != 0;
+    } else {
+      M.sell.release();
+    }
   }
-  (void)hipEventDestroy(ea);
-  (void)hipEventDestroy(eb);
   if (rc) {
     M.use_sell = false;
     M.sell.release();
@@ -302,6 +348,14 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
   const bool f32 = precision == CHR_FP32_CYCLE;
   const size_t s = f32 ? 4 : 8;
   const int32_t nlev = (int32_t)lv.size();
+  // Cache-cold autotuning of the large level matrices needs a 1 GiB flush buffer for the
+  // duration of the upload.
+  DevBuf flush;
+  if (lv[0].nd > 0 && (double)lv[0].A.rowptr[lv[0].nd] * (s + 4) >= 32e6) {
+    if ((rc = flush.alloc(size_t(1) << 30))) return rc;
+    HIPC(hipMemsetAsync(flush.p, 0, flush.bytes, h->stream));
+  }
+  const DevBuf *fl = flush.p ? &flush : nullptr;
   h->lev.resize(nlev);
   std::vector<double> inv_all;  // packed inverses of every direct block
   for (int32_t l = 0; l < nlev; ++l) {
@@ -313,21 +367,21 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
     L.nupre = in.nupre;
     L.nupost = in.nupost;
     if (in.nd > 0) {
-      if ((rc = L.A.upload(in.A, f32)) || (rc = tune_layout(L.A, in.A.rowptr, in.A.col, in.A.val, h->stream, zf)))
+      if ((rc = L.A.upload(in.A, f32)) || (rc = tune_layout(L.A, in.A.rowptr, in.A.col, in.A.val, h->stream, zf, true, fl)))
         return rc;
-      if ((rc = L.G.upload(in.G, f32)) || (rc = tune_layout(L.G, in.G.rowptr, in.G.col, in.G.val, h->stream, zf)))
+      if ((rc = L.G.upload(in.G, f32)) || (rc = tune_layout(L.G, in.G.rowptr, in.G.col, in.G.val, h->stream, zf, true, fl)))
         return rc;
-      if ((rc = L.P.upload(in.P, f32)) || (rc = tune_layout(L.P, in.P.rowptr, in.P.col, in.P.val, h->stream, zf)))
+      if ((rc = L.P.upload(in.P, f32)) || (rc = tune_layout(L.P, in.P.rowptr, in.P.col, in.P.val, h->stream, zf, true, fl)))
         return rc;
       std::vector<int32_t> trp, tci;
       std::vector<double> tv;
       transpose_host(in.G, trp, tci, tv);
       if ((rc = L.GT.upload(in.G.ncols, in.G.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
-          (rc = tune_layout(L.GT, trp.data(), tci.data(), tv.data(), h->stream, zf)))
+          (rc = tune_layout(L.GT, trp.data(), tci.data(), tv.data(), h->stream, zf, true, fl)))
         return rc;
       transpose_host(in.P, trp, tci, tv);
       if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
-          (rc = tune_layout(L.PT, trp.data(), tci.data(), tv.data(), h->stream, zf)))
+          (rc = tune_layout(L.PT, trp.data(), tci.data(), tv.data(), h->stream, zf, true, fl)))
         return rc;
       if (!in.omega_rows.empty()) {
         if ((rc = L.omega_rows.alloc(sizeof(double) * in.nd))) return rc;
@@ -366,7 +420,7 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
   }
   // Level-0 operator in FP64 for the outer Krylov loop.
   if ((rc = h->outer.A.upload(outer, false)) ||
-      (rc = tune_layout(h->outer.A, outer.rowptr, outer.col, outer.val, h->stream, zf, false)))
+      (rc = tune_layout(h->outer.A, outer.rowptr, outer.col, outer.val, h->stream, zf, false, fl)))
     return rc;
   if ((rc = h->outer.zero.alloc(sizeof(int)))) return rc;
   HIPC(hipMemset(h->outer.zero.p, 0, sizeof(int)));

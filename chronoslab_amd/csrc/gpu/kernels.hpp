@@ -208,9 +208,10 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
 // one row per lane; slice s stores its elements column by column, element (k, lane) at
 // sp[s] + 64 k + lane, so every load of a wave is one contiguous 64-element segment and
 // the row sum stays in a register (no LDS staging, no row pointers).  A slice is as long
-// as its longest row; shorter rows are padded with a PAD code.  Within windows of 2^wshift
-// slots the rows may be ordered by decreasing length to cut padding: perm then holds the
-// row of each slot as an offset from the window's first row (0xFFFF: empty slot).
+// as its longest row; shorter rows are padded with a PAD code.  Within sort windows of
+// 2^sshift slots the rows may be ordered by decreasing length to cut padding: perm then
+// holds the row of each slot as an offset from the sort window's first row (0xFFFF: empty
+// slot).  Dictionaries (CB = 8) belong to windows of 2^wshift >= 2^sshift slots.
 //
 // Column codes, as row-relative offsets d = row - col:
 //   CB = 8:  an index into the window's dictionary of at most 255 offsets (dict[w*256 +
@@ -238,11 +239,11 @@ struct SellCode<32> {
   static constexpr uint32_t pad = 0xFFFFFFFFu;
 };
 
-constexpr int kSellUnroll = 8;
-
-template <int CB, typename TV, typename TX, typename Epi>
+// U = elements per chunk: every load of a chunk issues before its gathers; 16 when the
+// layout has slices longer than 8 (one round trip for the FSAI factors' rows of <= 16).
+template <int CB, int U, typename TV, typename TX, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
-    int nslices, int nrows, int wshift, const int *__restrict__ sp,
+    int nslices, int nrows, int wshift, int sshift, const int *__restrict__ sp,
     const uint16_t *__restrict__ perm, const TV *__restrict__ val,
     const typename SellCode<CB>::type *__restrict__ code, const int *__restrict__ dict, int dmin,
     const TX *__restrict__ x, Epi epi, const int *__restrict__ stop) {
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
   if (perm) {
     const int q = perm[slot];
     valid = q != 0xFFFF;
-    row = ((slot >> wshift) << wshift) + q;
+    row = ((slot >> sshift) << sshift) + q;
   }
   const int rc = valid ? row : 0;  // PAD elements read x[rc] and drop the product
   const double pv = epi.pre(rc);
@@ -275,26 +276,26 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
     else if constexpr (CB == 16) return rc - (c == (CT)SellCode<16>::pad ? 0 : (int)c + dmin);
     else return c < 0 ? rc : (int)c;
   };
-  // Chunks of kSellUnroll elements; the last chunk clamps its loads to the row's last
+  // Chunks of U elements; the last chunk clamps its loads to the row's last
   // element and drops the surplus products, so every load of a chunk issues at once.
   TA acc = 0;
-  const int nch = (len + kSellUnroll - 1) / kSellUnroll;
+  const int nch = (len + U - 1) / U;
   for (int ch = 0; ch < nch; ++ch) {
-    TV v[kSellUnroll];
-    CT c[kSellUnroll];
-    TA xv[kSellUnroll];
+    TV v[U];
+    CT c[U];
+    TA xv[U];
 #pragma unroll
-    for (int u = 0; u < kSellUnroll; ++u) {
-      const int e = base + min(ch * kSellUnroll + u, len - 1) * kWave;
+    for (int u = 0; u < U; ++u) {
+      const int e = base + min(ch * U + u, len - 1) * kWave;
       v[u] = val[e];
       c[u] = code[e];
     }
 #pragma unroll
-    for (int u = 0; u < kSellUnroll; ++u) xv[u] = (TA)x[col_of(c[u])];
+    for (int u = 0; u < U; ++u) xv[u] = (TA)x[col_of(c[u])];
 #pragma unroll
-    for (int u = 0; u < kSellUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       const TA p = mul_rn((TA)v[u], xv[u]);
-      const bool live = ch * kSellUnroll + u < len && ((uint32_t)c[u] & SellCode<CB>::pad) != SellCode<CB>::pad;
+      const bool live = ch * U + u < len && ((uint32_t)c[u] & SellCode<CB>::pad) != SellCode<CB>::pad;
       acc = live ? acc + p : acc;
     }
   }

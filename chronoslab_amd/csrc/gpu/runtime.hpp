@@ -114,17 +114,18 @@ constexpr int kPers[] = {4, 8, 16};
 
 // Sliced-ELL copy of a square matrix for single-vector products (spmv_sell_kernel): 64-row
 // slices stored column by column, rows optionally ordered by decreasing length inside
-// windows of 2^wshift rows, column codes as described at the kernel.
+// sort windows of 2^sshift rows, column codes as described at the kernel.
 struct DevSell {
   int cb = 0;  // code bits (8, 16, 32); 0 = not built
-  int32_t nrows = 0, nslices = 0, wshift = 14, dmin = 0;
+  int32_t nrows = 0, nslices = 0, wshift = 14, sshift = 14, dmin = 0;
   int64_t elems = 0;  // stored elements, padding included
   bool sorted = false;
+  int unroll = 8;     // elements per chunk of the kernel (16 when a slice is longer than 8)
   DevBuf sp, perm, val, code, dict;
   // Builds the layout from host CSR (chrsell::build) and uploads it; leaves cb = 0 (and
   // returns CHR_OK) when the matrix is not a candidate.
   int build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *hci, const double *hv,
-            bool fp32);
+            bool fp32, int sort_shift = -1);
   void release() {
     sp = DevBuf();
     perm = DevBuf();
@@ -237,24 +238,29 @@ struct DevCsr {
   }
 };
 
-template <int CB, typename TV, typename TX, typename Epi>
+template <int CB, int U, typename TV, typename TX, typename Epi>
 hipError_t launch_sell_cb(const DevSell &S, const TX *x, Epi epi, const int *stop, hipStream_t s) {
   const unsigned grid = (unsigned)((S.nslices + chr::kBlock / chr::kWave - 1) / (chr::kBlock / chr::kWave));
-  hipLaunchKernelGGL((chr::spmv_sell_kernel<CB, TV, TX, Epi>), dim3(grid), dim3(chr::kBlock), 0, s,
-                     S.nslices, S.nrows, S.wshift, S.sp.as<int>(),
+  hipLaunchKernelGGL((chr::spmv_sell_kernel<CB, U, TV, TX, Epi>), dim3(grid), dim3(chr::kBlock), 0, s,
+                     S.nslices, S.nrows, S.wshift, S.sshift, S.sp.as<int>(),
                      S.sorted ? S.perm.as<uint16_t>() : nullptr, S.val.as<TV>(),
                      S.code.as<typename chr::SellCode<CB>::type>(), S.dict.as<int>(), S.dmin, x, epi,
                      stop);
   return hipGetLastError();
 }
 
+template <int U, typename TV, typename TX, typename Epi>
+hipError_t launch_sell_u(const DevSell &S, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  switch (S.cb) {
+    case 8: return launch_sell_cb<8, U, TV>(S, x, epi, stop, s);
+    case 16: return launch_sell_cb<16, U, TV>(S, x, epi, stop, s);
+    default: return launch_sell_cb<32, U, TV>(S, x, epi, stop, s);
+  }
+}
+
 template <typename TV, typename TX, typename Epi>
 hipError_t launch_sell(const DevSell &S, const TX *x, Epi epi, const int *stop, hipStream_t s) {
-  switch (S.cb) {
-    case 8: return launch_sell_cb<8, TV>(S, x, epi, stop, s);
-    case 16: return launch_sell_cb<16, TV>(S, x, epi, stop, s);
-    default: return launch_sell_cb<32, TV>(S, x, epi, stop, s);
-  }
+  return S.unroll == 16 ? launch_sell_u<16, TV>(S, x, epi, stop, s) : launch_sell_u<8, TV>(S, x, epi, stop, s);
 }
 
 template <int PER, typename TV, typename TX, typename Epi>
@@ -324,8 +330,10 @@ hipError_t launch_op(const DevCsr &M, const TX *x, int k, Epi epi, const int *st
 constexpr int kMaxBlockCols = 16;
 
 
+// flush: a device buffer of 1 GiB or so whose halves are copied onto each other to evict
+// the Infinity Cache before cache-cold timings (NULL: every candidate timed back to back).
 int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
-                const int *zero, bool tune_per = true);
+                const int *zero, bool tune_per = true, const DevBuf *flush = nullptr);
 
 uint64_t next_uid();
 

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "chronos_setup.h"
@@ -14,7 +15,7 @@
 
 namespace {
 
-constexpr int kBlock = 256, kUnroll = 8;
+constexpr int kBlock = 256;
 
 template <typename TA>
 int replay(const chrsell::Layout &L, const double *x, double *y) {
@@ -37,7 +38,7 @@ int replay(const chrsell::Layout &L, const double *x, double *y) {
         if (slot >= (int64_t)L.perm.size()) return -11;
         const int q = L.perm[(size_t)slot];
         valid = q != 0xFFFF;
-        row = ((slot >> L.wshift) << L.wshift) + q;
+        row = ((slot >> L.sshift) << L.sshift) + q;
       }
       if (valid && (row < 0 || row >= n)) return -12;
       const int64_t rc = valid ? row : 0;
@@ -45,6 +46,7 @@ int replay(const chrsell::Layout &L, const double *x, double *y) {
       if ((d1 - d0) % chrsell::kSlice) return -13;
       const int64_t base = d0 + lane, len = (d1 - d0) / chrsell::kSlice;
       TA acc = 0;
+      const int kUnroll = L.unroll;
       const int64_t nch = (len + kUnroll - 1) / kUnroll;
       for (int64_t ch = 0; ch < nch; ++ch)
         for (int u = 0; u < kUnroll; ++u) {
@@ -88,12 +90,16 @@ extern "C" int chs_sell_apply(int32_t n, int32_t ncols, const int32_t *rowptr, c
                               const double *val, int32_t fp32, const double *x, double *y,
                               int32_t *info, int64_t *elems) {
   chrsell::Layout L;
-  if (!chrsell::build(n, ncols, rowptr, col, val, fp32 ? 4 : 8, L)) return 1;
+  // CHRONOS_SELL_SORT fixes the sort window exactly as it does for the GPU library.
+  const char *env_sort = std::getenv("CHRONOS_SELL_SORT");
+  if (!chrsell::build(n, ncols, rowptr, col, val, fp32 ? 4 : 8, L, env_sort ? std::atoi(env_sort) : -1))
+    return 1;
   if (info) {
     info[0] = L.cb;
     info[1] = L.sorted ? 1 : 0;
     info[2] = L.wshift;
     info[3] = L.nslices;
+    info[4] = L.sshift;
   }
   if (elems) *elems = L.elems;
   if (fp32) {

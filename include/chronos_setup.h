@@ -143,7 +143,7 @@ int chs_build(const chs_mat *A, const chs_params *p, chs_hier **out);
  * counterpart): builds the layout the GPU library would upload for this CSR matrix and
  * walks it with the kernel's index arithmetic, checking every access against the layout's
  * buffers; y = A x with the kernel's summation order and rounding (fp32 != 0: FP32 values,
- * vector, products and sums).  info[4] = {code bits, sorted, log2 window, slices}; elems =
+ * vector, products and sums).  info[5] = {code bits, sorted, log2 dictionary window, slices, log2 sort window}; elems =
  * stored elements.  Returns 0, 1 when the matrix is not a sliced-ELL candidate, or a
  * negative code naming the first out-of-range access. */
 int chs_sell_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,

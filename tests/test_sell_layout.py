@@ -8,6 +8,7 @@ FP32 products and sums."""
 import ctypes as C
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import oracle
@@ -25,13 +26,13 @@ def sell_apply(A: H.Csr, x: np.ndarray, fp32: bool):
     v = np.ascontiguousarray(A.val, dtype=np.float64)
     xx = np.ascontiguousarray(x, dtype=np.float64)
     y = np.full(A.nrows, np.nan)
-    info = np.zeros(4, np.int32)
+    info = np.zeros(5, np.int32)
     elems = C.c_int64()
     rc = lib.chs_sell_apply(A.nrows, A.ncols, rp.ctypes.data_as(_I32P), ci.ctypes.data_as(_I32P),
                             v.ctypes.data_as(_F64P), int(fp32), xx.ctypes.data_as(_F64P),
                             y.ctypes.data_as(_F64P), info.ctypes.data_as(_I32P), C.byref(elems))
     return rc, y, dict(cb=int(info[0]), sorted=bool(info[1]), wshift=int(info[2]),
-                       nslices=int(info[3]), elems=elems.value)
+                       nslices=int(info[3]), sshift=int(info[4]), elems=elems.value)
 
 
 def csr_seq_fp32(A: H.Csr, x: np.ndarray) -> np.ndarray:
@@ -115,6 +116,33 @@ def test_ragged_rows_sorted_windows():
     A = random_csr(3000, lambda i: int(lens[i]), list(range(-60, 60)), rng)
     info, _ = check(A, rng)
     assert info["sorted"]
+
+
+@pytest.mark.parametrize("sort", ["0", "8", "10", "12", "14"])
+def test_sort_windows(monkeypatch, sort):
+    # Ragged rows over several dictionary windows, with each sort window forced in turn:
+    # rows are permuted only inside their sort window, and every row is written once.
+    rng = np.random.default_rng(7)
+    n = 40000
+    A = random_csr_fast(n, 9, list(range(-300, 300, 7)), rng)
+    keep = rng.random(A.nnz) < 0.7  # ragged rows
+    M = A.to_scipy().tocoo()
+    A = H.Csr.from_scipy(sp.csr_matrix((M.data[keep], (M.row[keep], M.col[keep])), shape=(n, n)))
+    monkeypatch.setenv("CHRONOS_SELL_SORT", sort)
+    info, _ = check(A, rng, fp32_exact=False)
+    assert info["wshift"] == 14
+    if sort == "0":
+        assert not info["sorted"]
+    else:
+        assert info["sorted"] and info["sshift"] == int(sort)
+
+
+def test_default_sort_window_choice():
+    rng = np.random.default_rng(8)
+    lens = rng.integers(0, 41, size=20000)
+    A = random_csr(20000, lambda i: int(lens[i]), list(range(-60, 60)), rng)
+    info, _ = check(A, rng)
+    assert info["sorted"] and 8 <= info["sshift"] <= info["wshift"]
 
 
 def test_sizes_not_multiple_of_slice_and_tiny():
