@@ -295,6 +295,62 @@ def computeAMG(A: H.Csr, param: H.SetupParams | None = None, precision="fp64", s
     return AMGPreconditioner(hier, precision), T_setup
 
 
+def SQMR(Afun, rhs, tol: float, itmax: int, lprec, rprec=None, x0=None):
+    """[pnew, info, relres, niter, resvec] = SQMR(Afun, rhs, tol, itmax, lprec, rprec, x0)
+    (sources/LinearSolver/SQMR.m:1-108): symmetric QMR with the preconditioner split as
+    lprec then rprec (rprec None: identity), the residual recurrence's relative norm as the
+    stopping test, info 0 converged / 1 itmax reached / 10 small sigma / 20 small rho0.  The
+    host loop runs on vectors of the caller's type; the preconditioner is typically the GPU
+    V-cycle (AMGPreconditioner)."""
+    rprec = rprec or (lambda v: v)
+    rhs = np.asarray(rhs, np.float64).ravel()
+    nn = rhs.shape[0]
+    d = np.zeros(nn)
+    p = np.zeros(nn) if x0 is None else np.array(x0, np.float64).ravel()
+    resvec = np.zeros(itmax)
+    info, niter, done = 0, 0, False
+    bnorm = np.linalg.norm(rhs)
+    r = rhs - Afun(p)
+    vscr = lprec(r)
+    q = rprec(vscr)
+    tau = np.linalg.norm(vscr)
+    rho0 = r @ q
+    theta0 = 0.0
+    while not done and niter < itmax:
+        niter += 1
+        vscr = Afun(q)
+        sigma = q @ vscr
+        if abs(sigma) < 1e-20 * rho0:
+            info = 10
+            break
+        alpha = rho0 / sigma
+        r = r - alpha * vscr
+        vscr = lprec(r)
+        theta1 = np.linalg.norm(vscr) / tau
+        gamma = 1.0 / np.sqrt(1.0 + theta1 * theta1)
+        tau = tau * theta1 * gamma
+        d = d * gamma * gamma * theta0 * theta0 + gamma * gamma * alpha * q
+        theta0 = theta1
+        p = p + d
+        resnorm = np.linalg.norm(r) / bnorm
+        if resnorm < tol:
+            done = True
+        else:
+            u = rprec(vscr)
+            rho1 = r @ u
+            if abs(rho0) < 1e-20 * rho1:
+                info = 20
+                break
+            beta = rho1 / rho0
+            rho0 = rho1
+            q = u + beta * q
+        resvec[niter - 1] = resnorm
+    if niter >= itmax:
+        info = 1
+    relres = np.linalg.norm(rhs - Afun(p)) / bnorm
+    return p, info, relres, niter, resvec[:niter]
+
+
 def PCG_CJ(prodM: AMGPreconditioner, prodA, b, x0, itmax: int, tol: float):
     """[x, iter, resvec, flag] = PCG_CJ(prodM, prodA, b, x0, itmax, tol) on the GPU.
 

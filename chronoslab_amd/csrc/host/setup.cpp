@@ -92,6 +92,7 @@ extern "C" void chs_params_default(chs_params *p) {
   p->nupre = 1;
   p->nupost = 1;
   p->smoother = 0;
+  p->prolong = 0;
   p->nthreads = 0;
   p->seed = 20240607ULL;
   p->verbose = 0;
@@ -594,6 +595,140 @@ extern "C" int chs_prol_classical(const chs_mat *Ap, const int8_t *strong,
 }
 
 // ----------------------------------------------------------------------------------------
+// Extended+i interpolation (De Sterck et al., NLAA 15 (2008) 115-139), as the reference's
+// EXTI core computes it (MEX_EXTI_Prol/ONELP_ProlStripe_EXTI.cpp:27-293, driven by
+// EXTI_prolongation.cpp:52-199).  Coarse rows are e_c.  A fine row i starts its
+// interpolatory set with its strong coarse neighbours (weights a_ij) and its denominator
+// with a_ii plus the weak coarse and weak fine entries; every strong fine neighbour k then
+// extends the set with k's strong coarse neighbours (distance two; a weak coarse neighbour
+// of i that is reached this way joins with its a_ij, which leaves the denominator) and
+// distributes a_ik over them in proportion to min(0, a_kj), i's own min(0, a_ki) share
+// going to the denominator; when that extended sum vanishes a_ik is spread evenly.  The
+// weights are -w/denom, sorted by coarse column.  Accumulations follow the row orders of A,
+// so the result is reproducible bit for bit.
+// ----------------------------------------------------------------------------------------
+extern "C" int chs_prol_exti(const chs_mat *Ap, const int8_t *strong, const int32_t *fcnode,
+                             int32_t nc, int nthreads, chs_mat **Pout) {
+  if (!Ap || !strong || !fcnode || !Pout) return fail(CHS_EARG, "null");
+  const chs_mat &A = *Ap;
+  const int32_t n = A.nrows;
+  nthreads = nthreads_or_default(nthreads);
+  const double eps = std::numeric_limits<double>::epsilon();
+  std::vector<std::vector<int32_t>> rc(n);
+  std::vector<std::vector<double>> rv(n);
+#pragma omp parallel num_threads(nthreads)
+  {
+    // mark[j] > 0: j is interpolatory for the current row, at set position mark[j] - 1;
+    // mark[j] < 0: j is a weak coarse neighbour whose entry is A.v[-mark[j] - 1]; 0: neither.
+    std::vector<int32_t> mark(n, 0);
+    std::vector<int32_t> set, weak, fs;
+    std::vector<double> w, fsv;
+#pragma omp for schedule(dynamic, 1024)
+    for (int32_t i = 0; i < n; ++i) {
+      if (fcnode[i] >= 0) {
+        rc[i].push_back(fcnode[i]);
+        rv[i].push_back(1.0);
+        continue;
+      }
+      set.clear();
+      w.clear();
+      weak.clear();
+      double aii = 0.0, denom = 0.0;
+      for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+        const int32_t j = A.ci[e];
+        if (j == i) {
+          aii = A.v[e];
+          denom += aii;
+        } else if (fcnode[j] >= 0) {
+          if (strong[e] > 0) {
+            set.push_back(j);
+            w.push_back(A.v[e]);
+            mark[j] = (int32_t)set.size();
+          } else {
+            denom += A.v[e];
+            mark[j] = -(e + 1);
+            weak.push_back(j);
+          }
+        } else if (strong[e] < 0) {
+          denom += A.v[e];
+        }
+      }
+      for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+        const int32_t k = A.ci[e];
+        if (strong[e] <= 0 || k == i || fcnode[k] >= 0) continue;
+        const double aik = A.v[e];
+        double aki = 0.0, ext = 0.0;
+        fs.clear();
+        fsv.clear();
+        for (int32_t f = A.rp[k]; f < A.rp[k + 1]; ++f) {
+          if (strong[f] <= 0) continue;
+          const int32_t l = A.ci[f];
+          if (l == i) {
+            aki = std::min(0.0, A.v[f]);
+            ext += aki;
+          } else if (fcnode[l] >= 0) {
+            const double akl = std::min(0.0, A.v[f]);
+            fs.push_back(l);
+            fsv.push_back(akl);
+            ext += akl;
+            if (mark[l] <= 0) {
+              double init = 0.0;
+              if (mark[l] < 0) {  // a weak coarse neighbour becomes interpolatory
+                init = A.v[-mark[l] - 1];
+                denom -= init;
+              }
+              set.push_back(l);
+              w.push_back(init);
+              mark[l] = (int32_t)set.size();
+            }
+          }
+        }
+        if (std::fabs(ext) > eps * std::fabs(aii)) {
+          for (size_t q = 0; q < fs.size(); ++q) w[mark[fs[q]] - 1] += (aik * fsv[q]) / ext;
+          denom += (aik * aki) / ext;
+        } else {
+          const double share = 1.0 / (double)(fs.size() + 1);
+          denom += aik * share;
+          for (size_t q = 0; q < fs.size(); ++q) w[mark[fs[q]] - 1] += aik * share;
+        }
+      }
+      if (std::fabs(denom) < eps) denom = std::fabs(aii) > eps ? aii : 1.0;
+      // Coarse numbering, weights, and the row sorted by coarse column.
+      std::vector<std::pair<int32_t, double>> row(set.size());
+      for (size_t q = 0; q < set.size(); ++q) {
+        mark[set[q]] = 0;
+        row[q] = {fcnode[set[q]], -w[q] / denom};
+      }
+      for (int32_t j : weak) mark[j] = 0;
+      std::sort(row.begin(), row.end(),
+                [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) { return a.first < b.first; });
+      auto &oc = rc[i];
+      auto &ov = rv[i];
+      oc.resize(row.size());
+      ov.resize(row.size());
+      for (size_t q = 0; q < row.size(); ++q) {
+        oc[q] = row[q].first;
+        ov[q] = row[q].second;
+      }
+    }
+  }
+  auto *P = new (std::nothrow) chs_mat;
+  if (!P) return fail(CHS_EALLOC, "alloc");
+  P->nrows = n;
+  P->ncols = nc;
+  P->rp.assign(n + 1, 0);
+  for (int32_t i = 0; i < n; ++i) P->rp[i + 1] = P->rp[i] + (int32_t)rc[i].size();
+  P->ci.resize(P->rp[n]);
+  P->v.resize(P->rp[n]);
+  for (int32_t i = 0; i < n; ++i) {
+    std::copy(rc[i].begin(), rc[i].end(), P->ci.begin() + P->rp[i]);
+    std::copy(rv[i].begin(), rv[i].end(), P->v.begin() + P->rp[i]);
+  }
+  *Pout = P;
+  return CHS_OK;
+}
+
+// ----------------------------------------------------------------------------------------
 // Adaptive symmetric FSAI (cpt_afsai_coef.cpp:316-554, compute_local_fsai.cpp:6-116).
 //
 // Row i grows a pattern P_i of columns j < i.  At every step the candidates are the
@@ -1051,7 +1186,8 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     if (nc <= 0 || nc >= n) return fail(CHS_ECONSIST, "coarsening produced no reduction at level " +
                                                          std::to_string(level));
     chs_mat *P = nullptr;
-    int rc = chs_prol_classical(&A, strong.data(), fc.data(), nc, nt, &P);
+    int rc = p.prolong == 1 ? chs_prol_exti(&A, strong.data(), fc.data(), nc, nt, &P)
+                            : chs_prol_classical(&A, strong.data(), fc.data(), nc, nt, &P);
     if (rc) return rc;
     double t3 = now_s();
     chs_mat *Ac = nullptr;

@@ -207,6 +207,16 @@ def prol_classical(A: Csr, strong: np.ndarray, fcnode: np.ndarray, nc: int, nthr
     return _take(out)
 
 
+def prol_exti(A: Csr, strong: np.ndarray, fcnode: np.ndarray, nc: int, nthreads: int = 0) -> Csr:
+    h = _to_handle(A)
+    strong = np.ascontiguousarray(strong, np.int8)
+    fcnode = np.ascontiguousarray(fcnode, np.int32)
+    out = C.c_void_p()
+    _check(_lib().chs_prol_exti(h.ptr, strong.ctypes.data_as(C.POINTER(C.c_int8)),
+                                fcnode.ctypes.data_as(_I32P), nc, nthreads, C.byref(out)))
+    return _take(out)
+
+
 def afsai(A: Csr, nstep: int = 15, step_size: int = 1, eps: float = 1e-3, nthreads: int = 0) -> tuple[Csr, bool]:
     h = _to_handle(A)
     out = C.c_void_p()
@@ -258,6 +268,7 @@ class SetupParams:
     nupre: int = 1
     nupost: int = 1
     smoother: str = "afsai_sym"
+    prolong: str = "clas"  # parm_PROLONG proltype: "clas" or "exti"
     nthreads: int = 0
     seed: int = 20240607
     verbose: bool = False
@@ -271,7 +282,7 @@ class SetupParams:
 class _CParams(C.Structure):
     _fields_ = [("nlev_max", C.c_int32), ("max_coarse", C.c_int32), ("soc_tau", C.c_double),
                 ("nstep", C.c_int32), ("step_size", C.c_int32), ("fsai_eps", C.c_double),
-                ("nupre", C.c_int32), ("nupost", C.c_int32), ("smoother", C.c_int32),
+                ("nupre", C.c_int32), ("nupost", C.c_int32), ("smoother", C.c_int32), ("prolong", C.c_int32),
                 ("nthreads", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32),
                 ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p), ("omega_fn", C.c_void_p),
                 ("omega_ctx", C.c_void_p)]
@@ -289,8 +300,9 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
         g = load_amg_lib()
         fn = C.cast(g.chr_setup_afsai, C.c_void_p).value if p.gpu_afsai else None
         ofn = C.cast(g.chr_setup_omega, C.c_void_p).value if p.gpu_omega else None
+    pr = {"clas": 0, "exti": 1}[p.prolong.lower()]
     cp = _CParams(p.nlev_max, p.max_coarse, p.soc_tau, p.nstep, p.step_size, p.fsai_eps, p.nupre,
-                  p.nupost, sm, p.nthreads, p.seed, int(p.verbose), fn, None, ofn, None)
+                  p.nupost, sm, pr, p.nthreads, p.seed, int(p.verbose), fn, None, ofn, None)
     lib = _lib()
     ha = _to_handle(A)
     hh = C.c_void_p()

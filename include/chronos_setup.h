@@ -16,6 +16,9 @@
  *                             + mark_isolated_nodes.m:1-8
  *   chs_prol_classical        sources/Preconditioner/AMG/prolong/MEX_CLAS_prol/
  *                             Classical_prolongation.cpp:40-147, ProlStripe_Classical.cpp:19-175
+ *   chs_prol_exti             sources/Preconditioner/AMG/prolong/MEX_EXTI_Prol/
+ *                             EXTI_prolongation.cpp:52-199, ONELP_ProlStripe_EXTI.cpp:27-293
+ *                             (called by cpt_ProlMEX_EXTI.m:1-38 from prolong.m:84-87)
  *   chs_afsai                 sources/Preconditioner/AMG/smoother/MEX_SYM_AFSAI/
  *                             compute_local_fsai.cpp:6-116, cpt_afsai_coef.cpp:316-554
  *   chs_fsai_omega            sources/Preconditioner/AMG/smoother/smoother.m:72-81
@@ -71,6 +74,7 @@ typedef struct chs_params {
   int32_t nupre;        /* parm_SMOOTH line 2 */
   int32_t nupost;       /* parm_SMOOTH line 3 */
   int32_t smoother;     /* 0 = afsai_sym, 1 = jacobi */
+  int32_t prolong;      /* parm_PROLONG proltype: 0 = CLAS, 1 = EXTI */
   int32_t nthreads;     /* OpenMP threads (<=0: runtime default) */
   uint64_t seed;        /* PMIS tie-break stream and Lanczos start vector */
   int32_t verbose;
@@ -122,6 +126,9 @@ int chs_pmis(const chs_mat *A, const int8_t *strong, const double *rand01,
 /* Classical direct interpolation with strong-fine correction. */
 int chs_prol_classical(const chs_mat *A, const int8_t *strong, const int32_t *fcnode,
                        int32_t nc, int nthreads, chs_mat **P);
+/* Extended+i distance-two interpolation (the reference's EXTI prolongation). */
+int chs_prol_exti(const chs_mat *A, const int8_t *strong, const int32_t *fcnode, int32_t nc,
+                  int nthreads, chs_mat **P);
 /* Adaptive symmetric FSAI (lower-triangular G, diagonal last in every row). */
 int chs_afsai(const chs_mat *A, int32_t nstep, int32_t step_size, double eps,
               int nthreads, chs_mat **G, int32_t *is_posdef);

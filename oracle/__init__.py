@@ -16,6 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libref_clas.so")
+REF_EXTI_LIB = os.path.join(HERE, "_ref", "libref_exti.so")
 
 _I32P = C.POINTER(C.c_int32)
 _F64P = C.POINTER(C.c_double)
@@ -33,6 +34,7 @@ class OrcLevel(C.Structure):
 
 _lib = None
 _ref = None
+_ref_exti = None
 
 
 def build(reference: bool = True) -> None:
@@ -208,4 +210,36 @@ def ref_classical_prol(A, strong: np.ndarray, fcnode: np.ndarray, nc: int, nthre
     ja = np.ctypeslib.as_array(jp, shape=(max(nt.value, 1),))[: nt.value].copy()
     co = np.ctypeslib.as_array(cp, shape=(max(nt.value, 1),))[: nt.value].copy()
     _ref.ref_free(ip, jp, cp)
+    return iat, ja, co
+
+
+# The reference's own extended+i prolongation core (oracle/_ref/libref_exti.so), when built.
+def ref_exti_available() -> bool:
+    return os.path.exists(REF_EXTI_LIB)
+
+
+def ref_exti_prol(A, strong: np.ndarray, fcnode: np.ndarray, nc: int, nthreads: int = 1):
+    global _ref_exti
+    if _ref_exti is None:
+        R = C.CDLL(REF_EXTI_LIB)
+        R.ref_exti_prol.argtypes = [C.c_int, C.c_int, _I32P, _I32P, _F64P, _I32P, _I32P, C.c_int,
+                                    _I32P, C.POINTER(_I32P), C.POINTER(_I32P), C.POINTER(_F64P)]
+        R.ref_exti_free.argtypes = [_I32P, _I32P, _F64P]
+        _ref_exti = R
+    rp = np.ascontiguousarray(A.rowptr, np.int32)
+    ci = np.ascontiguousarray(A.col, np.int32)
+    v = np.ascontiguousarray(A.val, np.float64)
+    s = np.ascontiguousarray(strong, np.int32)
+    fc = np.ascontiguousarray(fcnode, np.int32)
+    nt = C.c_int32()
+    ip, jp, cp = _I32P(), _I32P(), _F64P()
+    rc = _ref_exti.ref_exti_prol(nthreads, A.nrows, _p(rp, _I32P), _p(ci, _I32P), _p(v, _F64P), _p(s, _I32P),
+                                 _p(fc, _I32P), nc, C.byref(nt), C.byref(ip), C.byref(jp), C.byref(cp))
+    if rc != 0:
+        raise RuntimeError(f"reference EXTI prolongation failed: {rc}")
+    n = A.nrows
+    iat = np.ctypeslib.as_array(ip, shape=(n + 1,)).copy()
+    ja = np.ctypeslib.as_array(jp, shape=(max(nt.value, 1),))[: nt.value].copy()
+    co = np.ctypeslib.as_array(cp, shape=(max(nt.value, 1),))[: nt.value].copy()
+    _ref_exti.ref_exti_free(ip, jp, cp)
     return iat, ja, co

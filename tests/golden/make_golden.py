@@ -1,16 +1,19 @@
-"""Writes the golden fixtures of the classical interpolation from the reference's own core.
+"""Writes the golden fixtures of the classical and extended+i interpolations from the
+reference's own cores, and the reference test matrix the known-answer test runs on.
 
-Run in a container that holds the reference tree (oracle/_ref/libref_clas.so is built from
-it by `make -C oracle ref`):
+Run in a container that holds the reference tree (oracle/_ref/libref_clas.so and
+libref_exti.so are built from it by `make -C oracle ref`):
 
     python tests/golden/make_golden.py
 
-Each fixture stores an input (A in CSR, the strong-connection signs aligned with A, the C/F
-split) and the prolongation the reference's Classical_prolongation core
-(sources/Preconditioner/AMG/prolong/MEX_CLAS_prol/Classical_prolongation.cpp:40-147)
-returns for it.  Inputs: a 7-point Poisson cube, a heterogeneous-diffusion cube, and the
-leading 1200 x 1200 block of the reference test matrix examples/matrices/Flux.mat (the
-FlowCLAS_* tests' matrix, read as data with scipy.io.loadmat).
+Each clas_* / exti_* fixture stores an input (A in CSR, the strong-connection signs aligned
+with A, the C/F split) and the prolongation the reference's Classical_prolongation core
+(sources/Preconditioner/AMG/prolong/MEX_CLAS_prol/Classical_prolongation.cpp:40-147) or
+EXTI_prolongation core (MEX_EXTI_Prol/EXTI_prolongation.cpp:52-199) returns for it.
+Inputs: a 7-point Poisson cube, a heterogeneous-diffusion cube, and the leading 1200 x 1200
+block of the reference test matrix examples/matrices/Flux.mat (the FlowCLAS_* tests'
+matrix, read as data with scipy.io.loadmat).  flux.npz holds that whole matrix (Amat, as
+CSR) and its right-hand side b: the data of Tests/FlowCLAS_0.m.
 """
 import os
 import sys
@@ -40,8 +43,8 @@ def inputs():
 
 
 def main():
-    if not oracle.ref_available():
-        raise SystemExit("oracle/_ref/libref_clas.so is not built (make -C oracle ref)")
+    if not (oracle.ref_available() and oracle.ref_exti_available()):
+        raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
     for name, A, tau in inputs():
         strong = H.soc_cla(A, tau)
         r01 = H.rng_uniform(99, 100, A.nrows)
@@ -49,7 +52,22 @@ def main():
         iat, ja, co = oracle.ref_classical_prol(A, strong, fc, nc, nthreads=1)
         np.savez_compressed(os.path.join(HERE, f"clas_{name}.npz"), rowptr=A.rowptr, col=A.col, val=A.val,
                             strong=strong, fcnode=fc, nc=nc, P_rowptr=iat, P_col=ja, P_val=co)
-        print(name, A.nrows, A.nnz, nc, len(ja))
+        print("clas", name, A.nrows, A.nnz, nc, len(ja))
+        iat, ja, co = oracle.ref_exti_prol(A, strong, fc, nc, nthreads=1)
+        np.savez_compressed(os.path.join(HERE, f"exti_{name}.npz"), rowptr=A.rowptr, col=A.col, val=A.val,
+                            strong=strong, fcnode=fc, nc=nc, P_rowptr=iat, P_col=ja, P_val=co)
+        print("exti", name, A.nrows, A.nnz, nc, len(ja))
+    flux = os.path.join(REF, "examples", "matrices", "Flux.mat")
+    if os.path.exists(flux):
+        import scipy.io
+
+        d = scipy.io.loadmat(flux)
+        M = d["Amat"].tocsr()
+        M.sort_indices()
+        np.savez_compressed(os.path.join(HERE, "flux.npz"), rowptr=M.indptr.astype(np.int32),
+                            col=M.indices.astype(np.int32), val=M.data.astype(np.float64),
+                            b=d["b"].ravel().astype(np.float64))
+        print("flux", M.shape, M.nnz)
 
 
 if __name__ == "__main__":

@@ -1,6 +1,7 @@
-"""Host setup (hierarchy producer): generators, SoC, PMIS, classical interpolation, aFSAI.
+"""Host setup (hierarchy producer): generators, SoC, PMIS, classical and extended+i
+interpolation, aFSAI.
 
-Pins: the classical interpolation against the reference's own compiled core (golden
+Pins: both interpolations against the reference's own compiled cores (golden
 fixtures in tests/golden, and oracle/_ref live where it is built); aFSAI against its closed
 forms ((G A)_ij = 0 on the pattern, diag(G A G^T) = 1) and a dense brute-force replay of the
 greedy pattern growth; PMIS against the independent/dominating-set properties.
@@ -157,6 +158,54 @@ def test_classical_prolongation_invariants():
     g = np.arange(N ** 3)
     x, y, z = g % N, (g // N) % N, g // (N * N)
     interior = (x > 1) & (x < N - 2) & (y > 1) & (y < N - 2) & (z > 1) & (z < N - 2) & (fc < 0)
+    np.testing.assert_allclose(P[interior].sum(axis=1), 1.0, rtol=1e-13)
+
+
+# ------------------------------------------------------------- extended+i interpolation
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "exti_*.npz"))))
+def test_exti_prolongation_golden(path):
+    d = np.load(path)
+    n = d["rowptr"].shape[0] - 1
+    A = H.Csr(n, n, d["rowptr"], d["col"], d["val"])
+    P = H.prol_exti(A, d["strong"], d["fcnode"], int(d["nc"]), nthreads=3)
+    np.testing.assert_array_equal(P.rowptr, d["P_rowptr"])
+    np.testing.assert_array_equal(P.col, d["P_col"])
+    np.testing.assert_array_equal(P.val, d["P_val"])  # bit for bit
+
+
+def test_exti_golden_fixtures_present():
+    assert len(glob.glob(os.path.join(GOLDEN, "exti_*.npz"))) >= 3
+
+
+@pytest.mark.skipif(not oracle.ref_exti_available(), reason="oracle/_ref not built here")
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_exti_prolongation_vs_reference_core(seed):
+    A = H.heterogeneous_diffusion(12, 1.5 + seed / 2, seed)[0]
+    s = H.soc_cla(A, 0.25)
+    fc, nc = H.pmis(A, s, H.rng_uniform(seed, 100, A.nrows))
+    P = H.prol_exti(A, s, fc, nc)
+    iat, ja, co = oracle.ref_exti_prol(A, s, fc, nc, nthreads=2)
+    np.testing.assert_array_equal(P.rowptr, iat)
+    np.testing.assert_array_equal(P.col, ja)
+    np.testing.assert_array_equal(P.val, co)
+
+
+def test_exti_prolongation_invariants():
+    # Coarse rows are unit rows; interior fine rows of a zero-row-sum operator interpolate
+    # constants exactly (the extended+i weights and denominator add up to the row sum).
+    N = 8
+    A = H.poisson7(N)
+    s = H.soc_cla(A, 0.25)
+    fc, nc = H.pmis(A, s, H.rng_uniform(4, 100, A.nrows))
+    P = H.prol_exti(A, s, fc, nc).to_scipy().toarray()
+    for i in np.where(fc >= 0)[0]:
+        e = np.zeros(nc)
+        e[fc[i]] = 1
+        np.testing.assert_array_equal(P[i], e)
+    g = np.arange(N ** 3)
+    x, y, z = g % N, (g // N) % N, g // (N * N)
+    interior = (x > 1) & (x < N - 2) & (y > 1) & (y < N - 2) & (z > 1) & (z < N - 2) & (fc < 0)
+    assert interior.any()
     np.testing.assert_allclose(P[interior].sum(axis=1), 1.0, rtol=1e-13)
 
 
