@@ -325,3 +325,105 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
         lib.chs_hier_free(hh)
     # The finest level's A is the operator the caller passed (possibly BC-modified).
     return Hierarchy(levels, secs)
+
+
+# ---------------------------------------------------------------------------------------
+# On-disk format: the reference's Ext_bin binary files (little-endian).
+#   matrix (sources/utils/debug_stuff/dump_chronos/DumpChronos.cpp:27-35, read by
+#   read_ExtBinMat.m:1-36): int64 nrows, [int64 ncols when rectangular], int64 nterm, then
+#   nterm records {int64 i, int64 j, float64 v} with 1-based i, j;
+#   vector (read_ExtBinVec.m:1-21): int64 N, int64 M, then N*M float64, row by row.
+# A hierarchy is a directory: A_<l>.Ext_bin for every level, P_<l>.Ext_bin (rectangular
+# header) and G_<l>.Ext_bin for every level but the last, and hierarchy.txt with one line
+# per level "l n omega nupre nupost" (omega written with 17 significant digits, exact).
+# ---------------------------------------------------------------------------------------
+_EXT_REC = np.dtype([("i", "<i8"), ("j", "<i8"), ("v", "<f8")])
+
+
+def write_ext_bin_mat(path: str, M: Csr, rect: bool = False) -> None:
+    rows = np.repeat(np.arange(M.nrows, dtype=np.int64), np.diff(M.rowptr.astype(np.int64)))
+    rec = np.empty(M.nnz, _EXT_REC)
+    rec["i"] = rows + 1
+    rec["j"] = M.col.astype(np.int64) + 1
+    rec["v"] = M.val
+    head = [M.nrows, M.ncols, M.nnz] if rect else [M.nrows, M.nnz]
+    if not rect and M.nrows != M.ncols:
+        raise ValueError("a rectangular matrix needs rect=True")
+    with open(path, "wb") as f:
+        np.asarray(head, "<i8").tofile(f)
+        rec.tofile(f)
+
+
+def read_ext_bin_mat(path: str, rect: bool = False) -> Csr:
+    with open(path, "rb") as f:
+        nh = 3 if rect else 2
+        head = np.fromfile(f, "<i8", count=nh)
+        if head.size != nh:
+            raise ValueError(f"{path}: truncated header")
+        nrows = int(head[0])
+        ncols = int(head[1]) if rect else nrows
+        nterm = int(head[-1])
+        rec = np.fromfile(f, _EXT_REC, count=nterm)
+    if rec.size != nterm:
+        raise ValueError(f"{path}: {rec.size} of {nterm} entries")
+    i, j = rec["i"] - 1, rec["j"] - 1
+    if nterm and (i.min() < 0 or i.max() >= nrows or j.min() < 0 or j.max() >= ncols):
+        raise ValueError(f"{path}: index out of range")
+    order = np.lexsort((j, i))
+    i, j, v = i[order], j[order], rec["v"][order]
+    if nterm and np.any((np.diff(i) == 0) & (np.diff(j) == 0)):
+        raise ValueError(f"{path}: duplicate entries")
+    rp = np.zeros(nrows + 1, np.int64)
+    np.cumsum(np.bincount(i, minlength=nrows), out=rp[1:])
+    return Csr(nrows, ncols, rp.astype(np.int32), j.astype(np.int32), v.astype(np.float64))
+
+
+def write_ext_bin_vec(path: str, X: np.ndarray) -> None:
+    X = np.asarray(X, np.float64)
+    X2 = X.reshape(X.shape[0], -1)
+    with open(path, "wb") as f:
+        np.asarray([X2.shape[0], X2.shape[1]], "<i8").tofile(f)
+        np.ascontiguousarray(X2, "<f8").tofile(f)
+
+
+def read_ext_bin_vec(path: str) -> np.ndarray:
+    with open(path, "rb") as f:
+        n, m = (int(t) for t in np.fromfile(f, "<i8", count=2))
+        data = np.fromfile(f, "<f8", count=n * m)
+    if data.size != n * m:
+        raise ValueError(f"{path}: truncated")
+    return data.reshape(n, m) if m != 1 else data
+
+
+def save_hierarchy(directory: str, hier: Hierarchy) -> None:
+    """Writes every level of hier in the Ext_bin format (see above)."""
+    os.makedirs(directory, exist_ok=True)
+    lines = []
+    for l, L in enumerate(hier.levels):
+        write_ext_bin_mat(os.path.join(directory, f"A_{l}.Ext_bin"), L.A)
+        if L.P is not None:
+            write_ext_bin_mat(os.path.join(directory, f"P_{l}.Ext_bin"), L.P, rect=True)
+            write_ext_bin_mat(os.path.join(directory, f"G_{l}.Ext_bin"), L.G)
+        lines.append(f"{l} {L.A.nrows} {L.omega!r} {L.nupre} {L.nupost}")
+    with open(os.path.join(directory, "hierarchy.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def load_hierarchy(directory: str) -> Hierarchy:
+    """Reads a hierarchy written by save_hierarchy (values and patterns bit for bit)."""
+    with open(os.path.join(directory, "hierarchy.txt")) as f:
+        rows = [ln.split() for ln in f if ln.strip()]
+    levels = []
+    for k, (l, n, omega, nupre, nupost) in enumerate(rows):
+        if int(l) != k:
+            raise ValueError("hierarchy.txt: levels out of order")
+        A = read_ext_bin_mat(os.path.join(directory, f"A_{k}.Ext_bin"))
+        if A.nrows != int(n):
+            raise ValueError(f"level {k}: A has {A.nrows} rows, manifest says {n}")
+        if k < len(rows) - 1:
+            P = read_ext_bin_mat(os.path.join(directory, f"P_{k}.Ext_bin"), rect=True)
+            G = read_ext_bin_mat(os.path.join(directory, f"G_{k}.Ext_bin"))
+            levels.append(Level(A, P, G, float(omega), int(nupre), int(nupost)))
+        else:
+            levels.append(Level(A, omega=float(omega), nupre=int(nupre), nupost=int(nupost)))
+    return Hierarchy(levels, {})
