@@ -303,6 +303,8 @@ def main() -> int:
     tv = "float" if main_prec == "fp32" else "double"
     if g_layout[0] == "sell":
         kname = f"spmv_sell_kernel<{g_layout[1]}, {tv}, {tv}, EpiProbe<{tv}, 'G'>>"
+    elif isinstance(g_layout[0], tuple):
+        kname = f"spmv_pack_kernel<{g_layout[0][1]}, {tv}, {tv}, EpiProbe<{tv}, 'G'>>"
     else:
         kname = f"spmv_stream_kernel<{g_layout[0]}, {tv}, ..., EpiProbe<{tv}, 'G'>>"
     out = {

@@ -145,6 +145,7 @@ class AMGPreconditioner:
     def layout(self, stored_bytes: bool = False) -> list:
         """Per level, the layout chosen for the single-vector products of A, G, G^T, P, P^T:
         ("sell", code bits) for the sliced-ELL kernel, (PER, code bits) for the stream kernel,
+        (("pack", PER), 16) for the packed-code stream kernel,
         with the bytes one product reads of the stored matrix appended when stored_bytes."""
         out = []
         for l in range(len(self.hier.levels) - 1):
@@ -153,7 +154,8 @@ class AMGPreconditioner:
                 per, bits, nnz, sb = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
                 _check(_lib().chr_hier_layout(self.ptr, l, w.encode(), C.byref(per), C.byref(bits),
                                               C.byref(nnz), C.byref(sb)))
-                row[w] = ("sell" if per.value == 0 else per.value, bits.value) + ((sb.value,) if stored_bytes else ())
+                kind = "sell" if per.value == 0 else ("pack", -per.value) if per.value < 0 else per.value
+                row[w] = (kind, bits.value) + ((sb.value,) if stored_bytes else ())
             out.append(row)
         return out
 

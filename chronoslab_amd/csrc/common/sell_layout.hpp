@@ -4,7 +4,8 @@
 //
 // Layout:
 //   * slots are taken in slices of 64 (one wave, one row per lane); slice s stores element
-//     (k, lane) at sp[s] + 64 k + lane, k < len_s = the longest row of the slice;
+//     (k, lane) at sp[s] + 64 k + lane, k < len_s = the longest row of the slice, or of the
+//     whole matrix when `ulen` > 0 (uniform ELL: sp[s] = 64 ulen s, unsorted);
 //   * dictionary windows of 2^wshift slots (256 <= 2^wshift <= 16384); when `sorted`, the
 //     rows of each sort window of 2^sshift slots (256 <= 2^sshift <= 2^wshift) sit in its
 //     slots by decreasing length (stable) and perm[slot] is the row's offset from the sort
@@ -30,6 +31,7 @@ struct Layout {
   int cb = 0;  // 0: the matrix is not a candidate
   int32_t nrows = 0, nslices = 0, wshift = 8, sshift = 8, dmin = 0;
   int unroll = 8;  // kernel chunk: 16 when some slice is longer than 8 elements
+  int ulen = 0;    // > 0: uniform ELL (unsorted, every slice ulen elements long)
   int64_t elems = 0;
   bool sorted = false;
   std::vector<int32_t> sp, dict;
@@ -153,7 +155,11 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
   else if (sort_shift > 0) pick = 1;
   const bool sorted = pick > 0;
   const int sshift = sorted ? sshifts[(size_t)pick - 1] : ws;
-  const int64_t elems = sorted ? e_sorted_at[(size_t)pick - 1] : e_ident;
+  // Unsorted with every slice padded to the longest row (uniform ELL) when that costs at
+  // most 2% more elements: the kernel then computes slice offsets instead of reading them.
+  const int64_t e_uniform = (int64_t)nslices * maxlen * kSlice;
+  const bool uniform = !sorted && (double)e_uniform <= 1.02 * (double)e_ident;
+  const int64_t elems = sorted ? e_sorted_at[(size_t)pick - 1] : uniform ? e_uniform : e_ident;
   if (elems > (int64_t)INT32_MAX - kSlice || elems > nnz + nnz / 2) {
     L = Layout();
     L.nrows = n;
@@ -164,6 +170,7 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
   L.nslices = nslices;
   L.wshift = ws;
   L.sshift = sshift;
+  L.ulen = uniform ? maxlen : 0;
   L.dmin = cb == 16 ? (int32_t)dlo : 0;
   L.elems = elems;
   L.sorted = sorted;
@@ -187,6 +194,7 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
     }
     int32_t mx = 0;
     for (int l = 0; l < kSlice; ++l) mx = std::max(mx, rlen(map[(size_t)(s * kSlice + l)]));
+    if (uniform) mx = maxlen;
     for (int l = 0; l < kSlice; ++l) {
       const int32_t r = map[(size_t)(s * kSlice + l)];
       if (r < 0) continue;

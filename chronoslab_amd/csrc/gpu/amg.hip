@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "runtime.hpp"
+#include "pack_layout.hpp"
 #include "sell_layout.hpp"
 
 namespace chrrt {
@@ -72,9 +73,27 @@ int DevSell::build(int32_t n, int32_t nc, const int32_t *hrp, const int32_t *hci
   wshift = L.wshift;
   sshift = L.sshift;
   unroll = L.unroll;
+  ulen = L.ulen;
   dmin = L.dmin;
   elems = L.elems;
   sorted = L.sorted;
+  return CHR_OK;
+}
+
+int DevPack::build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *hci, int per_) {
+  release();
+  chrpack::Layout L;
+  if (!chrpack::build(n, ncols, hrp, hci, per_, L)) return CHR_OK;
+  int rc;
+  if ((rc = blk.alloc(sizeof(int32_t) * L.blk.size())) || (rc = code.alloc(sizeof(uint16_t) * L.code.size())) ||
+      (rc = dict.alloc(sizeof(int32_t) * L.dict.size())))
+    return rc;
+  HIPC(hipMemcpy(blk.p, L.blk.data(), sizeof(int32_t) * L.blk.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(code.p, L.code.data(), sizeof(uint16_t) * L.code.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dict.p, L.dict.data(), sizeof(int32_t) * L.dict.size(), hipMemcpyHostToDevice));
+  per = per_;
+  nblk = L.nblk;
+  built = true;
   return CHR_OK;
 }
 
@@ -82,7 +101,9 @@ int DevSell::build(int32_t n, int32_t nc, const int32_t *hrp, const int32_t *hci
 //  1. the CSR-stream layout with PER in {4, 8, 16} nonzeros per thread (tune_per = false
 //     keeps PER 8: the outer operator of PCG, whose fused dot products are summed over the
 //     stream row blocks, so that the solver's rounding never depends on a timing);
-//  2. the sliced-ELL layout, when the matrix admits it, unsorted, sorted in 256-row
+//  2. the packed-code stream layout (16-bit codes: row offset in the block and a window
+//     dictionary index), with the PER of step 1, when the matrix admits it;
+//  3. the sliced-ELL layout, when the matrix admits it, unsorted, sorted in 256-row
 //     windows, and with the builder's own choice of sort window.
 // Each candidate is timed with HIP events, 2 warm-up and 5 timed launches; a matrix of 32 MB
 // or more is timed cache-cold (a 512 MiB device copy before every timed launch, as the
@@ -92,12 +113,17 @@ int DevSell::build(int32_t n, int32_t nc, const int32_t *hrp, const int32_t *hci
 // changes a product.
 // Environment: CHRONOS_STREAM_PER=4|8|16 fixes PER (tune_per only); CHRONOS_SELL=0
 // excludes and CHRONOS_SELL=1 forces the sliced-ELL layout where the matrix admits it;
-// CHRONOS_SELL_SORT=0|8|10|12|14 fixes its sort window (0: unsorted).
+// CHRONOS_SELL_SORT=0|8|10|12|14 fixes its sort window (0: unsorted); CHRONOS_PACK=0
+// excludes and CHRONOS_PACK=1 forces the packed-code layout (when sliced ELL is not forced).
 int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
                 const int *zero, bool tune_per, const DevBuf *flush) {
   M.use_sell = false;
   M.sell.release();
+  M.use_pack = false;
+  M.pack.release();
   if (M.nnz == 0) return CHR_OK;
+  const char *env_pack = std::getenv("CHRONOS_PACK");
+  const int pack_mode = env_pack ? std::atoi(env_pack) : -1;  // -1 timed, 0 off, 1 forced
   const char *env_sell = std::getenv("CHRONOS_SELL");
   const int sell_mode = env_sell ? std::atoi(env_sell) : -1;  // -1 timed, 0 off, 1 forced
   const char *env_sort = std::getenv("CHRONOS_SELL_SORT");
@@ -174,8 +200,29 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
       if (best_per != M.per) rc = M.layout(best_per, hrp, hci);
     }
   }
-  // Step 2: sliced ELL.
-  if (!rc && sell_mode != 0) {
+  // Step 2: packed codes (the stream kernel's blocks with 2-byte codes).
+  if (!rc && pack_mode != 0 && sell_mode != 1 && !(rc = M.pack.build(M.nrows, M.ncols, hrp, hci, M.per)) &&
+      M.pack.built) {
+    if (pack_mode == 1) {
+      M.use_pack = true;
+    } else {
+      if (stream_ms < 0.f) rc = time_it(stream_ms);
+      float ms = 0.f;
+      if (!rc) {
+        M.use_pack = true;
+        rc = time_it(ms);
+        M.use_pack = false;
+      }
+      if (!rc && ms < 0.97f * stream_ms) {
+        M.use_pack = true;
+        stream_ms = ms;
+      } else {
+        M.pack.release();
+      }
+    }
+  }
+  // Step 3: sliced ELL.
+  if (!rc && sell_mode != 0 && !(pack_mode == 1 && M.use_pack)) {
     std::vector<int> sorts;
     if (env_sort) sorts.push_back(std::atoi(env_sort));
     else sorts = {-1, 0, 8};
@@ -204,6 +251,10 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     if (!rc && best_sort != -2) {
       if (best_sort != last_sort) rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32, best_sort);
       M.use_sell = !rc && M.sell.cb != 0;
+      if (M.use_sell) {
+        M.use_pack = false;
+        M.pack.release();
+      }
     } else {
       M.sell.release();
     }
@@ -211,6 +262,8 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
   if (rc) {
     M.use_sell = false;
     M.sell.release();
+    M.use_pack = false;
+    M.pack.release();
   }
   return rc;
 }
@@ -251,9 +304,11 @@ bool spd_inverse(int32_t n, std::vector<double> &a) {
   return true;
 }
 
-hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop, int k) {
+hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop, int k,
+                          const float *b0f) {
   if (h.fp32()) {
     Cycle<float> c{h, stop, h.stream, k};
+    c.b0f = b0f;
     return c.level<double, double>(0, b, out);
   }
   Cycle<double> c{h, stop, h.stream, k};
@@ -493,8 +548,8 @@ extern "C" int chr_hier_layout(const chr_hier *h, int32_t level, char which, int
   const DevCsr *M = which == 'A' ? &L.A : which == 'G' ? &L.G : which == 'T' ? &L.GT
                   : which == 'P' ? &L.P : which == 'R' ? &L.PT : nullptr;
   if (!M) return fail(CHR_EARG, "chr_hier_layout: which must be A, G, T, P or R");
-  if (per) *per = M->use_sell ? 0 : M->per;
-  if (code_bits) *code_bits = M->use_sell ? M->sell.cb : (M->idx16 ? 16 : 32);
+  if (per) *per = M->use_sell ? 0 : M->use_pack ? -M->pack.per : M->per;
+  if (code_bits) *code_bits = M->use_sell ? M->sell.cb : (M->use_pack || M->idx16) ? 16 : 32;
   if (nnz) *nnz = M->nnz;
   if (stored_bytes) *stored_bytes = M->stream_bytes();
   return CHR_OK;
@@ -631,7 +686,8 @@ hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity
   double *pold = parity ? w.p1.as<double>() : w.p0.as<double>();
   double *pnew = parity ? w.p0.as<double>() : w.p1.as<double>();
   hipError_t e;
-  if ((e = enqueue_vcycle(h, w.r.as<double>(), w.z.as<double>(), stop))) return e;
+  float *r32 = h.fp32() ? w.r32.as<float>() : nullptr;
+  if ((e = enqueue_vcycle(h, w.r.as<double>(), w.z.as<double>(), stop, 1, r32))) return e;
   double *part = w.part.as<double>(), *resvec = w.resvec.as<double>();
   hipLaunchKernelGGL(chr::pcg_zap_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
                      (const double *)w.z.as<double>(), (const double *)w.ap.as<double>(), part,
@@ -642,7 +698,7 @@ hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity
   if ((e = pcg_finalize(chr::kFinAlpha, A.nblk, w.part_p.as<double>(), st, resvec, h.stream))) return e;
   hipLaunchKernelGGL(chr::pcg_update_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
                      (const double *)pnew, (const double *)w.ap.as<double>(), w.x.as<double>(),
-                     w.r.as<double>(), part, (const chr::PcgState *)st);
+                     w.r.as<double>(), r32, part, (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
   if ((e = pcg_finalize(chr::kFinUpdate, (int)w.nblk, part, st, resvec, h.stream))) return e;
   return hipMemcpyAsync(&w.pinned[parity], st, sizeof(chr::PcgState), hipMemcpyDeviceToHost,
@@ -659,7 +715,8 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
     w.n = n;
     const size_t vb = 8 * (size_t)n;
     if ((rc = w.x.alloc(vb)) || (rc = w.r.alloc(vb)) || (rc = w.z.alloc(vb)) ||
-        (rc = w.p0.alloc(vb)) || (rc = w.p1.alloc(vb)) || (rc = w.ap.alloc(vb)))
+        (rc = w.p0.alloc(vb)) || (rc = w.p1.alloc(vb)) || (rc = w.ap.alloc(vb)) ||
+        (rc = w.r32.alloc(h.fp32() ? 4 * (size_t)n : 0)))
       return rc;
     // Fixed grid: enough blocks to fill 256 CUs several times, small enough for the
     // single-block finalisation of the partial sums.
@@ -746,6 +803,11 @@ extern "C" int chr_pcg_solve(chr_hier *h, chr_mat *Aop, const double *b, double 
   HIPC(hipMemcpyAsync(w.state.p, &st0, sizeof(st0), hipMemcpyHostToDevice, h->stream));
   HIPC(launch_spmv(A, w.x.as<double>(), chr::EpiResidual<double, double>{db, w.r.as<double>()},
                    h->zero.as<int>(), h->stream));
+  if (h->fp32()) {
+    hipLaunchKernelGGL((chr::convert_kernel<double, float>), dim3((n + chr::kBlock - 1) / chr::kBlock),
+                       dim3(chr::kBlock), 0, h->stream, n, (const double *)w.r.as<double>(), w.r32.as<float>());
+    HIPC(hipGetLastError());
+  }
   hipLaunchKernelGGL(chr::pcg_norms_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h->stream, n, db,
                      (const double *)w.r.as<double>(), w.part.as<double>());
   HIPC(hipGetLastError());

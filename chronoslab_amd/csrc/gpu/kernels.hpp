@@ -203,6 +203,74 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
 }
 
 // ---------------------------------------------------------------------------------------
+// CSR-stream SpMV on the packed-code layout (pack_layout.hpp): each nonzero's 16-bit code
+// holds its row's offset from the block's first row (high byte) and an index into the
+// window's dictionary of row-relative offsets (low byte), so col = r0 + (code >> 8) -
+// dict[code & 255].  Same block cut, staging, summation order and rounding as
+// spmv_stream_kernel; the block table's third word is the block's dictionary window, whose
+// 256 offsets are loaded into LDS beside the block's values and codes.
+// ---------------------------------------------------------------------------------------
+template <int PER, typename TV, typename TX, typename Epi>
+__global__ __launch_bounds__(kBlock) void spmv_pack_kernel(int nblk, const int4 *__restrict__ blk,
+                                                           const int *__restrict__ rp,
+                                                           const uint16_t *__restrict__ code,
+                                                           const TV *__restrict__ val,
+                                                           const int *__restrict__ dict,
+                                                           const TX *__restrict__ x, Epi epi,
+                                                           const int *__restrict__ stop) {
+  using TA = typename Acc<TV>::type;
+  __shared__ TA prod[PER * kBlock];
+  __shared__ int ld[256];
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 lo = blk[b], hi = blk[b + 1];
+  if (*stop) return;
+  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y;
+  const int tid = threadIdx.x;
+  ld[tid] = dict[lo.z * 256 + tid];
+  if (e1 - e0 > PER * kBlock) {  // one long row: strided sum + block reduction
+    __syncthreads();
+    TA s = 0;
+    for (int e = e0 + tid; e < e1; e += kBlock) {
+      const uint32_t c = code[e];
+      s += (TA)val[e] * (TA)x[r0 + (int)(c >> 8) - ld[c & 255u]];
+    }
+    s = block_sum_t<TA>(s, prod);
+    if (tid == 0) epi(r0, (double)s, epi.pre(r0));
+    return;
+  }
+  const int r = r0 + tid;
+  const int rc = min(r, r1 - 1);
+  const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
+  const double pv = epi.pre(rc);
+  if (e1 > e0) {
+    uint32_t c[PER];
+    TV v[PER];
+    TA xv[PER];
+    const int last = e1 - 1;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = min(e0 + k * kBlock + tid, last);
+      c[k] = code[e];
+      v[k] = val[e];
+    }
+    __syncthreads();  // dictionary in LDS
+#pragma unroll
+    for (int k = 0; k < PER; ++k) xv[k] = (TA)x[r0 + (int)(c[k] >> 8) - ld[c[k] & 255u]];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int o = k * kBlock + tid;
+      if (e0 + o < e1) prod[o] = (TA)v[k] * xv[k];
+    }
+  }
+  __syncthreads();
+  if (r < r1) {
+    TA s = 0;
+    for (int q = ra; q < rz; ++q) s += prod[q];
+    epi(r, (double)s, pv);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Sliced-ELL SpMV for square matrices with short rows (the smoother factors G, G^T and
 // the fine-level operators).  Rows are taken in slices of 64 slots, one wave per slice and
 // one row per lane; slice s stores its elements column by column, element (k, lane) at
@@ -241,9 +309,13 @@ struct SellCode<32> {
 
 // U = elements per chunk: every load of a chunk issues before its gathers; 16 when the
 // layout has slices longer than 8 (one round trip for the FSAI factors' rows of <= 16).
+// ulen > 0: every slice is ulen elements long (uniform ELL, unsorted), so slice offsets
+// are computed and the value/code loads need no table read first.  All loads that do not
+// depend on another (stop flag, slice offsets, row permutation, dictionary, and with ulen
+// the first chunk) are issued before the first wait.
 template <int CB, int U, typename TV, typename TX, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
-    int nslices, int nrows, int wshift, int sshift, const int *__restrict__ sp,
+    int nslices, int nrows, int wshift, int sshift, int ulen, const int *__restrict__ sp,
     const uint16_t *__restrict__ perm, const TV *__restrict__ val,
     const typename SellCode<CB>::type *__restrict__ code, const int *__restrict__ dict, int dmin,
     const TX *__restrict__ x, Epi epi, const int *__restrict__ stop) {
@@ -252,44 +324,65 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
   constexpr int kSlices = kBlock / kWave;
   __shared__ int ld[CB == 8 ? 256 : 1];
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  if (*stop) return;
   const int s = b * kSlices + threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int sc = min(s, nslices - 1);  // clamped: every load below stays in bounds
+  const int slot = sc * kWave + lane;
+  int e0, e1;
+  if (ulen > 0) {
+    e0 = sc * ulen * kWave;
+    e1 = e0 + ulen * kWave;
+  } else {
+    e0 = sp[sc];
+    e1 = sp[sc + 1];
+  }
+  const int q = perm ? (int)perm[slot] : 0;
+  int dv = 0;
+  if constexpr (CB == 8) dv = dict[(((b * kSlices) * kWave) >> wshift) * 256 + threadIdx.x];
+  const int base = e0 + lane, len = (e1 - e0) / kWave;
+  // First chunk's values and codes (clamped to the row's last element; an empty slice
+  // reads element 0, which every layout has, and uses none of it).
+  TV v[U];
+  CT c[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = len > 0 ? base + min(u, len - 1) * kWave : 0;
+    v[u] = val[e];
+    c[u] = code[e];
+  }
+  if (*stop) return;
   if constexpr (CB == 8) {
     // The block's slices lie in one window (windows hold a multiple of kBlock slots).
-    ld[threadIdx.x] = dict[(((b * kSlices) * kWave) >> wshift) * 256 + threadIdx.x];
+    ld[threadIdx.x] = dv;
     __syncthreads();
   }
   if (s >= nslices) return;
-  const int slot = s * kWave + lane;
   int row = slot;
   bool valid = slot < nrows;
   if (perm) {
-    const int q = perm[slot];
     valid = q != 0xFFFF;
     row = ((slot >> sshift) << sshift) + q;
   }
   const int rc = valid ? row : 0;  // PAD elements read x[rc] and drop the product
   const double pv = epi.pre(rc);
-  const int base = sp[s] + lane, len = (sp[s + 1] - sp[s]) / kWave;
-  auto col_of = [&](CT c) -> int {
-    if constexpr (CB == 8) return rc - ld[c];
-    else if constexpr (CB == 16) return rc - (c == (CT)SellCode<16>::pad ? 0 : (int)c + dmin);
-    else return c < 0 ? rc : (int)c;
+  auto col_of = [&](CT cc) -> int {
+    if constexpr (CB == 8) return rc - ld[cc];
+    else if constexpr (CB == 16) return rc - (cc == (CT)SellCode<16>::pad ? 0 : (int)cc + dmin);
+    else return cc < 0 ? rc : (int)cc;
   };
-  // Chunks of U elements; the last chunk clamps its loads to the row's last
-  // element and drops the surplus products, so every load of a chunk issues at once.
+  // Chunks of U elements; the last chunk clamps its loads to the row's last element and
+  // drops the surplus products.
   TA acc = 0;
   const int nch = (len + U - 1) / U;
   for (int ch = 0; ch < nch; ++ch) {
-    TV v[U];
-    CT c[U];
-    TA xv[U];
+    if (ch > 0) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = base + min(ch * U + u, len - 1) * kWave;
-      v[u] = val[e];
-      c[u] = code[e];
+      for (int u = 0; u < U; ++u) {
+        const int e = base + min(ch * U + u, len - 1) * kWave;
+        v[u] = val[e];
+        c[u] = code[e];
+      }
     }
+    TA xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xv[u] = (TA)x[col_of(c[u])];
 #pragma unroll
@@ -560,11 +653,12 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
   }
 }
 
-// x += alpha p ; r -= alpha Ap ; partials of r.r.
+// x += alpha p ; r -= alpha Ap ; partials of r.r; r32 (when not null) = (float) r.
 static __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, const double *__restrict__ p,
                                                             const double *__restrict__ ap,
                                                             double *__restrict__ x,
                                                             double *__restrict__ r,
+                                                            float *__restrict__ r32,
                                                             double *__restrict__ part,
                                                             const PcgState *st) {
   if (st->stop) return;
@@ -575,6 +669,7 @@ static __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, const 
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * ap[i];
     r[i] = ri;
+    if (r32) r32[i] = (float)ri;
     rr += ri * ri;
   }
   rr = block_sum(rr, lds);

@@ -121,6 +121,7 @@ struct DevSell {
   int64_t elems = 0;  // stored elements, padding included
   bool sorted = false;
   int unroll = 8;     // elements per chunk of the kernel (16 when a slice is longer than 8)
+  int ulen = 0;       // > 0: uniform ELL, every slice ulen elements long
   DevBuf sp, perm, val, code, dict;
   // Builds the layout from host CSR (chrsell::build) and uploads it; leaves cb = 0 (and
   // returns CHR_OK) when the matrix is not a candidate.
@@ -140,6 +141,23 @@ struct DevSell {
   }
 };
 
+// Packed-code index copy for single-vector stream products (spmv_pack_kernel): block table
+// with dictionary windows, one 16-bit code per nonzero, per-window dictionaries.  Values
+// and row pointers are the DevCsr's own (same nonzero order).
+struct DevPack {
+  int per = 8;
+  int32_t nblk = 0;
+  bool built = false;
+  DevBuf blk, code, dict;
+  int build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *hci, int per_);
+  void release() {
+    blk = DevBuf();
+    code = DevBuf();
+    dict = DevBuf();
+    built = false;
+  }
+};
+
 // Device CSR in FP32 or FP64, laid out for the stream kernel.
 struct DevCsr {
   int32_t nrows = 0, ncols = 0;
@@ -152,6 +170,8 @@ struct DevCsr {
   DevBuf ci;           // uint16 offsets when idx16, else int32 columns
   DevSell sell;        // sliced-ELL copy, used for single-vector products when use_sell
   bool use_sell = false;
+  DevPack pack;        // packed-code index copy, used for single-vector products when use_pack
+  bool use_pack = false;
   int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
              bool as_fp32, int per_ = 8) {
     nrows = nr;
@@ -228,13 +248,16 @@ struct DevCsr {
   // block table (stream layout) or slice offsets, permutation and dictionaries (sliced ELL).
   double stream_bytes() const {
     if (use_sell) return sell.bytes(fp32);
+    if (use_pack)
+      return (double)nnz * ((fp32 ? 4 : 8) + 2) + 4.0 * (nrows + 1) + 16.0 * (pack.nblk + 1) + (double)pack.dict.bytes;
     return (double)nnz * ((fp32 ? 4 : 8) + (idx16 ? 2 : 4)) + 4.0 * (nrows + 1) + 16.0 * (nblk + 1);
   }
   // The same with 4-byte column indices and no block table (the reference's CSR format).
   double csr_bytes() const { return (double)nnz * ((fp32 ? 4 : 8) + 4) + 4.0 * (nrows + 1); }
   double device_bytes() const {
     return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes + sell.sp.bytes + sell.perm.bytes +
-                    sell.val.bytes + sell.code.bytes + sell.dict.bytes);
+                    sell.val.bytes + sell.code.bytes + sell.dict.bytes + pack.blk.bytes + pack.code.bytes +
+                    pack.dict.bytes);
   }
 };
 
@@ -242,7 +265,7 @@ template <int CB, int U, typename TV, typename TX, typename Epi>
 hipError_t launch_sell_cb(const DevSell &S, const TX *x, Epi epi, const int *stop, hipStream_t s) {
   const unsigned grid = (unsigned)((S.nslices + chr::kBlock / chr::kWave - 1) / (chr::kBlock / chr::kWave));
   hipLaunchKernelGGL((chr::spmv_sell_kernel<CB, U, TV, TX, Epi>), dim3(grid), dim3(chr::kBlock), 0, s,
-                     S.nslices, S.nrows, S.wshift, S.sshift, S.sp.as<int>(),
+                     S.nslices, S.nrows, S.wshift, S.sshift, S.ulen, S.sp.as<int>(),
                      S.sorted ? S.perm.as<uint16_t>() : nullptr, S.val.as<TV>(),
                      S.code.as<typename chr::SellCode<CB>::type>(), S.dict.as<int>(), S.dmin, x, epi,
                      stop);
@@ -264,6 +287,14 @@ hipError_t launch_sell(const DevSell &S, const TX *x, Epi epi, const int *stop, 
 }
 
 template <int PER, typename TV, typename TX, typename Epi>
+hipError_t launch_pack_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  hipLaunchKernelGGL((chr::spmv_pack_kernel<PER, TV, TX, Epi>), dim3(M.pack.nblk), dim3(chr::kBlock), 0, s,
+                     M.pack.nblk, M.pack.blk.as<int4>(), M.rp.as<int>(), M.pack.code.as<uint16_t>(),
+                     M.val.as<TV>(), M.pack.dict.as<int>(), x, epi, stop);
+  return hipGetLastError();
+}
+
+template <int PER, typename TV, typename TX, typename Epi>
 hipError_t launch_spmv_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
   if (M.idx16)
     hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, uint16_t, TX, Epi>), dim3(M.nblk),
@@ -281,6 +312,13 @@ hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
                          hipStream_t s) {
   if (M.nrows == 0) return hipSuccess;
   if (M.use_sell) return launch_sell<TV>(M.sell, x, epi, stop, s);
+  if (M.use_pack) {
+    switch (M.pack.per) {
+      case 4: return launch_pack_p<4, TV>(M, x, epi, stop, s);
+      case 16: return launch_pack_p<16, TV>(M, x, epi, stop, s);
+      default: return launch_pack_p<8, TV>(M, x, epi, stop, s);
+    }
+  }
   switch (M.per) {
     case 4: return launch_spmv_p<4, TV>(M, x, epi, stop, s);
     case 16: return launch_spmv_p<16, TV>(M, x, epi, stop, s);
@@ -356,6 +394,7 @@ struct chr_mat {
 struct PcgWork {
   int32_t n = 0, itmax_cap = 0;
   DevBuf x, r, z, p0, p1, ap, part, part_p, state, resvec;
+  DevBuf r32;  // FP32 copy of r (FP32 cycle): the V-cycle's first smoother operand
   unsigned nblk = 0;
   chr::PcgState *pinned = nullptr;  // two slots
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -427,6 +466,10 @@ struct Cycle {
   const int *stop;
   hipStream_t s;
   int k = 1;
+  // Optional FP32 copy of the level-0 input, read by the first smoother product instead of
+  // the FP64 input (the product rounds its operand to FP32 either way: same bits, half the
+  // gathered bytes).
+  const float *b0f = nullptr;
 
   template <typename TX, typename Epi>
   hipError_t op(const DevCsr &M, const TX *x, Epi epi) {
@@ -471,7 +514,11 @@ struct Cycle {
     if (L.nd == 0) return hipSuccess;
     T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
     // Pre-smoothing from a zero initial guess, then nupre-1 further sweeps.
-    if ((e = op(L.G, b, chr::EpiStore<T>{y}))) return e;
+    if (l == 0 && b0f && k == 1 && std::is_same<T, float>::value)
+      e = op(L.G, b0f, chr::EpiStore<T>{y});
+    else
+      e = op(L.G, b, chr::EpiStore<T>{y});
+    if (e) return e;
     if ((e = op(L.GT, y, chr::EpiScale<T>{x, L.omega, L.omega_rows.as<double>(), k}))) return e;
     for (int q = 2; q <= L.nupre; ++q)
       if ((e = post_sweep(L, b, x))) return e;
@@ -512,7 +559,9 @@ struct LevelIn {
 int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out);
 
 // One V-cycle on FP64 row-major n_0 x k blocks (k <= kcap).
-hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop, int k = 1);
+// b0f (FP32 cycle, k = 1): an FP32 copy of b for the first smoother product.
+hipError_t enqueue_vcycle(chr_hier &h, const double *b, double *out, const int *stop, int k = 1,
+                          const float *b0f = nullptr);
 // Grows the cycle vectors to k columns (synchronises the handle's stream).
 int ensure_block_capacity(chr_hier &h, int k);
 

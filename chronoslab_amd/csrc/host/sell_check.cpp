@@ -1,7 +1,8 @@
-// Host replay of the GPU library's sliced-ELL product (spmv_sell_kernel): builds the layout
-// with the same builder (sell_layout.hpp) and walks it with the kernel's own index
-// arithmetic, workgroup by workgroup and lane by lane, checking every load against the
-// buffers the kernel would read.  It lets the layout and the kernel's addressing be
+// Host replays of the GPU library's sliced-ELL product (spmv_sell_kernel) and packed-code
+// stream product (spmv_pack_kernel): each builds the layout with the same builder
+// (sell_layout.hpp, pack_layout.hpp) and walks it with the kernel's own index arithmetic,
+// workgroup by workgroup and lane by lane, checking every load against the buffers the
+// kernel would read.  It lets the layout and the kernel's addressing be
 // verified without a GPU; the product it returns has the kernel's summation order and
 // rounding (FP32 or FP64 products and sums).
 #include <stdint.h>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "chronos_setup.h"
+#include "pack_layout.hpp"
 #include "sell_layout.hpp"
 
 namespace {
@@ -43,6 +45,7 @@ int replay(const chrsell::Layout &L, const double *x, double *y) {
       if (valid && (row < 0 || row >= n)) return -12;
       const int64_t rc = valid ? row : 0;
       const int64_t d0 = L.sp[(size_t)s], d1 = L.sp[(size_t)s + 1];
+      if (L.ulen > 0 && (d0 != s * L.ulen * chrsell::kSlice || d1 - d0 != L.ulen * chrsell::kSlice)) return -17;
       if ((d1 - d0) % chrsell::kSlice) return -13;
       const int64_t base = d0 + lane, len = (d1 - d0) / chrsell::kSlice;
       TA acc = 0;
@@ -100,6 +103,7 @@ extern "C" int chs_sell_apply(int32_t n, int32_t ncols, const int32_t *rowptr, c
     info[2] = L.wshift;
     info[3] = L.nslices;
     info[4] = L.sshift;
+    info[5] = L.ulen;
   }
   if (elems) *elems = L.elems;
   if (fp32) {
@@ -111,4 +115,92 @@ extern "C" int chs_sell_apply(int32_t n, int32_t ncols, const int32_t *rowptr, c
     return replay<float>(M, xf.data(), y);
   }
   return replay<double>(L, x, y);
+}
+
+namespace {
+
+template <typename TA>
+int replay_pack(const chrpack::Layout &L, const int32_t *rp, const double *val, int64_t nnz, const double *x,
+                double *y) {
+  const int32_t n = L.nrows;
+  const int cap = L.per * kBlock;
+  const int64_t ndict = (int64_t)L.dict.size();
+  std::vector<int> written((size_t)n, 0);
+  std::vector<TA> prod((size_t)cap);
+  auto col_of = [&](int64_t e, int32_t r0, int64_t w, int64_t &col) -> int {
+    if (e < 0 || e >= nnz) return -21;
+    const uint32_t c = L.code[(size_t)e];
+    const int64_t di = w * 256 + (c & 255u);
+    if (di < 0 || di >= ndict) return -22;
+    col = (int64_t)r0 + (c >> 8) - L.dict[(size_t)di];
+    if (col < 0 || col >= n) return -23;
+    return 0;
+  };
+  for (int32_t b = 0; b < L.nblk; ++b) {
+    const int32_t r0 = L.blk[4 * b], e0 = L.blk[4 * b + 1], w = L.blk[4 * b + 2];
+    const int32_t r1 = L.blk[4 * (b + 1)], e1 = L.blk[4 * (b + 1) + 1];
+    if (r1 - r0 > kBlock || r1 <= r0) return -24;
+    if (e1 - e0 > cap) {  // long row: strided partial sums, then the block reduction
+      if (r1 - r0 != 1) return -25;
+      TA part[kBlock];
+      for (int t = 0; t < kBlock; ++t) {
+        TA s = 0;
+        for (int64_t e = e0 + t; e < e1; e += kBlock) {
+          int64_t col;
+          if (int rc = col_of(e, r0, w, col)) return rc;
+          s += (TA)val[e] * (TA)x[col];
+        }
+        part[t] = s;
+      }
+      // block_sum_t: xor butterfly inside each wave of 64, then waves 0..3 in order
+      TA tot = 0;
+      for (int wv = 0; wv < kBlock / 64; ++wv) {
+        TA v[64];
+        for (int l = 0; l < 64; ++l) v[l] = part[wv * 64 + l];
+        for (int off = 32; off > 0; off >>= 1) {
+          TA nv[64];
+          for (int l = 0; l < 64; ++l) nv[l] = v[l] + v[l ^ off];
+          for (int l = 0; l < 64; ++l) v[l] = nv[l];
+        }
+        tot += v[0];
+      }
+      y[r0] = (double)tot;
+      written[(size_t)r0]++;
+      continue;
+    }
+    for (int64_t o = 0; o < e1 - e0; ++o) {
+      int64_t col;
+      if (int rc = col_of(e0 + o, r0, w, col)) return rc;
+      prod[(size_t)o] = (TA)val[e0 + o] * (TA)x[col];
+    }
+    for (int32_t r = r0; r < r1; ++r) {
+      const int64_t ra = rp[r] - e0, rz = rp[r + 1] - e0;
+      if (ra < 0 || rz > e1 - e0) return -26;
+      TA s = 0;
+      for (int64_t q = ra; q < rz; ++q) s += prod[(size_t)q];
+      y[r] = (double)s;
+      written[(size_t)r]++;
+    }
+  }
+  for (int32_t r = 0; r < n; ++r)
+    if (written[(size_t)r] != 1) return -27;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int chs_pack_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                              const double *val, int32_t per, int32_t fp32, const double *x, double *y,
+                              int32_t *nblk) {
+  chrpack::Layout L;
+  if (!chrpack::build(n, ncols, rowptr, col, per, L)) return 1;
+  if (nblk) *nblk = L.nblk;
+  const int64_t nnz = rowptr[n];
+  if (fp32) {
+    std::vector<double> vf((size_t)nnz), xf((size_t)n);
+    for (int64_t e = 0; e < nnz; ++e) vf[(size_t)e] = (double)(float)val[e];
+    for (int32_t i = 0; i < n; ++i) xf[(size_t)i] = (double)(float)x[i];
+    return replay_pack<float>(L, rowptr, vf.data(), nnz, xf.data(), y);
+  }
+  return replay_pack<double>(L, rowptr, val, nnz, x, y);
 }

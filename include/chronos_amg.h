@@ -96,11 +96,12 @@ int chr_hier_destroy(chr_hier *h);
 int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st);
 
 /* Layout the autotuner chose for the single-vector products of one level matrix (which =
- * 'A','G','T','P','R'): per = nonzeros per thread of the CSR-stream kernel, or 0 for the
- * sliced-ELL kernel; code_bits = bits per stored column code (stream: 16 for block
- * offsets, 32 for columns; sliced ELL: 8 for dictionary codes, 16 for row-relative
- * offsets, 32 for columns); nnz = nonzeros; stored_bytes = bytes one product reads of
- * the matrix in that layout.  Any output may be NULL. */
+ * 'A','G','T','P','R'): per = nonzeros per thread of the CSR-stream kernel, minus that for
+ * the packed-code stream kernel, or 0 for the sliced-ELL kernel; code_bits = bits per
+ * stored column code (stream: 16 for block offsets, 32 for columns; packed: 16; sliced
+ * ELL: 8 for dictionary codes, 16 for row-relative offsets, 32 for columns); nnz =
+ * nonzeros; stored_bytes = bytes one product reads of the matrix in that layout.  Any
+ * output may be NULL. */
 int chr_hier_layout(const chr_hier *h, int32_t level, char which, int32_t *per,
                     int32_t *code_bits, int64_t *nnz, double *stored_bytes);
 

@@ -150,12 +150,21 @@ int chs_build(const chs_mat *A, const chs_params *p, chs_hier **out);
  * counterpart): builds the layout the GPU library would upload for this CSR matrix and
  * walks it with the kernel's index arithmetic, checking every access against the layout's
  * buffers; y = A x with the kernel's summation order and rounding (fp32 != 0: FP32 values,
- * vector, products and sums).  info[5] = {code bits, sorted, log2 dictionary window, slices, log2 sort window}; elems =
+ * vector, products and sums).  info[6] = {code bits, sorted, log2 dictionary window, slices, log2 sort window, uniform
+ * slice length or 0}; elems =
  * stored elements.  Returns 0, 1 when the matrix is not a sliced-ELL candidate, or a
  * negative code naming the first out-of-range access. */
 int chs_sell_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
                    const double *val, int32_t fp32, const double *x, double *y, int32_t *info,
                    int64_t *elems);
+/* Host replay of the GPU library's packed-code stream product (no reference counterpart):
+ * builds the packed layout for PER nonzeros per thread and walks it with the kernel's
+ * index arithmetic and summation order, checking every access; y = A x.  nblk receives the
+ * block count.  Returns 0, 1 when the matrix is not a packed candidate, or a negative code
+ * naming the first out-of-range access. */
+int chs_pack_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                   const double *val, int32_t per, int32_t fp32, const double *x, double *y,
+                   int32_t *nblk);
 int chs_hier_nlev(const chs_hier *h);
 /* which: 'A', 'P' or 'G'.  Returned matrix is borrowed (owned by the hierarchy). */
 const chs_mat *chs_hier_mat(const chs_hier *h, int32_t level, char which);

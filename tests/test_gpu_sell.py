@@ -69,3 +69,43 @@ def test_vcycle_layouts_bit_identical(gpu, monkeypatch, prec):
     if prec == "fp64":
         xr = ref.vcycle(b)
         assert np.linalg.norm(Ms(b) - xr) <= 1e-12 * np.linalg.norm(xr)
+
+
+def _with_env(monkeypatch, env, make):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    try:
+        return make()
+    finally:
+        for k in env:
+            monkeypatch.delenv(k)
+
+
+@pytest.mark.parametrize("name,A", _mats())
+def test_spmv_packed_layout_equals_oracle(gpu, monkeypatch, name, A):
+    """The packed-code stream layout (CHRONOS_PACK=1, sliced ELL off) where the matrix admits
+    it: same bits as the stream layout and the oracle."""
+    x = np.random.default_rng(2).standard_normal(A.ncols)
+    ref = oracle.spmv(A, x)
+    M = _with_env(monkeypatch, {"CHRONOS_SELL": "0", "CHRONOS_PACK": "1"}, lambda: gpu.DeviceMatrix(A))
+    print(f"{name} CHRONOS_PACK=1", flush=True)
+    np.testing.assert_array_equal(M(x), ref)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_vcycle_packed_layout_bit_identical(gpu, monkeypatch, prec):
+    A, _ = H.heterogeneous_diffusion(24, 2.0, seed=5)
+    hier = H.build(A)
+    Mp = _with_env(monkeypatch, {"CHRONOS_SELL": "0", "CHRONOS_PACK": "1"},
+                   lambda: gpu.AMGPreconditioner(hier, prec))
+    Mc = _with_env(monkeypatch, {"CHRONOS_SELL": "0", "CHRONOS_PACK": "0"},
+                   lambda: gpu.AMGPreconditioner(hier, prec))
+    lp = Mp.layout()
+    print(prec, "packed layouts", lp[0], flush=True)
+    assert lp[0]["G"][0][0] == "pack" and lp[0]["T"][0][0] == "pack"
+    b = H.rng_uniform(7, 2, A.nrows, -1, 1)
+    np.testing.assert_array_equal(Mp(b), Mc(b))
+    xp, itp, rvp, fp_ = Mp.pcg(b, itmax=100, tol=1e-8)
+    xc, itc, rvc, fc = Mc.pcg(b, itmax=100, tol=1e-8)
+    assert itp == itc and fp_ == fc == 0
+    np.testing.assert_array_equal(xp, xc)

@@ -26,13 +26,14 @@ def sell_apply(A: H.Csr, x: np.ndarray, fp32: bool):
     v = np.ascontiguousarray(A.val, dtype=np.float64)
     xx = np.ascontiguousarray(x, dtype=np.float64)
     y = np.full(A.nrows, np.nan)
-    info = np.zeros(5, np.int32)
+    info = np.zeros(6, np.int32)
     elems = C.c_int64()
     rc = lib.chs_sell_apply(A.nrows, A.ncols, rp.ctypes.data_as(_I32P), ci.ctypes.data_as(_I32P),
                             v.ctypes.data_as(_F64P), int(fp32), xx.ctypes.data_as(_F64P),
                             y.ctypes.data_as(_F64P), info.ctypes.data_as(_I32P), C.byref(elems))
     return rc, y, dict(cb=int(info[0]), sorted=bool(info[1]), wshift=int(info[2]),
-                       nslices=int(info[3]), sshift=int(info[4]), elems=elems.value)
+                       nslices=int(info[3]), sshift=int(info[4]), ulen=int(info[5]),
+                       elems=elems.value)
 
 
 def csr_seq_fp32(A: H.Csr, x: np.ndarray) -> np.ndarray:
@@ -98,6 +99,16 @@ def test_stencil_dictionary_codes():
     A = H.poisson7(12)
     info, _ = check(A, np.random.default_rng(0), expect_cb=8)
     assert info["elems"] <= 1.2 * A.nnz
+
+
+def test_uniform_ell(monkeypatch):
+    # Unsorted layout of a matrix whose slices are all as long as its longest row: uniform
+    # ELL (computed slice offsets).  Rows of one length plus an empty last slice's worth of
+    # short rows exercise the zero-length and clamped loads.
+    monkeypatch.setenv("CHRONOS_SELL_SORT", "0")
+    A = H.poisson7(16)
+    info, _ = check(A, np.random.default_rng(5), expect_cb=8)
+    assert info["ulen"] == 7 and not info["sorted"] and info["elems"] == info["nslices"] * 7 * 64
 
 
 def test_fsai_factor_and_transpose():
