@@ -148,7 +148,7 @@ class AMGPreconditioner:
         (("pack", PER), 16) for the packed-code stream kernel,
         with the bytes one product reads of the stored matrix appended when stored_bytes."""
         out = []
-        for l in range(len(self.hier.levels) - 1):
+        for l in range(self.stats()["nlev"] - 1):  # device levels (the tail may be collapsed)
             row = {}
             for w in "AGTPR":
                 per, bits, nnz, sb = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()

@@ -489,6 +489,81 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
 
 }  // namespace chrrt
 
+namespace chrrt {
+
+// Tail collapse.  The V-cycle from a small level c down (zero initial guess, smoothing,
+// transfers, the coarsest solve) is a fixed linear map b_c -> x_c.  When n_c is at most
+// CHRONOS_TAIL_ROWS (default 2048; 0 disables), that map is formed once, column by column,
+// by running the cycle's own kernels from level c on the unit vectors, and level c becomes
+// the last level with that matrix applied as its dense FP64 direct solve.  The dozen small
+// kernels of the levels below c (each bound by dependent memory latency) become one GEMV;
+// the result equals the uncollapsed cycle's up to the rounding of one matrix-vector
+// product.  Single-system hierarchies only (batched hierarchies keep every level).
+int collapse_tail(chr_hier &h) {
+  const char *env = std::getenv("CHRONOS_TAIL_ROWS");
+  const int64_t limit = env ? std::atoll(env) : 2048;
+  const int32_t nlev = (int32_t)h.lev.size();
+  int32_t c = -1;
+  for (int32_t l = 1; l + 1 < nlev; ++l)
+    if (h.lev[l].nd == h.lev[l].n && h.lev[l].n <= limit) {
+      c = l;
+      break;
+    }
+  if (c < 0 || limit <= 0) return CHR_OK;
+  auto &L = h.lev[c];
+  const int32_t m = L.n;
+  const size_t sz = h.fp32() ? 4 : 8;
+  DevBuf cols;
+  int rc;
+  if ((rc = cols.alloc(sz * (size_t)m * m))) return rc;
+  const int *stop = h.zero.as<int>();
+  for (int32_t j = 0; j < m; ++j) {
+    HIPC(hipMemsetAsync(L.b.p, 0, sz * m, h.stream));
+    if (h.fp32()) {  // b = e_j (1.0f = 0x3F800000)
+      HIPC(hipMemsetD32Async((hipDeviceptr_t)(L.b.as<float>() + j), 0x3F800000u, 1, h.stream));
+      Cycle<float> cy{h, stop, h.stream, 1};
+      HIPC((cy.level<float, float>(c, L.b.as<float>(), cols.as<float>() + (size_t)j * m)));
+    } else {  // 1.0 = 0x3FF00000'00000000: high word of the little-endian double
+      HIPC(hipMemsetD32Async((hipDeviceptr_t)(reinterpret_cast<uint32_t *>(L.b.as<double>() + j) + 1),
+                             0x3FF00000u, 1, h.stream));
+      Cycle<double> cy{h, stop, h.stream, 1};
+      HIPC((cy.level<double, double>(c, L.b.as<double>(), cols.as<double>() + (size_t)j * m)));
+    }
+  }
+  HIPC(hipStreamSynchronize(h.stream));
+  // cols holds column j = M e_j contiguously; the direct solve wants row-major M.
+  std::vector<double> M((size_t)m * m);
+  if (h.fp32()) {
+    std::vector<float> t((size_t)m * m);
+    HIPC(hipMemcpy(t.data(), cols.p, sizeof(float) * t.size(), hipMemcpyDeviceToHost));
+    for (int32_t j = 0; j < m; ++j)
+      for (int32_t i = 0; i < m; ++i) M[(size_t)i * m + j] = (double)t[(size_t)j * m + i];
+  } else {
+    std::vector<double> t((size_t)m * m);
+    HIPC(hipMemcpy(t.data(), cols.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+    for (int32_t j = 0; j < m; ++j)
+      for (int32_t i = 0; i < m; ++i) M[(size_t)i * m + j] = t[(size_t)j * m + i];
+  }
+  // Level c becomes the last level: all its rows are direct, with M as their operator.
+  std::vector<int4> cmap((size_t)m, make_int4(0, m, 0, 0));
+  if ((rc = L.cmap.alloc(sizeof(int4) * m)) || (rc = h.Ainv.alloc(sizeof(double) * M.size()))) return rc;
+  HIPC(hipMemcpy(L.cmap.p, cmap.data(), sizeof(int4) * m, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(h.Ainv.p, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
+  L.nd = 0;
+  L.ncoarse_blocks = 1;
+  L.coarse_m2 = (int64_t)m * m;
+  L.G = DevCsr();
+  L.GT = DevCsr();
+  L.P = DevCsr();
+  L.PT = DevCsr();
+  h.lev.resize((size_t)c + 1);
+  h.nc = m;
+  fill_stats(h);
+  return CHR_OK;
+}
+
+}  // namespace chrrt
+
 extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t precision,
                                chr_hier **out) {
   if (nlev < 1 || nlev > 16 || !levels || !out)
@@ -523,7 +598,13 @@ extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t pr
       lv[l].direct.push_back({0, L.A});
     }
   }
-  return build_hier(lv, precision, levels[0].A, out);
+  int rc = build_hier(lv, precision, levels[0].A, out);
+  if (rc) return rc;
+  if ((rc = collapse_tail(**out))) {
+    chr_hier_destroy(*out);
+    *out = nullptr;
+  }
+  return rc;
 }
 
 extern "C" int chr_hier_destroy(chr_hier *h) {

@@ -82,3 +82,31 @@ def test_pcg_fresh_outer_operators_and_growing_itmax(gpu):
         np.testing.assert_allclose(x, x_ref, rtol=0, atol=1e-13 * np.abs(x_ref).max())
     x, it, rv, fl = M.pcg(b, itmax=400, tol=1e-10)
     assert it == it_ref and len(rv) == it + 1
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_tail_collapse(gpu, monkeypatch, prec):
+    """The V-cycle from the first level of <= CHRONOS_TAIL_ROWS rows down becomes one dense
+    FP64 operator: the cycle equals the uncollapsed one to rounding, PCG takes the same
+    iterations, and the collapsed level is the device hierarchy's last."""
+    A = _problem("diffusion", 32, seed=8)
+    hier = H.build(A)
+    nrows = [L.A.nrows for L in hier.levels]
+    c = next(l for l in range(1, hier.nlev - 1) if nrows[l] <= 4096)
+    monkeypatch.setenv("CHRONOS_TAIL_ROWS", "0")
+    Mfull = gpu.AMGPreconditioner(hier, prec)
+    monkeypatch.setenv("CHRONOS_TAIL_ROWS", "4096")
+    Mtail = gpu.AMGPreconditioner(hier, prec)
+    print(f"levels {nrows}, collapse at level {c}: device levels {Mfull.stats()['nlev']} -> "
+          f"{Mtail.stats()['nlev']}", flush=True)
+    assert Mfull.stats()["nlev"] == hier.nlev and Mtail.stats()["nlev"] == c + 1
+    b = H.rng_uniform(9, 2, A.nrows, -1, 1)
+    xf, xt = Mfull(b), Mtail(b)
+    rel = np.linalg.norm(xt - xf) / np.linalg.norm(xf)
+    print(f"{prec}: collapsed vs full V-cycle rel diff {rel:.3e}", flush=True)
+    assert rel <= (1e-13 if prec == "fp64" else 1e-5)
+    if prec == "fp64":
+        assert np.linalg.norm(xt - oracle.Hier(hier).vcycle(b)) <= 1e-12 * np.linalg.norm(xf)
+    _, itf, _, flf = Mfull.pcg(b, itmax=100, tol=1e-8)
+    _, itt, _, flt = Mtail.pcg(b, itmax=100, tol=1e-8)
+    assert flf == flt == 0 and abs(itf - itt) <= (0 if prec == "fp64" else 1)
