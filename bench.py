@@ -237,11 +237,14 @@ def main() -> int:
 
     res = {}
     precs = ["fp32", "fp64"] if args.only == "both" else [args.only]
+    upload = {}
     for prec in precs:
+        t0 = time.perf_counter()
         M = amg.AMGPreconditioner(hier, prec)
+        upload[prec] = time.perf_counter() - t0
         times, its, xs, tres = run_solves(M, args.steps, args.warmup)
         st = M.stats()
-        log(f"[bench] {prec} layouts (PER or sell, code bits): {M.layout()}")
+        log(f"[bench] {prec}: upload + autotune {upload[prec]:.2f} s; layouts (PER or sell, code bits): {M.layout()}")
         vms = M.time_vcycle(20)
         gms, gbytes = M.time_spmv(0, "G", 20)
         tms, tbytes = M.time_spmv(0, "T", 20)
@@ -323,7 +326,8 @@ def main() -> int:
         "config": {"workload": f"heterogeneous diffusion {N}^3 FV 7-pt, ln k~N(0,{args.sigma:g}^2), "
                                f"AMG(CLA+PMIS+classical, aFSAI) PCG_CJ to {args.tol:g}",
                    "n": A.nrows, "nnz": A.nnz, "levels": hier.nlev, "iterations": r["its"][0],
-                   "setup_s": t_setup, "setup_parts_s": setup_parts, "setup_all_host_s": t_setup_host},
+                   "setup_s": t_setup, "setup_parts_s": setup_parts, "setup_all_host_s": t_setup_host,
+                   "upload_and_autotune_s": upload},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": _measured_traffic(A.nrows) if main_prec == "fp32" else None,

@@ -102,9 +102,11 @@ int DevPack::build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *
 //     keeps PER 8: the outer operator of PCG, whose fused dot products are summed over the
 //     stream row blocks, so that the solver's rounding never depends on a timing);
 //  2. the packed-code stream layout (16-bit codes: row offset in the block and a window
-//     dictionary index), with the PER of step 1, when the matrix admits it;
-//  3. the sliced-ELL layout, when the matrix admits it, unsorted, sorted in 256-row
-//     windows, and with the builder's own choice of sort window.
+//     dictionary index), with the PER of step 1, when the matrix admits it and its blocks
+//     span too many columns for 16-bit offsets;
+//  3. the sliced-ELL layout, when the matrix admits it: unsorted, then (unless unsorted
+//     pads by at most 12%) sorted in 256-row windows and with the builder's own choice of
+//     sort window.
 // Each candidate is timed with HIP events, 2 warm-up and 5 timed launches; a matrix of 32 MB
 // or more is timed cache-cold (a 512 MiB device copy before every timed launch, as the
 // other products of a V-cycle evict it), smaller ones back to back.  The fastest candidate
@@ -201,8 +203,10 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     }
   }
   // Step 2: packed codes (the stream kernel's blocks with 2-byte codes).
-  if (!rc && pack_mode != 0 && sell_mode != 1 && !(rc = M.pack.build(M.nrows, M.ncols, hrp, hci, M.per)) &&
-      M.pack.built) {
+  // (Only where 16-bit block offsets do not apply: otherwise the stream layout's index is
+  // already 2 bytes.)
+  if (!rc && pack_mode != 0 && sell_mode != 1 && (!M.idx16 || pack_mode == 1) &&
+      !(rc = M.pack.build(M.nrows, M.ncols, hrp, hci, M.per)) && M.pack.built) {
     if (pack_mode == 1) {
       M.use_pack = true;
     } else {
@@ -223,9 +227,11 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
   }
   // Step 3: sliced ELL.
   if (!rc && sell_mode != 0 && !(pack_mode == 1 && M.use_pack)) {
+    // Unsorted first; when its padding is within 12% of the nonzeros the sorted variants
+    // (fewer padded bytes, scattered rows) are not tried.  The builder's choice last.
     std::vector<int> sorts;
     if (env_sort) sorts.push_back(std::atoi(env_sort));
-    else sorts = {-1, 0, 8};
+    else sorts = {0, 8, -1};
     if (sell_mode == 1 && !env_sort) sorts = {-1};
     if (stream_ms < 0.f && sell_mode != 1) rc = time_it(stream_ms);
     float best_ms = sell_mode == 1 ? 1e30f : 0.97f * stream_ms;
@@ -233,8 +239,9 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     std::vector<std::pair<int, int>> seen;  // (sorted, sshift) of the layouts built so far
     for (size_t c = 0; c < sorts.size() && !rc; ++c) {
       if ((rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32, sorts[c]))) break;
-      if (!M.sell.cb) break;  // not a candidate
+      if (!M.sell.cb) continue;  // not a candidate with this sort window
       last_sort = sorts[c];
+      const bool tight = sorts[c] == 0 && (double)M.sell.elems <= 1.12 * (double)M.nnz;
       const std::pair<int, int> key{M.sell.sorted ? 1 : 0, M.sell.sorted ? M.sell.sshift : 0};
       if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
       seen.push_back(key);
@@ -247,6 +254,7 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
         best_sort = sorts[c];
         if (sell_mode == 1) break;
       }
+      if (tight) break;
     }
     if (!rc && best_sort != -2) {
       if (best_sort != last_sort) rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32, best_sort);
