@@ -34,6 +34,15 @@ def _lib():
     return load_amg_lib()
 
 
+def _release(fn: str, ptr) -> None:
+    """Destroys a library object; at interpreter exit the module globals may already be gone,
+    and the process teardown releases the device memory anyway."""
+    try:
+        getattr(_lib(), fn)(ptr)
+    except (TypeError, AttributeError):
+        pass
+
+
 def _check(rc: int, what: str = ""):
     if rc != 0:
         raise RuntimeError(f"chronos_amg error {rc} {what}: {_lib().chr_last_error().decode()}")
@@ -104,7 +113,7 @@ class DeviceMatrix:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            _lib().chr_mat_destroy(self.ptr)
+            _release("chr_mat_destroy", self.ptr)
             self.ptr = None
 
     def __call__(self, x: np.ndarray) -> np.ndarray:
@@ -131,7 +140,7 @@ class AMGPreconditioner:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            _lib().chr_hier_destroy(self.ptr)
+            _release("chr_hier_destroy", self.ptr)
             self.ptr = None
 
     def stats(self) -> dict:
@@ -229,7 +238,7 @@ class BatchSolver:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            _lib().chr_batch_destroy(self.ptr)
+            _release("chr_batch_destroy", self.ptr)
             self.ptr = None
 
     def vcycle(self, B: np.ndarray) -> np.ndarray:

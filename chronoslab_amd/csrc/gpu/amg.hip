@@ -670,8 +670,8 @@ extern "C" int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t
   int64_t lds = ldb;
   if (ptr_kind == CHR_PTR_HOST) {
     if ((rc = cm.alloc(8 * (size_t)n * k))) return rc;
-    HIPC(hipMemcpy2DAsync(cm.p, 8 * (size_t)n, b, 8 * (size_t)ldb, 8 * (size_t)n, k,
-                          hipMemcpyHostToDevice, h->stream));
+    HIPC(copy_cols(cm.p, 8 * (size_t)n, b, 8 * (size_t)ldb, 8 * (size_t)n, k, hipMemcpyHostToDevice,
+                   h->stream));
     bsrc = cm.as<double>();
     lds = n;
   }
@@ -684,8 +684,8 @@ extern "C" int chr_vcycle_apply(chr_hier *h, int32_t k, const double *b, int64_t
     hipLaunchKernelGGL(chr::rm_to_cm_kernel, dim3(grid), dim3(chr::kBlock), 0, h->stream, n, k,
                        (const double *)h->io_x.as<double>(), cm.as<double>(), (int64_t)n);
     HIPC(hipGetLastError());
-    HIPC(hipMemcpy2DAsync(x, 8 * (size_t)ldx, cm.p, 8 * (size_t)n, 8 * (size_t)n, k,
-                          hipMemcpyDeviceToHost, h->stream));
+    HIPC(copy_cols(x, 8 * (size_t)ldx, cm.p, 8 * (size_t)n, 8 * (size_t)n, k, hipMemcpyDeviceToHost,
+                   h->stream));
   } else {
     hipLaunchKernelGGL(chr::rm_to_cm_kernel, dim3(grid), dim3(chr::kBlock), 0, h->stream, n, k,
                        (const double *)h->io_x.as<double>(), x, ldx);

@@ -475,42 +475,68 @@ static __global__ __launch_bounds__(kBlock) void rm_to_cm_kernel(int n, int k, c
 }
 
 // Per-block partial Gram products: part[g][p*b + q] = sum over the block's rows of
-// X[i*a + p] * Y[i*b + q] (a*b <= kBlock).  Rows are split in fixed chunks, so the partials
-// (and the in-order reduction that follows) do not depend on timing.
+// X[i*a + p] * Y[i*b + q] in row order (a, b <= 16).  Rows are split in fixed chunks, so
+// the partials (and the fixed-order reduction that follows) do not depend on timing.  Row
+// tiles of X and Y are staged in LDS with coalesced loads; thread p*b + q then walks the
+// tile's rows.
+constexpr int kGramTile = 64;
 static __global__ __launch_bounds__(kBlock) void gram_partial_kernel(int n, int a, const double *__restrict__ X,
                                                               int b, const double *__restrict__ Y,
                                                               int rows_per_block,
                                                               double *__restrict__ part) {
+  __shared__ double xs[kGramTile * 16], ys[kGramTile * 16];
   const int t = threadIdx.x;
   const int ab = a * b;
-  if (t >= ab) return;
   const int p = t / b, q = t % b;
   const int i0 = blockIdx.x * rows_per_block, i1 = min(n, i0 + rows_per_block);
   double s = 0.0;
-  for (int i = i0; i < i1; ++i) s += X[(size_t)i * a + p] * Y[(size_t)i * b + q];
-  part[(size_t)blockIdx.x * ab + t] = s;
+  for (int r0 = i0; r0 < i1; r0 += kGramTile) {
+    const int rows = min(kGramTile, i1 - r0);
+    for (int e = t; e < rows * a; e += kBlock) xs[e] = X[(size_t)r0 * a + e];
+    for (int e = t; e < rows * b; e += kBlock) ys[e] = Y[(size_t)r0 * b + e];
+    __syncthreads();
+    if (t < ab)
+      for (int r = 0; r < rows; ++r) s += xs[r * a + p] * ys[r * b + q];
+    __syncthreads();
+  }
+  if (t < ab) part[(size_t)blockIdx.x * ab + t] = s;
 }
+// out[t] = sum over the nparts partials of entry t: one wave per entry, lane l adding
+// partials l, l + 64, ... in order, then a fixed butterfly across the wave.
 static __global__ __launch_bounds__(kBlock) void gram_reduce_kernel(int nparts, int ab,
                                                              const double *__restrict__ part,
                                                              double *__restrict__ out) {
-  const int t = threadIdx.x;
+  const int t = (blockIdx.x * kBlock + threadIdx.x) / kWave, lane = threadIdx.x % kWave;
   if (t >= ab) return;
   double s = 0.0;
-  for (int g = 0; g < nparts; ++g) s += part[(size_t)g * ab + t];
-  out[t] = s;  // row-major a x b: out[p*b + q]
+  for (int g = lane; g < nparts; g += kWave) s += part[(size_t)g * ab + t];
+  s = group_sum<kWave>(s);
+  if (lane == 0) out[t] = s;  // row-major a x b: out[p*b + q]
 }
 
-// Z (n x b) = beta * Z + X (n x a) * Cm, Cm column-major a x b.
+// Z (n x b) = beta * Z + X (n x a) * Cm, Cm column-major a x b (a, b <= 48).  Cm is staged
+// in LDS once per workgroup and X in tiles of 32 rows; output (i, q) sums over p in order.
+constexpr int kGemmTile = 32;
 static __global__ __launch_bounds__(kBlock) void block_gemm_kernel(int n, int a, const double *__restrict__ X,
                                                             int b, const double *__restrict__ Cm,
                                                             double beta, double *__restrict__ Z) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= (int64_t)n * b) return;
-  const int64_t i = t / b;
-  const int q = (int)(t % b);
-  double s = 0.0;
-  for (int p = 0; p < a; ++p) s += X[i * a + p] * Cm[p + (size_t)q * a];
-  Z[t] = (beta != 0.0 ? beta * Z[t] : 0.0) + s;
+  __shared__ double cs[48 * 48], xs[kGemmTile * 48];
+  const int t = threadIdx.x;
+  for (int e = t; e < a * b; e += kBlock) cs[e] = Cm[e];
+  __syncthreads();
+  for (int r0 = blockIdx.x * kGemmTile; r0 < n; r0 += gridDim.x * kGemmTile) {
+    const int rows = min(kGemmTile, n - r0);
+    for (int e = t; e < rows * a; e += kBlock) xs[e] = X[(size_t)r0 * a + e];
+    __syncthreads();
+    for (int o = t; o < rows * b; o += kBlock) {
+      const int r = o / b, q = o % b;
+      double s = 0.0;
+      for (int p = 0; p < a; ++p) s += xs[r * a + p] * cs[p + q * a];
+      const size_t zi = (size_t)(r0 + r) * b + q;
+      Z[zi] = (beta != 0.0 ? beta * Z[zi] : 0.0) + s;
+    }
+    __syncthreads();
+  }
 }
 
 // R = AX - X diag(lam)   (n x k)

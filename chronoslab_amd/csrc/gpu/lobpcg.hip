@@ -36,10 +36,12 @@ struct Lob {
 
   // out (column-major a x b, host) = X^T Y for row-major n x a and n x b blocks.
   int gram(const double *X, int a, const double *Y, int b, std::vector<double> &out) {
+    if (a < 1 || b < 1 || a > 16 || b > 16) return fail(CHR_EARG, "lobpcg: Gram block wider than 16");
     hipLaunchKernelGGL(chr::gram_partial_kernel, dim3(nparts), dim3(chr::kBlock), 0, s, n, a, X, b, Y,
                        rows_per_part, part.as<double>());
     HIPC(hipGetLastError());
-    hipLaunchKernelGGL(chr::gram_reduce_kernel, dim3(1), dim3(chr::kBlock), 0, s, nparts, a * b,
+    const unsigned rgrid = (unsigned)((a * b * chr::kWave + chr::kBlock - 1) / chr::kBlock);
+    hipLaunchKernelGGL(chr::gram_reduce_kernel, dim3(rgrid), dim3(chr::kBlock), 0, s, nparts, a * b,
                        (const double *)part.as<double>(), gout.as<double>());
     HIPC(hipGetLastError());
     std::vector<double> g((size_t)a * b);
@@ -51,9 +53,19 @@ struct Lob {
     return CHR_OK;
   }
   // Z (n x b) = beta Z + X (n x a) Cm (column-major a x b, host).
+  // Small host operands are copied with the stream drained afterwards: the host arrays
+  // are temporaries of the Rayleigh-Ritz step, and a copy from pageable memory that is
+  // still pending when they change or go away would read the wrong data.
+  int upload_small(void *dst, const void *src, size_t bytes) {
+    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+    return CHR_OK;
+  }
   int gemm(const double *X, int a, const double *Cm, int b, double beta, double *Z) {
-    HIPC(hipMemcpyAsync(cdev.p, Cm, sizeof(double) * a * b, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(chr::block_gemm_kernel, dim3(grid((int64_t)n * b)), dim3(chr::kBlock), 0, s, n, a,
+    if (int rc = upload_small(cdev.p, Cm, sizeof(double) * a * b)) return rc;
+    if (a < 1 || b < 1 || a > 48 || b > 48) return fail(CHR_EARG, "lobpcg: block product wider than 48");
+    const unsigned ggrid = (unsigned)std::min<int64_t>(2048, ((int64_t)n + chr::kGemmTile - 1) / chr::kGemmTile);
+    hipLaunchKernelGGL(chr::block_gemm_kernel, dim3(ggrid), dim3(chr::kBlock), 0, s, n, a,
                        X, b, (const double *)cdev.as<double>(), beta, Z);
     HIPC(hipGetLastError());
     return CHR_OK;
@@ -70,7 +82,7 @@ struct Lob {
     return CHR_OK;
   }
   int resid(const double *AX, const double *X, const double *lam_host, int k, double *R) {
-    HIPC(hipMemcpyAsync(cdev.p, lam_host, sizeof(double) * k, hipMemcpyHostToDevice, s));
+    if (int rc = upload_small(cdev.p, lam_host, sizeof(double) * k)) return rc;
     hipLaunchKernelGGL(chr::block_resid_kernel, dim3(grid((int64_t)n * k)), dim3(chr::kBlock), 0, s, n, k,
                        AX, X, (const double *)cdev.as<double>(), R);
     HIPC(hipGetLastError());
@@ -78,7 +90,7 @@ struct Lob {
   }
   int gather(const double *X, int k, const std::vector<int> &cols, double *Y) {
     const int m = (int)cols.size();
-    HIPC(hipMemcpyAsync(idx.p, cols.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+    if (int rc = upload_small(idx.p, cols.data(), sizeof(int) * m)) return rc;
     hipLaunchKernelGGL(chr::block_gather_kernel, dim3(grid((int64_t)n * m)), dim3(chr::kBlock), 0, s, n, k,
                        X, m, (const int *)idx.as<int>(), Y);
     HIPC(hipGetLastError());
@@ -150,7 +162,7 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
   const double *xsrc = X;
   int64_t lds = ldx;
   if (ptr_kind == CHR_PTR_HOST) {
-    HIPC(hipMemcpy2DAsync(dT, 8 * (size_t)n, X, 8 * (size_t)ldx, 8 * (size_t)n, neig, hipMemcpyHostToDevice, s));
+    HIPC(copy_cols(dT, 8 * (size_t)n, X, 8 * (size_t)ldx, 8 * (size_t)n, neig, hipMemcpyHostToDevice, s));
     xsrc = dT;
     lds = n;
   }
@@ -361,7 +373,7 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
                      (const double *)dX, xdst, ldd);
   HIPC(hipGetLastError());
   if (ptr_kind == CHR_PTR_HOST)
-    HIPC(hipMemcpy2DAsync(X, 8 * (size_t)ldx, dT, 8 * (size_t)n, 8 * (size_t)n, neig, hipMemcpyDeviceToHost, s));
+    HIPC(copy_cols(X, 8 * (size_t)ldx, dT, 8 * (size_t)n, 8 * (size_t)n, neig, hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));
   if (lambda_out) std::copy(lam.begin(), lam.end(), lambda_out);
   if (niter_out) *niter_out = iter - 1;

@@ -109,6 +109,19 @@ inline void transpose_host(const chr_csr &M, std::vector<int32_t> &rp, std::vect
     }
 }
 
+// Copies ncols columns of `bytes` bytes each between column-major blocks with leading
+// dimensions (in bytes) ldd and lds: one copy when both are packed, else one per column.
+inline hipError_t copy_cols(void *dst, size_t ldd, const void *src, size_t lds, size_t bytes, int ncols,
+                            hipMemcpyKind kind, hipStream_t s) {
+  if (ldd == bytes && lds == bytes) return hipMemcpyAsync(dst, src, bytes * (size_t)ncols, kind, s);
+  for (int c = 0; c < ncols; ++c) {
+    hipError_t e = hipMemcpyAsync(static_cast<char *>(dst) + ldd * c, static_cast<const char *>(src) + lds * c,
+                                  bytes, kind, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 // Candidate nonzeros-per-thread of the stream kernel (block capacity PER * 256).
 constexpr int kPers[] = {4, 8, 16};
 
