@@ -54,7 +54,13 @@ def load_setup_lib():
             _sig(lib, "chs_gen_poisson7", C.c_int, [_I32, C.POINTER(_VP)])
             _sig(lib, "chs_gen_diffusion", C.c_int, [_I32, _F64P, C.POINTER(_VP)])
             _sig(lib, "chs_gen_lnk", None, [_I32, _F64, _U64, _F64P])
+            _sig(lib, "chs_gen_elasticity", C.c_int, [_I32, _F64, _F64, _I32, C.POINTER(_VP), _F64P])
+            _sig(lib, "chs_gen_rbm", None, [_I32, _F64P])
             _sig(lib, "chs_soc_cla", C.c_int, [_VP, _F64, _I8P])
+            _sig(lib, "chs_soc_dom", C.c_int, [_VP, _F64, _I8P])
+            _sig(lib, "chs_prol_bamg", C.c_int, [_VP, _I8P, _I32P, _I32, _I32, _F64P, _VP, C.c_int,
+                                                 C.POINTER(_VP), _I32P, C.POINTER(_I32)])
+            _sig(lib, "chs_orthonormalize", C.c_int, [C.c_int64, _I32, _F64P])
             _sig(lib, "chs_pmis", C.c_int, [_VP, _I8P, _F64P, _I32P, C.POINTER(_I32)])
             _sig(lib, "chs_prol_classical", C.c_int, [_VP, _I8P, _I32P, _I32, C.c_int, C.POINTER(_VP)])
             _sig(lib, "chs_prol_exti", C.c_int, [_VP, _I8P, _I32P, _I32, C.c_int, C.POINTER(_VP)])

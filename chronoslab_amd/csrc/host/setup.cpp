@@ -3,6 +3,7 @@
 // This is the classical + adaptive-FSAI setup that ChronosLab runs before its solve phase
 // (sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:68-351), restated from the reference's
 // behaviour.  The GPU solve phase consumes its output through chronos_amg.h.
+#include "bamg.hpp"
 #include "chronos_setup.h"
 
 #include <algorithm>
@@ -100,6 +101,18 @@ extern "C" void chs_params_default(chs_params *p) {
   p->afsai_ctx = nullptr;
   p->omega_fn = nullptr;
   p->omega_ctx = nullptr;
+  p->soc_type = 0;
+  p->ntv = 6;
+  p->ntv0 = 0;
+  p->tv = nullptr;
+  p->bamg.itmax_vol = 100;
+  p->bamg.tol_vol = 0.01;
+  p->bamg.dist_min = 1;
+  p->bamg.dist_max = 10;
+  p->bamg.maxcond = 1e13;
+  p->bamg.maxrownrm = 5.0;
+  p->bamg.eps_prol = 1e-10;
+  p->prol_smooth = 0;
 }
 
 extern "C" const char *chs_last_error(void) { return g_err.c_str(); }
@@ -248,6 +261,180 @@ extern "C" int chs_gen_diffusion(int32_t N, const double *k, chs_mat **out) {
 }
 
 // ----------------------------------------------------------------------------------------
+// Linear elasticity on trilinear hexahedra (Q1), unit cube of ne^3 elements, (ne+1)^3
+// nodes with 3 displacement dofs each (dof 3*node + component, node = ix + m*(iy + m*iz),
+// m = ne + 1).  Isotropic material (E, nu): D in Voigt order (xx, yy, zz, yz, xz, xy) with
+// engineering shears; the element matrix is the 2x2x2 Gauss rule of B^T D B, the same for
+// every element, and rows are assembled node by node over the (up to) 8 elements around the
+// node, so the pattern holds all 27 neighbour nodes x 3 components (81 per interior row).
+// A uniform body force -1 in z gives the load, each element contributing a eighth of its
+// volume to the z dof of each of its nodes.  clamp != 0: the dofs of the z = 0 face become
+// unit rows and columns (zero load), as the reference's elasticity tests keep them.
+// ----------------------------------------------------------------------------------------
+namespace {
+
+void q1_element_matrix(double h, double E, double nu, double Ke[24][24]) {
+  const double c = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  double D[6][6] = {};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) D[i][j] = c * (i == j ? 1.0 - nu : nu);
+  for (int i = 3; i < 6; ++i) D[i][i] = c * (1.0 - 2.0 * nu) / 2.0;
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) Ke[i][j] = 0.0;
+  const double g = 1.0 / std::sqrt(3.0);
+  const double detJ = (h / 2.0) * (h / 2.0) * (h / 2.0);
+  for (int gp = 0; gp < 8; ++gp) {
+    const double xi = (gp & 1) ? g : -g, eta = (gp & 2) ? g : -g, zeta = (gp & 4) ? g : -g;
+    double dN[8][3];
+    for (int a = 0; a < 8; ++a) {
+      const double sa = (a & 1) ? 1.0 : -1.0, ta = (a & 2) ? 1.0 : -1.0, ua = (a & 4) ? 1.0 : -1.0;
+      // dN/dx = (2/h) dN/dxi
+      dN[a][0] = (2.0 / h) * 0.125 * sa * (1.0 + eta * ta) * (1.0 + zeta * ua);
+      dN[a][1] = (2.0 / h) * 0.125 * ta * (1.0 + xi * sa) * (1.0 + zeta * ua);
+      dN[a][2] = (2.0 / h) * 0.125 * ua * (1.0 + xi * sa) * (1.0 + eta * ta);
+    }
+    double B[6][24] = {};
+    for (int a = 0; a < 8; ++a) {
+      const double bx = dN[a][0], by = dN[a][1], bz = dN[a][2];
+      B[0][3 * a] = bx;
+      B[1][3 * a + 1] = by;
+      B[2][3 * a + 2] = bz;
+      B[3][3 * a + 1] = bz;
+      B[3][3 * a + 2] = by;
+      B[4][3 * a] = bz;
+      B[4][3 * a + 2] = bx;
+      B[5][3 * a] = by;
+      B[5][3 * a + 1] = bx;
+    }
+    double DB[6][24];
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 24; ++j) {
+        double t = 0.0;
+        for (int k = 0; k < 6; ++k) t += D[i][k] * B[k][j];
+        DB[i][j] = t;
+      }
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) {
+        double t = 0.0;
+        for (int k = 0; k < 6; ++k) t += B[k][i] * DB[k][j];
+        Ke[i][j] += t * detJ;
+      }
+  }
+}
+
+}  // namespace
+
+extern "C" int chs_gen_elasticity(int32_t ne, double E, double nu, int32_t clamp, chs_mat **out,
+                                  double *rhs) {
+  if (ne <= 0 || !out || E <= 0.0 || nu <= -1.0 || nu >= 0.5) return fail(CHS_EARG, "elasticity: arguments");
+  const int64_t m = (int64_t)ne + 1, nnode = m * m * m, n = 3 * nnode;
+  if (n > INT32_MAX / 81) return fail(CHS_EARG, "elasticity: too large");
+  const double h = 1.0 / ne;
+  double Ke[24][24];
+  q1_element_matrix(h, E, nu, Ke);
+  auto *mat = new (std::nothrow) chs_mat;
+  if (!mat) return fail(CHS_EALLOC, "alloc");
+  mat->nrows = mat->ncols = (int32_t)n;
+  auto clamped = [&](int64_t node) { return clamp && node / (m * m) == 0; };
+  // Row lengths: 3 per existing neighbour node (clamped neighbours dropped), 1 for clamped rows.
+  mat->rp.assign(n + 1, 0);
+  for (int64_t node = 0; node < nnode; ++node) {
+    const int64_t ix = node % m, iy = (node / m) % m, iz = node / (m * m);
+    int32_t cnt = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int64_t jx = ix + dx, jy = iy + dy, jz = iz + dz;
+          if (jx < 0 || jy < 0 || jz < 0 || jx >= m || jy >= m || jz >= m) continue;
+          if (!clamped(jx + m * (jy + m * jz))) cnt += 3;
+        }
+    for (int c = 0; c < 3; ++c) mat->rp[3 * node + c + 1] = clamped(node) ? 1 : cnt;
+  }
+  for (int64_t i = 0; i < n; ++i) mat->rp[i + 1] += mat->rp[i];
+  mat->ci.resize(mat->rp[n]);
+  mat->v.resize(mat->rp[n]);
+#pragma omp parallel for schedule(static)
+  for (int64_t node = 0; node < nnode; ++node) {
+    const int64_t ix = node % m, iy = (node / m) % m, iz = node / (m * m);
+    if (clamped(node)) {
+      for (int c = 0; c < 3; ++c) {
+        mat->ci[mat->rp[3 * node + c]] = (int32_t)(3 * node + c);
+        mat->v[mat->rp[3 * node + c]] = 1.0;
+      }
+      continue;
+    }
+    double blk[27][3][3] = {};
+    bool has[27] = {};
+    // Elements around the node: corner (ix - i, iy - j, iz - k), the node is local corner a.
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          const int64_t ex = ix - i, ey = iy - j, ez = iz - k;
+          if (ex < 0 || ey < 0 || ez < 0 || ex >= ne || ey >= ne || ez >= ne) continue;
+          const int a = i + 2 * j + 4 * k;
+          for (int b = 0; b < 8; ++b) {
+            const int bi = b & 1, bj = (b >> 1) & 1, bk = (b >> 2) & 1;
+            const int slot = (bi - i + 1) + 3 * (bj - j + 1) + 9 * (bk - k + 1);
+            has[slot] = true;
+            for (int c = 0; c < 3; ++c)
+              for (int d = 0; d < 3; ++d) blk[slot][c][d] += Ke[3 * a + c][3 * b + d];
+          }
+        }
+    for (int c = 0; c < 3; ++c) {
+      int64_t e = mat->rp[3 * node + c];
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int64_t jx = ix + dx, jy = iy + dy, jz = iz + dz;
+            if (jx < 0 || jy < 0 || jz < 0 || jx >= m || jy >= m || jz >= m) continue;
+            const int64_t jn = jx + m * (jy + m * jz);
+            if (clamped(jn)) continue;
+            const int slot = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+            for (int d = 0; d < 3; ++d) {
+              mat->ci[e] = (int32_t)(3 * jn + d);
+              mat->v[e] = has[slot] ? blk[slot][c][d] : 0.0;
+              ++e;
+            }
+          }
+    }
+  }
+  if (rhs) {
+    const double f = -h * h * h / 8.0;  // body force -1 in z, an eighth of each element volume
+#pragma omp parallel for schedule(static)
+    for (int64_t node = 0; node < nnode; ++node) {
+      const int64_t ix = node % m, iy = (node / m) % m, iz = node / (m * m);
+      const int64_t nel = (int64_t)(1 + (ix > 0 && ix < ne)) * (1 + (iy > 0 && iy < ne)) * (1 + (iz > 0 && iz < ne));
+      rhs[3 * node] = 0.0;
+      rhs[3 * node + 1] = 0.0;
+      rhs[3 * node + 2] = clamped(node) ? 0.0 : f * (double)nel;
+    }
+  }
+  *out = mat;
+  return CHS_OK;
+}
+
+// Rigid-body modes of the (ne+1)^3 nodes (sources/utils/mk_rbm_3d.m:1-27): three translations and three
+// rotations about the centroid, column-major n x 6.
+extern "C" void chs_gen_rbm(int32_t ne, double *rbm) {
+  const int64_t m = (int64_t)ne + 1, nnode = m * m * m, n = 3 * nnode;
+  const double h = 1.0 / ne, xb = 0.5;  // the centroid of the unit cube's nodes
+  for (int64_t q = 0; q < 6 * n; ++q) rbm[q] = 0.0;
+  for (int64_t node = 0; node < nnode; ++node) {
+    const double x = (node % m) * h - xb, y = ((node / m) % m) * h - xb, z = (node / (m * m)) * h - xb;
+    const int64_t d = 3 * node;
+    rbm[d + 0 * n] = 1.0;
+    rbm[d + 1 + 1 * n] = 1.0;
+    rbm[d + 2 + 2 * n] = 1.0;
+    rbm[d + 1 + 3 * n] = -z;
+    rbm[d + 2 + 3 * n] = y;
+    rbm[d + 0 + 4 * n] = z;
+    rbm[d + 2 + 4 * n] = -x;
+    rbm[d + 0 + 5 * n] = -y;
+    rbm[d + 1 + 5 * n] = x;
+  }
+}
+
+// ----------------------------------------------------------------------------------------
 // Sparse helpers
 // ----------------------------------------------------------------------------------------
 static void transpose_into(const chs_mat &M, chs_mat &T) {
@@ -389,6 +576,71 @@ extern "C" int chs_soc_cla(const chs_mat *Ap, double tau, int8_t *strong) {
       strong[e] = (j == i) ? 1 : (s > tau ? 1 : -1);
     }
   return CHS_OK;
+}
+
+// Strength by diagonal dominance (cpt_SoC_dom.m:26-52, tau >= 0): s_ij = |a_ij| /
+// sqrt(a_ii a_jj); entries with s_ij <= tau are weak.  The diagonal (s_ii = 1) is kept.
+extern "C" int chs_soc_dom(const chs_mat *Ap, double tau, int8_t *strong) {
+  if (!Ap || !strong) return fail(CHS_EARG, "null");
+  if (tau < 0.0) return fail(CHS_EARG, "soc_dom: tau < 0 (kept-entry count) is not supported");
+  const chs_mat &A = *Ap;
+  const int32_t n = A.nrows;
+  std::vector<double> d(n, 0.0);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e)
+      if (A.ci[e] == i) d[i] = A.v[e];
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+      const int32_t j = A.ci[e];
+      const double sij = std::fabs(A.v[e]) / std::sqrt(d[i] * d[j]);
+      strong[e] = (j == i) ? 1 : (sij > tau ? 1 : -1);
+    }
+  return CHS_OK;
+}
+
+extern "C" int chs_prol_bamg(const chs_mat *Ap, const int8_t *strong, const int32_t *fcnode,
+                             int32_t nc, int32_t ntv, const double *tv, const chs_bamg_params *bp,
+                             int nthreads, chs_mat **Pout, int32_t *c_mark, int32_t *nmarked) {
+  if (!Ap || !strong || !fcnode || !tv || !bp || !Pout) return fail(CHS_EARG, "null");
+  if (ntv < 1 || bp->dist_min < 0 || bp->dist_max < bp->dist_min)
+    return fail(CHS_EARG, "bamg: arguments");
+  const chs_mat &A = *Ap;
+  const int32_t n = A.nrows;
+  nthreads = nthreads_or_default(nthreads);
+  // Strength pattern with the diagonal, in A's (ascending) column order.
+  std::vector<int32_t> srp(n + 1, 0), sci;
+  sci.reserve(A.nnz());
+  for (int32_t i = 0; i < n; ++i) {
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e)
+      if (strong[e] > 0) sci.push_back(A.ci[e]);
+    srp[i + 1] = (int32_t)sci.size();
+  }
+  std::vector<double> tvr((size_t)n * ntv);  // row-major copy (cpt_Prolongation_BAMG.cpp:43-54)
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t j = 0; j < ntv; ++j) tvr[(size_t)i * ntv + j] = tv[(size_t)j * n + i];
+  chsb::Params prm{bp->itmax_vol, bp->tol_vol, bp->dist_min, bp->dist_max, ntv,
+                   bp->maxcond, bp->maxrownrm, bp->eps_prol};
+  std::vector<int32_t> mark(n, 0);
+  auto P = std::make_unique<chs_mat>();
+  P->nrows = n;
+  P->ncols = nc;
+  std::string err;
+  int rc = chsb::prolong(n, srp.data(), sci.data(), fcnode, ntv, tvr.data(), prm, nthreads, P->rp,
+                         P->ci, P->v, mark.data(), err);
+  if (rc) return fail(CHS_ENUM, err);
+  int32_t nm = 0;
+  for (int32_t i = 0; i < n; ++i) nm += mark[i];
+  if (c_mark) std::copy(mark.begin(), mark.end(), c_mark);
+  if (nmarked) *nmarked = nm;
+  *Pout = P.release();
+  return CHS_OK;
+}
+
+extern "C" int chs_orthonormalize(int64_t n, int32_t k, double *V) {
+  if (!V || n < 0 || k < 0) return fail(CHS_EARG, "orthonormalize: arguments");
+  return chsb::orthonormalize(n, k, V);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1116,6 +1368,61 @@ static int jacobi_factor(const chs_mat &A, chs_mat &G) {
   return CHS_OK;
 }
 
+// Prolongation smoothing (prolong.m:180-197): P <- P - omega D^-1 A P with omega =
+// 1.9 / lambda_max(D^-1 A); lambda_max by the same Lanczos as the smoother damping, on
+// D^-1/2 A D^-1/2 (the same spectrum).  The product is formed in MATLAB's order: the scaled
+// operator omega (a_ij / a_ii) first, then its product with P, then the difference, exact
+// zeros dropped.
+static int smooth_prolongation(const chs_mat &A, chs_mat &P, uint64_t seed, int nt, double *omega_out) {
+  chs_mat Dh;
+  int rc = jacobi_factor(A, Dh);
+  if (rc) return rc;
+  double lam = 0.0;
+  rc = chs_fsai_omega(&A, &Dh, seed, nt, nullptr, &lam);
+  if (rc) return rc;
+  const double omega = 1.9 / lam;
+  chs_mat W = A;
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int32_t i = 0; i < A.nrows; ++i) {
+    double d = 0.0;
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e)
+      if (A.ci[e] == i) d = A.v[e];
+    for (int32_t e = W.rp[i]; e < W.rp[i + 1]; ++e) W.v[e] = omega * (A.v[e] / d);
+  }
+  chs_mat WP;
+  spgemm(W, P, WP, nt);
+  chs_mat R;
+  R.nrows = P.nrows;
+  R.ncols = P.ncols;
+  R.rp.assign((size_t)P.nrows + 1, 0);
+  std::vector<std::vector<std::pair<int32_t, double>>> rows(P.nrows);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+  for (int32_t i = 0; i < P.nrows; ++i) {
+    auto &r = rows[i];
+    int32_t a = P.rp[i], ae = P.rp[i + 1], b = WP.rp[i], be = WP.rp[i + 1];
+    while (a < ae || b < be) {
+      const int32_t ca = a < ae ? P.ci[a] : INT32_MAX, cb = b < be ? WP.ci[b] : INT32_MAX;
+      double v;
+      int32_t c;
+      if (ca == cb) { c = ca; v = P.v[a++] - WP.v[b++]; }
+      else if (ca < cb) { c = ca; v = P.v[a++]; }
+      else { c = cb; v = -WP.v[b++]; }
+      if (v != 0.0) r.emplace_back(c, v);
+    }
+  }
+  for (int32_t i = 0; i < P.nrows; ++i) R.rp[i + 1] = R.rp[i] + (int32_t)rows[i].size();
+  R.ci.resize(R.rp[P.nrows]);
+  R.v.resize(R.rp[P.nrows]);
+  for (int32_t i = 0; i < P.nrows; ++i)
+    for (size_t q = 0; q < rows[i].size(); ++q) {
+      R.ci[R.rp[i] + q] = rows[i][q].first;
+      R.v[R.rp[i] + q] = rows[i][q].second;
+    }
+  P = std::move(R);
+  if (omega_out) *omega_out = omega;
+  return CHS_OK;
+}
+
 // ----------------------------------------------------------------------------------------
 // The recursion of cpt_aspAMG.m:68-351 for the classical + afsai_sym / jacobi path.
 // ----------------------------------------------------------------------------------------
@@ -1127,6 +1434,19 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
   auto h = std::make_unique<chs_hier>();
   const double t_start = now_s();
   chs_mat A = *A0;
+  // Test space (cpt_aspAMG.m:29-46): the caller's columns, padded with uniform draws, then
+  // orthonormalised at every level (tspace method NONE) and injected to the coarse nodes.
+  std::vector<double> tv;
+  if (p.prolong == 2) {
+    if (p.ntv < 1) return fail(CHS_EARG, "bamg: ntv < 1");
+    const int64_t n0 = A.nrows;
+    tv.assign((size_t)n0 * p.ntv, 0.0);
+    const int32_t ncopy = p.tv ? std::min(p.ntv0, p.ntv) : 0;
+    for (int32_t j = 0; j < ncopy; ++j)
+      std::copy(p.tv + (size_t)j * n0, p.tv + (size_t)(j + 1) * n0, tv.begin() + (size_t)j * n0);
+    if (ncopy < p.ntv)
+      chs_rng_uniform(p.seed, 7, 0, (int64_t)(p.ntv - ncopy) * n0, 0.0, 1.0, tv.data() + (size_t)ncopy * n0);
+  }
   for (int level = 1;; ++level) {
     chs_hier::Level L;
     const int32_t n = A.nrows;
@@ -1176,7 +1496,9 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     }
     double t1 = now_s();
     std::vector<int8_t> strong(A.nnz());
-    chs_soc_cla(&A, p.soc_tau, strong.data());
+    int rcs = p.soc_type == 1 ? chs_soc_dom(&A, p.soc_tau, strong.data())
+                              : chs_soc_cla(&A, p.soc_tau, strong.data());
+    if (rcs) return rcs;
     std::vector<double> r01(n);
     chs_rng_uniform(p.seed, 100 + level, 0, n, 0.0, 1.0, r01.data());
     std::vector<int32_t> fc(n);
@@ -1186,9 +1508,31 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     if (nc <= 0 || nc >= n) return fail(CHS_ECONSIST, "coarsening produced no reduction at level " +
                                                          std::to_string(level));
     chs_mat *P = nullptr;
-    int rc = p.prolong == 1 ? chs_prol_exti(&A, strong.data(), fc.data(), nc, nt, &P)
-                            : chs_prol_classical(&A, strong.data(), fc.data(), nc, nt, &P);
+    int32_t nmarked = 0;
+    int rc;
+    if (p.prolong == 2) {
+      chs_orthonormalize(n, p.ntv, tv.data());
+      rc = chs_prol_bamg(&A, strong.data(), fc.data(), nc, p.ntv, tv.data(), &p.bamg, nt, &P,
+                         nullptr, &nmarked);
+      if (!rc) {  // next level's test space: the coarse nodes' rows (cpt_aspAMG.m:242-243)
+        std::vector<double> tvc((size_t)nc * p.ntv);
+        for (int32_t j = 0; j < p.ntv; ++j)
+          for (int32_t i = 0; i < n; ++i)
+            if (fc[i] >= 0) tvc[(size_t)j * nc + fc[i]] = tv[(size_t)j * n + i];
+        tv.swap(tvc);
+      }
+    } else {
+      rc = p.prolong == 1 ? chs_prol_exti(&A, strong.data(), fc.data(), nc, nt, &P)
+                          : chs_prol_classical(&A, strong.data(), fc.data(), nc, nt, &P);
+    }
     if (rc) return rc;
+    if (p.prol_smooth == 1) {
+      rc = smooth_prolongation(A, *P, p.seed + 1000 + level, nt, nullptr);
+      if (rc) { delete P; return rc; }
+    } else if (p.prol_smooth != 0) {
+      delete P;
+      return fail(CHS_EARG, "prolongation energy minimisation (EMIN) is not supported");
+    }
     double t3 = now_s();
     chs_mat *Ac = nullptr;
     rc = chs_galerkin(&A, P, nt, &Ac);
@@ -1200,9 +1544,9 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     h->t[3] += t4 - t3;
     if (p.verbose)
       std::fprintf(stderr, "level %d: n=%d nnz=%lld nnzG=%lld nc=%d nnzP=%lld omega=%.6f "
-                   "(smoother %.2fs coarsen %.2fs prol %.2fs galerkin %.2fs)\n",
+                   "inexact=%d (smoother %.2fs coarsen %.2fs prol %.2fs galerkin %.2fs)\n",
                    level, n, (long long)A.nnz(), (long long)L.G.nnz(), nc,
-                   (long long)P->nnz(), L.omega, t1 - t0, t2 - t1, t3 - t2, t4 - t3);
+                   (long long)P->nnz(), L.omega, nmarked, t1 - t0, t2 - t1, t3 - t2, t4 - t3);
     L.A = std::move(A);
     L.P = std::move(*P);
     delete P;

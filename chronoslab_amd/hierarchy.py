@@ -171,6 +171,26 @@ def diffusion(N: int, k: np.ndarray) -> Csr:
     return _take(out)
 
 
+def elasticity(ne: int, E: float = 1.0, nu: float = 0.3, clamp: bool = True) -> tuple[Csr, np.ndarray]:
+    """Q1 linear elasticity on ne^3 cube elements ((ne+1)^3 nodes x 3 dofs), uniform body force
+    -1 in z, the z = 0 face clamped as unit rows (the elasticity configuration).  Returns
+    (A, rhs)."""
+    n = 3 * (ne + 1) ** 3
+    rhs = np.empty(n, np.float64)
+    out = C.c_void_p()
+    _check(_lib().chs_gen_elasticity(ne, C.c_double(E), C.c_double(nu), int(clamp), C.byref(out),
+                                     rhs.ctypes.data_as(_F64P)))
+    return _take(out), rhs
+
+
+def rigid_body_modes(ne: int) -> np.ndarray:
+    """The 6 rigid-body modes of the elasticity grid's nodes (mk_rbm_3d.m), n x 6."""
+    n = 3 * (ne + 1) ** 3
+    R = np.empty((n, 6), np.float64, order="F")
+    _lib().chs_gen_rbm(ne, R.ctypes.data_as(_F64P))
+    return R
+
+
 def heterogeneous_diffusion(N: int, sigma: float, seed: int) -> tuple[Csr, np.ndarray]:
     lnk = lnk_field(N, sigma, seed)
     return diffusion(N, np.exp(lnk)), lnk
@@ -184,6 +204,38 @@ def soc_cla(A: Csr, tau: float) -> np.ndarray:
     out = np.empty(A.nnz, np.int8)
     _check(_lib().chs_soc_cla(h.ptr, C.c_double(tau), out.ctypes.data_as(C.POINTER(C.c_int8))))
     return out
+
+
+def soc_dom(A: Csr, tau: float) -> np.ndarray:
+    h = _to_handle(A)
+    out = np.empty(A.nnz, np.int8)
+    _check(_lib().chs_soc_dom(h.ptr, C.c_double(tau), out.ctypes.data_as(C.POINTER(C.c_int8))))
+    return out
+
+
+def prol_bamg(A: Csr, strong: np.ndarray, fcnode: np.ndarray, nc: int, tv: np.ndarray,
+              params: SetupParams | None = None, nthreads: int = 0) -> tuple[Csr, np.ndarray]:
+    """BAMG least-squares prolongation from the test space tv (n x k); returns (P, c_mark)."""
+    p = params or SetupParams()
+    h = _to_handle(A)
+    strong = np.ascontiguousarray(strong, np.int8)
+    fcnode = np.ascontiguousarray(fcnode, np.int32)
+    tv = np.asfortranarray(tv, dtype=np.float64)
+    bp = _CBamg(p.itmax_vol, p.tol_vol, p.dist_min, p.dist_max, p.maxcond, p.maxrownrm, p.eps_prol)
+    mark = np.zeros(A.nrows, np.int32)
+    nm = C.c_int32()
+    out = C.c_void_p()
+    _check(_lib().chs_prol_bamg(h.ptr, strong.ctypes.data_as(C.POINTER(C.c_int8)), fcnode.ctypes.data_as(_I32P),
+                                nc, tv.shape[1], tv.ctypes.data_as(_F64P), C.cast(C.pointer(bp), C.c_void_p),
+                                nthreads, C.byref(out), mark.ctypes.data_as(_I32P), C.byref(nm)))
+    return _take(out), mark
+
+
+def orthonormalize(V: np.ndarray) -> np.ndarray:
+    """Q factor (positive R diagonal) of the thin QR of V, as the test-space set-up forms it."""
+    Q = np.array(V, dtype=np.float64, order="F", copy=True)
+    _lib().chs_orthonormalize(Q.shape[0], Q.shape[1], Q.ctypes.data_as(_F64P))
+    return Q
 
 
 def pmis(A: Csr, strong: np.ndarray, rand01: np.ndarray) -> tuple[np.ndarray, int]:
@@ -268,7 +320,8 @@ class SetupParams:
     nupre: int = 1
     nupost: int = 1
     smoother: str = "afsai_sym"
-    prolong: str = "clas"  # parm_PROLONG proltype: "clas" or "exti"
+    prolong: str = "clas"  # parm_PROLONG proltype: "clas", "exti" or "bamg"
+    soc: str = "cla"  # parm_COARSEN SoC_type: "cla" or "dom" (tau in soc_tau)
     nthreads: int = 0
     seed: int = 20240607
     verbose: bool = False
@@ -277,6 +330,25 @@ class SetupParams:
     gpu_afsai: bool = False
     # Compute the smoother damping's lambda_max by Lanczos on the GPU (chr_setup_omega).
     gpu_omega: bool = False
+    # BAMG: finest-level test space (n x k; padded with uniform draws up to ntv columns) and
+    # parm_PROLONG's BAMG fields.
+    tv: np.ndarray | None = None
+    ntv: int = 6
+    itmax_vol: int = 100
+    tol_vol: float = 0.01
+    dist_min: int = 1
+    dist_max: int = 10
+    maxcond: float = 1e13
+    maxrownrm: float = 5.0
+    eps_prol: float = 1e-10
+    # parm_PROLONG prol_emin: "none" or "smooth" (EMIN is not restated)
+    prol_emin: str = "none"
+
+
+class _CBamg(C.Structure):
+    _fields_ = [("itmax_vol", C.c_int32), ("tol_vol", C.c_double), ("dist_min", C.c_int32),
+                ("dist_max", C.c_int32), ("maxcond", C.c_double), ("maxrownrm", C.c_double),
+                ("eps_prol", C.c_double)]
 
 
 class _CParams(C.Structure):
@@ -285,7 +357,8 @@ class _CParams(C.Structure):
                 ("nupre", C.c_int32), ("nupost", C.c_int32), ("smoother", C.c_int32), ("prolong", C.c_int32),
                 ("nthreads", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32),
                 ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p), ("omega_fn", C.c_void_p),
-                ("omega_ctx", C.c_void_p)]
+                ("omega_ctx", C.c_void_p), ("soc_type", C.c_int32), ("ntv", C.c_int32), ("ntv0", C.c_int32),
+                ("tv", _F64P), ("bamg", _CBamg), ("prol_smooth", C.c_int32)]
 
 
 def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
@@ -300,9 +373,16 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
         g = load_amg_lib()
         fn = C.cast(g.chr_setup_afsai, C.c_void_p).value if p.gpu_afsai else None
         ofn = C.cast(g.chr_setup_omega, C.c_void_p).value if p.gpu_omega else None
-    pr = {"clas": 0, "exti": 1}[p.prolong.lower()]
+    pr = {"clas": 0, "exti": 1, "bamg": 2}[p.prolong.lower()]
+    soc = {"cla": 0, "dom": 1}[p.soc.lower()]
+    tv = None
+    if p.tv is not None:
+        tv = np.asfortranarray(p.tv, dtype=np.float64).reshape(A.nrows, -1, order="F")
     cp = _CParams(p.nlev_max, p.max_coarse, p.soc_tau, p.nstep, p.step_size, p.fsai_eps, p.nupre,
-                  p.nupost, sm, pr, p.nthreads, p.seed, int(p.verbose), fn, None, ofn, None)
+                  p.nupost, sm, pr, p.nthreads, p.seed, int(p.verbose), fn, None, ofn, None, soc, p.ntv,
+                  0 if tv is None else tv.shape[1], None if tv is None else tv.ctypes.data_as(_F64P),
+                  _CBamg(p.itmax_vol, p.tol_vol, p.dist_min, p.dist_max, p.maxcond, p.maxrownrm, p.eps_prol),
+                  {"none": 0, "smooth": 1, "emin": 2}[p.prol_emin.lower()])
     lib = _lib()
     ha = _to_handle(A)
     hh = C.c_void_p()

@@ -12,6 +12,11 @@
  * Reference counterparts (ChronosLab, paths relative to the reference root):
  *   chs_build                 sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:68-351
  *   chs_soc_cla               sources/Preconditioner/AMG/coarsen/cpt_SoC_cla.m:1-90
+ *   chs_soc_dom               sources/Preconditioner/AMG/coarsen/cpt_SoC_dom.m:1-80
+ *   chs_prol_bamg             sources/Preconditioner/AMG/prolong/MEX_BAMG_Prol/BAMG.cpp:16-241
+ *                             (called by cpt_ProlMEX_BAMG.m:1-70 from prolong.m:37-41)
+ *   chs_orthonormalize        sources/Preconditioner/AMG/tspace/tspace.m ('none': qr(V0,0)),
+ *                             cpt_aspAMG.m:29-46
  *   chs_pmis                  sources/Preconditioner/AMG/coarsen/cpt_PMIS.m:1-102
  *                             + mark_isolated_nodes.m:1-8
  *   chs_prol_classical        sources/Preconditioner/AMG/prolong/MEX_CLAS_prol/
@@ -24,6 +29,7 @@
  *   chs_fsai_omega            sources/Preconditioner/AMG/smoother/smoother.m:72-81
  *   chs_galerkin              sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:277
  *   chs_bc_fix                sources/utils/computeAMG.m:5-10
+ *   chs_gen_rbm               sources/utils/mk_rbm_3d.m:1-27
  */
 #ifndef CHRONOS_SETUP_H
 #define CHRONOS_SETUP_H
@@ -39,7 +45,8 @@ enum {
   CHS_EARG = -1,     /* bad argument */
   CHS_EALLOC = -2,   /* allocation failure */
   CHS_ENOTSPD = -4,  /* FSAI found a non positive definite block (reference: is_posdef=false) */
-  CHS_ECONSIST = -6  /* F/C count inconsistency (reference: cpt_aspAMG.m:212-215) */
+  CHS_ECONSIST = -6, /* F/C count inconsistency (reference: cpt_aspAMG.m:212-215) */
+  CHS_ENUM = -7      /* singular least-squares fit (reference: "error in LAPACKE_dgels") */
 };
 
 /* Opaque CSR matrix owned by the library. */
@@ -64,6 +71,19 @@ typedef int (*chs_omega_fn)(void *ctx, int32_t n, const int32_t *rowptr_A, const
 
 /* Setup parameters: the fields of the reference's parm_AMG / parm_COARSEN / parm_SMOOTH
  * files that the classical + aFSAI path reads (Tests/parm_*, examples/AMG/parm_*). */
+/* parm_PROLONG's BAMG fields (Tests/parm_PROLONG lines 4-10; defaults in brackets).  The
+ * number of interpolation points per row is bounded by the number of test vectors
+ * (cpt_ProlMEX_BAMG.m:45). */
+typedef struct chs_bamg_params {
+  int32_t itmax_vol;    /* maximum volume iterations [100] */
+  double tol_vol;       /* maximum volume tolerance [0.01] */
+  int32_t dist_min;     /* belts of strong neighbours before the first fit [1] */
+  int32_t dist_max;     /* belts at most [10] */
+  double maxcond;       /* conditioning bound of the selected test-vector rows [1e13] */
+  double maxrownrm;     /* bound on a row's weight 2-norm [5] */
+  double eps_prol;      /* relative fit residual taken as exact [1e-10] */
+} chs_bamg_params;
+
 typedef struct chs_params {
   int32_t nlev_max;     /* parm_AMG line 1  (nLevMax, 10) */
   int32_t max_coarse;   /* parm_AMG line 2  (maxCoarseSZ, 1000) */
@@ -74,7 +94,7 @@ typedef struct chs_params {
   int32_t nupre;        /* parm_SMOOTH line 2 */
   int32_t nupost;       /* parm_SMOOTH line 3 */
   int32_t smoother;     /* 0 = afsai_sym, 1 = jacobi */
-  int32_t prolong;      /* parm_PROLONG proltype: 0 = CLAS, 1 = EXTI */
+  int32_t prolong;      /* parm_PROLONG proltype: 0 = CLAS, 1 = EXTI, 2 = BAMG */
   int32_t nthreads;     /* OpenMP threads (<=0: runtime default) */
   uint64_t seed;        /* PMIS tie-break stream and Lanczos start vector */
   int32_t verbose;
@@ -82,6 +102,14 @@ typedef struct chs_params {
   void *afsai_ctx;       /* passed to afsai_fn */
   chs_omega_fn omega_fn; /* NULL: chs_fsai_omega on the host */
   void *omega_ctx;       /* passed to omega_fn */
+  int32_t soc_type;      /* parm_COARSEN SoC_type: 0 = CLA, 1 = DOM (tau in soc_tau) */
+  int32_t ntv;           /* parm_TSPACE ntv: test vectors (BAMG) */
+  int32_t ntv0;          /* columns of tv */
+  const double *tv;      /* finest-level test space, n x ntv0 column-major; columns beyond
+                            ntv0 (or all, when NULL) are uniform [0,1) draws of stream 7 */
+  chs_bamg_params bamg;  /* prolong = 2 */
+  int32_t prol_smooth;   /* parm_PROLONG prol_emin: 0 = NONE, 1 = SMOOTH (P - 1.9/lambda_max
+                            D^-1 A P, prolong.m:180-197); EMIN is not restated */
 } chs_params;
 
 void chs_params_default(chs_params *p);
@@ -114,11 +142,36 @@ int chs_gen_poisson7(int32_t N, chs_mat **out);
 int chs_gen_diffusion(int32_t N, const double *k, chs_mat **out);
 /* ln k ~ N(0, sigma^2) i.i.d. from the RNG stream (seed, 1). */
 void chs_gen_lnk(int32_t N, double sigma, uint64_t seed, double *lnk);
+/* Linear elasticity, trilinear hexahedra on a unit cube of ne^3 elements: (ne+1)^3 nodes x
+ * 3 dofs (dof 3*node + component, node = ix + m*(iy + m*iz), m = ne+1), Young's modulus
+ * E, Poisson ratio nu, 2x2x2 Gauss rule, all 27 neighbour nodes x 3 components in each
+ * row.  rhs (may be NULL, n entries): uniform body force -1 in z.  clamp != 0: the z = 0
+ * face's dofs become unit rows and columns with zero load. */
+int chs_gen_elasticity(int32_t ne, double E, double nu, int32_t clamp, chs_mat **out,
+                       double *rhs);
+/* Rigid-body modes of those nodes about the cube's centroid (mk_rbm_3d.m): column-major
+ * n x 6, translations x, y, z then rotations. */
+void chs_gen_rbm(int32_t ne, double *rbm);
 
 /* ---- setup components ------------------------------------------------------------- */
 /* Classical strength of connection.  strong[e] = +1 strong / -1 weak for every stored
  * entry e of A (diagonal forced to +1), i.e. the sign pattern S_patt aligned with A. */
 int chs_soc_cla(const chs_mat *A, double tau, int8_t *strong);
+/* Strength by diagonal dominance: s_ij = |a_ij| / sqrt(a_ii a_jj), strong when s_ij > tau
+ * (tau >= 0; the reference's tau < 0 variant, a kept-entry count, is not restated).  Same
+ * output convention as chs_soc_cla. */
+int chs_soc_dom(const chs_mat *A, double tau, int8_t *strong);
+/* Bootstrap-AMG least-squares prolongation.  Strength pattern: the entries of A with
+ * strong[e] > 0.  tv: n x ntv column-major test space.  P (n x nc) interpolates every fine
+ * node from at most ntv coarse nodes of its neighbourhood, grown belt by belt (dist_min to
+ * dist_max), with the weights fitting the test vectors in least squares.  c_mark (n entries,
+ * may be NULL) flags the fine rows whose fit stays inexact; *nmarked counts them. */
+int chs_prol_bamg(const chs_mat *A, const int8_t *strong, const int32_t *fcnode, int32_t nc,
+                  int32_t ntv, const double *tv, const chs_bamg_params *bp, int nthreads,
+                  chs_mat **P, int32_t *c_mark, int32_t *nmarked);
+/* Orthonormal basis of the columns of V (n x k column-major, in place), the Q of a thin QR
+ * with positive R diagonal (tspace method NONE).  Returns 1 if the columns are dependent. */
+int chs_orthonormalize(int64_t n, int32_t k, double *V);
 /* Dirichlet marking + sequential PMIS.  rand01[i] is the tie-break added to node i's
  * weight.  Output fcnode[i] = c >= 0 (coarse index) or -1 (fine); returns nc in *nc. */
 int chs_pmis(const chs_mat *A, const int8_t *strong, const double *rand01,

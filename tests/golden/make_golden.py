@@ -13,7 +13,9 @@ EXTI_prolongation core (MEX_EXTI_Prol/EXTI_prolongation.cpp:52-199) returns for 
 Inputs: a 7-point Poisson cube, a heterogeneous-diffusion cube, and the leading 1200 x 1200
 block of the reference test matrix examples/matrices/Flux.mat (the FlowCLAS_* tests'
 matrix, read as data with scipy.io.loadmat).  flux.npz holds that whole matrix (Amat, as
-CSR) and its right-hand side b: the data of Tests/FlowCLAS_0.m.
+CSR) and its right-hand side b: the data of Tests/FlowCLAS_0.m.  cubo4820.npz holds
+examples/matrices/Cubo_4820.mat (the upper triangle of the symmetric A, the test space TV0
+and rhs): the data of Tests/CubeBAMG_NONE.m and Tests/CubeBAMG_SMOOTH.m.
 """
 import os
 import sys
@@ -42,7 +44,27 @@ def inputs():
         yield "flux_block1200", H.Csr.from_scipy(A), 0.25
 
 
+def data_fixtures():
+    """The reference test matrices the known-answer tests run on, as plain arrays."""
+    import scipy.io
+    import scipy.sparse as sp
+
+    cubo = os.path.join(REF, "examples", "matrices", "Cubo_4820.mat")
+    if os.path.exists(cubo):
+        d = scipy.io.loadmat(cubo, variable_names=["A", "TV0", "rhs"])
+        A = d["A"].tocsr()
+        assert abs(A - A.T).max() == 0.0  # symmetric: the upper triangle holds it
+        U = sp.triu(A).tocsr()
+        U.sort_indices()
+        np.savez_compressed(os.path.join(HERE, "cubo4820.npz"), n=A.shape[0],
+                            U_rowptr=U.indptr.astype(np.int32), U_col=U.indices.astype(np.int32),
+                            U_val=U.data.astype(np.float64), TV0=np.asarray(d["TV0"], np.float64),
+                            rhs=d["rhs"].ravel().astype(np.float64))
+        print("cubo4820", A.shape, A.nnz)
+
+
 def main():
+    data_fixtures()
     if not (oracle.ref_available() and oracle.ref_exti_available()):
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
     for name, A, tau in inputs():
