@@ -551,5 +551,12 @@ extern "C" int chr_setup_afsai(void *ctx, int32_t n, const int32_t *rowptr, cons
                                int32_t *is_posdef) {
   (void)ctx;
   const chr_csr A{n, n, rowptr, col, val};
+  // Rows too long for one workgroup's LDS work space: the host set-up factors this level.
+  if (nstep >= 0 && step_size >= 1 && nstep * step_size <= chr::kFsaiMaxM && n > 0 && rowptr) {
+    int dev = 0, max_lds = 0;
+    HIPC(hipGetDevice(&dev));
+    HIPC(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (fsai_shape(A, std::max(1, nstep * step_size)).lds_bytes() > (size_t)max_lds) return 1;
+  }
   return chr_afsai(&A, nstep, step_size, eps, rowptr_G, col_G, val_G, nnz_G, is_posdef);
 }

@@ -1459,33 +1459,41 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     if (p.smoother == 1) {
       int rc = jacobi_factor(A, L.G);
       if (rc) return rc;
-    } else if (p.afsai_fn) {
-      if (p.nstep < 0 || p.step_size < 1) return fail(CHS_EARG, "afsai: arguments");
-      const int64_t kmax = (int64_t)std::max(1, p.nstep * p.step_size) + 1;
-      int32_t pd = 1;
-      int64_t nnz_g = 0;
-      L.G.nrows = L.G.ncols = n;
-      L.G.rp.assign((size_t)n + 1, 0);
-      L.G.ci.resize((size_t)n * kmax);
-      L.G.v.resize((size_t)n * kmax);
-      int rc = p.afsai_fn(p.afsai_ctx, n, A.rp.data(), A.ci.data(), A.v.data(), p.nstep, p.step_size,
-                          p.fsai_eps, L.G.rp.data(), L.G.ci.data(), L.G.v.data(), &nnz_g, &pd);
-      if (rc) return fail(rc, std::string("afsai (external): error ") + std::to_string(rc));
-      L.G.ci.resize((size_t)nnz_g);
-      L.G.v.resize((size_t)nnz_g);
-      L.G.ci.shrink_to_fit();
-      L.G.v.shrink_to_fit();
-      if (!pd) return fail(CHS_ENOTSPD, "afsai_sym: matrix deemed indefinite at level " +
-                                            std::to_string(level));
     } else {
-      chs_mat *G = nullptr;
-      int32_t pd = 1;
-      int rc = chs_afsai(&A, p.nstep, p.step_size, p.fsai_eps, nt, &G, &pd);
-      if (rc) return rc;
-      L.G = std::move(*G);
-      delete G;
-      if (!pd) return fail(CHS_ENOTSPD, "afsai_sym: matrix deemed indefinite at level " +
-                                            std::to_string(level));
+      bool host = true;
+      if (p.afsai_fn) {
+        if (p.nstep < 0 || p.step_size < 1) return fail(CHS_EARG, "afsai: arguments");
+        const int64_t kmax = (int64_t)std::max(1, p.nstep * p.step_size) + 1;
+        int32_t pd = 1;
+        int64_t nnz_g = 0;
+        L.G.nrows = L.G.ncols = n;
+        L.G.rp.assign((size_t)n + 1, 0);
+        L.G.ci.resize((size_t)n * kmax);
+        L.G.v.resize((size_t)n * kmax);
+        int rc = p.afsai_fn(p.afsai_ctx, n, A.rp.data(), A.ci.data(), A.v.data(), p.nstep, p.step_size,
+                            p.fsai_eps, L.G.rp.data(), L.G.ci.data(), L.G.v.data(), &nnz_g, &pd);
+        if (rc < 0) return fail(rc, std::string("afsai (external): error ") + std::to_string(rc));
+        if (rc == 0) {
+          host = false;
+          L.G.ci.resize((size_t)nnz_g);
+          L.G.v.resize((size_t)nnz_g);
+          L.G.ci.shrink_to_fit();
+          L.G.v.shrink_to_fit();
+          if (!pd) return fail(CHS_ENOTSPD, "afsai_sym: matrix deemed indefinite at level " +
+                                                std::to_string(level));
+        }
+        // rc == 1: the replacement declined this level's row lengths; the host factors it.
+      }
+      if (host) {
+        chs_mat *G = nullptr;
+        int32_t pd = 1;
+        int rc = chs_afsai(&A, p.nstep, p.step_size, p.fsai_eps, nt, &G, &pd);
+        if (rc) return rc;
+        L.G = std::move(*G);
+        delete G;
+        if (!pd) return fail(CHS_ENOTSPD, "afsai_sym: matrix deemed indefinite at level " +
+                                              std::to_string(level));
+      }
     }
     if (p.omega_fn) {
       int rc = p.omega_fn(p.omega_ctx, n, A.rp.data(), A.ci.data(), A.v.data(), L.G.rp.data(),

@@ -57,7 +57,8 @@ typedef struct chs_hier chs_hier;
 /* Replacement of the aFSAI factor computation inside chs_build (e.g. the GPU kernel of
  * libchronos_amg, chr_setup_afsai in chronos_amg.h).  Contract of chr_afsai: A in CSR,
  * outputs rowptr_G[n+1] and col_G / val_G of n * (nstep*step_size + 1) entries, nnz_G the
- * entries written; returns 0 or a negative code. */
+ * entries written; returns 0, a negative code, or 1 when the level's rows are longer than
+ * the replacement handles (chs_build then factors that level on the host). */
 typedef int (*chs_afsai_fn)(void *ctx, int32_t n, const int32_t *rowptr, const int32_t *col,
                             const double *val, int32_t nstep, int32_t step_size, double eps,
                             int32_t *rowptr_G, int32_t *col_G, double *val_G, int64_t *nnz_G,

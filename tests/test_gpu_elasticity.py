@@ -1,0 +1,95 @@
+"""GPU parity on the elasticity hierarchies (BAMG prolongation with rigid-body test vectors,
+prolongation smoothing, aFSAI): the HIP V-cycle and PCG against the CPU oracle on the same
+hierarchy, the reference's Cubo_4820 known answers (Tests/CubeBAMG_NONE.m: 43 SQMR
+iterations, Tests/CubeBAMG_SMOOTH.m: 29) with the GPU V-cycle, and the GPU set-up on levels
+whose rows are too long for the per-row aFSAI kernel (the host factors those levels; the
+factors of every level equal the all-host set-up's bit for bit)."""
+import numpy as np
+import pytest
+
+import oracle
+from chronoslab_amd import amg, hierarchy as H
+from test_bamg import cubo, rule
+
+pytestmark = pytest.mark.gpu
+
+
+def _elasticity(ne, **kw):
+    A, rhs = H.elasticity(ne)
+    Afix, _ = H.bc_fix(A, 1)
+    p = H.SetupParams(prolong="bamg", soc="dom", soc_tau=0.01, prol_emin="smooth", tv=H.rigid_body_modes(ne), **kw)
+    return A, rhs, H.build(Afix, p)
+
+
+@pytest.mark.parametrize("ne", [12])
+def test_vcycle_matches_oracle(gpu, ne):
+    A, rhs, hier = _elasticity(ne)
+    ref = oracle.Hier(hier)
+    b = H.rng_uniform(5, 2, A.nrows, -1, 1)
+    xr = ref.vcycle(b)
+    x64 = gpu.AMGPreconditioner(hier, "fp64")(b)
+    x32 = gpu.AMGPreconditioner(hier, "fp32")(b)
+    r64 = np.linalg.norm(x64 - xr) / np.linalg.norm(xr)
+    r32 = np.linalg.norm(x32 - xr) / np.linalg.norm(xr)
+    print(f"elasticity {ne}: levels {[l.A.nrows for l in hier.levels]} V-cycle rel diff fp64 {r64:.2e} "
+          f"fp32 {r32:.2e}", flush=True)
+    assert r64 <= 1e-12 and r32 <= 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pcg_matches_oracle(gpu, prec):
+    A, rhs, hier = _elasticity(16)
+    ref = oracle.Hier(hier)
+    xr, itr, rvr, flr = ref.pcg(A, rhs, itmax=200, tol=1e-8)
+    M = gpu.AMGPreconditioner(hier, prec)
+    x, it, rv, fl = M.pcg(rhs, itmax=200, tol=1e-8)
+    err = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    tres = np.linalg.norm(rhs - A.matvec(x)) / np.linalg.norm(rhs)
+    print(f"elasticity 16 {prec}: it {it} oracle {itr} err {err:.3e} true-res {tres:.3e}", flush=True)
+    assert fl == 0 and abs(it - itr) <= 1
+    assert tres <= 1e-8
+    if prec == "fp64":
+        assert err <= 1e-10
+        np.testing.assert_allclose(rv[: min(it, itr) + 1], rvr[: min(it, itr) + 1], rtol=1e-7)
+
+
+@pytest.mark.parametrize("emin,ref_iter", [("none", 43), ("smooth", 29)])
+def test_cube_known_answer_gpu_vcycle(gpu, emin, ref_iter):
+    As, TV0, rhs = cubo()
+    Afix, _ = H.bc_fix(H.Csr.from_scipy(As), 1)
+    hier = H.build(Afix, H.SetupParams(prolong="bamg", soc="dom", soc_tau=0.01, seed=1, prol_emin=emin, tv=TV0))
+    xo, info_o, _, it_o, _ = amg.SQMR(lambda v: As @ v, rhs, 1e-6, 100, oracle.Hier(hier).vcycle)
+    for prec in ("fp64", "fp32"):
+        M = gpu.AMGPreconditioner(hier, prec)
+        x, info, relres, niter, rv = amg.SQMR(lambda v: As @ v, rhs, 1e-6, 100, M)
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        print(f"Cubo_4820 BAMG {emin} {prec}: SQMR {niter} (oracle cycle {it_o}, reference {ref_iter}), "
+              f"relres {relres:.2e}, err vs oracle-cycle solve {err:.2e}", flush=True)
+        assert info == 0 and rule(ref_iter, niter)
+        if prec == "fp64":
+            assert niter == it_o and err <= 1e-10
+        else:
+            assert abs(niter - it_o) <= 1
+
+
+def test_gpu_setup_long_rows_declined_to_host(gpu):
+    ne = 16
+    _, _, hh = _elasticity(ne)
+    _, _, hg = _elasticity(ne, gpu_afsai=True, gpu_omega=True)
+    assert hh.nlev == hg.nlev
+    for l, (a, b) in enumerate(zip(hh.levels[:-1], hg.levels[:-1])):
+        assert np.array_equal(a.G.rowptr, b.G.rowptr) and np.array_equal(a.G.col, b.G.col)
+        nd = int(np.sum(a.G.val != b.G.val))
+        if nd:  # name the side that moved: direct host and GPU factors of the same level
+            gh, _ = H.afsai(a.A)
+            gg, _ = gpu.afsai(a.A)
+            bad = np.flatnonzero(a.G.val != b.G.val)
+            rows = np.unique(np.searchsorted(a.G.rowptr, bad, side="right") - 1)
+            print(f"level {l}: {nd} entries differ in rows {rows[:10].tolist()}...; direct host vs "
+                  f"build-host {int(np.sum(gh.val != a.G.val))}, direct GPU vs build-GPU "
+                  f"{int(np.sum(gg.val != b.G.val))}, direct host vs direct GPU {int(np.sum(gh.val != gg.val))}; "
+                  f"max rel {np.max(np.abs(a.G.val - b.G.val) / np.abs(a.G.val).clip(1e-300)):.2e}", flush=True)
+        assert nd == 0
+        assert abs(a.omega - b.omega) <= 1e-6 * a.omega
+    # the second level's rows (hundreds of entries) exceed the per-row kernel's work space
+    assert np.diff(hh.levels[1].A.rowptr).max() > 300
