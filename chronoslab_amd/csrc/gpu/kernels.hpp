@@ -132,6 +132,24 @@ __device__ __forceinline__ T block_sum_t(T v, T *lds) {
   return s;
 }
 
+// Sum of prod[ra..rz) in index order.  The LDS reads are issued 16 at a time ahead of the
+// additions, so a long row waits for one LDS round trip per 16 elements instead of one per
+// element; the additions themselves stay sequential (same bits as a plain loop).
+template <typename T>
+__device__ __forceinline__ T row_sum(const T *prod, int ra, int rz) {
+  T s = 0;
+  for (int q = ra; q < rz; q += 16) {
+    const int n = min(16, rz - q);
+    T v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = prod[q + min(k, n - 1)];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < n) s += v[k];
+  }
+  return s;
+}
+
 // Stages the products of the block's nonzeros [e0, e1) in prod[].  Indices are clamped to
 // the last nonzero so that every load is unconditional: all 2*PER index/value loads issue
 // back to back, then the PER gathers, then the LDS stores (masked).
@@ -195,11 +213,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   const double pv = epi.pre(rc);
   stream_products<PER, TA>(e0, e1, base, ci, val, ColLoad<TA, TX>{x}, prod);
   __syncthreads();
-  if (r < r1) {
-    TA s = 0;
-    for (int q = ra; q < rz; ++q) s += prod[q];
-    epi(r, (double)s, pv);
-  }
+  if (r < r1) epi(r, (double)row_sum<TA>(prod, ra, rz), pv);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -263,11 +277,7 @@ __global__ __launch_bounds__(kBlock) void spmv_pack_kernel(int nblk, const int4 
     }
   }
   __syncthreads();
-  if (r < r1) {
-    TA s = 0;
-    for (int q = ra; q < rz; ++q) s += prod[q];
-    epi(r, (double)s, pv);
-  }
+  if (r < r1) epi(r, (double)row_sum<TA>(prod, ra, rz), pv);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -662,8 +672,7 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
     stream_products<PER, double>(e0, e1, base, ci, val, cv, prod);
     __syncthreads();
     if (row < r1) {
-      double s = 0.0;
-      for (int q = ra; q < rz; ++q) s += prod[q];
+      const double s = row_sum<double>(prod, ra, rz);
       p_new[row] = pi;
       ap[row] = s;
       pap = pi * s;

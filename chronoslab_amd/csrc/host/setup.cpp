@@ -1265,11 +1265,30 @@ double tridiag_max_eig(int k, const double *a, const double *b) {
   return hi;
 }
 
+// x . y with partial sums over fixed 4096-entry blocks added in block order: the same bits
+// for any thread count and schedule (an OpenMP reduction clause combines its thread
+// partials in an unspecified order).
+double det_dot(int32_t n, const double *x, const double *y, std::vector<double> &part) {
+  constexpr int32_t kBlk = 4096;
+  const int32_t nb = (n + kBlk - 1) / kBlk;
+  part.resize(nb);
+#pragma omp parallel for schedule(static)
+  for (int32_t q = 0; q < nb; ++q) {
+    double t = 0.0;
+    const int32_t e = std::min(n, (q + 1) * kBlk);
+    for (int32_t i = q * kBlk; i < e; ++i) t += x[i] * y[i];
+    part[q] = t;
+  }
+  double s = 0.0;
+  for (int32_t q = 0; q < nb; ++q) s += part[q];
+  return s;
+}
+
 template <class Op>
 double lanczos_max(int32_t n, Op op, uint64_t seed, int maxit, double tol) {
   maxit = std::max(2, std::min<int>(maxit, n));
   std::vector<std::vector<double>> V;
-  std::vector<double> a, b, v(n), w(n);
+  std::vector<double> a, b, v(n), w(n), part;
   chs_rng_uniform(seed, 7, 0, n, -1.0, 1.0, v.data());
   double nv = 0.0;
   for (double t : v) nv += t * t;
@@ -1279,23 +1298,16 @@ double lanczos_max(int32_t n, Op op, uint64_t seed, int maxit, double tol) {
   for (int j = 0; j < maxit; ++j) {
     V.push_back(v);
     op(v.data(), w.data());
-    double alpha = 0.0;
-#pragma omp parallel for reduction(+ : alpha)
-    for (int32_t i = 0; i < n; ++i) alpha += w[i] * v[i];
+    const double alpha = det_dot(n, w.data(), v.data(), part);
     a.push_back(alpha);
     // Full reorthogonalisation (twice is enough).
     for (int pass = 0; pass < 2; ++pass)
       for (auto &u : V) {
-        double d = 0.0;
-#pragma omp parallel for reduction(+ : d)
-        for (int32_t i = 0; i < n; ++i) d += w[i] * u[i];
+        const double d = det_dot(n, w.data(), u.data(), part);
 #pragma omp parallel for
         for (int32_t i = 0; i < n; ++i) w[i] -= d * u[i];
       }
-    double beta = 0.0;
-#pragma omp parallel for reduction(+ : beta)
-    for (int32_t i = 0; i < n; ++i) beta += w[i] * w[i];
-    beta = std::sqrt(beta);
+    const double beta = std::sqrt(det_dot(n, w.data(), w.data(), part));
     lam = tridiag_max_eig((int)a.size(), a.data(), b.data());
     if (j > 3 && std::fabs(lam - lam_old) <= tol * std::fabs(lam)) break;
     lam_old = lam;

@@ -187,3 +187,18 @@ def test_elasticity_bamg_smooth_converges(ne):
     print(f"ne {ne}: levels {[l.A.nrows for l in hier.levels]}, PCG {it}")
     assert fl == 0 and it <= 20
     assert np.linalg.norm(rhs - A.matvec(x)) <= 1e-8 * np.linalg.norm(rhs)
+
+
+def test_setup_independent_of_thread_count():
+    # Every reduction of the set-up (Lanczos dot products included) has a fixed order, so the
+    # hierarchy is the same bits for any OpenMP thread count.
+    A, _ = H.elasticity(8)
+    A, _ = H.bc_fix(A, 1)
+    R = H.rigid_body_modes(8)
+    hs = [H.build(A, H.SetupParams(prolong="bamg", soc="dom", soc_tau=0.01, prol_emin="smooth", tv=R,
+                                   nthreads=nt)) for nt in (1, 3, 8)]
+    for h in hs[1:]:
+        for a, b in zip(hs[0].levels, h.levels):
+            assert np.array_equal(a.A.val, b.A.val) and a.omega == b.omega
+            if a.G is not None:
+                assert np.array_equal(a.G.val, b.G.val) and np.array_equal(a.P.val, b.P.val)

@@ -81,15 +81,20 @@ def test_gpu_setup_long_rows_declined_to_host(gpu):
         assert np.array_equal(a.G.rowptr, b.G.rowptr) and np.array_equal(a.G.col, b.G.col)
         nd = int(np.sum(a.G.val != b.G.val))
         if nd:  # name the side that moved: direct host and GPU factors of the same level
+            same_a = a.A.nnz == b.A.nnz and np.array_equal(a.A.val, b.A.val)
             gh, _ = H.afsai(a.A)
-            gg, _ = gpu.afsai(a.A)
+            try:
+                gg, _ = gpu.afsai(a.A)
+                ggd = int(np.sum(gg.val != b.G.val))
+            except RuntimeError:  # rows too long for the kernel: the build factored on the host
+                ggd = None
             bad = np.flatnonzero(a.G.val != b.G.val)
             rows = np.unique(np.searchsorted(a.G.rowptr, bad, side="right") - 1)
-            print(f"level {l}: {nd} entries differ in rows {rows[:10].tolist()}...; direct host vs "
-                  f"build-host {int(np.sum(gh.val != a.G.val))}, direct GPU vs build-GPU "
-                  f"{int(np.sum(gg.val != b.G.val))}, direct host vs direct GPU {int(np.sum(gh.val != gg.val))}; "
-                  f"max rel {np.max(np.abs(a.G.val - b.G.val) / np.abs(a.G.val).clip(1e-300)):.2e}", flush=True)
+            print(f"level {l}: {nd} entries differ in rows {rows[:10].tolist()}...; same A {same_a}; "
+                  f"direct host vs build-host {int(np.sum(gh.val != a.G.val))}, direct GPU vs build-GPU "
+                  f"{ggd}; max rel {np.max(np.abs(a.G.val - b.G.val) / np.abs(a.G.val).clip(1e-300)):.2e}",
+                  flush=True)
         assert nd == 0
         assert abs(a.omega - b.omega) <= 1e-6 * a.omega
-    # the second level's rows (hundreds of entries) exceed the per-row kernel's work space
-    assert np.diff(hh.levels[1].A.rowptr).max() > 300
+    # the second level's rows (about a thousand entries) exceed the per-row kernel's work space
+    assert np.diff(hh.levels[1].A.rowptr).max() > 600
