@@ -138,8 +138,16 @@ __device__ __forceinline__ T block_sum_t(T v, T *lds) {
 template <typename T>
 __device__ __forceinline__ T row_sum(const T *prod, int ra, int rz) {
   T s = 0;
-  for (int q = ra; q < rz; q += 16) {
-    const int n = min(16, rz - q);
+  int q = ra;
+  for (; q + 16 <= rz; q += 16) {  // full groups: a plain chain of adds
+    T v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = prod[q + k];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += v[k];
+  }
+  if (q < rz) {
+    const int n = rz - q;
     T v[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = prod[q + min(k, n - 1)];
@@ -147,6 +155,24 @@ __device__ __forceinline__ T row_sum(const T *prod, int ra, int rz) {
     for (int k = 0; k < 16; ++k)
       if (k < n) s += v[k];
   }
+  return s;
+}
+
+// Strided partial sum of one long row over the workgroup (thread t takes e0 + t, e0 + t +
+// kBlock, ...), each thread adding its elements in that order; the loads of four steps are
+// issued before their adds.
+template <typename TA, typename Load>
+__device__ __forceinline__ TA strided_row_sum(int e0, int e1, Load load) {
+  TA s = 0;
+  int e = e0 + (int)threadIdx.x;
+  for (; e + 3 * kBlock < e1; e += 4 * kBlock) {
+    TA v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = load(e + k * kBlock);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s += v[k];
+  }
+  for (; e < e1; e += kBlock) s += load(e);
   return s;
 }
 
@@ -200,8 +226,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
   const int tid = threadIdx.x;
   if (e1 - e0 > PER * kBlock) {  // one long row: strided sum + block reduction
-    TA s = 0;
-    for (int e = e0 + tid; e < e1; e += kBlock) s += (TA)val[e] * (TA)x[base + (int)ci[e]];
+    TA s = strided_row_sum<TA>(e0, e1, [&](int e) { return (TA)val[e] * (TA)x[base + (int)ci[e]]; });
     s = block_sum_t<TA>(s, prod);
     if (tid == 0) epi(r0, (double)s, epi.pre(r0));
     return;
@@ -243,11 +268,10 @@ __global__ __launch_bounds__(kBlock) void spmv_pack_kernel(int nblk, const int4 
   ld[tid] = dict[lo.z * 256 + tid];
   if (e1 - e0 > PER * kBlock) {  // one long row: strided sum + block reduction
     __syncthreads();
-    TA s = 0;
-    for (int e = e0 + tid; e < e1; e += kBlock) {
+    TA s = strided_row_sum<TA>(e0, e1, [&](int e) {
       const uint32_t c = code[e];
-      s += (TA)val[e] * (TA)x[r0 + (int)(c >> 8) - ld[c & 255u]];
-    }
+      return (TA)val[e] * (TA)x[r0 + (int)(c >> 8) - ld[c & 255u]];
+    });
     s = block_sum_t<TA>(s, prod);
     if (tid == 0) epi(r0, (double)s, epi.pre(r0));
     return;
@@ -654,8 +678,7 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
   PColVal cv{z, p_old, beta};
   double pap = 0.0, prr = 0.0;
   if (e1 - e0 > PER * kBlock) {
-    double s = 0.0;
-    for (int e = e0 + tid; e < e1; e += kBlock) s += val[e] * cv(base + (int)ci[e]);
+    double s = strided_row_sum<double>(e0, e1, [&](int e) { return val[e] * cv(base + (int)ci[e]); });
     s = block_sum_t<double>(s, prod);
     if (tid == 0) {
       const double pi = cv(r0);
