@@ -507,6 +507,13 @@ namespace chrrt {
 // kernels of the levels below c (each bound by dependent memory latency) become one GEMV;
 // the result equals the uncollapsed cycle's up to the rounding of one matrix-vector
 // product.  Single-system hierarchies only (batched hierarchies keep every level).
+// b = e_j (length m).
+template <typename T>
+__global__ __launch_bounds__(chr::kBlock) void unit_vector_kernel(int m, int j, T *__restrict__ b) {
+  const int i = blockIdx.x * chr::kBlock + threadIdx.x;
+  if (i < m) b[i] = i == j ? T(1) : T(0);
+}
+
 int collapse_tail(chr_hier &h) {
   const char *env = std::getenv("CHRONOS_TAIL_ROWS");
   const int64_t limit = env ? std::atoll(env) : 2048;
@@ -525,15 +532,18 @@ int collapse_tail(chr_hier &h) {
   int rc;
   if ((rc = cols.alloc(sz * (size_t)m * m))) return rc;
   const int *stop = h.zero.as<int>();
+  const unsigned ug = (unsigned)((m + chr::kBlock - 1) / chr::kBlock);
   for (int32_t j = 0; j < m; ++j) {
-    HIPC(hipMemsetAsync(L.b.p, 0, sz * m, h.stream));
-    if (h.fp32()) {  // b = e_j (1.0f = 0x3F800000)
-      HIPC(hipMemsetD32Async((hipDeviceptr_t)(L.b.as<float>() + j), 0x3F800000u, 1, h.stream));
+    if (h.fp32()) {
+      hipLaunchKernelGGL(unit_vector_kernel<float>, dim3(ug), dim3(chr::kBlock), 0, h.stream, m, j,
+                         L.b.as<float>());
+      HIPC(hipGetLastError());
       Cycle<float> cy{h, stop, h.stream, 1};
       HIPC((cy.level<float, float>(c, L.b.as<float>(), cols.as<float>() + (size_t)j * m)));
-    } else {  // 1.0 = 0x3FF00000'00000000: high word of the little-endian double
-      HIPC(hipMemsetD32Async((hipDeviceptr_t)(reinterpret_cast<uint32_t *>(L.b.as<double>() + j) + 1),
-                             0x3FF00000u, 1, h.stream));
+    } else {
+      hipLaunchKernelGGL(unit_vector_kernel<double>, dim3(ug), dim3(chr::kBlock), 0, h.stream, m, j,
+                         L.b.as<double>());
+      HIPC(hipGetLastError());
       Cycle<double> cy{h, stop, h.stream, 1};
       HIPC((cy.level<double, double>(c, L.b.as<double>(), cols.as<double>() + (size_t)j * m)));
     }

@@ -735,9 +735,19 @@ static __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, const 
 }
 
 // Deterministic sum of count partials by one workgroup of kFinBlock threads (thread 0).
+// Each thread adds its strided partials in order, with the loads of eight steps issued
+// before their adds.
 __device__ __forceinline__ double fin_sum(const double *part, int count, double *lds) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < count; i += kFinBlock) s += part[i];
+  int i = threadIdx.x;
+  for (; i + 7 * kFinBlock < count; i += 8 * kFinBlock) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[i + k * kFinBlock];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; i < count; i += kFinBlock) s += part[i];
   s = group_sum<kWave>(s);
   const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
   if (l == 0) lds[w] = s;
