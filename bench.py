@@ -164,7 +164,8 @@ def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) 
     Afix, _ = H.bc_fix(A, 1)  # computeAMG.m:5-10 (the clamped dofs are unit rows)
     t = time.perf_counter()
     hier = H.build(Afix, H.SetupParams(prolong="bamg", soc="dom", soc_tau=0.01, prol_emin="smooth",
-                                       tv=H.rigid_body_modes(ne), gpu_afsai=True, gpu_omega=True))
+                                       tv=H.rigid_body_modes(ne), gpu_afsai=True, gpu_omega=True,
+                                       gpu_galerkin=True))
     t_setup = time.perf_counter() - t
     log(f"[bench] elasticity {ne}^3: n={A.nrows} nnz={A.nnz} setup {t_setup:.1f}s {hier.setup_seconds}\n{hier.describe()}")
     b = torch.from_numpy(rhs).to(dev)
@@ -208,7 +209,8 @@ def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) 
             "solve": res,
             "speedup_fp32_vs_fp64": (res["fp64"]["mean_s"] / res["fp32"]["mean_s"]) if len(res) == 2 else None,
             "cpu_oracle_s": t_cpu, "cpu_oracle_iterations": ito, "cpu_threads": cores,
-            "note": "BAMG + SMOOTH prolongation (EMIN not restated), aFSAI; host set-up with GPU aFSAI/omega"}
+            "note": "BAMG + SMOOTH prolongation (EMIN not restated), aFSAI; host coarsening and "
+                    "interpolation, GPU aFSAI, omega and Galerkin products"}
 
 
 def main() -> int:
@@ -263,10 +265,11 @@ def main() -> int:
     N = args.N
     A, lnk = H.heterogeneous_diffusion(N, args.sigma, args.seed)
     b_host = H.rng_uniform(args.seed, 2, A.nrows, -1.0, 1.0)
-    # Setup: host classical coarsening / interpolation / Galerkin, aFSAI factors and omega on
-    # the GPU (bit-identical factors to the all-host setup, timed beside it).
+    # Setup: host classical coarsening and interpolation; aFSAI factors, omega and the Galerkin
+    # products on the GPU (bit-identical factors and operators to the all-host setup, timed
+    # beside it).
     t0 = time.perf_counter()
-    hier = H.build(A, H.SetupParams(gpu_afsai=True, gpu_omega=True))
+    hier = H.build(A, H.SetupParams(gpu_afsai=True, gpu_omega=True, gpu_galerkin=True))
     t_setup = time.perf_counter() - t0
     setup_parts = dict(hier.setup_seconds)
     t_setup_host = None
@@ -275,7 +278,9 @@ def main() -> int:
         hh = H.build(A)
         t_setup_host = time.perf_counter() - t0
         same = all(np.array_equal(a.G.val, b.G.val) for a, b in zip(hh.levels[:-1], hier.levels[:-1]))
-        log(f"[bench] all-host setup {t_setup_host:.2f}s (aFSAI factors identical: {same})")
+        same_a = all(np.array_equal(a.A.val, b.A.val) for a, b in zip(hh.levels, hier.levels))
+        log(f"[bench] all-host setup {t_setup_host:.2f}s (aFSAI factors identical: {same}, "
+            f"Galerkin operators identical: {same_a})")
         del hh
     log(f"[bench] {N}^3 n={A.nrows} nnz={A.nnz} setup {t_setup:.2f}s {setup_parts}\n{hier.describe()}")
 

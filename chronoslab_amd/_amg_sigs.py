@@ -50,6 +50,9 @@ EXPORTS = {
     "chr_fsai_omega": (C.c_int, [C.POINTER(ChrCsr), C.POINTER(ChrCsr), C.c_uint64, _F64P, _F64P]),
     "chr_setup_omega": (C.c_int, [_VP, _I32, _I32P, _I32P, _F64P, _I32P, _I32P, _F64P, C.c_uint64, _F64P,
                                   _F64P]),
+    "chr_galerkin": (C.c_int, [C.POINTER(ChrCsr), C.POINTER(ChrCsr), _I32P, _VP, _VP]),
+    "chr_setup_galerkin": (C.c_int, [_VP, _I32, _I32P, _I32P, _F64P, _I32, _I32P, _I32P, _F64P, _I32P, _VP,
+                                     _VP]),
     "chr_time_vcycle": (C.c_int, [_VP, _I32, _F64P]),
     "chr_time_spmv": (C.c_int, [_VP, _I32, C.c_char, _I32, _F64P, _F64P]),
 }

@@ -69,6 +69,36 @@ def init(device: int = 0) -> None:
     _check(_lib().chr_init(device), "chr_init")
 
 
+_RESERVE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.POINTER(C.c_int32)),
+                       C.POINTER(C.POINTER(C.c_double)))
+
+
+def galerkin(A: H.Csr, P: H.Csr) -> H.Csr | None:
+    """Galerkin coarse operator P^T (A P) on the GPU (chr_galerkin; cpt_aspAMG.m:277), equal bit
+    for bit to hierarchy.galerkin's host product.  None when the GPU declines (a product row
+    with more than 4096 distinct columns)."""
+    keep = []
+    ca, cp = _csr(A, keep), _csr(P, keep)
+    rp = np.empty(P.ncols + 1, np.int32)
+    out = {}
+
+    def reserve(_sink, nnz, col, val):
+        out["c"] = np.empty(max(nnz, 1), np.int32)
+        out["v"] = np.empty(max(nnz, 1), np.float64)
+        out["n"] = nnz
+        col[0] = out["c"].ctypes.data_as(_I32P)
+        val[0] = out["v"].ctypes.data_as(_F64P)
+        return 0
+
+    cb = _RESERVE(reserve)
+    rc = _lib().chr_galerkin(C.byref(ca), C.byref(cp), rp.ctypes.data_as(_I32P), None, C.cast(cb, C.c_void_p))
+    if rc == 1:
+        return None
+    _check(rc, "chr_galerkin")
+    n = out["n"]
+    return H.Csr(P.ncols, P.ncols, rp, out["c"][:n].copy(), out["v"][:n].copy())
+
+
 def afsai(A: H.Csr, nstep: int = 15, step_size: int = 1, eps: float = 1e-3) -> tuple[H.Csr, bool]:
     """Adaptive symmetric FSAI factor G of A on the GPU (chr_afsai): one batched launch over
     all rows.  Mirrors the reference's SYM_aFSAI_compute gateway

@@ -36,7 +36,7 @@ class FVDiffusionSolve(torch.autograd.Function):
         k_host = np.exp(lnk.detach().double().cpu().numpy())
         A = H.diffusion(N, k_host)
         if params is None:  # aFSAI factors and omega on the GPU
-            params = H.SetupParams(gpu_afsai=True, gpu_omega=True)
+            params = H.SetupParams(gpu_afsai=True, gpu_omega=True, gpu_galerkin=True)
         M = amg.AMGPreconditioner(H.build(A, params), precision)
         bb = b.detach().double().contiguous()
         x = torch.zeros_like(bb)

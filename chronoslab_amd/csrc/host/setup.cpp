@@ -113,6 +113,8 @@ extern "C" void chs_params_default(chs_params *p) {
   p->bamg.maxrownrm = 5.0;
   p->bamg.eps_prol = 1e-10;
   p->prol_smooth = 0;
+  p->galerkin_fn = nullptr;
+  p->galerkin_ctx = nullptr;
 }
 
 extern "C" const char *chs_last_error(void) { return g_err.c_str(); }
@@ -1555,8 +1557,31 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     }
     double t3 = now_s();
     chs_mat *Ac = nullptr;
-    rc = chs_galerkin(&A, P, nt, &Ac);
-    if (rc) { delete P; return rc; }
+    if (p.galerkin_fn) {
+      auto C = std::make_unique<chs_mat>();
+      C->nrows = C->ncols = nc;
+      C->rp.assign((size_t)nc + 1, 0);
+      auto reserve = [](void *sink, int64_t nnz, int32_t **col, double **val) -> int {
+        auto *m = static_cast<chs_mat *>(sink);
+        if (nnz < 0 || nnz > INT32_MAX) return -1;
+        m->ci.resize((size_t)nnz);
+        m->v.resize((size_t)nnz);
+        *col = m->ci.data();
+        *val = m->v.data();
+        return 0;
+      };
+      rc = p.galerkin_fn(p.galerkin_ctx, n, A.rp.data(), A.ci.data(), A.v.data(), nc, P->rp.data(),
+                         P->ci.data(), P->v.data(), C->rp.data(), C.get(), reserve);
+      if (rc < 0) {
+        delete P;
+        return fail(rc, std::string("galerkin (external): error ") + std::to_string(rc));
+      }
+      if (rc == 0) Ac = C.release();  // rc == 1: declined, formed on the host below
+    }
+    if (!Ac) {
+      rc = chs_galerkin(&A, P, nt, &Ac);
+      if (rc) { delete P; return rc; }
+    }
     double t4 = now_s();
     h->t[0] += t1 - t0;
     h->t[1] += t2 - t1;

@@ -343,6 +343,8 @@ class SetupParams:
     eps_prol: float = 1e-10
     # parm_PROLONG prol_emin: "none" or "smooth" (EMIN is not restated)
     prol_emin: str = "none"
+    # Form the Galerkin products P^T (A P) on the GPU (chr_setup_galerkin); same bits.
+    gpu_galerkin: bool = False
 
 
 class _CBamg(C.Structure):
@@ -358,7 +360,8 @@ class _CParams(C.Structure):
                 ("nthreads", C.c_int32), ("seed", C.c_uint64), ("verbose", C.c_int32),
                 ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p), ("omega_fn", C.c_void_p),
                 ("omega_ctx", C.c_void_p), ("soc_type", C.c_int32), ("ntv", C.c_int32), ("ntv0", C.c_int32),
-                ("tv", _F64P), ("bamg", _CBamg), ("prol_smooth", C.c_int32)]
+                ("tv", _F64P), ("bamg", _CBamg), ("prol_smooth", C.c_int32), ("galerkin_fn", C.c_void_p),
+                ("galerkin_ctx", C.c_void_p)]
 
 
 def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
@@ -366,13 +369,14 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
     if p.nupre != 1 or p.nupost != 1:
         raise ValueError("the V-cycle supports nupre = nupost = 1 (the reference's parameter files)")
     sm = {"afsai_sym": 0, "jacobi": 1}[p.smoother.lower()]
-    fn = ofn = None
-    if p.gpu_afsai or p.gpu_omega:
+    fn = ofn = gfn = None
+    if p.gpu_afsai or p.gpu_omega or p.gpu_galerkin:
         from ._lib import load_amg_lib
 
         g = load_amg_lib()
         fn = C.cast(g.chr_setup_afsai, C.c_void_p).value if p.gpu_afsai else None
         ofn = C.cast(g.chr_setup_omega, C.c_void_p).value if p.gpu_omega else None
+        gfn = C.cast(g.chr_setup_galerkin, C.c_void_p).value if p.gpu_galerkin else None
     pr = {"clas": 0, "exti": 1, "bamg": 2}[p.prolong.lower()]
     soc = {"cla": 0, "dom": 1}[p.soc.lower()]
     tv = None
@@ -382,7 +386,7 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
                   p.nupost, sm, pr, p.nthreads, p.seed, int(p.verbose), fn, None, ofn, None, soc, p.ntv,
                   0 if tv is None else tv.shape[1], None if tv is None else tv.ctypes.data_as(_F64P),
                   _CBamg(p.itmax_vol, p.tol_vol, p.dist_min, p.dist_max, p.maxcond, p.maxrownrm, p.eps_prol),
-                  {"none": 0, "smooth": 1, "emin": 2}[p.prol_emin.lower()])
+                  {"none": 0, "smooth": 1, "emin": 2}[p.prol_emin.lower()], gfn, None)
     lib = _lib()
     ha = _to_handle(A)
     hh = C.c_void_p()

@@ -165,6 +165,26 @@ int chr_batch_vcycle(chr_batch *bt, int32_t k, const double *b, double *x, int32
 int chr_batch_pcg(chr_batch *bt, int32_t k, const double *b, double *x, int32_t itmax,
                   double tol, int32_t ptr_kind, int32_t *iters, int32_t *flags, double *resvec);
 
+/* Storage for a result whose size is known only at the end: called once with the entry
+ * count, it sets *col and *val to caller-owned arrays of that many entries (or returns
+ * non-zero to abort). */
+typedef int (*chr_reserve_fn)(void *sink, int64_t nnz, int32_t **col, double **val);
+
+/* Galerkin coarse operator A_c = P^T (A P) on the GPU (replaces `Anext = Pf'*(A*Pf)`,
+ * sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:277): A n x n, P n x nc, both CSR with
+ * sorted columns.  Writes rowptr_C[nc+1], obtains col/val storage through reserve, and
+ * fills it.  Bit-identical to the host set-up's product (Gustavson rows, products in the
+ * left factor's column order, rows sorted, exact zeros dropped).  Returns 0, a negative
+ * code, or 1 when a row of the product has more than 4096 distinct columns (nothing is
+ * written; the host forms such products). */
+int chr_galerkin(const chr_csr *A, const chr_csr *P, int32_t *rowptr_C, void *sink,
+                 chr_reserve_fn reserve);
+/* chs_galerkin_fn adapter for chs_build (chronos_setup.h); ctx unused. */
+int chr_setup_galerkin(void *ctx, int32_t n, const int32_t *rowptr_A, const int32_t *col_A,
+                       const double *val_A, int32_t nc, const int32_t *rowptr_P,
+                       const int32_t *col_P, const double *val_P, int32_t *rowptr_C,
+                       void *sink, chr_reserve_fn reserve);
+
 /* Adaptive symmetric FSAI factor of A computed on the GPU, every row's pattern growth and
  * dense solves in one batched launch (one wave per row).  Replaces the reference's native
  * setup kernel [nterm_G, iat_G, ja_G, coef_G, is_posdef] = SYM_aFSAI_compute(...)

@@ -64,6 +64,19 @@ typedef int (*chs_afsai_fn)(void *ctx, int32_t n, const int32_t *rowptr, const i
                             int32_t *rowptr_G, int32_t *col_G, double *val_G, int64_t *nnz_G,
                             int32_t *is_posdef);
 
+/* Storage callback for results sized at the end (see chs_galerkin_fn). */
+typedef int (*chs_reserve_fn)(void *sink, int64_t nnz, int32_t **col, double **val);
+
+/* Replacement of the Galerkin product inside chs_build (e.g. chr_setup_galerkin of
+ * libchronos_amg): A_c = P^T (A P) for the level's A (n x n) and P (n x nc), written as
+ * rowptr_C[nc+1] plus col/val storage obtained from reserve(sink, nnz, ...).  Returns 0,
+ * a negative code, or 1 when it declines (chs_build then forms the product on the host). */
+typedef int (*chs_galerkin_fn)(void *ctx, int32_t n, const int32_t *rowptr_A,
+                               const int32_t *col_A, const double *val_A, int32_t nc,
+                               const int32_t *rowptr_P, const int32_t *col_P,
+                               const double *val_P, int32_t *rowptr_C, void *sink,
+                               chs_reserve_fn reserve);
+
 /* Replacement of the smoother damping computation inside chs_build (e.g. chr_setup_omega
  * of libchronos_amg): omega = min(1, 1.9 / lambda_max(G A G^T)) for the level's A and G. */
 typedef int (*chs_omega_fn)(void *ctx, int32_t n, const int32_t *rowptr_A, const int32_t *col_A,
@@ -111,6 +124,8 @@ typedef struct chs_params {
   chs_bamg_params bamg;  /* prolong = 2 */
   int32_t prol_smooth;   /* parm_PROLONG prol_emin: 0 = NONE, 1 = SMOOTH (P - 1.9/lambda_max
                             D^-1 A P, prolong.m:180-197); EMIN is not restated */
+  chs_galerkin_fn galerkin_fn; /* NULL: chs_galerkin on the host */
+  void *galerkin_ctx;          /* passed to galerkin_fn */
 } chs_params;
 
 void chs_params_default(chs_params *p);
