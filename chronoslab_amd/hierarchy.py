@@ -1,9 +1,11 @@
 """Host-side hierarchy producer and problem generators (ctypes over libchronos_setup.so).
 
-The setup mirrors ChronosLab's `cpt_aspAMG` recursion for the classical + afsai_sym path
-(sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:68-351): classical strength of connection,
-PMIS, classical interpolation, adaptive FSAI smoother with omega = min(1, 1.9/lambda_max),
-Galerkin coarse operators, and a last level of size <= maxCoarseSZ or at level nLevMax.
+The setup mirrors ChronosLab's `cpt_aspAMG` recursion
+(sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:68-351): classical (CLA) or diagonal-
+dominance (DOM) strength of connection, PMIS, classical, extended+i or BAMG interpolation
+(optionally smoothed), adaptive FSAI or Jacobi smoother with omega = min(1, 1.9/lambda_max),
+Galerkin coarse operators, and a last level of size <= maxCoarseSZ or at level nLevMax.  The
+aFSAI factors, omega and the Galerkin products can be formed on the GPU (same results).
 
 Matrices cross this module as `Csr` (0-based int32 row pointers and columns, float64 values,
 columns sorted).  `scipy.sparse` is used only for convenience conversions.
