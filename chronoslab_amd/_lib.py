@@ -58,6 +58,7 @@ def load_setup_lib():
             _sig(lib, "chs_gen_rbm", None, [_I32, _F64P])
             _sig(lib, "chs_soc_cla", C.c_int, [_VP, _F64, _I8P])
             _sig(lib, "chs_soc_dom", C.c_int, [_VP, _F64, _I8P])
+            _sig(lib, "chs_agg_coarsen", C.c_int, [_VP, _I8P, _U64, _I32, _I8P])
             _sig(lib, "chs_prol_bamg", C.c_int, [_VP, _I8P, _I32P, _I32, _I32, _F64P, _VP, C.c_int,
                                                  C.POINTER(_VP), _I32P, C.POINTER(_I32)])
             _sig(lib, "chs_orthonormalize", C.c_int, [C.c_int64, _I32, _F64P])

@@ -115,6 +115,7 @@ extern "C" void chs_params_default(chs_params *p) {
   p->prol_smooth = 0;
   p->galerkin_fn = nullptr;
   p->galerkin_ctx = nullptr;
+  p->nl_agg = 0;
 }
 
 extern "C" const char *chs_last_error(void) { return g_err.c_str(); }
@@ -653,8 +654,18 @@ extern "C" int chs_orthonormalize(int64_t n, int32_t k, double *V) {
 // larger weight, and then the nodes it strongly influences become fine.  Removal swaps the
 // last list entry into the visited slot, which is then visited next.
 // ----------------------------------------------------------------------------------------
+static int pmis_core(const chs_mat *Ap, const int8_t *strong, const double *rand01, bool dirichlet,
+                     int32_t *fcnode, int32_t *nc_out);
+
 extern "C" int chs_pmis(const chs_mat *Ap, const int8_t *strong, const double *rand01,
                         int32_t *fcnode, int32_t *nc_out) {
+  return pmis_core(Ap, strong, rand01, true, fcnode, nc_out);
+}
+
+// dirichlet: mark the nodes whose column of A holds one entry as fine first
+// (mark_isolated_nodes.m); the second pass of aggressive coarsening starts from all zeros.
+static int pmis_core(const chs_mat *Ap, const int8_t *strong, const double *rand01, bool dirichlet,
+                     int32_t *fcnode, int32_t *nc_out) {
   if (!Ap || !strong || !rand01 || !fcnode) return fail(CHS_EARG, "null");
   const chs_mat &A = *Ap;
   const int32_t n = A.nrows;
@@ -697,7 +708,7 @@ extern "C" int chs_pmis(const chs_mat *Ap, const int8_t *strong, const double *r
   }
   std::vector<double> w(n), wfree(n);
   for (int32_t i = 0; i < n; ++i) {
-    fcnode[i] = (colcnt[i] == 1) ? -1 : 0;  // isolated (Dirichlet) nodes are fine
+    fcnode[i] = (dirichlet && colcnt[i] == 1) ? -1 : 0;  // isolated (Dirichlet) nodes are fine
     w[i] = (double)wcnt[i] + rand01[i];
     wfree[i] = fcnode[i] == -1 ? -1.0 : w[i];
   }
@@ -1382,6 +1393,122 @@ static int jacobi_factor(const chs_mat &A, chs_mat &G) {
   return CHS_OK;
 }
 
+// Aggressive coarsening (agg_coarsen.m:1-100): PMIS on the strength pattern S splits the
+// nodes into coarse candidates and fine nodes Ff; a second PMIS on the candidates, over the
+// distance-two pattern S_Fc = S_sub^T S_sub with S_sub the candidate columns of |S| + |S^T|
+// (mk_SoC_FcC.m; its values are positive, so its pattern is the structural product's), keeps
+// the coarse nodes C and leaves the other candidates as Fc.  kind[i] = 2 (C), 1 (Fc), 0 (Ff).
+static int agg_coarsen(const chs_mat &A, const int8_t *strong, uint64_t seed, int level,
+                       std::vector<int8_t> &kind) {
+  const int32_t n = A.nrows;
+  std::vector<double> r01(n);
+  chs_rng_uniform(seed, 100 + level, 0, n, 0.0, 1.0, r01.data());
+  std::vector<int32_t> fc1(n);
+  int32_t ncand = 0;
+  int rc = pmis_core(&A, strong, r01.data(), true, fc1.data(), &ncand);
+  if (rc) return rc;
+  // S_sym pattern (strong off-diagonal entries of A and their transposes), rows.
+  std::vector<std::vector<int32_t>> sym(n);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+      const int32_t j = A.ci[e];
+      if (j != i && strong[e] > 0) {
+        sym[i].push_back(j);
+        sym[j].push_back(i);
+      }
+    }
+  for (auto &r : sym) {
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+  }
+  // S_Fc over the candidates (numbered fc1): a ~ b when some node m has both as S_sym
+  // neighbours, i.e. m in the S_sym rows of both (S_sym is symmetric).
+  chs_mat S;
+  S.nrows = S.ncols = ncand;
+  S.rp.assign((size_t)ncand + 1, 0);
+  std::vector<std::vector<int32_t>> rows(ncand);
+  std::vector<int32_t> cand;
+  cand.reserve(ncand);
+  for (int32_t i = 0; i < n; ++i)
+    if (fc1[i] >= 0) cand.push_back(i);
+#pragma omp parallel
+  {
+    std::vector<int32_t> mark(ncand, -1);
+#pragma omp for schedule(dynamic, 256)
+    for (int32_t a = 0; a < ncand; ++a) {
+      auto &out = rows[a];
+      for (int32_t m : sym[cand[a]])
+        for (int32_t b : sym[m])
+          if (fc1[b] >= 0 && mark[fc1[b]] != a) {
+            mark[fc1[b]] = a;
+            out.push_back(fc1[b]);
+          }
+      std::sort(out.begin(), out.end());
+    }
+  }
+  for (int32_t a = 0; a < ncand; ++a) S.rp[a + 1] = S.rp[a] + (int32_t)rows[a].size();
+  S.ci.resize(S.rp[ncand]);
+  S.v.assign(S.rp[ncand], 1.0);
+  for (int32_t a = 0; a < ncand; ++a) std::copy(rows[a].begin(), rows[a].end(), S.ci.begin() + S.rp[a]);
+  std::vector<int8_t> sstrong(S.ci.size(), 1);
+  std::vector<double> r2(ncand);
+  chs_rng_uniform(seed, 300 + level, 0, ncand, 0.0, 1.0, r2.data());
+  std::vector<int32_t> fc2(ncand);
+  int32_t nc = 0;
+  if ((rc = pmis_core(&S, sstrong.data(), r2.data(), false, fc2.data(), &nc))) return rc;
+  kind.assign(n, 0);
+  for (int32_t i = 0; i < n; ++i)
+    if (fc1[i] >= 0) kind[i] = fc2[fc1[i]] >= 0 ? 2 : 1;
+  return CHS_OK;
+}
+
+extern "C" int chs_agg_coarsen(const chs_mat *Ap, const int8_t *strong, uint64_t seed, int32_t level,
+                               int8_t *kind) {
+  if (!Ap || !strong || !kind) return fail(CHS_EARG, "null");
+  std::vector<int8_t> k;
+  int rc = agg_coarsen(*Ap, strong, seed, level, k);
+  if (rc) return rc;
+  std::copy(k.begin(), k.end(), kind);
+  return CHS_OK;
+}
+
+// Two-stage extended+i prolongation (cpt_TwoStageProlMEX_ExtI.m:1-60): P1 interpolates the
+// Ff nodes from the candidates C + Fc, P2 the Fc nodes from C (the Ff nodes taking part only
+// as fine neighbours, without rows), and P = P1 P2.
+static int two_stage_exti(const chs_mat &A, const int8_t *strong, const std::vector<int8_t> &kind,
+                          int nt, chs_mat **Pout) {
+  const int32_t n = A.nrows;
+  std::vector<int32_t> f1(n), f2(n);
+  int32_t n1 = 0, n2 = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    f1[i] = kind[i] > 0 ? n1++ : -1;
+    f2[i] = kind[i] == 2 ? n2++ : (kind[i] == 1 ? -1 : -2);
+  }
+  chs_mat *P1 = nullptr, *P2f = nullptr;
+  int rc = chs_prol_exti(&A, strong, f1.data(), n1, nt, &P1);
+  if (rc) return rc;
+  std::unique_ptr<chs_mat> g1(P1);
+  if ((rc = chs_prol_exti(&A, strong, f2.data(), n2, nt, &P2f))) return rc;
+  std::unique_ptr<chs_mat> g2(P2f);
+  // P2: the rows of the candidates, in node order (the Ff rows are not part of it).
+  chs_mat P2;
+  P2.nrows = n1;
+  P2.ncols = n2;
+  P2.rp.assign((size_t)n1 + 1, 0);
+  for (int32_t i = 0, r = 0; i < n; ++i)
+    if (kind[i] > 0) {
+      P2.rp[r + 1] = P2.rp[r] + (P2f->rp[i + 1] - P2f->rp[i]);
+      P2.ci.insert(P2.ci.end(), P2f->ci.begin() + P2f->rp[i], P2f->ci.begin() + P2f->rp[i + 1]);
+      P2.v.insert(P2.v.end(), P2f->v.begin() + P2f->rp[i], P2f->v.begin() + P2f->rp[i + 1]);
+      ++r;
+    }
+  auto *P = new (std::nothrow) chs_mat;
+  if (!P) return fail(CHS_EALLOC, "alloc");
+  spgemm(*P1, P2, *P, nt);
+  *Pout = P;
+  return CHS_OK;
+}
+
 // Prolongation smoothing (prolong.m:180-197): P <- P - omega D^-1 A P with omega =
 // 1.9 / lambda_max(D^-1 A); lambda_max by the same Lanczos as the smoother damping, on
 // D^-1/2 A D^-1/2 (the same spectrum).  The product is formed in MATLAB's order: the scaled
@@ -1521,11 +1648,20 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     int rcs = p.soc_type == 1 ? chs_soc_dom(&A, p.soc_tau, strong.data())
                               : chs_soc_cla(&A, p.soc_tau, strong.data());
     if (rcs) return rcs;
-    std::vector<double> r01(n);
-    chs_rng_uniform(p.seed, 100 + level, 0, n, 0.0, 1.0, r01.data());
     std::vector<int32_t> fc(n);
     int32_t nc = 0;
-    chs_pmis(&A, strong.data(), r01.data(), fc.data(), &nc);
+    const bool aggressive = level <= p.nl_agg;  // cpt_aspAMG.m:183
+    std::vector<int8_t> kind;
+    if (aggressive) {
+      if (p.prolong == 0)
+        return fail(CHS_EARG, "aggressive coarsening needs the EXTI or BAMG prolongation (prolong.m:101-165)");
+      if ((rcs = agg_coarsen(A, strong.data(), p.seed, level, kind))) return rcs;
+      for (int32_t i = 0; i < n; ++i) fc[i] = kind[i] == 2 ? nc++ : -1;
+    } else {
+      std::vector<double> r01(n);
+      chs_rng_uniform(p.seed, 100 + level, 0, n, 0.0, 1.0, r01.data());
+      chs_pmis(&A, strong.data(), r01.data(), fc.data(), &nc);
+    }
     double t2 = now_s();
     if (nc <= 0 || nc >= n) return fail(CHS_ECONSIST, "coarsening produced no reduction at level " +
                                                          std::to_string(level));
@@ -1543,6 +1679,8 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
             if (fc[i] >= 0) tvc[(size_t)j * nc + fc[i]] = tv[(size_t)j * n + i];
         tv.swap(tvc);
       }
+    } else if (aggressive) {
+      rc = two_stage_exti(A, strong.data(), kind, nt, &P);
     } else {
       rc = p.prolong == 1 ? chs_prol_exti(&A, strong.data(), fc.data(), nc, nt, &P)
                           : chs_prol_classical(&A, strong.data(), fc.data(), nc, nt, &P);

@@ -215,6 +215,16 @@ def soc_dom(A: Csr, tau: float) -> np.ndarray:
     return out
 
 
+def agg_coarsen(A: Csr, strong: np.ndarray, seed: int, level: int = 1) -> np.ndarray:
+    """Aggressive coarsening (agg_coarsen.m): per node 2 = coarse, 1 = candidate left fine, 0 = fine."""
+    h = _to_handle(A)
+    strong = np.ascontiguousarray(strong, np.int8)
+    kind = np.empty(A.nrows, np.int8)
+    _check(_lib().chs_agg_coarsen(h.ptr, strong.ctypes.data_as(C.POINTER(C.c_int8)), C.c_uint64(seed), level,
+                                  kind.ctypes.data_as(C.POINTER(C.c_int8))))
+    return kind
+
+
 def prol_bamg(A: Csr, strong: np.ndarray, fcnode: np.ndarray, nc: int, tv: np.ndarray,
               params: SetupParams | None = None, nthreads: int = 0) -> tuple[Csr, np.ndarray]:
     """BAMG least-squares prolongation from the test space tv (n x k); returns (P, c_mark)."""
@@ -347,6 +357,8 @@ class SetupParams:
     prol_emin: str = "none"
     # Form the Galerkin products P^T (A P) on the GPU (chr_setup_galerkin); same bits.
     gpu_galerkin: bool = False
+    # parm_COARSEN nl_agg: levels 1..nl_agg use aggressive coarsening (two-stage EXTI).
+    nl_agg: int = 0
 
 
 class _CBamg(C.Structure):
@@ -363,7 +375,7 @@ class _CParams(C.Structure):
                 ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p), ("omega_fn", C.c_void_p),
                 ("omega_ctx", C.c_void_p), ("soc_type", C.c_int32), ("ntv", C.c_int32), ("ntv0", C.c_int32),
                 ("tv", _F64P), ("bamg", _CBamg), ("prol_smooth", C.c_int32), ("galerkin_fn", C.c_void_p),
-                ("galerkin_ctx", C.c_void_p)]
+                ("galerkin_ctx", C.c_void_p), ("nl_agg", C.c_int32)]
 
 
 def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
@@ -388,7 +400,7 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
                   p.nupost, sm, pr, p.nthreads, p.seed, int(p.verbose), fn, None, ofn, None, soc, p.ntv,
                   0 if tv is None else tv.shape[1], None if tv is None else tv.ctypes.data_as(_F64P),
                   _CBamg(p.itmax_vol, p.tol_vol, p.dist_min, p.dist_max, p.maxcond, p.maxrownrm, p.eps_prol),
-                  {"none": 0, "smooth": 1, "emin": 2}[p.prol_emin.lower()], gfn, None)
+                  {"none": 0, "smooth": 1, "emin": 2}[p.prol_emin.lower()], gfn, None, p.nl_agg)
     lib = _lib()
     ha = _to_handle(A)
     hh = C.c_void_p()

@@ -13,6 +13,8 @@
  *   chs_build                 sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:68-351
  *   chs_soc_cla               sources/Preconditioner/AMG/coarsen/cpt_SoC_cla.m:1-90
  *   chs_soc_dom               sources/Preconditioner/AMG/coarsen/cpt_SoC_dom.m:1-80
+ *   chs_agg_coarsen           sources/Preconditioner/AMG/coarsen/agg_coarsen.m:1-100,
+ *                             mk_SoC_FcC.m:1-20
  *   chs_prol_bamg             sources/Preconditioner/AMG/prolong/MEX_BAMG_Prol/BAMG.cpp:16-241
  *                             (called by cpt_ProlMEX_BAMG.m:1-70 from prolong.m:37-41)
  *   chs_orthonormalize        sources/Preconditioner/AMG/tspace/tspace.m ('none': qr(V0,0)),
@@ -126,6 +128,8 @@ typedef struct chs_params {
                             D^-1 A P, prolong.m:180-197); EMIN is not restated */
   chs_galerkin_fn galerkin_fn; /* NULL: chs_galerkin on the host */
   void *galerkin_ctx;          /* passed to galerkin_fn */
+  int32_t nl_agg;        /* parm_COARSEN nl_agg: levels 1..nl_agg use aggressive coarsening
+                            (agg_coarsen.m) with the two-stage EXTI (or BAMG) prolongation */
 } chs_params;
 
 void chs_params_default(chs_params *p);
@@ -177,6 +181,13 @@ int chs_soc_cla(const chs_mat *A, double tau, int8_t *strong);
  * (tau >= 0; the reference's tau < 0 variant, a kept-entry count, is not restated).  Same
  * output convention as chs_soc_cla. */
 int chs_soc_dom(const chs_mat *A, double tau, int8_t *strong);
+/* Aggressive coarsening (agg_coarsen.m): PMIS on the strength pattern (draws of stream
+ * 100 + level), then PMIS over the coarse candidates' distance-two pattern |S|+|S^T| (draws
+ * of stream 300 + level, no Dirichlet marking).  kind[i] = 2 coarse, 1 candidate left fine
+ * (Fc), 0 fine (Ff).  chs_build uses it on levels 1..nl_agg with the two-stage EXTI
+ * prolongation P = P1 P2 (cpt_TwoStageProlMEX_ExtI.m). */
+int chs_agg_coarsen(const chs_mat *A, const int8_t *strong, uint64_t seed, int32_t level,
+                    int8_t *kind);
 /* Bootstrap-AMG least-squares prolongation.  Strength pattern: the entries of A with
  * strong[e] > 0.  tv: n x ntv column-major test space.  P (n x nc) interpolates every fine
  * node from at most ntv coarse nodes of its neighbourhood, grown belt by belt (dist_min to

@@ -205,7 +205,7 @@ def ref_classical_prol(A, strong: np.ndarray, fcnode: np.ndarray, nc: int, nthre
                                  _p(fc, _I32P), nc, C.byref(nt), C.byref(ip), C.byref(jp), C.byref(cp))
     if rc != 0:
         raise RuntimeError(f"reference classical prolongation failed: {rc}")
-    n = A.nrows
+    n = int(np.sum(fc >= -1))  # rows: the nodes not marked -2
     iat = np.ctypeslib.as_array(ip, shape=(n + 1,)).copy()
     ja = np.ctypeslib.as_array(jp, shape=(max(nt.value, 1),))[: nt.value].copy()
     co = np.ctypeslib.as_array(cp, shape=(max(nt.value, 1),))[: nt.value].copy()
@@ -237,7 +237,7 @@ def ref_exti_prol(A, strong: np.ndarray, fcnode: np.ndarray, nc: int, nthreads: 
                                  _p(fc, _I32P), nc, C.byref(nt), C.byref(ip), C.byref(jp), C.byref(cp))
     if rc != 0:
         raise RuntimeError(f"reference EXTI prolongation failed: {rc}")
-    n = A.nrows
+    n = int(np.sum(fc >= -1))  # rows: the nodes not marked -2
     iat = np.ctypeslib.as_array(ip, shape=(n + 1,)).copy()
     ja = np.ctypeslib.as_array(jp, shape=(max(nt.value, 1),))[: nt.value].copy()
     co = np.ctypeslib.as_array(cp, shape=(max(nt.value, 1),))[: nt.value].copy()

@@ -2,8 +2,9 @@
 // (sources/Preconditioner/AMG/prolong/MEX_EXTI_Prol/EXTI_prolongation.cpp:52-199), compiled
 // from the reference tree by oracle/Makefile into oracle/_ref/, through a C ABI so that
 // tests can compare the host setup's EXTI interpolation with it.  It replaces the MATLAB
-// wrapper cpt_ProlMEX_EXTI.m:20-38 (0-based int32 inputs, every fine node marked -1, one
-// interpolation row per node).
+// wrappers cpt_ProlMEX_EXTI.m:20-38 (0-based int32 inputs, fine nodes marked -1) and, for
+// the second stage of cpt_TwoStageProlMEX_ExtI.m:41-56, nodes marked -2, which take part as
+// fine neighbours but get no row: the row count is that of the nodes marked >= -1.
 int EXTI_prolongation(const int level, const int nthreads, const int nn_A, const int nt_A,
                       const int *const iat_A, const int *const ja_A, const double *const coef_A,
                       const int *const coef_S, const int *const fcnodes, const int nr_I,
@@ -14,7 +15,9 @@ extern "C" int ref_exti_prol(int nthreads, int n, const int *iat, const int *ja,
                              int **ja_out, double **coef_out) {
   int nt = 0, *ip = nullptr, *jp = nullptr;
   double *cp = nullptr;
-  const int ierr = EXTI_prolongation(0, nthreads, n, iat[n], iat, ja, coef, strong, fcnodes, n, nc, nt,
+  int nr = 0;
+  for (int i = 0; i < n; ++i) nr += fcnodes[i] >= -1 ? 1 : 0;
+  const int ierr = EXTI_prolongation(0, nthreads, n, iat[n], iat, ja, coef, strong, fcnodes, nr, nc, nt,
                                      ip, jp, cp);
   *nt_out = nt;
   *iat_out = ip;

@@ -1,4 +1,4 @@
-"""The reference's own end-to-end known answer: Tests/FlowCLAS_0.m.
+"""The reference's own end-to-end known answers: Tests/FlowCLAS_0.m and Tests/FlowCLAS_AGG.m.
 
 The reference test loads examples/matrices/Flux.mat (n = 20,757, held here as
 tests/golden/flux.npz, written by tests/golden/make_golden.py), builds the AMG
@@ -39,6 +39,21 @@ def flux_problem(seed: int):
     return A, d["b"], H.build(Afix, params)
 
 
+REF_ITER_AGG = 14  # Tests/FlowCLAS_AGG.m:3
+
+
+def flux_agg_problem(seed: int):
+    """Tests/FlowCLAS_AGG.m: FlowCLAS_0's parameters with nl_agg = 1 (aggressive coarsening on
+    the first level, two-stage extended+i prolongation)."""
+    d = np.load(os.path.join(GOLDEN, "flux.npz"))
+    n = d["rowptr"].shape[0] - 1
+    A = H.Csr(n, n, d["rowptr"], d["col"], d["val"])
+    Afix, _ = H.bc_fix(A, 1)
+    params = H.SetupParams(smoother="jacobi", prolong="exti", soc_tau=0.25, nl_agg=1, nlev_max=10,
+                           max_coarse=1000, seed=seed)
+    return A, d["b"], H.build(Afix, params)
+
+
 def test_reference_rule():
     assert reference_rule(12) and reference_rule(10) and reference_rule(14)
     assert not reference_rule(9) and not reference_rule(15)
@@ -73,3 +88,87 @@ def test_flux_sqmr_gpu_vcycle(gpu, precision):
         assert niter == it_o and err <= 1e-10
     else:
         assert abs(niter - it_o) <= 1 and relres < 1e-5
+
+
+@pytest.mark.parametrize("seed", [20240607, 2, 3])
+def test_flux_agg_sqmr_oracle_vcycle(seed):
+    A, b, hier = flux_agg_problem(seed)
+    oh = oracle.Hier(hier)
+    As = A.to_scipy()
+    x, info, relres, niter, rv = amg.SQMR(lambda v: As @ v, b, 1e-6, 100, oh.vcycle)
+    print(f"seed {seed}: levels {[l.A.nrows for l in hier.levels]}, SQMR niter {niter} (reference 14)")
+    assert info == 0 and rv[-1] < 1e-6
+    assert not (abs(REF_ITER_AGG / niter - 1) > 0.05 and abs(REF_ITER_AGG - niter) - 2 > 0), niter
+
+
+def _flux_fixed():
+    d = np.load(os.path.join(GOLDEN, "flux.npz"))
+    n = d["rowptr"].shape[0] - 1
+    A, _ = H.bc_fix(H.Csr(n, n, d["rowptr"], d["col"], d["val"]), 1)
+    return A
+
+
+def test_agg_coarsen_invariants():
+    A = _flux_fixed()
+    s = H.soc_cla(A, 0.25)
+    kind = H.agg_coarsen(A, s, 7)
+    # First pass: the candidates (C + Fc) are PMIS's coarse set on S.
+    fc, nc = H.pmis(A, s, H.rng_uniform(7, 101, A.nrows))
+    np.testing.assert_array_equal(kind > 0, fc >= 0)
+    # Second pass over the distance-two pattern of |S| + |S^T| among the candidates: C is
+    # independent and every Fc candidate has a C node at distance two.
+    import scipy.sparse as sp
+    rows = np.repeat(np.arange(A.nrows), np.diff(A.rowptr))
+    keep = (s > 0) & (rows != A.col)
+    S = sp.csr_matrix((np.ones(keep.sum()), (rows[keep], A.col[keep])), shape=(A.nrows, A.nrows))
+    Ssym = ((S + S.T) > 0).astype(float)
+    cand = np.flatnonzero(kind > 0)
+    Sub = Ssym[:, cand]
+    SFc = (Sub.T @ Sub).tocsr()
+    SFc.setdiag(0)
+    SFc.eliminate_zeros()
+    isC = kind[cand] == 2
+    Cc = SFc[isC][:, isC]
+    assert Cc.nnz == 0
+    reach = SFc[~isC][:, isC]
+    assert np.all(np.diff(reach.indptr) > 0)
+
+
+@pytest.mark.skipif(not oracle.ref_exti_available(), reason="oracle/_ref not built (make -C oracle ref)")
+def test_two_stage_exti_matches_reference_core():
+    # Both stages of cpt_TwoStageProlMEX_ExtI.m through the reference's own EXTI core: the
+    # first with Ff marked -1, the second with Fc marked -1 and Ff marked -2 (no rows).
+    A = _flux_fixed()
+    s = H.soc_cla(A, 0.25)
+    kind = H.agg_coarsen(A, s, 7)
+    for stage in (1, 2):
+        if stage == 1:
+            f = np.where(kind > 0, np.cumsum(kind > 0) - 1, -1).astype(np.int32)
+        else:
+            f = np.where(kind == 2, np.cumsum(kind == 2) - 1, np.where(kind == 1, -1, -2)).astype(np.int32)
+        nc = int(f.max()) + 1
+        P = H.prol_exti(A, s, f, nc).to_scipy()[np.flatnonzero(f >= -1)].tocsr()
+        P.sort_indices()
+        iat, ja, co = oracle.ref_exti_prol(A, s, f, nc, nthreads=1)
+        np.testing.assert_array_equal(iat, P.indptr)
+        np.testing.assert_array_equal(ja, P.indices)
+        np.testing.assert_array_equal(co, P.data)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_flux_agg_sqmr_gpu_vcycle(gpu, precision):
+    A, b, hier = flux_agg_problem(20240607)
+    As = A.to_scipy()
+    xo, info_o, _, it_o, _ = amg.SQMR(lambda v: As @ v, b, 1e-6, 100, oracle.Hier(hier).vcycle)
+    M = amg.AMGPreconditioner(hier, precision)
+    x, info, relres, niter, rv = amg.SQMR(lambda v: As @ v, b, 1e-6, 100, M)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"FlowCLAS_AGG {precision}: SQMR niter {niter} (oracle V-cycle {it_o}, reference 14), "
+          f"err vs oracle-cycle solve {err:.2e}", flush=True)
+    assert info == 0
+    assert not (abs(REF_ITER_AGG / niter - 1) > 0.05 and abs(REF_ITER_AGG - niter) - 2 > 0)
+    if precision == "fp64":
+        assert niter == it_o and err <= 1e-10
+    else:
+        assert abs(niter - it_o) <= 1
