@@ -405,10 +405,15 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
   };
   // Chunks of U elements; the last chunk clamps its loads to the row's last element and
   // drops the surplus products.
+  // FP64 values: the next chunk's values and codes are requested right after this chunk's
+  // gathers, so a multi-chunk slice keeps one chunk of loads in flight while it waits (with
+  // FP32 values the extra registers cost more occupancy than the overlap gains, measured).
+  // Either way the products are added chunk by chunk in element order.
+  constexpr bool kAhead = sizeof(TV) == 8;
   TA acc = 0;
   const int nch = (len + U - 1) / U;
   for (int ch = 0; ch < nch; ++ch) {
-    if (ch > 0) {
+    if (!kAhead && ch > 0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int e = base + min(ch * U + u, len - 1) * kWave;
@@ -419,11 +424,29 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
     TA xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xv[u] = (TA)x[col_of(c[u])];
+    const bool more = kAhead && ch + 1 < nch;  // uniform over the wave (one slice)
+    TV vn[U];
+    CT cn[U];
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = base + min((ch + 1) * U + u, len - 1) * kWave;
+        vn[u] = val[e];
+        cn[u] = code[e];
+      }
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const TA p = mul_rn((TA)v[u], xv[u]);
       const bool live = ch * U + u < len && ((uint32_t)c[u] & SellCode<CB>::pad) != SellCode<CB>::pad;
       acc = live ? acc + p : acc;
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        v[u] = vn[u];
+        c[u] = cn[u];
+      }
     }
   }
   if (valid) epi(row, (double)acc, pv);
