@@ -379,7 +379,7 @@ def main() -> int:
     g_layout = r["g_layout"]
     tv = "float" if main_prec == "fp32" else "double"
     if g_layout[0] == "sell":
-        kname = f"spmv_sell_kernel<{g_layout[1]}, {tv}, {tv}, EpiProbe<{tv}, 'G'>>"
+        kname = f"spmv_sell_kernel<{g_layout[1]}, U, {tv}, {tv}, EpiProbe<{tv}, 'G'>> (U: elements per chunk)"
     elif isinstance(g_layout[0], tuple):
         kname = f"spmv_pack_kernel<{g_layout[0][1]}, {tv}, {tv}, EpiProbe<{tv}, 'G'>>"
     else:
