@@ -453,10 +453,15 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
 }
 
 // ---------------------------------------------------------------------------------------
-// Block product Y = M X for row-major n x k blocks (element (i, c) at i*k + c, k <= KL):
-// KL lanes per row, one per column; each lane sums its row in nonzero order with the
-// product rounded before the add, so that column c is bit for bit the single-vector stream
-// kernel's result on column c.  Epilogues see flat indices i*k + c.
+// Block product Y = M X for row-major n x k blocks (element (i, c) at i*k + c, k <= KL), on
+// the stream layout's row blocks.  The block's values and column indices are staged in LDS
+// with coalesced loads (cap = the layout's PER * kBlock nonzeros, in dynamic LDS); then each
+// (row, column) pair is one lane, KL lanes per row, so the KL lanes of a row gather one
+// contiguous k-vector of X per nonzero.  A lane reads 16 staged nonzeros and issues their 16
+// gathers before it adds the products, in nonzero order and each rounded before the add:
+// column c is bit for bit the single-vector stream kernel's result on column c.  A block
+// holding one row longer than cap (the stream layout's long-row blocks) is summed by its KL
+// lanes straight from memory, in the same order.  Epilogues see flat indices i*k + c.
 // ---------------------------------------------------------------------------------------
 
 template <int KL, typename TV, typename TI, typename TX, typename Epi>
@@ -464,21 +469,54 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__re
                                                       const int *__restrict__ rp,
                                                       const TI *__restrict__ ci,
                                                       const TV *__restrict__ val,
-                                                      const TX *__restrict__ x, int k, Epi epi,
+                                                      const TX *__restrict__ x, int k, int cap, Epi epi,
                                                       const int *__restrict__ stop) {
   using TA = typename Acc<TV>::type;
+  extern __shared__ __align__(16) char smem[];
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 lo = blk[b], hi = blk[b + 1];
   if (*stop) return;
-  const int r0 = lo.x, r1 = hi.x, base = lo.z;
-  const int c = threadIdx.x % KL;
-  if (c >= k) return;  // no cross-lane operations below
-  for (int r = r0 + threadIdx.x / KL; r < r1; r += kBlock / KL) {
+  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
+  const int tid = threadIdx.x;
+  if (e1 - e0 > cap) {  // one long row
+    const int c = tid % KL;
+    if (tid >= KL || c >= k) return;  // no cross-lane operations below
+    const int i = r0 * k + c;
+    const double pv = epi.pre(i);
+    TA s = 0;
+    for (int e = rp[r0]; e < rp[r0 + 1]; ++e) s = s + mul_rn((TA)val[e], (TA)x[(base + (int)ci[e]) * k + c]);
+    epi(i, (double)s, pv);
+    return;
+  }
+  TV *sv = reinterpret_cast<TV *>(smem);
+  TI *sc = reinterpret_cast<TI *>(smem + (size_t)cap * sizeof(TV));
+  for (int t = tid; t < e1 - e0; t += kBlock) {
+    sv[t] = val[e0 + t];
+    sc[t] = ci[e0 + t];
+  }
+  __syncthreads();
+  const int npair = (r1 - r0) * KL;
+  for (int p = tid; p < npair; p += kBlock) {
+    const int r = r0 + p / KL, c = p % KL;
+    if (c >= k) continue;
     const int i = r * k + c;
     const double pv = epi.pre(i);
-    const int e1 = rp[r + 1];
+    const int ra = rp[r] - e0, rz = rp[r + 1] - e0;
     TA s = 0;
-    for (int e = rp[r]; e < e1; ++e) s = s + mul_rn((TA)val[e], (TA)x[(base + (int)ci[e]) * k + c]);
+    for (int q = ra; q < rz; q += 16) {
+      const int n = min(16, rz - q);
+      TV vv[16];
+      TA xv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int qq = q + min(u, n - 1);
+        vv[u] = sv[qq];
+        xv[u] = (TA)x[(base + (int)sc[qq]) * k + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (u < n) s = s + mul_rn((TA)vv[u], xv[u]);
+    }
     epi(i, (double)s, pv);
   }
 }
