@@ -111,12 +111,15 @@ __global__ __launch_bounds__(chr::kBlock) void bpcg_update_kernel(const int4 *__
 }
 
 // p_new = z + beta p_old formed at every gathered column, Ap = A p_new (row-major n x k,
-// beta per solve: the row's system and the column).
+// beta per solve: the row's system and the column).  As in the block product, the stream
+// block's values and columns are staged in LDS (cap = PER * kBlock nonzeros) and each (row,
+// column) pair is a lane that issues the gathers of 8 staged nonzeros before adding their
+// products in nonzero order; a block holding one longer row is summed straight from memory.
 template <int KL, typename TI>
 __global__ __launch_bounds__(chr::kBlock) void bpcg_p_kernel(int nblk, const int4 *__restrict__ blk,
                                                              const int *__restrict__ rp,
                                                              const TI *__restrict__ ci,
-                                                             const double *__restrict__ val, int k,
+                                                             const double *__restrict__ val, int k, int cap,
                                                              const int *__restrict__ off, int nsys,
                                                              const double *__restrict__ beta,
                                                              const double *__restrict__ z,
@@ -124,18 +127,53 @@ __global__ __launch_bounds__(chr::kBlock) void bpcg_p_kernel(int nblk, const int
                                                              double *__restrict__ p_new,
                                                              double *__restrict__ ap,
                                                              const BState *st) {
+  extern __shared__ __align__(16) char smem[];
   const int b = chr::xcd_block(blockIdx.x, nblk);
   const int4 lo = blk[b], hi = blk[b + 1];
   if (st->stop) return;
-  const int c = threadIdx.x % KL;
-  if (c >= k) return;
-  const int base = lo.z;
-  for (int r = lo.x + threadIdx.x / KL; r < hi.x; r += chr::kBlock / KL) {
+  const int base = lo.z, e0 = lo.y, e1 = hi.y, tid = threadIdx.x;
+  if (e1 - e0 > cap) {  // one long row
+    const int c = tid % KL, r = lo.x;
+    if (tid >= KL || c >= k) return;  // no cross-lane operations below
     const double bq = beta[sys_of(r, off, nsys) * k + c];
     double s = 0.0;
     for (int e = rp[r]; e < rp[r + 1]; ++e) {
       const size_t j = (size_t)(base + (int)ci[e]) * k + c;
       s = s + chr::mul_rn(val[e], z[j] + bq * p_old[j]);
+    }
+    const size_t i = (size_t)r * k + c;
+    p_new[i] = z[i] + bq * p_old[i];
+    ap[i] = s;
+    return;
+  }
+  double *sv = reinterpret_cast<double *>(smem);
+  TI *sc = reinterpret_cast<TI *>(smem + (size_t)cap * sizeof(double));
+  for (int t = tid; t < e1 - e0; t += chr::kBlock) {
+    sv[t] = val[e0 + t];
+    sc[t] = ci[e0 + t];
+  }
+  __syncthreads();
+  const int npair = (hi.x - lo.x) * KL;
+  for (int p = tid; p < npair; p += chr::kBlock) {
+    const int r = lo.x + p / KL, c = p % KL;
+    if (c >= k) continue;
+    const double bq = beta[sys_of(r, off, nsys) * k + c];
+    const int ra = rp[r] - e0, rz = rp[r + 1] - e0;
+    double s = 0.0;
+    for (int q = ra; q < rz; q += 8) {
+      const int n = min(8, rz - q);
+      double vv[8], zz[8], pp[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int qq = q + min(u, n - 1);
+        const size_t j = (size_t)(base + (int)sc[qq]) * k + c;
+        vv[u] = sv[qq];
+        zz[u] = z[j];
+        pp[u] = p_old[j];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < n) s = s + chr::mul_rn(vv[u], zz[u] + bq * pp[u]);
     }
     const size_t i = (size_t)r * k + c;
     p_new[i] = z[i] + bq * p_old[i];
@@ -473,9 +511,11 @@ int batch_prepare(chr_batch &bt, int k, int itmax) {
 hipError_t launch_bp(chr_batch &bt, int k, const double *pold, double *pnew, hipStream_t s) {
   const DevCsr &A = bt.h->outer.A;
   auto *st = bt.state.as<BState>();
+  const int cap = A.per * chr::kBlock;
 #define BP(KL, TI)                                                                                        \
-  hipLaunchKernelGGL((bpcg_p_kernel<KL, TI>), dim3(A.nblk), dim3(chr::kBlock), 0, s, A.nblk, A.blk.as<int4>(), \
-                     A.rp.as<int>(), A.ci.as<TI>(), A.val.as<double>(), k, bt.off_dev.as<int>(), bt.nsys,  \
+  hipLaunchKernelGGL((bpcg_p_kernel<KL, TI>), dim3(A.nblk), dim3(chr::kBlock),                           \
+                     (size_t)cap * (sizeof(double) + sizeof(TI)), s, A.nblk, A.blk.as<int4>(),            \
+                     A.rp.as<int>(), A.ci.as<TI>(), A.val.as<double>(), k, cap, bt.off_dev.as<int>(), bt.nsys, \
                      (const double *)bt.a.beta, (const double *)bt.z.as<double>(), pold, pnew,            \
                      bt.ap.as<double>(), (const BState *)st)
   if (A.idx16) {
