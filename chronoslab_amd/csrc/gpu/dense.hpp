@@ -1,6 +1,6 @@
 // Small dense FP64 linear algebra on the host for the Rayleigh-Ritz steps of LOBPCG
 // (matrices of order <= 48): Cholesky, the symmetric-definite generalized eigenproblem by
-// Cholesky reduction + cyclic Jacobi (Golub & Van Loan, Alg. 8.5.3), condition numbers.
+// Cholesky reduction + the symmetric QR algorithm, condition numbers.
 // All matrices are column-major.
 #pragma once
 
@@ -39,44 +39,90 @@ inline void symmetrize(int m, double *M) {
     }
 }
 
-// Cyclic Jacobi: S (symmetric) is destroyed; w = eigenvalues (unsorted), Q = eigenvectors.
-inline void jacobi_eig(int m, double *S, double *w, double *Q) {
+// Symmetric eigen-decomposition: S (symmetric) is destroyed; w = eigenvalues (unsorted),
+// Q = orthonormal eigenvectors.  Householder reduction to tridiagonal form, then implicit
+// QR steps with the Wilkinson shift and Givens bulge chasing (Golub & Van Loan, Algs. 8.3.1
+// and 8.3.3); both accumulate into Q.
+inline void sym_eig(int m, double *S, double *w, double *Q) {
   for (int j = 0; j < m; ++j)
     for (int i = 0; i < m; ++i) Q[i + j * m] = (i == j) ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < m; ++i) {
-        const double v = S[i + j * m] * S[i + j * m];
-        tot += v;
-        if (i != j) off += v;
-      }
-    if (off == 0.0 || off <= 1e-32 * tot) break;
-    for (int p = 0; p < m - 1; ++p)
-      for (int q = p + 1; q < m; ++q) {
-        const double apq = S[p + q * m];
-        if (apq == 0.0) continue;
-        const double tau = (S[q + q * m] - S[p + p * m]) / (2.0 * apq);
-        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
-        const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
-        for (int k = 0; k < m; ++k) {
-          const double skp = S[k + p * m], skq = S[k + q * m];
-          S[k + p * m] = c * skp - s * skq;
-          S[k + q * m] = s * skp + c * skq;
-        }
-        for (int k = 0; k < m; ++k) {
-          const double spk = S[p + k * m], sqk = S[q + k * m];
-          S[p + k * m] = c * spk - s * sqk;
-          S[q + k * m] = s * spk + c * sqk;
-        }
-        for (int k = 0; k < m; ++k) {
-          const double qkp = Q[k + p * m], qkq = Q[k + q * m];
-          Q[k + p * m] = c * qkp - s * qkq;
-          Q[k + q * m] = s * qkp + c * qkq;
-        }
-      }
+  std::vector<double> v(m), y(m);
+  for (int k = 0; k + 2 < m; ++k) {
+    // Reflector H = I - beta v v^T on indices k+1..m-1 mapping S[k+1.., k] onto e_1.
+    double sig = 0.0;
+    for (int i = k + 2; i < m; ++i) sig += S[i + k * m] * S[i + k * m];
+    if (sig == 0.0) continue;
+    const double x0 = S[k + 1 + k * m], alpha = std::sqrt(x0 * x0 + sig);
+    const double v0 = x0 + (x0 >= 0.0 ? alpha : -alpha);
+    v[k + 1] = v0;
+    for (int i = k + 2; i < m; ++i) v[i] = S[i + k * m];
+    const double beta = 2.0 / (v0 * v0 + sig);
+    // S <- S H (all rows), then S <- H S (all columns), Q <- Q H.
+    for (int i = 0; i < m; ++i) {
+      double t = 0.0;
+      for (int j = k + 1; j < m; ++j) t += S[i + j * m] * v[j];
+      y[i] = beta * t;
+    }
+    for (int j = k + 1; j < m; ++j)
+      for (int i = 0; i < m; ++i) S[i + j * m] -= y[i] * v[j];
+    for (int j = 0; j < m; ++j) {
+      double t = 0.0;
+      for (int i = k + 1; i < m; ++i) t += v[i] * S[i + j * m];
+      t *= beta;
+      for (int i = k + 1; i < m; ++i) S[i + j * m] -= t * v[i];
+    }
+    for (int i = 0; i < m; ++i) {
+      double t = 0.0;
+      for (int j = k + 1; j < m; ++j) t += Q[i + j * m] * v[j];
+      y[i] = beta * t;
+    }
+    for (int j = k + 1; j < m; ++j)
+      for (int i = 0; i < m; ++i) Q[i + j * m] -= y[i] * v[j];
   }
-  for (int i = 0; i < m; ++i) w[i] = S[i + i * m];
+  // Tridiagonal T: diagonal d, off-diagonal e[i] = T(i+1, i).
+  std::vector<double> d(m), e(m, 0.0);
+  for (int i = 0; i < m; ++i) d[i] = S[i + i * m];
+  for (int i = 0; i + 1 < m; ++i) e[i] = 0.5 * (S[i + 1 + i * m] + S[i + (i + 1) * m]);
+  const double eps = 2.220446049250313e-16;
+  int hi = m - 1, guard = 0;
+  while (hi > 0 && guard < 60 * m) {
+    if (std::fabs(e[hi - 1]) <= eps * (std::fabs(d[hi - 1]) + std::fabs(d[hi]))) {
+      e[hi - 1] = 0.0;
+      --hi;
+      continue;
+    }
+    int lo = hi - 1;
+    while (lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(d[lo - 1]) + std::fabs(d[lo]))) --lo;
+    if (lo > 0) e[lo - 1] = 0.0;
+    ++guard;
+    // Wilkinson shift from the trailing 2x2 block of T(lo..hi).
+    const double dd = 0.5 * (d[hi - 1] - d[hi]), b = e[hi - 1];
+    const double mu = d[hi] - b * b / (dd + (dd >= 0.0 ? 1.0 : -1.0) * std::hypot(dd, b));
+    double x = d[lo] - mu, z = e[lo], bulge = 0.0;
+    for (int k = lo; k < hi; ++k) {
+      // Givens rotation on rows/columns k, k+1 zeroing z against x.
+      const double r = std::hypot(x, z);
+      const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
+      if (k > lo) e[k - 1] = r;
+      // Rotate the 2x2 block [d_k e_k; e_k d_k1] and the entries beside it.
+      const double dk = d[k], dk1 = d[k + 1], ek = e[k];
+      d[k] = c * c * dk + 2.0 * c * s * ek + s * s * dk1;
+      d[k + 1] = s * s * dk - 2.0 * c * s * ek + c * c * dk1;
+      e[k] = c * s * (dk1 - dk) + (c * c - s * s) * ek;
+      if (k + 1 < hi) {
+        bulge = s * e[k + 1];
+        e[k + 1] *= c;
+        x = e[k];
+        z = bulge;
+      }
+      for (int i = 0; i < m; ++i) {
+        const double qk = Q[i + k * m], qk1 = Q[i + (k + 1) * m];
+        Q[i + k * m] = c * qk + s * qk1;
+        Q[i + (k + 1) * m] = -s * qk + c * qk1;
+      }
+    }
+  }
+  for (int i = 0; i < m; ++i) w[i] = d[i];
 }
 
 // A v = lambda B v, A symmetric, B SPD: ascending eigenvalues w, B-orthonormal V.
@@ -96,7 +142,7 @@ inline bool sygv(int m, const double *A, const double *B, double *w, double *V) 
       C[i + j * m] = s / L[j + j * m];
     }
   symmetrize(m, C.data());
-  jacobi_eig(m, C.data(), wt.data(), Q.data());
+  sym_eig(m, C.data(), wt.data(), Q.data());
   std::vector<int> ord(m);
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return wt[a] < wt[b]; });
@@ -117,7 +163,7 @@ inline bool sygv(int m, const double *A, const double *B, double *w, double *V) 
 // 2-norm condition number of a symmetric matrix (|eigenvalue| ratio).
 inline double cond_sym(int m, const double *M) {
   Mat S(M, M + (size_t)m * m), Q((size_t)m * m), w(m);
-  jacobi_eig(m, S.data(), w.data(), Q.data());
+  sym_eig(m, S.data(), w.data(), Q.data());
   double mx = 0.0, mn = 1e308;
   for (double x : w) {
     mx = std::max(mx, std::fabs(x));

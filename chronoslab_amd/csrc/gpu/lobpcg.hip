@@ -5,6 +5,8 @@
 // the preconditioner is the block V-cycle, and the Gram matrices are tall-skinny products
 // reduced in a fixed order.  The Rayleigh-Ritz steps (order <= 3*neig) run in FP64 on the
 // host: Cholesky, the generalized symmetric-definite eigenproblem and condition numbers.
+#include <cstring>
+
 #include "chronos_amg.h"
 #include "dense.hpp"
 #include "runtime.hpp"
@@ -19,54 +21,100 @@ struct Lob {
   const int *zero;   // stop flag that is never set
   hipStream_t s;
   int n;
-  DevBuf part, cdev, gout, idx;
+  DevBuf part, gout, stage_dev;
   int nparts = 1, rows_per_part = 1;
+  // Small host operands (Rayleigh-Ritz coefficients, eigenvalues, column lists) go through a
+  // pinned staging area, each piece copied asynchronously to its own slot of stage_dev.
+  // Slots are reused only after a stream synchronisation (fetch), so no pending copy ever
+  // reads a slot the host is rewriting.  Gram matrices are reduced into slots of gout and
+  // brought back together by one copy and one synchronisation per batch.
+  static constexpr size_t kStage = 16384;  // doubles
+  static constexpr int kSlots = 32, kSlot = 256;
+  double *hstage = nullptr, *hout = nullptr;
+  size_t staged = 0;
+  int nslot = 0;
 
+  ~Lob() {
+    if (hstage) (void)hipHostFree(hstage);
+    if (hout) (void)hipHostFree(hout);
+  }
   int init() {
     nparts = std::max(1, std::min(1024, (n + 255) / 256));
     rows_per_part = (n + nparts - 1) / nparts;
     int rc;
     if ((rc = part.alloc(sizeof(double) * (size_t)nparts * chr::kBlock))) return rc;
-    if ((rc = cdev.alloc(sizeof(double) * 48 * 48))) return rc;
-    if ((rc = gout.alloc(sizeof(double) * chr::kBlock))) return rc;
-    if ((rc = idx.alloc(sizeof(int) * 64))) return rc;
+    if ((rc = gout.alloc(sizeof(double) * kSlots * kSlot))) return rc;
+    if ((rc = stage_dev.alloc(sizeof(double) * kStage))) return rc;
+    HIPC(hipHostMalloc((void **)&hstage, sizeof(double) * kStage, hipHostMallocDefault));
+    HIPC(hipHostMalloc((void **)&hout, sizeof(double) * kSlots * kSlot, hipHostMallocDefault));
     return CHR_OK;
   }
   static unsigned grid(int64_t cnt) { return (unsigned)((cnt + chr::kBlock - 1) / chr::kBlock); }
 
-  // out (column-major a x b, host) = X^T Y for row-major n x a and n x b blocks.
-  int gram(const double *X, int a, const double *Y, int b, std::vector<double> &out) {
-    if (a < 1 || b < 1 || a > 16 || b > 16) return fail(CHR_EARG, "lobpcg: Gram block wider than 16");
+  // Copies bytes from src to a fresh staging slot and returns its device address (null on error).
+  const void *stage(const void *src, size_t bytes) {
+    const size_t nd = (bytes + 31) / 32 * 4;  // whole 32-byte units
+    if (nd > kStage) return nullptr;
+    if (staged + nd > kStage) {
+      if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
+      staged = 0;
+    }
+    std::memcpy(hstage + staged, src, bytes);
+    void *dst = stage_dev.as<double>() + staged;
+    if (hipMemcpyAsync(dst, hstage + staged, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return nullptr;
+    staged += nd;
+    return dst;
+  }
+  // Enqueues X^T Y (row-major n x a and n x b blocks) into the next Gram slot; returns the
+  // slot, or -1 on error.
+  int gram_launch(const double *X, int a, const double *Y, int b) {
+    if (a < 1 || b < 1 || a > 16 || b > 16 || nslot >= kSlots) {
+      fail(CHR_EARG, "lobpcg: Gram block wider than 16 or too many pending Gram matrices");
+      return -1;
+    }
     hipLaunchKernelGGL(chr::gram_partial_kernel, dim3(nparts), dim3(chr::kBlock), 0, s, n, a, X, b, Y,
                        rows_per_part, part.as<double>());
-    HIPC(hipGetLastError());
     const unsigned rgrid = (unsigned)((a * b * chr::kWave + chr::kBlock - 1) / chr::kBlock);
     hipLaunchKernelGGL(chr::gram_reduce_kernel, dim3(rgrid), dim3(chr::kBlock), 0, s, nparts, a * b,
-                       (const double *)part.as<double>(), gout.as<double>());
-    HIPC(hipGetLastError());
-    std::vector<double> g((size_t)a * b);
-    HIPC(hipMemcpyAsync(g.data(), gout.p, sizeof(double) * a * b, hipMemcpyDeviceToHost, s));
+                       (const double *)part.as<double>(), gout.as<double>() + (size_t)nslot * kSlot);
+    if (hipGetLastError() != hipSuccess) {
+      fail(CHR_EHIP, "lobpcg: Gram launch");
+      return -1;
+    }
+    return nslot++;
+  }
+  // Brings the pending Gram slots to the host (one copy, one synchronisation).
+  int fetch() {
+    if (nslot > 0)
+      HIPC(hipMemcpyAsync(hout, gout.p, sizeof(double) * (size_t)nslot * kSlot, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    staged = 0;
+    nslot = 0;
+    return CHR_OK;
+  }
+  // out (column-major a x b) = a fetched slot.
+  void get(int slot, int a, int b, std::vector<double> &out) const {
+    const double *g = hout + (size_t)slot * kSlot;
     out.assign((size_t)a * b, 0.0);
     for (int p = 0; p < a; ++p)
       for (int q = 0; q < b; ++q) out[p + (size_t)q * a] = g[(size_t)p * b + q];
+  }
+  // out (column-major a x b, host) = X^T Y.
+  int gram(const double *X, int a, const double *Y, int b, std::vector<double> &out) {
+    const int slot = gram_launch(X, a, Y, b);
+    if (slot < 0) return CHR_EARG;
+    if (int rc = fetch()) return rc;
+    get(slot, a, b, out);
     return CHR_OK;
   }
   // Z (n x b) = beta Z + X (n x a) Cm (column-major a x b, host).
-  // Small host operands are copied with the stream drained afterwards: the host arrays
-  // are temporaries of the Rayleigh-Ritz step, and a copy from pageable memory that is
-  // still pending when they change or go away would read the wrong data.
-  int upload_small(void *dst, const void *src, size_t bytes) {
-    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    HIPC(hipStreamSynchronize(s));
-    return CHR_OK;
-  }
   int gemm(const double *X, int a, const double *Cm, int b, double beta, double *Z) {
-    if (int rc = upload_small(cdev.p, Cm, sizeof(double) * a * b)) return rc;
     if (a < 1 || b < 1 || a > 48 || b > 48) return fail(CHR_EARG, "lobpcg: block product wider than 48");
+    const double *cdev = static_cast<const double *>(stage(Cm, sizeof(double) * a * b));
+    if (!cdev) return fail(CHR_EHIP, "lobpcg: staging a coefficient block");
     const unsigned ggrid = (unsigned)std::min<int64_t>(2048, ((int64_t)n + chr::kGemmTile - 1) / chr::kGemmTile);
     hipLaunchKernelGGL(chr::block_gemm_kernel, dim3(ggrid), dim3(chr::kBlock), 0, s, n, a,
-                       X, b, (const double *)cdev.as<double>(), beta, Z);
+                       X, b, cdev, beta, Z);
     HIPC(hipGetLastError());
     return CHR_OK;
   }
@@ -82,17 +130,19 @@ struct Lob {
     return CHR_OK;
   }
   int resid(const double *AX, const double *X, const double *lam_host, int k, double *R) {
-    if (int rc = upload_small(cdev.p, lam_host, sizeof(double) * k)) return rc;
+    const double *ldev = static_cast<const double *>(stage(lam_host, sizeof(double) * k));
+    if (!ldev) return fail(CHR_EHIP, "lobpcg: staging eigenvalues");
     hipLaunchKernelGGL(chr::block_resid_kernel, dim3(grid((int64_t)n * k)), dim3(chr::kBlock), 0, s, n, k,
-                       AX, X, (const double *)cdev.as<double>(), R);
+                       AX, X, ldev, R);
     HIPC(hipGetLastError());
     return CHR_OK;
   }
   int gather(const double *X, int k, const std::vector<int> &cols, double *Y) {
     const int m = (int)cols.size();
-    if (int rc = upload_small(idx.p, cols.data(), sizeof(int) * m)) return rc;
+    const int *idev = static_cast<const int *>(stage(cols.data(), sizeof(int) * m));
+    if (!idev) return fail(CHR_EHIP, "lobpcg: staging a column list");
     hipLaunchKernelGGL(chr::block_gather_kernel, dim3(grid((int64_t)n * m)), dim3(chr::kBlock), 0, s, n, k,
-                       X, m, (const int *)idx.as<int>(), Y);
+                       X, m, idev, Y);
     HIPC(hipGetLastError());
     return CHR_OK;
   }
@@ -181,7 +231,11 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
   auto rayleigh_ritz_X = [&]() -> int {  // X, AX <- X E, AX E with E from (X'AX, X'X)
     std::vector<double> xax, xx;
     int r;
-    if ((r = L.gram(dX, neig, dAX, neig, xax)) || (r = L.gram(dX, neig, dX, neig, xx))) return r;
+    const int s0 = L.gram_launch(dX, neig, dAX, neig), s1 = L.gram_launch(dX, neig, dX, neig);
+    if (s0 < 0 || s1 < 0) return CHR_EARG;
+    if ((r = L.fetch())) return r;
+    L.get(s0, neig, neig, xax);
+    L.get(s1, neig, neig, xx);
     chrdense::symmetrize(neig, xax.data());
     chrdense::symmetrize(neig, xx.data());
     w.resize(neig);
@@ -270,39 +324,44 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
           if (sym) M[(c0 + jj) + (size_t)(r0 + ii) * m] = v;
         }
     };
-    std::vector<double> g;
-    if ((rc = L.gram(dX, neig, dAX, neig, g))) return rc;
-    chrdense::symmetrize(neig, g.data());
-    put(gA, 0, 0, neig, neig, g, false);
-    if ((rc = L.gram(dX, neig, dAAR, na, g))) return rc;
-    put(gA, 0, neig, neig, na, g, true);
-    if ((rc = L.gram(dAR, na, dAAR, na, g))) return rc;
-    chrdense::symmetrize(na, g.data());
-    put(gA, neig, neig, na, na, g, false);
-    if ((rc = L.gram(dX, neig, dX, neig, g))) return rc;
-    chrdense::symmetrize(neig, g.data());
-    put(gB, 0, 0, neig, neig, g, false);
-    if ((rc = L.gram(dX, neig, dAR, na, g))) return rc;
-    put(gB, 0, neig, neig, na, g, true);
-    if ((rc = L.gram(dAR, na, dAR, na, g))) return rc;
-    chrdense::symmetrize(na, g.data());
-    put(gB, neig, neig, na, na, g, false);
+    // All Gram blocks of the step are reduced on the device, then fetched together.
+    struct GB {
+      const double *X;
+      int a;
+      const double *Y;
+      int b;
+      bool toA;
+      int r0, c0;
+      bool diag;
+      int slot;
+    };
+    const int c2 = neig + na;
+    std::vector<GB> blocks = {{dX, neig, dAX, neig, true, 0, 0, true, -1},
+                              {dX, neig, dAAR, na, true, 0, neig, false, -1},
+                              {dAR, na, dAAR, na, true, neig, neig, true, -1},
+                              {dX, neig, dX, neig, false, 0, 0, true, -1},
+                              {dX, neig, dAR, na, false, 0, neig, false, -1},
+                              {dAR, na, dAR, na, false, neig, neig, true, -1}};
     if (!restart) {
-      const int c2 = neig + na;
-      if ((rc = L.gram(dX, neig, dAAP, na, g))) return rc;
-      put(gA, 0, c2, neig, na, g, true);
-      if ((rc = L.gram(dAR, na, dAAP, na, g))) return rc;
-      put(gA, neig, c2, na, na, g, true);
-      if ((rc = L.gram(dAPa, na, dAAP, na, g))) return rc;
-      chrdense::symmetrize(na, g.data());
-      put(gA, c2, c2, na, na, g, false);
-      if ((rc = L.gram(dX, neig, dAPa, na, g))) return rc;
-      put(gB, 0, c2, neig, na, g, true);
-      if ((rc = L.gram(dAR, na, dAPa, na, g))) return rc;
-      put(gB, neig, c2, na, na, g, true);
-      if ((rc = L.gram(dAPa, na, dAPa, na, g))) return rc;
-      chrdense::symmetrize(na, g.data());
-      put(gB, c2, c2, na, na, g, false);
+      blocks.push_back({dX, neig, dAAP, na, true, 0, c2, false, -1});
+      blocks.push_back({dAR, na, dAAP, na, true, neig, c2, false, -1});
+      blocks.push_back({dAPa, na, dAAP, na, true, c2, c2, true, -1});
+      blocks.push_back({dX, neig, dAPa, na, false, 0, c2, false, -1});
+      blocks.push_back({dAR, na, dAPa, na, false, neig, c2, false, -1});
+      blocks.push_back({dAPa, na, dAPa, na, false, c2, c2, true, -1});
+    }
+    for (auto &bk : blocks)
+      if ((bk.slot = L.gram_launch(bk.X, bk.a, bk.Y, bk.b)) < 0) return CHR_EARG;
+    if ((rc = L.fetch())) return rc;
+    std::vector<double> g;
+    for (const auto &bk : blocks) {
+      L.get(bk.slot, bk.a, bk.b, g);
+      if (bk.diag) {
+        chrdense::symmetrize(bk.a, g.data());
+        put(bk.toA ? gA : gB, bk.r0, bk.c0, bk.a, bk.b, g, false);
+      } else {
+        put(bk.toA ? gA : gB, bk.r0, bk.c0, bk.a, bk.b, g, true);
+      }
     }
     w.resize(m);
     E.resize((size_t)m * m);
