@@ -4,6 +4,7 @@
 // (sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:68-351), restated from the reference's
 // behaviour.  The GPU solve phase consumes its output through chronos_amg.h.
 #include "bamg.hpp"
+#include "emin.hpp"
 #include "chronos_setup.h"
 
 #include <algorithm>
@@ -116,6 +117,12 @@ extern "C" void chs_params_default(chs_params *p) {
   p->galerkin_fn = nullptr;
   p->galerkin_ctx = nullptr;
   p->nl_agg = 0;
+  p->emin.patt_pow = 2;
+  p->emin.patt_tau = 0.01;
+  p->emin.nnzr_max = 30;
+  p->emin.itmax = 10;
+  p->emin.energ_tol = 1e-3;
+  p->emin.condmax = 1e10;
 }
 
 extern "C" const char *chs_last_error(void) { return g_err.c_str(); }
@@ -637,6 +644,52 @@ extern "C" int chs_prol_bamg(const chs_mat *Ap, const int8_t *strong, const int3
   for (int32_t i = 0; i < n; ++i) nm += mark[i];
   if (c_mark) std::copy(mark.begin(), mark.end(), c_mark);
   if (nmarked) *nmarked = nm;
+  *Pout = P.release();
+  return CHS_OK;
+}
+
+namespace {
+chse::Params emin_params(const chs_emin_params &e) {
+  return chse::Params{e.patt_pow, e.patt_tau, e.nnzr_max, e.itmax, e.energ_tol, e.condmax};
+}
+}  // namespace
+
+extern "C" int chs_emin_pattern(const chs_mat *Ap, const chs_mat *Pp, const int32_t *fcnode, int32_t ntv,
+                                const chs_emin_params *prm, int nthreads, chs_mat **patt) {
+  if (!Ap || !Pp || !fcnode || !prm || !patt) return fail(CHS_EARG, "null");
+  if (Pp->nrows != Ap->nrows || Ap->nrows != Ap->ncols || ntv < 1)
+    return fail(CHS_EARG, "emin pattern: dimensions");
+  auto T = std::make_unique<chs_mat>();
+  T->nrows = Pp->nrows;
+  T->ncols = Pp->ncols;
+  chse::pattern(Ap->nrows, Pp->ncols, Ap->rp.data(), Ap->ci.data(), Ap->v.data(), Pp->rp.data(),
+                Pp->ci.data(), Pp->v.data(), fcnode, ntv, emin_params(*prm),
+                nthreads_or_default(nthreads), T->rp, T->ci);
+  T->v.assign(T->ci.size(), 1.0);
+  *patt = T.release();
+  return CHS_OK;
+}
+
+extern "C" int chs_emin_improve(const chs_mat *Ap, const chs_mat *Pp, const chs_mat *patt,
+                                const int32_t *fcnode, int32_t ntv, const double *tv,
+                                const chs_emin_params *prm, int nthreads, chs_mat **Pout,
+                                int32_t *iter) {
+  if (!Ap || !Pp || !patt || !fcnode || !tv || !prm || !Pout) return fail(CHS_EARG, "null");
+  if (Pp->nrows != Ap->nrows || Ap->nrows != Ap->ncols || patt->nrows != Pp->nrows ||
+      patt->ncols != Pp->ncols || ntv < 1)
+    return fail(CHS_EARG, "emin: dimensions");
+  auto P = std::make_unique<chs_mat>();
+  P->nrows = Pp->nrows;
+  P->ncols = Pp->ncols;
+  int32_t it = 0;
+  std::string err;
+  const int rc = chse::improve(Ap->nrows, Pp->ncols, Ap->rp.data(), Ap->ci.data(), Ap->v.data(),
+                               Pp->rp.data(), Pp->ci.data(), Pp->v.data(), patt->rp.data(),
+                               patt->ci.data(), fcnode, ntv, tv, emin_params(*prm),
+                               nthreads_or_default(nthreads), P->rp, P->ci, P->v, it, err);
+  if (rc == 1) return fail(CHS_EARG, err);
+  if (rc) return fail(CHS_ENUM, err);
+  if (iter) *iter = it;
   *Pout = P.release();
   return CHS_OK;
 }
@@ -1668,10 +1721,12 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     chs_mat *P = nullptr;
     int32_t nmarked = 0;
     int rc;
+    std::vector<double> tv_f;  // this level's test space, kept for EMIN
     if (p.prolong == 2) {
       chs_orthonormalize(n, p.ntv, tv.data());
       rc = chs_prol_bamg(&A, strong.data(), fc.data(), nc, p.ntv, tv.data(), &p.bamg, nt, &P,
                          nullptr, &nmarked);
+      if (!rc && p.prol_smooth == 2) tv_f = tv;
       if (!rc) {  // next level's test space: the coarse nodes' rows (cpt_aspAMG.m:242-243)
         std::vector<double> tvc((size_t)nc * p.ntv);
         for (int32_t j = 0; j < p.ntv; ++j)
@@ -1689,9 +1744,23 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
     if (p.prol_smooth == 1) {
       rc = smooth_prolongation(A, *P, p.seed + 1000 + level, nt, nullptr);
       if (rc) { delete P; return rc; }
+    } else if (p.prol_smooth == 2) {
+      // prolong.m:207-232: pattern, then energy minimisation with the level's test space.
+      if (tv_f.empty()) {
+        delete P;
+        return fail(CHS_EARG, "EMIN needs the BAMG prolongation's test space (prolong = 2)");
+      }
+      chs_mat *T = nullptr, *Pe = nullptr;
+      int32_t it = 0;
+      rc = chs_emin_pattern(&A, P, fc.data(), p.ntv, &p.emin, nt, &T);
+      if (!rc) rc = chs_emin_improve(&A, P, T, fc.data(), p.ntv, tv_f.data(), &p.emin, nt, &Pe, &it);
+      delete T;
+      delete P;
+      if (rc) return rc;
+      P = Pe;
     } else if (p.prol_smooth != 0) {
       delete P;
-      return fail(CHS_EARG, "prolongation energy minimisation (EMIN) is not supported");
+      return fail(CHS_EARG, "prol_smooth: 0 (NONE), 1 (SMOOTH) or 2 (EMIN)");
     }
     double t3 = now_s();
     chs_mat *Ac = nullptr;

@@ -243,6 +243,37 @@ def prol_bamg(A: Csr, strong: np.ndarray, fcnode: np.ndarray, nc: int, tv: np.nd
     return _take(out), mark
 
 
+def emin_pattern(A: Csr, P: Csr, fcnode: np.ndarray, ntv: int, params: SetupParams | None = None,
+                 nthreads: int = 0) -> Csr:
+    """EMIN interpolation pattern (MEX_cptProlPatt.cpp): fine rows of P grown by the strongest
+    entries of S^(patt_pow-1) P, at most max(nnzr_max, ntv) per row; values 1."""
+    p = params or SetupParams()
+    ha, hp = _to_handle(A), _to_handle(P)
+    fcnode = np.ascontiguousarray(fcnode, np.int32)
+    ep = _cemin(p)
+    out = C.c_void_p()
+    _check(_lib().chs_emin_pattern(ha.ptr, hp.ptr, fcnode.ctypes.data_as(_I32P), ntv,
+                                   C.cast(C.pointer(ep), C.c_void_p), nthreads, C.byref(out)))
+    return _take(out)
+
+
+def emin_improve(A: Csr, P: Csr, patt: Csr, fcnode: np.ndarray, tv: np.ndarray,
+                 params: SetupParams | None = None, nthreads: int = 0) -> tuple[Csr, int]:
+    """Energy-minimised P over the pattern patt under the test-space constraint of every fine
+    row (EMIN_matfree.cpp with solv_emin = 4, prec_emin = 1); returns (P, CG iterations)."""
+    p = params or SetupParams()
+    ha, hp, ht = _to_handle(A), _to_handle(P), _to_handle(patt)
+    fcnode = np.ascontiguousarray(fcnode, np.int32)
+    tv = np.asfortranarray(tv, dtype=np.float64)
+    ep = _cemin(p)
+    out = C.c_void_p()
+    it = C.c_int32()
+    _check(_lib().chs_emin_improve(ha.ptr, hp.ptr, ht.ptr, fcnode.ctypes.data_as(_I32P), tv.shape[1],
+                                   tv.ctypes.data_as(_F64P), C.cast(C.pointer(ep), C.c_void_p), nthreads,
+                                   C.byref(out), C.byref(it)))
+    return _take(out), it.value
+
+
 def orthonormalize(V: np.ndarray) -> np.ndarray:
     """Q factor (positive R diagonal) of the thin QR of V, as the test-space set-up forms it."""
     Q = np.array(V, dtype=np.float64, order="F", copy=True)
@@ -353,8 +384,15 @@ class SetupParams:
     maxcond: float = 1e13
     maxrownrm: float = 5.0
     eps_prol: float = 1e-10
-    # parm_PROLONG prol_emin: "none" or "smooth" (EMIN is not restated)
+    # parm_PROLONG prol_emin: "none", "smooth" or "emin" (EMIN needs prolong="bamg")
     prol_emin: str = "none"
+    # parm_PROLONG's EMIN fields (prec_emin = 1 Jacobi, solv_emin = 4 matrix product).
+    patt_pow: int = 2
+    patt_tau: float = 0.01
+    nnzr_max: int = 30
+    itmax_emin: int = 10
+    energ_tol: float = 1e-3
+    condmax_emin: float = 1e10
     # Form the Galerkin products P^T (A P) on the GPU (chr_setup_galerkin); same bits.
     gpu_galerkin: bool = False
     # parm_COARSEN nl_agg: levels 1..nl_agg use aggressive coarsening (two-stage EXTI).
@@ -367,6 +405,15 @@ class _CBamg(C.Structure):
                 ("eps_prol", C.c_double)]
 
 
+class _CEmin(C.Structure):
+    _fields_ = [("patt_pow", C.c_int32), ("patt_tau", C.c_double), ("nnzr_max", C.c_int32),
+                ("itmax", C.c_int32), ("energ_tol", C.c_double), ("condmax", C.c_double)]
+
+
+def _cemin(p: SetupParams) -> _CEmin:
+    return _CEmin(p.patt_pow, p.patt_tau, p.nnzr_max, p.itmax_emin, p.energ_tol, p.condmax_emin)
+
+
 class _CParams(C.Structure):
     _fields_ = [("nlev_max", C.c_int32), ("max_coarse", C.c_int32), ("soc_tau", C.c_double),
                 ("nstep", C.c_int32), ("step_size", C.c_int32), ("fsai_eps", C.c_double),
@@ -375,7 +422,7 @@ class _CParams(C.Structure):
                 ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p), ("omega_fn", C.c_void_p),
                 ("omega_ctx", C.c_void_p), ("soc_type", C.c_int32), ("ntv", C.c_int32), ("ntv0", C.c_int32),
                 ("tv", _F64P), ("bamg", _CBamg), ("prol_smooth", C.c_int32), ("galerkin_fn", C.c_void_p),
-                ("galerkin_ctx", C.c_void_p), ("nl_agg", C.c_int32)]
+                ("galerkin_ctx", C.c_void_p), ("nl_agg", C.c_int32), ("emin", _CEmin)]
 
 
 def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
@@ -400,7 +447,8 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
                   p.nupost, sm, pr, p.nthreads, p.seed, int(p.verbose), fn, None, ofn, None, soc, p.ntv,
                   0 if tv is None else tv.shape[1], None if tv is None else tv.ctypes.data_as(_F64P),
                   _CBamg(p.itmax_vol, p.tol_vol, p.dist_min, p.dist_max, p.maxcond, p.maxrownrm, p.eps_prol),
-                  {"none": 0, "smooth": 1, "emin": 2}[p.prol_emin.lower()], gfn, None, p.nl_agg)
+                  {"none": 0, "smooth": 1, "emin": 2}[p.prol_emin.lower()], gfn, None, p.nl_agg,
+                  _cemin(p))
     lib = _lib()
     ha = _to_handle(A)
     hh = C.c_void_p()

@@ -17,6 +17,12 @@
  *                             mk_SoC_FcC.m:1-20
  *   chs_prol_bamg             sources/Preconditioner/AMG/prolong/MEX_BAMG_Prol/BAMG.cpp:16-241
  *                             (called by cpt_ProlMEX_BAMG.m:1-70 from prolong.m:37-41)
+ *   chs_emin_pattern          sources/Preconditioner/AMG/prolong/EMIN/MEX_EMIN/
+ *                             MEX_cptProlPatt.cpp:239-430 (prolong.m:213-228)
+ *   chs_emin_improve          sources/Preconditioner/AMG/prolong/EMIN/Cpp_EMIN/
+ *                             EMIN_matfree.cpp:61-300, gather_B_QR.cpp:30-480,
+ *                             DEFL_PCG_matfree.cpp:28-250 (MEX_EMIN_enhance_prol.m:1-105,
+ *                             prolong.m:231)
  *   chs_orthonormalize        sources/Preconditioner/AMG/tspace/tspace.m ('none': qr(V0,0)),
  *                             cpt_aspAMG.m:29-46
  *   chs_pmis                  sources/Preconditioner/AMG/coarsen/cpt_PMIS.m:1-102
@@ -100,6 +106,18 @@ typedef struct chs_bamg_params {
   double eps_prol;      /* relative fit residual taken as exact [1e-10] */
 } chs_bamg_params;
 
+/* Energy minimisation of the prolongation (parm_PROLONG lines 12-17; prec_emin = 1 Jacobi
+ * and solv_emin = 4 matrix product are the variants restated). */
+typedef struct chs_emin_params {
+  int32_t patt_pow;     /* powers of the strength pattern grown onto P's pattern [2] */
+  double patt_tau;      /* DOM strength threshold of that pattern [0.01] */
+  int32_t nnzr_max;     /* pattern entries per fine row at most, at least ntv [30] */
+  int32_t itmax;        /* itmax_emin: CG iterations at most [10] */
+  double energ_tol;     /* stop once an iteration's energy decrease falls below energ_tol
+                           times the first one's [1e-3] */
+  double condmax;       /* condmax_emin: conditioning bound of a row's constraint block [1e10] */
+} chs_emin_params;
+
 typedef struct chs_params {
   int32_t nlev_max;     /* parm_AMG line 1  (nLevMax, 10) */
   int32_t max_coarse;   /* parm_AMG line 2  (maxCoarseSZ, 1000) */
@@ -125,11 +143,13 @@ typedef struct chs_params {
                             ntv0 (or all, when NULL) are uniform [0,1) draws of stream 7 */
   chs_bamg_params bamg;  /* prolong = 2 */
   int32_t prol_smooth;   /* parm_PROLONG prol_emin: 0 = NONE, 1 = SMOOTH (P - 1.9/lambda_max
-                            D^-1 A P, prolong.m:180-197); EMIN is not restated */
+                            D^-1 A P, prolong.m:180-197), 2 = EMIN (prolong.m:207-232, with
+                            the level's test space; needs prolong = 2) */
   chs_galerkin_fn galerkin_fn; /* NULL: chs_galerkin on the host */
   void *galerkin_ctx;          /* passed to galerkin_fn */
   int32_t nl_agg;        /* parm_COARSEN nl_agg: levels 1..nl_agg use aggressive coarsening
                             (agg_coarsen.m) with the two-stage EXTI (or BAMG) prolongation */
+  chs_emin_params emin;  /* prol_smooth = 2 */
 } chs_params;
 
 void chs_params_default(chs_params *p);
@@ -199,6 +219,23 @@ int chs_prol_bamg(const chs_mat *A, const int8_t *strong, const int32_t *fcnode,
 /* Orthonormal basis of the columns of V (n x k column-major, in place), the Q of a thin QR
  * with positive R diagonal (tspace method NONE).  Returns 1 if the columns are dependent. */
 int chs_orthonormalize(int64_t n, int32_t k, double *V);
+/* EMIN prolongation pattern (MEX_cptProlPatt.cpp:239-430, called at prolong.m:228): the
+ * fine rows of P's pattern (stored zeros ignored) grown to at most max(nnzr_max, ntv)
+ * entries by the largest |(S^(k-1) P)_ij| outside it, ties to the lower column, S the DOM
+ * strength matrix of A above patt_tau with values |a_ij|/sqrt(a_ii a_jj); coarse rows are
+ * empty.  *patt: n x nc, values 1. */
+int chs_emin_pattern(const chs_mat *A, const chs_mat *P, const int32_t *fcnode, int32_t ntv,
+                     const chs_emin_params *prm, int nthreads, chs_mat **patt);
+/* EMIN (MEX_EMIN_enhance_prol.m -> EMIN_Prolong_compute.cpp -> EMIN_matfree.cpp:61-300 with
+ * solv_emin = 4, prec_emin = 1; symmetric A): P's fine rows copied into the pattern, each
+ * row corrected onto its test-space constraint P_i TV_C = TV_i (QR, SVD below full rank;
+ * gather_B_QR.cpp), then the projected Jacobi-PCG on trace(P^T A P) over the pattern
+ * (DEFL_PCG_matfree.cpp) for at most itmax iterations.  tv: n x ntv column-major.  *Pout:
+ * n x nc with coarse rows e_c and exact zeros dropped; *iter: CG iterations. */
+int chs_emin_improve(const chs_mat *A, const chs_mat *P, const chs_mat *patt,
+                     const int32_t *fcnode, int32_t ntv, const double *tv,
+                     const chs_emin_params *prm, int nthreads, chs_mat **Pout, int32_t *iter);
+
 /* Dirichlet marking + sequential PMIS.  rand01[i] is the tie-break added to node i's
  * weight.  Output fcnode[i] = c >= 0 (coarse index) or -1 (fine); returns nc in *nc. */
 int chs_pmis(const chs_mat *A, const int8_t *strong, const double *rand01,

@@ -173,8 +173,6 @@ def test_smoothed_prolongation():
     lam = spla.eigsh(Dh @ As @ Dh, k=1, which="LA", tol=1e-10)[0][0]
     Pref = P0 - (1.9 / lam) * (Dm @ As @ P0)
     assert abs(P1 - Pref).max() <= 1e-6 * abs(Pref).max()
-    with pytest.raises(Exception):
-        H.build(A, H.SetupParams(prol_emin="emin", **base))
 
 
 @pytest.mark.parametrize("ne", [8, 16])

@@ -1,7 +1,8 @@
 """GPU parity on the elasticity hierarchies (BAMG prolongation with rigid-body test vectors,
-prolongation smoothing, aFSAI): the HIP V-cycle and PCG against the CPU oracle on the same
-hierarchy, the reference's Cubo_4820 known answers (Tests/CubeBAMG_NONE.m: 43 SQMR
-iterations, Tests/CubeBAMG_SMOOTH.m: 29) with the GPU V-cycle, and the GPU set-up on levels
+energy minimisation (EMIN) or prolongation smoothing, aFSAI): the HIP V-cycle and PCG against
+the CPU oracle on the same hierarchy, the reference's Cubo_4820 known answers
+(Tests/CubeBAMG_NONE.m: 43 SQMR iterations, Tests/CubeBAMG_SMOOTH.m: 29,
+Tests/CubeBAMG_EMIN.m: 20) with the GPU V-cycle, and the GPU set-up on levels
 whose rows are too long for the per-row aFSAI kernel (the host factors those levels; the
 factors of every level equal the all-host set-up's bit for bit)."""
 import numpy as np
@@ -14,10 +15,10 @@ from test_bamg import cubo, rule
 pytestmark = pytest.mark.gpu
 
 
-def _elasticity(ne, **kw):
+def _elasticity(ne, emin="emin", **kw):
     A, rhs = H.elasticity(ne)
     Afix, _ = H.bc_fix(A, 1)
-    p = H.SetupParams(prolong="bamg", soc="dom", soc_tau=0.01, prol_emin="smooth", tv=H.rigid_body_modes(ne), **kw)
+    p = H.SetupParams(prolong="bamg", soc="dom", soc_tau=0.01, prol_emin=emin, tv=H.rigid_body_modes(ne), **kw)
     return A, rhs, H.build(Afix, p)
 
 
@@ -36,16 +37,17 @@ def test_vcycle_matches_oracle(gpu, ne):
     assert r64 <= 1e-12 and r32 <= 1e-4
 
 
+@pytest.mark.parametrize("emin", ["emin", "smooth"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_pcg_matches_oracle(gpu, prec):
-    A, rhs, hier = _elasticity(16)
+def test_pcg_matches_oracle(gpu, prec, emin):
+    A, rhs, hier = _elasticity(16, emin)
     ref = oracle.Hier(hier)
     xr, itr, rvr, flr = ref.pcg(A, rhs, itmax=200, tol=1e-8)
     M = gpu.AMGPreconditioner(hier, prec)
     x, it, rv, fl = M.pcg(rhs, itmax=200, tol=1e-8)
     err = np.linalg.norm(x - xr) / np.linalg.norm(xr)
     tres = np.linalg.norm(rhs - A.matvec(x)) / np.linalg.norm(rhs)
-    print(f"elasticity 16 {prec}: it {it} oracle {itr} err {err:.3e} true-res {tres:.3e}", flush=True)
+    print(f"elasticity 16 {emin} {prec}: it {it} oracle {itr} err {err:.3e} true-res {tres:.3e}", flush=True)
     assert fl == 0 and abs(it - itr) <= 1
     assert tres <= 1e-8
     if prec == "fp64":
@@ -53,7 +55,7 @@ def test_pcg_matches_oracle(gpu, prec):
         np.testing.assert_allclose(rv[: min(it, itr) + 1], rvr[: min(it, itr) + 1], rtol=1e-7)
 
 
-@pytest.mark.parametrize("emin,ref_iter", [("none", 43), ("smooth", 29)])
+@pytest.mark.parametrize("emin,ref_iter", [("none", 43), ("smooth", 29), ("emin", 20)])
 def test_cube_known_answer_gpu_vcycle(gpu, emin, ref_iter):
     As, TV0, rhs = cubo()
     Afix, _ = H.bc_fix(H.Csr.from_scipy(As), 1)
@@ -74,8 +76,8 @@ def test_cube_known_answer_gpu_vcycle(gpu, emin, ref_iter):
 
 def test_gpu_setup_long_rows_declined_to_host(gpu):
     ne = 16
-    _, _, hh = _elasticity(ne)
-    _, _, hg = _elasticity(ne, gpu_afsai=True, gpu_omega=True)
+    _, _, hh = _elasticity(ne, "smooth")
+    _, _, hg = _elasticity(ne, "smooth", gpu_afsai=True, gpu_omega=True)
     assert hh.nlev == hg.nlev
     for l, (a, b) in enumerate(zip(hh.levels[:-1], hg.levels[:-1])):
         assert np.array_equal(a.G.rowptr, b.G.rowptr) and np.array_equal(a.G.col, b.G.col)
