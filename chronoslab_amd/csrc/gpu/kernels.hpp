@@ -795,29 +795,46 @@ static __global__ __launch_bounds__(kBlock) void pcg_update_kernel(int n, const 
   if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
-// Deterministic sum of count partials by one workgroup of kFinBlock threads (thread 0).
-// Each thread adds its strided partials in order, with the loads of eight steps issued
-// before their adds.
-__device__ __forceinline__ double fin_sum(const double *part, int count, double *lds) {
-  double s = 0.0;
+// Deterministic sums of N arrays of count partials (array k at part + k * count) by one
+// workgroup of kFinBlock threads; the results land in t[] of thread 0.  Each thread adds
+// its strided partials of every array in order, with the loads of sixteen steps of all N
+// arrays issued before their adds; then a fixed butterfly per wave and the waves in order.
+// The order of each sum does not depend on N or on the look-ahead.
+template <int N>
+__device__ __forceinline__ void fin_sum(const double *part, int count, double *lds, double (&t)[N]) {
+  constexpr int kAhead = 16;
+  double s[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) s[k] = 0.0;
   int i = threadIdx.x;
-  for (; i + 7 * kFinBlock < count; i += 8 * kFinBlock) {
-    double v[8];
+  for (; i + (kAhead - 1) * kFinBlock < count; i += kAhead * kFinBlock) {
+    double v[N][kAhead];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = part[i + k * kFinBlock];
+    for (int k = 0; k < N; ++k)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) s += v[k];
+      for (int a = 0; a < kAhead; ++a) v[k][a] = part[(size_t)k * count + i + a * kFinBlock];
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+      for (int a = 0; a < kAhead; ++a) s[k] += v[k][a];
   }
-  for (; i < count; i += kFinBlock) s += part[i];
-  s = group_sum<kWave>(s);
+  for (; i < count; i += kFinBlock)
+#pragma unroll
+    for (int k = 0; k < N; ++k) s[k] += part[(size_t)k * count + i];
   const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
-  if (l == 0) lds[w] = s;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    s[k] = group_sum<kWave>(s[k]);
+    if (l == 0) lds[k * (kFinBlock / kWave) + w] = s[k];
+  }
   __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < kFinBlock / kWave; ++i) t += lds[i];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    t[k] = 0.0;
+    if (threadIdx.x == 0)
+      for (int j = 0; j < kFinBlock / kWave; ++j) t[k] += lds[k * (kFinBlock / kWave) + j];
+  }
   __syncthreads();
-  return t;
 }
 
 enum { kFinInit = 0, kFinBeta = 1, kFinAlpha = 2, kFinUpdate = 3 };
@@ -827,14 +844,23 @@ static __global__ __launch_bounds__(kFinBlock) void pcg_finalize_kernel(int kind
                                                                  const double *__restrict__ part,
                                                                  PcgState *st,
                                                                  double *__restrict__ resvec) {
-  __shared__ double lds[kFinBlock / kWave];
+  __shared__ double lds[2 * (kFinBlock / kWave)];
   if (kind != kFinInit && st->stop) return;
   if (kind == kFinBeta && st->iter == 0) {
     if (threadIdx.x == 0) st->beta = 0.0;  // first iteration: p = z (PCG_CJ.m:95-96)
     return;
   }
-  const double s0 = fin_sum(part, count, lds);
-  const double s1 = (kind == kFinInit || kind == kFinAlpha) ? fin_sum(part + count, count, lds) : 0.0;
+  double s0, s1 = 0.0;
+  if (kind == kFinInit || kind == kFinAlpha) {
+    double t[2];
+    fin_sum<2>(part, count, lds, t);
+    s0 = t[0];
+    s1 = t[1];
+  } else {
+    double t[1];
+    fin_sum<1>(part, count, lds, t);
+    s0 = t[0];
+  }
   if (threadIdx.x != 0) return;
   switch (kind) {
     case kFinInit:
