@@ -4,6 +4,7 @@
 // that follows the product in the V-cycle (scaling by omega, residual, correction) so that
 // no intermediate vector makes an extra trip through HBM.
 #pragma once
+#include <type_traits>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -714,16 +715,35 @@ static __global__ __launch_bounds__(kBlock) void pcg_zap_kernel(int n, const dou
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// p_new = z + beta p_old (formed on the fly at every gathered column), Ap = A p_new, and
-// per-block partials of p.Ap and p.r over the row blocks of A (as in spmv_stream_kernel).
+// p_new = z + beta p_old, Ap = A p_new, and per-block partials of p.Ap and p.r over the row
+// blocks of A (as in spmv_stream_kernel).  FORMED = false: p_new is formed on the fly at
+// every gathered column (z and p_old gathered) and written for the block's rows; FORMED =
+// true: p_new was formed by pcg_pnew_kernel and is the one gathered operand.  Both forms
+// compute every p_i by the same expression, so they give the same bits.
 struct PColVal {
   const double *__restrict__ z;
   const double *__restrict__ p;
   double beta;
   __device__ __forceinline__ double operator()(int j) const { return z[j] + beta * p[j]; }
 };
+struct PColFormed {
+  const double *__restrict__ p;
+  __device__ __forceinline__ double operator()(int j) const { return p[j]; }
+};
 
-template <int PER, typename TI>
+// p_new = z + beta p_old (PCG_CJ.m:45), formed before the product so that the product
+// gathers one operand per nonzero.
+static __global__ __launch_bounds__(kBlock) void pcg_pnew_kernel(int n, const double *__restrict__ z,
+                                                          const double *__restrict__ p_old,
+                                                          double *__restrict__ p_new,
+                                                          const PcgState *st) {
+  if (st->stop) return;
+  const double beta = st->beta;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    p_new[i] = z[i] + beta * p_old[i];
+}
+
+template <int PER, typename TI, bool FORMED>
 __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
     int nblk, const int4 *__restrict__ blk, const int *__restrict__ rp, const TI *__restrict__ ci,
     const double *__restrict__ val, const double *__restrict__ z, const double *__restrict__ p_old,
@@ -736,14 +756,17 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
   const double beta = st->beta;
   const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
   const int tid = threadIdx.x;
-  PColVal cv{z, p_old, beta};
+  using CV = typename std::conditional<FORMED, PColFormed, PColVal>::type;
+  CV cv;
+  if constexpr (FORMED) cv = CV{p_new};
+  else cv = CV{z, p_old, beta};
   double pap = 0.0, prr = 0.0;
   if (e1 - e0 > PER * kBlock) {
     double s = strided_row_sum<double>(e0, e1, [&](int e) { return val[e] * cv(base + (int)ci[e]); });
     s = block_sum_t<double>(s, prod);
     if (tid == 0) {
       const double pi = cv(r0);
-      p_new[r0] = pi;
+      if constexpr (!FORMED) p_new[r0] = pi;
       ap[r0] = s;
       pap = pi * s;
       prr = pi * r[r0];
@@ -757,7 +780,7 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
     __syncthreads();
     if (row < r1) {
       const double s = row_sum<double>(prod, ra, rz);
-      p_new[row] = pi;
+      if constexpr (!FORMED) p_new[row] = pi;
       ap[row] = s;
       pap = pi * s;
       prr = pi * ri;
