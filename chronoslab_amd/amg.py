@@ -143,7 +143,8 @@ class DeviceMatrix:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            _release("chr_mat_destroy", self.ptr)
+            if _release is not None:  # None once the module is torn down at interpreter exit
+                _release("chr_mat_destroy", self.ptr)
             self.ptr = None
 
     def __call__(self, x: np.ndarray) -> np.ndarray:
@@ -170,7 +171,8 @@ class AMGPreconditioner:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            _release("chr_hier_destroy", self.ptr)
+            if _release is not None:  # None once the module is torn down at interpreter exit
+                _release("chr_hier_destroy", self.ptr)
             self.ptr = None
 
     def stats(self) -> dict:
@@ -268,7 +270,8 @@ class BatchSolver:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            _release("chr_batch_destroy", self.ptr)
+            if _release is not None:  # None once the module is torn down at interpreter exit
+                _release("chr_batch_destroy", self.ptr)
             self.ptr = None
 
     def vcycle(self, B: np.ndarray) -> np.ndarray:

@@ -110,12 +110,33 @@ __global__ __launch_bounds__(chr::kBlock) void bpcg_update_kernel(const int4 *__
   }
 }
 
-// p_new = z + beta p_old formed at every gathered column, Ap = A p_new (row-major n x k,
-// beta per solve: the row's system and the column).  As in the block product, the stream
-// block's values and columns are staged in LDS (cap = PER * kBlock nonzeros) and each (row,
-// column) pair is a lane that issues the gathers of 8 staged nonzeros before adding their
-// products in nonzero order; a block holding one longer row is summed straight from memory.
-template <int KL, typename TI>
+// p_new = z + beta p_old (row-major n x k, beta per solve: the chunk's system and the
+// column), formed before the product so that the product gathers one operand per nonzero.
+__global__ __launch_bounds__(chr::kBlock) void bpcg_pnew_kernel(const int4 *__restrict__ chunks, int k,
+                                                                const double *__restrict__ beta,
+                                                                const double *__restrict__ z,
+                                                                const double *__restrict__ p_old,
+                                                                double *__restrict__ p_new,
+                                                                const BState *st) {
+  if (st->stop) return;
+  const int4 ch = chunks[blockIdx.x];
+  const int c = threadIdx.x % k, g = threadIdx.x / k, ng = chr::kBlock / k;
+  if (threadIdx.x >= ng * k) return;
+  const double bq = beta[ch.z * k + c];
+  for (int row = ch.x + g; row < ch.y; row += ng) {
+    const size_t i = (size_t)row * k + c;
+    p_new[i] = z[i] + bq * p_old[i];
+  }
+}
+
+// Ap = A p_new (row-major n x k).  FORMED = false: p_new = z + beta p_old formed at every
+// gathered column and written for the block's rows; FORMED = true: p_new formed by
+// bpcg_pnew_kernel (the same expression, so the same bits).  As in the block product, the
+// stream block's values and columns are staged in LDS (cap = PER * kBlock nonzeros) and
+// each (row, column) pair is a lane that issues the gathers of 8 staged nonzeros before
+// adding their products in nonzero order; a block holding one longer row is summed
+// straight from memory.
+template <int KL, typename TI, bool FORMED>
 __global__ __launch_bounds__(chr::kBlock) void bpcg_p_kernel(int nblk, const int4 *__restrict__ blk,
                                                              const int *__restrict__ rp,
                                                              const TI *__restrict__ ci,
@@ -135,14 +156,14 @@ __global__ __launch_bounds__(chr::kBlock) void bpcg_p_kernel(int nblk, const int
   if (e1 - e0 > cap) {  // one long row
     const int c = tid % KL, r = lo.x;
     if (tid >= KL || c >= k) return;  // no cross-lane operations below
-    const double bq = beta[sys_of(r, off, nsys) * k + c];
+    const double bq = FORMED ? 0.0 : beta[sys_of(r, off, nsys) * k + c];
     double s = 0.0;
     for (int e = rp[r]; e < rp[r + 1]; ++e) {
       const size_t j = (size_t)(base + (int)ci[e]) * k + c;
-      s = s + chr::mul_rn(val[e], z[j] + bq * p_old[j]);
+      s = s + chr::mul_rn(val[e], FORMED ? p_new[j] : z[j] + bq * p_old[j]);
     }
     const size_t i = (size_t)r * k + c;
-    p_new[i] = z[i] + bq * p_old[i];
+    if constexpr (!FORMED) p_new[i] = z[i] + bq * p_old[i];
     ap[i] = s;
     return;
   }
@@ -157,26 +178,29 @@ __global__ __launch_bounds__(chr::kBlock) void bpcg_p_kernel(int nblk, const int
   for (int p = tid; p < npair; p += chr::kBlock) {
     const int r = lo.x + p / KL, c = p % KL;
     if (c >= k) continue;
-    const double bq = beta[sys_of(r, off, nsys) * k + c];
+    const double bq = FORMED ? 0.0 : beta[sys_of(r, off, nsys) * k + c];
     const int ra = rp[r] - e0, rz = rp[r + 1] - e0;
     double s = 0.0;
     for (int q = ra; q < rz; q += 8) {
       const int n = min(8, rz - q);
-      double vv[8], zz[8], pp[8];
+      double vv[8], pc[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int qq = q + min(u, n - 1);
         const size_t j = (size_t)(base + (int)sc[qq]) * k + c;
         vv[u] = sv[qq];
-        zz[u] = z[j];
-        pp[u] = p_old[j];
+        if constexpr (FORMED) {
+          pc[u] = p_new[j];
+        } else {
+          pc[u] = z[j] + bq * p_old[j];
+        }
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (u < n) s = s + chr::mul_rn(vv[u], zz[u] + bq * pp[u]);
+        if (u < n) s = s + chr::mul_rn(vv[u], pc[u]);
     }
     const size_t i = (size_t)r * k + c;
-    p_new[i] = z[i] + bq * p_old[i];
+    if constexpr (!FORMED) p_new[i] = z[i] + bq * p_old[i];
     ap[i] = s;
   }
 }
@@ -512,12 +536,26 @@ hipError_t launch_bp(chr_batch &bt, int k, const double *pold, double *pnew, hip
   const DevCsr &A = bt.h->outer.A;
   auto *st = bt.state.as<BState>();
   const int cap = A.per * chr::kBlock;
+  // p = z + beta p_old in a pass of its own (CHRONOS_PCG_FORMED=0: formed inside the product).
+  const char *env = std::getenv("CHRONOS_PCG_FORMED");
+  const bool formed = !env || std::atoi(env) != 0;
+  if (formed)
+    hipLaunchKernelGGL(bpcg_pnew_kernel, dim3(bt.nchunks), dim3(chr::kBlock), 0, s, bt.chunks.as<int4>(), k,
+                       (const double *)bt.a.beta, (const double *)bt.z.as<double>(), pold, pnew,
+                       (const BState *)st);
 #define BP(KL, TI)                                                                                        \
-  hipLaunchKernelGGL((bpcg_p_kernel<KL, TI>), dim3(A.nblk), dim3(chr::kBlock),                           \
+  do {                                                                                                    \
+  if (formed) hipLaunchKernelGGL((bpcg_p_kernel<KL, TI, true>), dim3(A.nblk), dim3(chr::kBlock),         \
                      (size_t)cap * (sizeof(double) + sizeof(TI)), s, A.nblk, A.blk.as<int4>(),            \
                      A.rp.as<int>(), A.ci.as<TI>(), A.val.as<double>(), k, cap, bt.off_dev.as<int>(), bt.nsys, \
                      (const double *)bt.a.beta, (const double *)bt.z.as<double>(), pold, pnew,            \
-                     bt.ap.as<double>(), (const BState *)st)
+                     bt.ap.as<double>(), (const BState *)st);                                             \
+  else hipLaunchKernelGGL((bpcg_p_kernel<KL, TI, false>), dim3(A.nblk), dim3(chr::kBlock),               \
+                     (size_t)cap * (sizeof(double) + sizeof(TI)), s, A.nblk, A.blk.as<int4>(),            \
+                     A.rp.as<int>(), A.ci.as<TI>(), A.val.as<double>(), k, cap, bt.off_dev.as<int>(), bt.nsys, \
+                     (const double *)bt.a.beta, (const double *)bt.z.as<double>(), pold, pnew,            \
+                     bt.ap.as<double>(), (const BState *)st);                                             \
+  } while (0)
   if (A.idx16) {
     if (k <= 2) BP(2, uint16_t); else if (k <= 4) BP(4, uint16_t); else if (k <= 8) BP(8, uint16_t); else BP(16, uint16_t);
   } else {
