@@ -597,6 +597,64 @@ static __global__ __launch_bounds__(kBlock) void gram_partial_kernel(int n, int 
   }
   if (t < ab) part[(size_t)blockIdx.x * ab + t] = s;
 }
+// Several Gram products of up to six row-major blocks in one pass: block b (n x w[b],
+// w[b] <= 16) is staged once per 64-row tile, and pair k contributes the entries
+// part[g][po[k] + p*w[pj[k]] + q] = sum over the part's rows of blk[pi[k]][i][p] *
+// blk[pj[k]][i][q], in row order with the same row split as gram_partial_kernel, so every
+// entry equals that kernel's partial bit for bit.
+constexpr int kGramMaxBlocks = 6, kGramMaxPairs = 12, kGramMaxOut = 12;  // 12 * kBlock outputs
+struct GramSet {
+  const double *blk[kGramMaxBlocks];
+  int w[kGramMaxBlocks];
+  int loff[kGramMaxBlocks];  // first LDS column of block b (prefix sum of the widths)
+  int nblk;
+  int pi[kGramMaxPairs], pj[kGramMaxPairs], po[kGramMaxPairs + 1];
+  int npair;
+};
+static __global__ __launch_bounds__(kBlock) void gram_multi_partial_kernel(int n, GramSet gs, int rows_per_block,
+                                                                    double *__restrict__ part) {
+  __shared__ double tile[kGramTile * kGramMaxBlocks * 16];
+  const int t = threadIdx.x, total = gs.po[gs.npair];
+  const int i0 = blockIdx.x * rows_per_block, i1 = min(n, i0 + rows_per_block);
+  double acc[kGramMaxOut];
+  int ax[kGramMaxOut], wx[kGramMaxOut], ay[kGramMaxOut], wy[kGramMaxOut];
+#pragma unroll
+  for (int m = 0; m < kGramMaxOut; ++m) {
+    acc[m] = 0.0;
+    ax[m] = ay[m] = 0;
+    wx[m] = wy[m] = 0;
+    const int o = t + m * kBlock;
+    if (o < total) {
+      int k = 0;
+      while (o >= gs.po[k + 1]) ++k;
+      const int bx = gs.pi[k], by = gs.pj[k], local = o - gs.po[k];
+      wx[m] = gs.w[bx];
+      wy[m] = gs.w[by];
+      ax[m] = kGramTile * gs.loff[bx] + local / wy[m];
+      ay[m] = kGramTile * gs.loff[by] + local % wy[m];
+    }
+  }
+  for (int r0 = i0; r0 < i1; r0 += kGramTile) {
+    const int rows = min(kGramTile, i1 - r0);
+    for (int b = 0; b < gs.nblk; ++b) {
+      const double *src = gs.blk[b] + (size_t)r0 * gs.w[b];
+      double *dst = tile + kGramTile * gs.loff[b];
+      for (int e = t; e < rows * gs.w[b]; e += kBlock) dst[e] = src[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kGramMaxOut; ++m)
+      if (wx[m] > 0)
+        for (int r = 0; r < rows; ++r) acc[m] += tile[ax[m] + r * wx[m]] * tile[ay[m] + r * wy[m]];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < kGramMaxOut; ++m) {
+    const int o = t + m * kBlock;
+    if (o < total) part[(size_t)blockIdx.x * total + o] = acc[m];
+  }
+}
+
 // out[t] = sum over the nparts partials of entry t: one wave per entry, lane l adding
 // partials l, l + 64, ... in order, then a fixed butterfly across the wave.
 static __global__ __launch_bounds__(kBlock) void gram_reduce_kernel(int nparts, int ab,
@@ -635,6 +693,57 @@ static __global__ __launch_bounds__(kBlock) void block_gemm_kernel(int n, int a,
   }
 }
 
+// The closing update of a LOBPCG iteration (lobpcg.m:389-400) in one pass, for the
+// search-direction side (R, Pa, X) or the operator side (AR, APa, AX):
+//   Pout = R ER (+ Pa EP),   Xout = Pout + X EX,
+// R and Pa n x na, X n x k, ER and EP column-major na x k, EX k x k (na, k <= 16).  Each
+// product sums over its inner index in order (as block_gemm_kernel does) and the terms are
+// added as ((R ER + Pa EP) + X EX), the order of separate products followed by a sum.  X and Xout may alias: a tile's rows are staged before
+// any of its outputs is written.
+static __global__ __launch_bounds__(kBlock) void lob_update_kernel(int n, int na, int k, const double *__restrict__ R,
+                                                            const double *__restrict__ Pa,
+                                                            const double *X, const double *__restrict__ ER,
+                                                            const double *__restrict__ EP,
+                                                            const double *__restrict__ EX,
+                                                            double *__restrict__ Pout, double *Xout) {
+  __shared__ double cr[16 * 16], cp[16 * 16], cx[16 * 16], rs[kGemmTile * 16], ps[kGemmTile * 16],
+      xs[kGemmTile * 16];
+  const int t = threadIdx.x;
+  for (int e = t; e < na * k; e += kBlock) {
+    cr[e] = ER[e];
+    if (Pa) cp[e] = EP[e];
+  }
+  for (int e = t; e < k * k; e += kBlock) cx[e] = EX[e];
+  __syncthreads();
+  for (int r0 = blockIdx.x * kGemmTile; r0 < n; r0 += gridDim.x * kGemmTile) {
+    const int rows = min(kGemmTile, n - r0);
+    for (int e = t; e < rows * na; e += kBlock) {
+      rs[e] = R[(size_t)r0 * na + e];
+      if (Pa) ps[e] = Pa[(size_t)r0 * na + e];
+    }
+    for (int e = t; e < rows * k; e += kBlock) xs[e] = X[(size_t)r0 * k + e];
+    __syncthreads();
+    for (int o = t; o < rows * k; o += kBlock) {
+      const int r = o / k, q = o % k;
+      double s1 = 0.0;
+      for (int p = 0; p < na; ++p) s1 += rs[r * na + p] * cr[p + q * na];
+      double pv = 0.0 + s1;
+      if (Pa) {
+        double s2 = 0.0;
+        for (int p = 0; p < na; ++p) s2 += ps[r * na + p] * cp[p + q * na];
+        pv = 1.0 * pv + s2;
+      }
+      double s3 = 0.0;
+      for (int p = 0; p < k; ++p) s3 += xs[r * k + p] * cx[p + q * k];
+      const double tv = 0.0 + s3;
+      const size_t zi = (size_t)(r0 + r) * k + q;
+      Pout[zi] = pv;
+      Xout[zi] = pv + tv;
+    }
+    __syncthreads();
+  }
+}
+
 // R = AX - X diag(lam)   (n x k)
 static __global__ __launch_bounds__(kBlock) void block_resid_kernel(int n, int k, const double *__restrict__ AX,
                                                              const double *__restrict__ X,
@@ -653,14 +762,6 @@ static __global__ __launch_bounds__(kBlock) void block_gather_kernel(int n, int 
   if (t >= (int64_t)n * m) return;
   const int64_t i = t / m;
   Y[t] = X[i * k + idx[t % m]];
-}
-
-// Z = X + Y (elementwise, n*k entries)
-static __global__ __launch_bounds__(kBlock) void block_add_kernel(int64_t nk, const double *__restrict__ X,
-                                                           const double *__restrict__ Y,
-                                                           double *__restrict__ Z) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t < nk) Z[t] = X[t] + Y[t];
 }
 
 // Strided block column copy with type conversion: dst[i] = src[i] (one column).

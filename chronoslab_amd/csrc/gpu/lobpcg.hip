@@ -5,6 +5,7 @@
 // the preconditioner is the block V-cycle, and the Gram matrices are tall-skinny products
 // reduced in a fixed order.  The Rayleigh-Ritz steps (order <= 3*neig) run in FP64 on the
 // host: Cholesky, the generalized symmetric-definite eigenproblem and condition numbers.
+#include <cstdlib>
 #include <cstring>
 
 #include "chronos_amg.h"
@@ -42,7 +43,7 @@ struct Lob {
     nparts = std::max(1, std::min(1024, (n + 255) / 256));
     rows_per_part = (n + nparts - 1) / nparts;
     int rc;
-    if ((rc = part.alloc(sizeof(double) * (size_t)nparts * chr::kBlock))) return rc;
+    if ((rc = part.alloc(sizeof(double) * (size_t)nparts * chr::kGramMaxOut * chr::kBlock))) return rc;
     if ((rc = gout.alloc(sizeof(double) * kSlots * kSlot))) return rc;
     if ((rc = stage_dev.alloc(sizeof(double) * kStage))) return rc;
     HIPC(hipHostMalloc((void **)&hstage, sizeof(double) * kStage, hipHostMallocDefault));
@@ -82,6 +83,72 @@ struct Lob {
       return -1;
     }
     return nslot++;
+  }
+  // Enqueues the Gram products X_k^T Y_k of pairs (X_k, a_k, Y_k, b_k) in one pass over the
+  // distinct blocks (at most six, widths <= 16; gram_multi_partial_kernel), each entry equal
+  // to the one gram_launch would give.  Returns the first slot, or -1 on error; product k
+  // starts offs[k] entries after that slot's start, row-major a_k x b_k.
+  struct GramPair {
+    const double *X;
+    int a;
+    const double *Y;
+    int b;
+  };
+  int gram_multi_launch(const std::vector<GramPair> &pairs, std::vector<int> &offs) {
+    chr::GramSet gs{};
+    auto id = [&](const double *p, int w) -> int {
+      for (int b = 0; b < gs.nblk; ++b)
+        if (gs.blk[b] == p) return gs.w[b] == w ? b : -1;
+      if (gs.nblk == chr::kGramMaxBlocks || w < 1 || w > 16) return -1;
+      gs.blk[gs.nblk] = p;
+      gs.w[gs.nblk] = w;
+      gs.loff[gs.nblk] = gs.nblk ? gs.loff[gs.nblk - 1] + gs.w[gs.nblk - 1] : 0;
+      return gs.nblk++;
+    };
+    if (pairs.empty() || (int)pairs.size() > chr::kGramMaxPairs) {
+      fail(CHR_EARG, "lobpcg: Gram batch size");
+      return -1;
+    }
+    offs.clear();
+    int total = 0;
+    for (size_t k = 0; k < pairs.size(); ++k) {
+      const int bx = id(pairs[k].X, pairs[k].a), by = id(pairs[k].Y, pairs[k].b);
+      if (bx < 0 || by < 0) {
+        fail(CHR_EARG, "lobpcg: Gram batch operands (at most six blocks of width <= 16)");
+        return -1;
+      }
+      gs.pi[k] = bx;
+      gs.pj[k] = by;
+      gs.po[k] = total;
+      offs.push_back(total);
+      total += pairs[k].a * pairs[k].b;
+    }
+    gs.npair = (int)pairs.size();
+    gs.po[gs.npair] = total;
+    const int slots = (total + kSlot - 1) / kSlot;
+    if (total > chr::kGramMaxOut * chr::kBlock || nslot + slots > kSlots) {
+      fail(CHR_EARG, "lobpcg: Gram batch too large");
+      return -1;
+    }
+    hipLaunchKernelGGL(chr::gram_multi_partial_kernel, dim3(nparts), dim3(chr::kBlock), 0, s, n, gs,
+                       rows_per_part, part.as<double>());
+    const unsigned rgrid = (unsigned)(((int64_t)total * chr::kWave + chr::kBlock - 1) / chr::kBlock);
+    hipLaunchKernelGGL(chr::gram_reduce_kernel, dim3(rgrid), dim3(chr::kBlock), 0, s, nparts, total,
+                       (const double *)part.as<double>(), gout.as<double>() + (size_t)nslot * kSlot);
+    if (hipGetLastError() != hipSuccess) {
+      fail(CHR_EHIP, "lobpcg: Gram batch launch");
+      return -1;
+    }
+    const int base = nslot;
+    nslot += slots;
+    return base;
+  }
+  // out (column-major a x b) = the fetched product at offset off of slot base.
+  void get_at(int base, int off, int a, int b, std::vector<double> &out) const {
+    const double *g = hout + (size_t)base * kSlot + off;
+    out.assign((size_t)a * b, 0.0);
+    for (int p = 0; p < a; ++p)
+      for (int q = 0; q < b; ++q) out[p + (size_t)q * a] = g[(size_t)p * b + q];
   }
   // Brings the pending Gram slots to the host (one copy, one synchronisation).
   int fetch() {
@@ -129,6 +196,21 @@ struct Lob {
     HIPC(hipMemcpyAsync(X, tmp, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToDevice, s));
     return CHR_OK;
   }
+  // Pout = R ER (+ Pa EP), X <- Pout + X EX in one pass (lob_update_kernel); Pa null on a
+  // restart.  ER, EP column-major na x k, EX k x k (host).
+  int update(const double *R, const double *Pa, int na, double *X, int k, const double *ER,
+             const double *EP, const double *EX, double *Pout) {
+    if (na < 1 || k < 1 || na > 16 || k > 16) return fail(CHR_EARG, "lobpcg: update block wider than 16");
+    const double *er = static_cast<const double *>(stage(ER, sizeof(double) * na * k));
+    const double *ep = Pa ? static_cast<const double *>(stage(EP, sizeof(double) * na * k)) : er;
+    const double *ex = static_cast<const double *>(stage(EX, sizeof(double) * k * k));
+    if (!er || !ep || !ex) return fail(CHR_EHIP, "lobpcg: staging the update coefficients");
+    const unsigned ggrid = (unsigned)std::min<int64_t>(2048, ((int64_t)n + chr::kGemmTile - 1) / chr::kGemmTile);
+    hipLaunchKernelGGL(chr::lob_update_kernel, dim3(ggrid), dim3(chr::kBlock), 0, s, n, na, k, R, Pa,
+                       (const double *)X, er, Pa ? ep : nullptr, ex, Pout, X);
+    HIPC(hipGetLastError());
+    return CHR_OK;
+  }
   int resid(const double *AX, const double *X, const double *lam_host, int k, double *R) {
     const double *ldev = static_cast<const double *>(stage(lam_host, sizeof(double) * k));
     if (!ldev) return fail(CHR_EHIP, "lobpcg: staging eigenvalues");
@@ -143,12 +225,6 @@ struct Lob {
     if (!idev) return fail(CHR_EHIP, "lobpcg: staging a column list");
     hipLaunchKernelGGL(chr::block_gather_kernel, dim3(grid((int64_t)n * m)), dim3(chr::kBlock), 0, s, n, k,
                        X, m, idev, Y);
-    HIPC(hipGetLastError());
-    return CHR_OK;
-  }
-  int add(const double *X, const double *Y, double *Z, int k) {
-    hipLaunchKernelGGL(chr::block_add_kernel, dim3(grid((int64_t)n * k)), dim3(chr::kBlock), 0, s,
-                       (int64_t)n * k, X, Y, Z);
     HIPC(hipGetLastError());
     return CHR_OK;
   }
@@ -199,13 +275,13 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
   Lob L{h, A, zero.as<int>(), s, n};
   if ((rc = L.init())) return rc;
   const size_t blk = sizeof(double) * (size_t)n * neig;
-  DevBuf bX, bAX, bP, bAP, bR, bAR, bAAR, bAPa, bAAP, bBest, bT, bT2;
-  for (DevBuf *b : {&bX, &bAX, &bP, &bAP, &bR, &bAR, &bAAR, &bAPa, &bAAP, &bBest, &bT, &bT2})
+  DevBuf bX, bAX, bP, bAP, bR, bAR, bAAR, bAPa, bAAP, bBest, bT;
+  for (DevBuf *b : {&bX, &bAX, &bP, &bAP, &bR, &bAR, &bAAR, &bAPa, &bAAP, &bBest, &bT})
     if ((rc = b->alloc(blk))) return rc;
   double *dX = bX.as<double>(), *dAX = bAX.as<double>(), *dP = bP.as<double>(), *dAP = bAP.as<double>();
   double *dR = bR.as<double>(), *dAR = bAR.as<double>(), *dAAR = bAAR.as<double>();
   double *dAPa = bAPa.as<double>(), *dAAP = bAAP.as<double>(), *dBest = bBest.as<double>();
-  double *dT = bT.as<double>(), *dT2 = bT2.as<double>();
+  double *dT = bT.as<double>();
   HIPC(hipMemsetAsync(dP, 0, blk, s));
   HIPC(hipMemsetAsync(dAP, 0, blk, s));
   // X0 (column-major) -> row-major.
@@ -350,12 +426,29 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
       blocks.push_back({dAR, na, dAPa, na, false, neig, c2, false, -1});
       blocks.push_back({dAPa, na, dAPa, na, false, c2, c2, true, -1});
     }
-    for (auto &bk : blocks)
-      if ((bk.slot = L.gram_launch(bk.X, bk.a, bk.Y, bk.b)) < 0) return CHR_EARG;
+    std::vector<Lob::GramPair> pairs;
+    for (const auto &bk : blocks) pairs.push_back({bk.X, bk.a, bk.Y, bk.b});
+    std::vector<int> offs;
+    int base;
+    if (const char *env = std::getenv("CHRONOS_GRAM_MULTI"); env && std::atoi(env) == 0) {
+      // one pass per product (same entries)
+      base = -1;
+      offs.clear();
+      for (const auto &pr : pairs) {
+        const int sl = L.gram_launch(pr.X, pr.a, pr.Y, pr.b);
+        if (sl < 0) return CHR_EARG;
+        if (base < 0) base = sl;
+        offs.push_back((sl - base) * Lob::kSlot);
+      }
+    } else {
+      base = L.gram_multi_launch(pairs, offs);
+      if (base < 0) return CHR_EARG;
+    }
     if ((rc = L.fetch())) return rc;
     std::vector<double> g;
-    for (const auto &bk : blocks) {
-      L.get(bk.slot, bk.a, bk.b, g);
+    for (size_t k = 0; k < blocks.size(); ++k) {
+      const auto &bk = blocks[k];
+      L.get_at(base, offs[k], bk.a, bk.b, g);
       if (bk.diag) {
         chrdense::symmetrize(bk.a, g.data());
         put(bk.toA ? gA : gB, bk.r0, bk.c0, bk.a, bk.b, g, false);
@@ -380,14 +473,10 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
       if (!restart)
         for (int i = 0; i < na; ++i) EP[i + (size_t)j * na] = e[neig + na + i];
     }
-    if ((rc = L.gemm(dAR, na, ER.data(), neig, 0.0, dP))) return rc;
-    if ((rc = L.gemm(dAAR, na, ER.data(), neig, 0.0, dAP))) return rc;
-    if (!restart) {
-      if ((rc = L.gemm(dAPa, na, EP.data(), neig, 1.0, dP))) return rc;
-      if ((rc = L.gemm(dAAP, na, EP.data(), neig, 1.0, dAP))) return rc;
-    }
-    if ((rc = L.gemm(dX, neig, EX.data(), neig, 0.0, dT)) || (rc = L.add(dP, dT, dX, neig))) return rc;
-    if ((rc = L.gemm(dAX, neig, EX.data(), neig, 0.0, dT2)) || (rc = L.add(dAP, dT2, dAX, neig))) return rc;
+    // P = AR ER + APa EP, X = P + X EX; AP and AX alike (lobpcg.m:389-400).
+    if ((rc = L.update(dAR, restart ? nullptr : dAPa, na, dX, neig, ER.data(), EP.data(), EX.data(), dP)) ||
+        (rc = L.update(dAAR, restart ? nullptr : dAAP, na, dAX, neig, ER.data(), EP.data(), EX.data(), dAP)))
+      return rc;
   }
   if (ierr != 1) {
     if (n_active > 0) {
