@@ -7,6 +7,7 @@
 // host: Cholesky, the generalized symmetric-definite eigenproblem and condition numbers.
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 
 #include "chronos_amg.h"
 #include "dense.hpp"
@@ -189,11 +190,12 @@ struct Lob {
     HIPC(launch_op(*A, X, k, chr::EpiStore<double>{Y}, zero, s));
     return CHR_OK;
   }
-  // X <- M X (block V-cycle), tmp: n x k scratch.
-  int applyM(double *X, int k, double *tmp) {
+  // X <- M X (block V-cycle) into the scratch block tmp, whose buffer then becomes X's
+  // (the two pointers swap instead of a copy back).
+  int applyM(double *&X, int k, double *&tmp) {
     if (!h) return CHR_OK;
     HIPC(enqueue_vcycle(*h, X, tmp, h->zero.as<int>(), k));
-    HIPC(hipMemcpyAsync(X, tmp, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToDevice, s));
+    std::swap(X, tmp);
     return CHR_OK;
   }
   // Pout = R ER (+ Pa EP), X <- Pout + X EX in one pass (lob_update_kernel); Pa null on a
@@ -232,9 +234,10 @@ struct Lob {
     HIPC(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToDevice, s));
     return CHR_OK;
   }
-  // B_orthonormalize.m with prodB = []: X <- X U^-1 with U^T U = X^T X; returns 1 when
-  // cond(X^T X) > 1/(k eps) (the reference's failure), U (column-major upper) otherwise.
-  int borth(double *X, int k, std::vector<double> &U, double *tmp, int &failed) {
+  // B_orthonormalize.m with prodB = []: X <- X U^-1 with U^T U = X^T X (formed in tmp,
+  // then the pointers swap); returns 1 when cond(X^T X) > 1/(k eps) (the reference's
+  // failure), U (column-major upper) otherwise.
+  int borth(double *&X, int k, std::vector<double> &U, double *&tmp, int &failed) {
     std::vector<double> M;
     int rc;
     if ((rc = gram(X, k, X, k, M))) return rc;
@@ -249,7 +252,8 @@ struct Lob {
       for (int i = 0; i <= j; ++i) U[i + (size_t)j * k] = M[j + (size_t)i * k];
     std::vector<double> Ui = chrdense::inv_upper(k, U.data());
     if ((rc = gemm(X, k, Ui.data(), k, 0.0, tmp))) return rc;
-    return copy(X, tmp, k);
+    std::swap(X, tmp);
+    return CHR_OK;
   }
 };
 
@@ -317,8 +321,10 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     w.resize(neig);
     E.resize((size_t)neig * neig);
     if (!chrdense::sygv(neig, xax.data(), xx.data(), w.data(), E.data())) return 1;
-    if ((r = L.gemm(dX, neig, E.data(), neig, 0.0, dT)) || (r = L.copy(dX, dT, neig))) return r;
-    if ((r = L.gemm(dAX, neig, E.data(), neig, 0.0, dT)) || (r = L.copy(dAX, dT, neig))) return r;
+    if ((r = L.gemm(dX, neig, E.data(), neig, 0.0, dT))) return r;
+    std::swap(dX, dT);
+    if ((r = L.gemm(dAX, neig, E.data(), neig, 0.0, dT))) return r;
+    std::swap(dAX, dT);
     for (int j = 0; j < neig; ++j) lam[j] = w[j];
     return 0;
   };
@@ -384,7 +390,8 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
         restart = 1;
       } else {
         std::vector<double> Ui = chrdense::inv_upper(na, U.data());
-        if ((rc = L.gemm(dAAP, na, Ui.data(), na, 0.0, dT)) || (rc = L.copy(dAAP, dT, na))) return rc;
+        if ((rc = L.gemm(dAAP, na, Ui.data(), na, 0.0, dT))) return rc;
+        std::swap(dAAP, dT);
         restart = forced;
       }
     }
