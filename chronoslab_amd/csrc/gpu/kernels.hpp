@@ -522,6 +522,101 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__re
   }
 }
 
+// Block product for k a multiple of 4 with X 16-byte aligned: as spmm_kernel, but each lane
+// takes four adjacent columns of its row (KL / 4 lanes per row) with one 16-byte load per
+// nonzero (two for FP64) and one read of the staged value and column for all four.  Every
+// column is still summed in nonzero order with each product rounded before the add, so the
+// results equal spmm_kernel's bit for bit.
+template <typename T>
+struct Quad;
+template <>
+struct Quad<float> {
+  static constexpr int kUnroll = 16;
+  __device__ __forceinline__ static void load(const float *p, float (&o)[4]) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  }
+};
+template <>
+struct Quad<double> {
+  static constexpr int kUnroll = 8;
+  __device__ __forceinline__ static void load(const double *p, double (&o)[4]) {
+    const double2 a = *reinterpret_cast<const double2 *>(p), b = *reinterpret_cast<const double2 *>(p + 2);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+  }
+};
+
+template <int KL, typename TV, typename TI, typename TX, typename Epi>
+__global__ __launch_bounds__(kBlock) void spmm4_kernel(int nblk, const int4 *__restrict__ blk,
+                                                       const int *__restrict__ rp,
+                                                       const TI *__restrict__ ci,
+                                                       const TV *__restrict__ val,
+                                                       const TX *__restrict__ x, int k, int cap, Epi epi,
+                                                       const int *__restrict__ stop) {
+  static_assert(KL % 4 == 0, "four columns per lane");
+  using TA = typename Acc<TV>::type;
+  constexpr int LR = KL / 4, U = Quad<TX>::kUnroll;
+  extern __shared__ __align__(16) char smem[];
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int4 lo = blk[b], hi = blk[b + 1];
+  if (*stop) return;
+  const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
+  const int tid = threadIdx.x;
+  if (e1 - e0 > cap) {  // one long row: KL lanes, one column each (as spmm_kernel)
+    const int c = tid % KL;
+    if (tid >= KL || c >= k) return;  // no cross-lane operations below
+    const int i = r0 * k + c;
+    const double pv = epi.pre(i);
+    TA s = 0;
+    for (int e = rp[r0]; e < rp[r0 + 1]; ++e) s = s + mul_rn((TA)val[e], (TA)x[(base + (int)ci[e]) * k + c]);
+    epi(i, (double)s, pv);
+    return;
+  }
+  TV *sv = reinterpret_cast<TV *>(smem);
+  TI *sc = reinterpret_cast<TI *>(smem + (size_t)cap * sizeof(TV));
+  for (int t = tid; t < e1 - e0; t += kBlock) {
+    sv[t] = val[e0 + t];
+    sc[t] = ci[e0 + t];
+  }
+  __syncthreads();
+  const int npair = (r1 - r0) * LR;
+  for (int p = tid; p < npair; p += kBlock) {
+    const int r = r0 + p / LR, c0 = 4 * (p % LR);
+    if (c0 >= k) continue;
+    const int i0 = r * k + c0;
+    double pv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) pv[j] = epi.pre(i0 + j);
+    const int ra = rp[r] - e0, rz = rp[r + 1] - e0;
+    TA s[4] = {0, 0, 0, 0};
+    for (int q = ra; q < rz; q += U) {
+      const int n = min(U, rz - q);
+      TV vv[U];
+      TX xv[U][4];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int qq = q + min(u, n - 1);
+        vv[u] = sv[qq];
+        Quad<TX>::load(x + (size_t)(base + (int)sc[qq]) * k + c0, xv[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (u < n) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) s[j] = s[j] + mul_rn((TA)vv[u], (TA)xv[u][j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) epi(i0 + j, (double)s[j], pv[j]);
+  }
+}
+
 // Direct solves on a level's coarse rows [nd, nd + nrows): each belongs to a dense block
 // (a system whose coarsest level this is) with its FP64 inverse (row-major m x m) packed in
 // Ainv.  cmap[t] = {first row of the block, m, offset of the block's inverse, 0} for coarse

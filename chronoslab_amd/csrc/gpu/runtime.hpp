@@ -349,6 +349,20 @@ template <int KL, typename TV, typename TX, typename Epi>
 hipError_t launch_spmm_kl(const DevCsr &M, const TX *x, int k, Epi epi, const int *stop,
                           hipStream_t s) {
   const int cap = M.per * chr::kBlock;  // nonzeros of a stream block (longer: one long row)
+  if constexpr (KL >= 4) {
+    // Four columns per lane when k allows it and X takes 16-byte loads (same bits).
+    if (k % 4 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 && !std::getenv("CHRONOS_SPMM_SCALAR")) {
+      if (M.idx16)
+        hipLaunchKernelGGL((chr::spmm4_kernel<KL, TV, uint16_t, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
+                           (size_t)cap * (sizeof(TV) + sizeof(uint16_t)), s, M.nblk, M.blk.as<int4>(),
+                           M.rp.as<int>(), M.ci.as<uint16_t>(), M.val.as<TV>(), x, k, cap, epi, stop);
+      else
+        hipLaunchKernelGGL((chr::spmm4_kernel<KL, TV, int, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
+                           (size_t)cap * (sizeof(TV) + sizeof(int)), s, M.nblk, M.blk.as<int4>(),
+                           M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, k, cap, epi, stop);
+      return hipGetLastError();
+    }
+  }
   if (M.idx16)
     hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, uint16_t, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
                        (size_t)cap * (sizeof(TV) + sizeof(uint16_t)), s, M.nblk, M.blk.as<int4>(),
