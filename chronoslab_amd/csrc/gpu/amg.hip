@@ -118,7 +118,7 @@ int DevPack::build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *
 // CHRONOS_SELL_SORT=0|8|10|12|14 fixes its sort window (0: unsorted); CHRONOS_PACK=0
 // excludes and CHRONOS_PACK=1 forces the packed-code layout (when sliced ELL is not forced).
 int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
-                const int *zero, bool tune_per, const DevBuf *flush) {
+                const int *zero, bool tune_per, const DevBuf *flush, int tune_k) {
   M.use_sell = false;
   M.sell.release();
   M.use_pack = false;
@@ -137,7 +137,8 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
   const bool cold = flush && flush->p && M.csr_bytes() >= 32e6;
   DevBuf xb, yb;
   int rc;
-  if ((rc = xb.alloc(sz * std::max(M.ncols, 1))) || (rc = yb.alloc(sz * std::max(M.nrows, 1)))) return rc;
+  const int kt = std::max(1, std::min(tune_k, kMaxBlockCols));
+  if ((rc = xb.alloc(sz * kt * std::max(M.ncols, 1))) || (rc = yb.alloc(sz * kt * std::max(M.nrows, 1)))) return rc;
   HIPC(hipMemsetAsync(xb.p, 0, xb.bytes, s));
   hipEvent_t ev[12];
   for (auto &e : ev) HIPC(hipEventCreate(&e));
@@ -149,8 +150,8 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
   } evg{ev};
   auto time_it = [&](float &ms) -> int {
     auto run = [&]() -> hipError_t {
-      if (M.fp32) return launch_spmv(M, xb.as<float>(), chr::EpiStore<float>{yb.as<float>()}, zero, s);
-      return launch_spmv(M, xb.as<double>(), chr::EpiStore<double>{yb.as<double>()}, zero, s);
+      if (M.fp32) return launch_op(M, xb.as<float>(), kt, chr::EpiStore<float>{yb.as<float>()}, zero, s);
+      return launch_op(M, xb.as<double>(), kt, chr::EpiStore<double>{yb.as<double>()}, zero, s);
     };
     hipError_t e = hipSuccess;
     for (int i = 0; i < 2 && !e; ++i) e = run();
@@ -202,6 +203,7 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
       if (best_per != M.per) rc = M.layout(best_per, hrp, hci);
     }
   }
+  if (kt > 1) return rc;  // block products use the stream blocks only
   // Step 2: packed codes (the stream kernel's blocks with 2-byte codes).
   // (Only where 16-bit block offsets do not apply: otherwise the stream layout's index is
   // already 2 bytes.)
@@ -400,7 +402,7 @@ extern "C" int chr_init(int32_t device) {
 namespace chrrt {
 
 // Builds a hierarchy on the device from per-level host descriptions (see chr_hier::Level).
-int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out) {
+int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out, int tune_k) {
   auto h = std::make_unique<chr_hier>();
   h->precision = precision;
   HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -430,21 +432,21 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
     L.nupre = in.nupre;
     L.nupost = in.nupost;
     if (in.nd > 0) {
-      if ((rc = L.A.upload(in.A, f32)) || (rc = tune_layout(L.A, in.A.rowptr, in.A.col, in.A.val, h->stream, zf, true, fl)))
+      if ((rc = L.A.upload(in.A, f32)) || (rc = tune_layout(L.A, in.A.rowptr, in.A.col, in.A.val, h->stream, zf, true, fl, tune_k)))
         return rc;
-      if ((rc = L.G.upload(in.G, f32)) || (rc = tune_layout(L.G, in.G.rowptr, in.G.col, in.G.val, h->stream, zf, true, fl)))
+      if ((rc = L.G.upload(in.G, f32)) || (rc = tune_layout(L.G, in.G.rowptr, in.G.col, in.G.val, h->stream, zf, true, fl, tune_k)))
         return rc;
-      if ((rc = L.P.upload(in.P, f32)) || (rc = tune_layout(L.P, in.P.rowptr, in.P.col, in.P.val, h->stream, zf, true, fl)))
+      if ((rc = L.P.upload(in.P, f32)) || (rc = tune_layout(L.P, in.P.rowptr, in.P.col, in.P.val, h->stream, zf, true, fl, tune_k)))
         return rc;
       std::vector<int32_t> trp, tci;
       std::vector<double> tv;
       transpose_host(in.G, trp, tci, tv);
       if ((rc = L.GT.upload(in.G.ncols, in.G.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
-          (rc = tune_layout(L.GT, trp.data(), tci.data(), tv.data(), h->stream, zf, true, fl)))
+          (rc = tune_layout(L.GT, trp.data(), tci.data(), tv.data(), h->stream, zf, true, fl, tune_k)))
         return rc;
       transpose_host(in.P, trp, tci, tv);
       if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
-          (rc = tune_layout(L.PT, trp.data(), tci.data(), tv.data(), h->stream, zf, true, fl)))
+          (rc = tune_layout(L.PT, trp.data(), tci.data(), tv.data(), h->stream, zf, true, fl, tune_k)))
         return rc;
       if (!in.omega_rows.empty()) {
         if ((rc = L.omega_rows.alloc(sizeof(double) * in.nd))) return rc;

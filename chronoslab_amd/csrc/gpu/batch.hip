@@ -435,7 +435,9 @@ extern "C" int chr_batch_create(int32_t nsys, const int32_t *nlev, const chr_lev
   }
   for (int32_t t = 0; t < nsys; ++t) mA0[0].append(levels[bt->order[t]][0].A, off[0][t]);
   mA0[0].ncols = lv[0].n;
-  int rc = build_hier(lv, precision, mA0[0].view(), &bt->h);
+  // The batch's level products are block products (8 right-hand sides in its configuration),
+  // so the layouts are tuned with 8-column products.
+  int rc = build_hier(lv, precision, mA0[0].view(), &bt->h, 8);
   if (rc) return rc;
   // Offsets in both orders, row chunks that never straddle systems.
   bt->n0.resize(nsys);

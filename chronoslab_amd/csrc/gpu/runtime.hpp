@@ -398,8 +398,10 @@ constexpr int kMaxBlockCols = 16;
 
 // flush: a device buffer of 1 GiB or so whose halves are copied onto each other to evict
 // the Infinity Cache before cache-cold timings (NULL: every candidate timed back to back).
+// tune_k > 1: the matrix serves block products of that many columns, so the stream blocks'
+// PER is timed with those and the single-vector layouts (packed, sliced ELL) are not built.
 int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
-                const int *zero, bool tune_per = true, const DevBuf *flush = nullptr);
+                const int *zero, bool tune_per = true, const DevBuf *flush = nullptr, int tune_k = 1);
 
 uint64_t next_uid();
 
@@ -584,7 +586,10 @@ struct LevelIn {
   };
   std::vector<Direct> direct;       // rows [nd, n), in order
 };
-int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out);
+// tune_k: block width the level matrices are tuned for (1 for single systems; the batch
+// tunes for its block products).
+int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out,
+               int tune_k = 1);
 
 // One V-cycle on FP64 row-major n_0 x k blocks (k <= kcap).
 // b0f (FP32 cycle, k = 1): an FP32 copy of b for the first smoother product.
