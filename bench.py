@@ -244,8 +244,8 @@ def main() -> int:
         return 0
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "the solve phase runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design only, not built)"}))
         return 2
 
     import numpy as np

@@ -139,7 +139,7 @@ int chr_lobpcg(chr_hier *h, chr_mat *A, int32_t n, int32_t neig, double *X, int6
  * proportional to k_i): given the forward field x (A x = b) and the adjoint field lam
  * (A^T lam = A lam = dL/dx, e.g. from chr_pcg_solve on the same hierarchy), writes
  * g = dL/d(ln k) on A's sparsity pattern in one fused kernel.  A: NULL = the level-0
- * operator of h.  No reference counterpart (the differentiation row of this build). */
+ * operator of h.  No reference counterpart (it serves the differentiable solve). */
 int chr_fv_grad(chr_hier *h, chr_mat *A, const double *k, const double *x, const double *lam,
                 double *g, int32_t ptr_kind);
 

@@ -18,7 +18,7 @@ def _built(path):
 
 @pytest.fixture(scope="session", autouse=True)
 def _native_libs():
-    """Build what is missing (never rebuild what exists: the GPU box uses prebuilt files)."""
+    """Build what is missing (never rebuild what exists: GPU runs use the prebuilt files)."""
     if not _built("chronoslab_amd/libchronos_setup.so"):
         from chronoslab_amd import build as B
 
