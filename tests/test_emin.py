@@ -153,6 +153,21 @@ def _values(Pm, ent):
     return np.array([Pd[i, j] for i, j in ent])
 
 
+def test_svd_rows_reproduce_least_squares():
+    # patt_pow = 1: the pattern is BAMG's P, at most ntv entries per row, so every row's block
+    # goes to the rank-revealing SVD branch; a full-row-rank block leaves no freedom and the
+    # result is the least-squares correction of P itself.
+    A, fc, nc, tv, P = _level(ne=3, seed=2)
+    prm = H.SetupParams(patt_pow=1)
+    T = H.emin_pattern(A, P, fc, tv.shape[1], prm).to_scipy()
+    assert np.diff(T.indptr).max() <= tv.shape[1]
+    ent, x0, xk, energy = _kkt_reference(A, P, T, fc, tv, prm.condmax_emin)
+    Pe, it = H.emin_improve(A, P, H.Csr.from_scipy(T), fc, tv, prm)
+    x = _values(Pe, ent)
+    np.testing.assert_allclose(x, x0, rtol=0, atol=1e-12 * np.abs(x0).max())
+    np.testing.assert_allclose(x, xk, rtol=0, atol=1e-9 * np.abs(xk).max())
+
+
 def test_minimiser_matches_dense_kkt():
     A, fc, nc, tv, P = _level(ne=3, seed=2)
     T = H.emin_pattern(A, P, fc, tv.shape[1]).to_scipy()
