@@ -164,7 +164,7 @@ def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) 
     Afix, _ = H.bc_fix(A, 1)  # computeAMG.m:5-10 (the clamped dofs are unit rows)
     t = time.perf_counter()
     hier = H.build(Afix, H.SetupParams(prolong="bamg", soc="dom", soc_tau=0.01, prol_emin="emin",
-                                       tv=H.rigid_body_modes(ne), gpu_afsai=True, gpu_omega=True,
+                                       tv=H.rigid_body_modes(ne), gpu_emin=True, gpu_afsai=True, gpu_omega=True,
                                        gpu_galerkin=True))
     t_setup = time.perf_counter() - t
     log(f"[bench] elasticity {ne}^3: n={A.nrows} nnz={A.nnz} setup {t_setup:.1f}s {hier.setup_seconds}\n{hier.describe()}")
@@ -209,8 +209,8 @@ def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) 
             "solve": res,
             "speedup_fp32_vs_fp64": (res["fp64"]["mean_s"] / res["fp32"]["mean_s"]) if len(res) == 2 else None,
             "cpu_oracle_s": t_cpu, "cpu_oracle_iterations": ito, "cpu_threads": cores,
-            "note": "BAMG + EMIN prolongation, aFSAI; host coarsening, interpolation and energy "
-                    "minimisation, GPU aFSAI, omega and Galerkin products"}
+            "note": "BAMG + EMIN prolongation, aFSAI; host coarsening, interpolation and EMIN "
+                    "constraints, GPU EMIN operator, aFSAI, omega and Galerkin products"}
 
 
 def main() -> int:

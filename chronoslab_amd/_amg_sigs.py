@@ -51,6 +51,9 @@ EXPORTS = {
     "chr_setup_omega": (C.c_int, [_VP, _I32, _I32P, _I32P, _F64P, _I32P, _I32P, _F64P, C.c_uint64, _F64P,
                                   _F64P]),
     "chr_galerkin": (C.c_int, [C.POINTER(ChrCsr), C.POINTER(ChrCsr), _I32P, _VP, _VP]),
+    "chr_setup_emin_open": (C.c_int, [_VP, _I32, _I32, _I32P, _I32P, _F64P, _I32P, _I32P, C.POINTER(_VP)]),
+    "chr_setup_emin_apply": (C.c_int, [_VP, _F64P, _F64P]),
+    "chr_setup_emin_close": (None, [_VP]),
     "chr_setup_galerkin": (C.c_int, [_VP, _I32, _I32P, _I32P, _F64P, _I32, _I32P, _I32P, _F64P, _I32P, _VP,
                                      _VP]),
     "chr_time_vcycle": (C.c_int, [_VP, _I32, _F64P]),

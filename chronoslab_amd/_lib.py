@@ -65,6 +65,8 @@ def load_setup_lib():
             _sig(lib, "chs_emin_pattern", C.c_int, [_VP, _VP, _I32P, _I32, _VP, C.c_int, C.POINTER(_VP)])
             _sig(lib, "chs_emin_improve", C.c_int, [_VP, _VP, _VP, _I32P, _I32, _F64P, _VP, C.c_int,
                                                     C.POINTER(_VP), C.POINTER(_I32)])
+            _sig(lib, "chs_emin_improve_ops", C.c_int, [_VP, _VP, _VP, _I32P, _I32, _F64P, _VP, _VP, C.c_int,
+                                                        C.POINTER(_VP), C.POINTER(_I32)])
             _sig(lib, "chs_pmis", C.c_int, [_VP, _I8P, _F64P, _I32P, C.POINTER(_I32)])
             _sig(lib, "chs_prol_classical", C.c_int, [_VP, _I8P, _I32P, _I32, C.c_int, C.POINTER(_VP)])
             _sig(lib, "chs_prol_exti", C.c_int, [_VP, _I8P, _I32P, _I32, C.c_int, C.POINTER(_VP)])

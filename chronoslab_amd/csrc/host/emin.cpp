@@ -329,7 +329,8 @@ int improve(int32_t n, int32_t nc, const int32_t *arp, const int32_t *aci, const
             const int32_t *prp, const int32_t *pci, const double *pv, const int32_t *trp,
             const int32_t *tci, const int32_t *fc, int32_t ntv, const double *tv,
             const Params &prm, int nthreads, std::vector<int32_t> &orp, std::vector<int32_t> &oci,
-            std::vector<double> &ov, int32_t &iter, std::string &err) {
+            std::vector<double> &ov, int32_t &iter, std::string &err,
+            const std::function<int(const double *, double *)> &kp_ext) {
   iter = 0;
   const int64_t nt = trp[n];
   std::vector<int32_t> c2g((size_t)nc, -1);
@@ -406,7 +407,8 @@ int improve(int32_t n, int32_t nc, const int32_t *arp, const int32_t *aci, const
       if (hit != e && *hit == i) f[k] = -av[arp[r] + (hit - b)];
     }
   // y = (A X) on the pattern, X on the pattern (KP_spmat.cpp).
-  auto kp = [&](const double *x, double *y) {
+  int kp_fail = 0;
+  auto kp_host = [&](const double *x, double *y) {
 #pragma omp parallel num_threads(nthreads)
     {
       std::vector<int32_t> wn((size_t)nc, -1);
@@ -428,6 +430,13 @@ int improve(int32_t n, int32_t nc, const int32_t *arp, const int32_t *aci, const
         }
         for (int32_t k = c0; k < c1; ++k) wn[tci[k]] = -1;
       }
+    }
+  };
+  auto kp = [&](const double *x, double *y) {
+    if (kp_ext) {
+      if (!kp_fail && kp_ext(x, y)) kp_fail = 1;
+    } else {
+      kp_host(x, y);
     }
   };
   // y = (I - Q Q^T) x row by row (Orth_Q.cpp).
@@ -487,6 +496,10 @@ int improve(int32_t n, int32_t nc, const int32_t *arp, const int32_t *aci, const
     }
     if (iter == 1) de0 = dek;
     done = iter == prm.itmax || dek / de0 < prm.en_tol;
+  }
+  if (kp_fail) {
+    err = "emin: the external masked product failed";
+    return 5;
   }
   for (int64_t k = 0; k < nt; ++k) p0[k] -= dp[k];
   // Coarse rows back as e_c (Prol_add_Cnodes.cpp); exact zeros dropped as MATLAB's sparse()

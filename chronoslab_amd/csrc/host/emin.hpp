@@ -1,6 +1,7 @@
 // Energy-minimising prolongation (host setup library, internal interface).
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -28,12 +29,14 @@ void pattern(int32_t n, int32_t nc, const int32_t *arp, const int32_t *aci, cons
 // Energy minimisation of P (n x nc, coarse rows identity) over the fine-row pattern
 // (trp, tci) under the test-space constraint of every fine row.  tv: n x ntv column-major.
 // Fills the CSR arrays of the improved P (coarse rows e_c, exact zeros dropped) and the
-// CG iteration count.  Returns 0, or 4 (a constraint factorisation failed) / 1 (bad input)
-// with err set.
+// CG iteration count.  kp_ext, when given, replaces the host's masked product y = (A X) on
+// the pattern (it must sum in the same order).  Returns 0, or 4 (a constraint factorisation
+// failed) / 1 (bad input) / 5 (kp_ext failed) with err set.
 int improve(int32_t n, int32_t nc, const int32_t *arp, const int32_t *aci, const double *av,
             const int32_t *prp, const int32_t *pci, const double *pv, const int32_t *trp,
             const int32_t *tci, const int32_t *fc, int32_t ntv, const double *tv,
             const Params &prm, int nthreads, std::vector<int32_t> &orp, std::vector<int32_t> &oci,
-            std::vector<double> &ov, int32_t &iter, std::string &err);
+            std::vector<double> &ov, int32_t &iter, std::string &err,
+            const std::function<int(const double *, double *)> &kp_ext = nullptr);
 
 }  // namespace chse

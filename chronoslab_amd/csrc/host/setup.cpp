@@ -123,6 +123,10 @@ extern "C" void chs_params_default(chs_params *p) {
   p->emin.itmax = 10;
   p->emin.energ_tol = 1e-3;
   p->emin.condmax = 1e10;
+  p->emin_kp.open = nullptr;
+  p->emin_kp.apply = nullptr;
+  p->emin_kp.close = nullptr;
+  p->emin_kp.ctx = nullptr;
 }
 
 extern "C" const char *chs_last_error(void) { return g_err.c_str(); }
@@ -670,14 +674,22 @@ extern "C" int chs_emin_pattern(const chs_mat *Ap, const chs_mat *Pp, const int3
   return CHS_OK;
 }
 
-extern "C" int chs_emin_improve(const chs_mat *Ap, const chs_mat *Pp, const chs_mat *patt,
-                                const int32_t *fcnode, int32_t ntv, const double *tv,
-                                const chs_emin_params *prm, int nthreads, chs_mat **Pout,
-                                int32_t *iter) {
+namespace {
+// EMIN on (A, P, pattern), with the CG operator from ops when ops->open accepts the level.
+int emin_improve(const chs_mat *Ap, const chs_mat *Pp, const chs_mat *patt, const int32_t *fcnode,
+                 int32_t ntv, const double *tv, const chs_emin_params *prm, const chs_emin_kp_ops *ops,
+                 int nthreads, chs_mat **Pout, int32_t *iter) {
   if (!Ap || !Pp || !patt || !fcnode || !tv || !prm || !Pout) return fail(CHS_EARG, "null");
   if (Pp->nrows != Ap->nrows || Ap->nrows != Ap->ncols || patt->nrows != Pp->nrows ||
       patt->ncols != Pp->ncols || ntv < 1)
     return fail(CHS_EARG, "emin: dimensions");
+  void *h = nullptr;
+  if (ops && ops->open && ops->apply &&
+      ops->open(ops->ctx, Ap->nrows, Pp->ncols, Ap->rp.data(), Ap->ci.data(), Ap->v.data(), patt->rp.data(),
+                patt->ci.data(), &h) != 0)
+    h = nullptr;  // declined: the host product
+  std::function<int(const double *, double *)> ext;
+  if (h) ext = [&](const double *x, double *y) { return ops->apply(h, x, y); };
   auto P = std::make_unique<chs_mat>();
   P->nrows = Pp->nrows;
   P->ncols = Pp->ncols;
@@ -686,12 +698,28 @@ extern "C" int chs_emin_improve(const chs_mat *Ap, const chs_mat *Pp, const chs_
   const int rc = chse::improve(Ap->nrows, Pp->ncols, Ap->rp.data(), Ap->ci.data(), Ap->v.data(),
                                Pp->rp.data(), Pp->ci.data(), Pp->v.data(), patt->rp.data(),
                                patt->ci.data(), fcnode, ntv, tv, emin_params(*prm),
-                               nthreads_or_default(nthreads), P->rp, P->ci, P->v, it, err);
+                               nthreads_or_default(nthreads), P->rp, P->ci, P->v, it, err, ext);
+  if (h && ops->close) ops->close(h);
   if (rc == 1) return fail(CHS_EARG, err);
   if (rc) return fail(CHS_ENUM, err);
   if (iter) *iter = it;
   *Pout = P.release();
   return CHS_OK;
+}
+}  // namespace
+
+extern "C" int chs_emin_improve(const chs_mat *Ap, const chs_mat *Pp, const chs_mat *patt,
+                                const int32_t *fcnode, int32_t ntv, const double *tv,
+                                const chs_emin_params *prm, int nthreads, chs_mat **Pout,
+                                int32_t *iter) {
+  return emin_improve(Ap, Pp, patt, fcnode, ntv, tv, prm, nullptr, nthreads, Pout, iter);
+}
+
+extern "C" int chs_emin_improve_ops(const chs_mat *Ap, const chs_mat *Pp, const chs_mat *patt,
+                                    const int32_t *fcnode, int32_t ntv, const double *tv,
+                                    const chs_emin_params *prm, const chs_emin_kp_ops *ops,
+                                    int nthreads, chs_mat **Pout, int32_t *iter) {
+  return emin_improve(Ap, Pp, patt, fcnode, ntv, tv, prm, ops, nthreads, Pout, iter);
 }
 
 extern "C" int chs_orthonormalize(int64_t n, int32_t k, double *V) {
@@ -1753,7 +1781,7 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
       chs_mat *T = nullptr, *Pe = nullptr;
       int32_t it = 0;
       rc = chs_emin_pattern(&A, P, fc.data(), p.ntv, &p.emin, nt, &T);
-      if (!rc) rc = chs_emin_improve(&A, P, T, fc.data(), p.ntv, tv_f.data(), &p.emin, nt, &Pe, &it);
+      if (!rc) rc = emin_improve(&A, P, T, fc.data(), p.ntv, tv_f.data(), &p.emin, &p.emin_kp, nt, &Pe, &it);
       delete T;
       delete P;
       if (rc) return rc;

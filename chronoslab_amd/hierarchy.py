@@ -266,11 +266,12 @@ def emin_improve(A: Csr, P: Csr, patt: Csr, fcnode: np.ndarray, tv: np.ndarray,
     fcnode = np.ascontiguousarray(fcnode, np.int32)
     tv = np.asfortranarray(tv, dtype=np.float64)
     ep = _cemin(p)
+    ops = _cemin_ops(p.gpu_emin)
     out = C.c_void_p()
     it = C.c_int32()
-    _check(_lib().chs_emin_improve(ha.ptr, hp.ptr, ht.ptr, fcnode.ctypes.data_as(_I32P), tv.shape[1],
-                                   tv.ctypes.data_as(_F64P), C.cast(C.pointer(ep), C.c_void_p), nthreads,
-                                   C.byref(out), C.byref(it)))
+    _check(_lib().chs_emin_improve_ops(ha.ptr, hp.ptr, ht.ptr, fcnode.ctypes.data_as(_I32P), tv.shape[1],
+                                       tv.ctypes.data_as(_F64P), C.cast(C.pointer(ep), C.c_void_p),
+                                       C.cast(C.pointer(ops), C.c_void_p), nthreads, C.byref(out), C.byref(it)))
     return _take(out), it.value
 
 
@@ -393,6 +394,9 @@ class SetupParams:
     itmax_emin: int = 10
     energ_tol: float = 1e-3
     condmax_emin: float = 1e10
+    # Run EMIN's CG operator (the masked product A P on the pattern) on the GPU
+    # (chr_setup_emin_open/apply/close); same bits as the host product.
+    gpu_emin: bool = False
     # Form the Galerkin products P^T (A P) on the GPU (chr_setup_galerkin); same bits.
     gpu_galerkin: bool = False
     # parm_COARSEN nl_agg: levels 1..nl_agg use aggressive coarsening (two-stage EXTI).
@@ -414,6 +418,20 @@ def _cemin(p: SetupParams) -> _CEmin:
     return _CEmin(p.patt_pow, p.patt_tau, p.nnzr_max, p.itmax_emin, p.energ_tol, p.condmax_emin)
 
 
+class _CEminOps(C.Structure):
+    _fields_ = [("open", C.c_void_p), ("apply", C.c_void_p), ("close", C.c_void_p), ("ctx", C.c_void_p)]
+
+
+def _cemin_ops(gpu: bool) -> _CEminOps:
+    if not gpu:
+        return _CEminOps(None, None, None, None)
+    from ._lib import load_amg_lib
+
+    g = load_amg_lib()
+    addr = lambda f: C.cast(f, C.c_void_p).value
+    return _CEminOps(addr(g.chr_setup_emin_open), addr(g.chr_setup_emin_apply), addr(g.chr_setup_emin_close), None)
+
+
 class _CParams(C.Structure):
     _fields_ = [("nlev_max", C.c_int32), ("max_coarse", C.c_int32), ("soc_tau", C.c_double),
                 ("nstep", C.c_int32), ("step_size", C.c_int32), ("fsai_eps", C.c_double),
@@ -422,7 +440,7 @@ class _CParams(C.Structure):
                 ("afsai_fn", C.c_void_p), ("afsai_ctx", C.c_void_p), ("omega_fn", C.c_void_p),
                 ("omega_ctx", C.c_void_p), ("soc_type", C.c_int32), ("ntv", C.c_int32), ("ntv0", C.c_int32),
                 ("tv", _F64P), ("bamg", _CBamg), ("prol_smooth", C.c_int32), ("galerkin_fn", C.c_void_p),
-                ("galerkin_ctx", C.c_void_p), ("nl_agg", C.c_int32), ("emin", _CEmin)]
+                ("galerkin_ctx", C.c_void_p), ("nl_agg", C.c_int32), ("emin", _CEmin), ("emin_kp", _CEminOps)]
 
 
 def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
@@ -448,7 +466,7 @@ def build(A: Csr, params: SetupParams | None = None) -> Hierarchy:
                   0 if tv is None else tv.shape[1], None if tv is None else tv.ctypes.data_as(_F64P),
                   _CBamg(p.itmax_vol, p.tol_vol, p.dist_min, p.dist_max, p.maxcond, p.maxrownrm, p.eps_prol),
                   {"none": 0, "smooth": 1, "emin": 2}[p.prol_emin.lower()], gfn, None, p.nl_agg,
-                  _cemin(p))
+                  _cemin(p), _cemin_ops(p.gpu_emin))
     lib = _lib()
     ha = _to_handle(A)
     hh = C.c_void_p()

@@ -185,6 +185,17 @@ int chr_setup_galerkin(void *ctx, int32_t n, const int32_t *rowptr_A, const int3
                        const int32_t *col_P, const double *val_P, int32_t *rowptr_C,
                        void *sink, chr_reserve_fn reserve);
 
+/* EMIN's CG operator for chs_build (the open / apply / close of chs_emin_kp_ops in
+ * chronos_setup.h): y = (A X) restricted to the fine rows' interpolation pattern (rowptr_T,
+ * col_T; columns ascending), each entry summed over A's row in order, bit for bit the host
+ * product (KP_spmat.cpp as restated in the host library).  open uploads A and the pattern;
+ * apply moves X (pattern order) in and y out; close frees the level.  No reference
+ * counterpart (the reference runs this product on the CPU). */
+int chr_setup_emin_open(void *ctx, int32_t n, int32_t nc, const int32_t *rowptr_A, const int32_t *col_A,
+                        const double *val_A, const int32_t *rowptr_T, const int32_t *col_T, void **handle);
+int chr_setup_emin_apply(void *handle, const double *x, double *y);
+void chr_setup_emin_close(void *handle);
+
 /* Adaptive symmetric FSAI factor of A computed on the GPU, every row's pattern growth and
  * dense solves in one batched launch (one wave per row).  Replaces the reference's native
  * setup kernel [nterm_G, iat_G, ja_G, coef_G, is_posdef] = SYM_aFSAI_compute(...)

@@ -85,6 +85,19 @@ typedef int (*chs_galerkin_fn)(void *ctx, int32_t n, const int32_t *rowptr_A,
                                const double *val_P, int32_t *rowptr_C, void *sink,
                                chs_reserve_fn reserve);
 
+/* Replacement of EMIN's CG operator inside chs_build (e.g. chr_setup_emin_open / _apply /
+ * _close of libchronos_amg): y = (A X) restricted to the pattern, X and y on the pattern
+ * (trp, tci: the fine rows' pattern, n rows), each entry summed over A's row in order like
+ * the host product.  open prepares a level (0, or nonzero to decline: the host then runs
+ * the product), apply runs one product, close releases the level. */
+typedef struct chs_emin_kp_ops {
+  int (*open)(void *ctx, int32_t n, int32_t nc, const int32_t *rowptr_A, const int32_t *col_A,
+              const double *val_A, const int32_t *rowptr_T, const int32_t *col_T, void **handle);
+  int (*apply)(void *handle, const double *x, double *y);
+  void (*close)(void *handle);
+  void *ctx;
+} chs_emin_kp_ops;
+
 /* Replacement of the smoother damping computation inside chs_build (e.g. chr_setup_omega
  * of libchronos_amg): omega = min(1, 1.9 / lambda_max(G A G^T)) for the level's A and G. */
 typedef int (*chs_omega_fn)(void *ctx, int32_t n, const int32_t *rowptr_A, const int32_t *col_A,
@@ -150,6 +163,7 @@ typedef struct chs_params {
   int32_t nl_agg;        /* parm_COARSEN nl_agg: levels 1..nl_agg use aggressive coarsening
                             (agg_coarsen.m) with the two-stage EXTI (or BAMG) prolongation */
   chs_emin_params emin;  /* prol_smooth = 2 */
+  chs_emin_kp_ops emin_kp; /* open = NULL: EMIN's CG operator on the host */
 } chs_params;
 
 void chs_params_default(chs_params *p);
@@ -235,6 +249,12 @@ int chs_emin_pattern(const chs_mat *A, const chs_mat *P, const int32_t *fcnode, 
 int chs_emin_improve(const chs_mat *A, const chs_mat *P, const chs_mat *patt,
                      const int32_t *fcnode, int32_t ntv, const double *tv,
                      const chs_emin_params *prm, int nthreads, chs_mat **Pout, int32_t *iter);
+/* chs_emin_improve with the CG operator from ops (see chs_emin_kp_ops; NULL or a declined
+ * open: the host product).  The same result bit for bit. */
+int chs_emin_improve_ops(const chs_mat *A, const chs_mat *P, const chs_mat *patt,
+                         const int32_t *fcnode, int32_t ntv, const double *tv,
+                         const chs_emin_params *prm, const chs_emin_kp_ops *ops, int nthreads,
+                         chs_mat **Pout, int32_t *iter);
 
 /* Dirichlet marking + sequential PMIS.  rand01[i] is the tie-break added to node i's
  * weight.  Output fcnode[i] = c >= 0 (coarse index) or -1 (fine); returns nc in *nc. */

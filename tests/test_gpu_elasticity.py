@@ -100,3 +100,25 @@ def test_gpu_setup_long_rows_declined_to_host(gpu):
         assert abs(a.omega - b.omega) <= 1e-6 * a.omega
     # the second level's rows (about a thousand entries) exceed the per-row kernel's work space
     assert np.diff(hh.levels[1].A.rowptr).max() > 600
+
+
+def test_gpu_emin_operator_same_bits(gpu):
+    """EMIN with its CG operator on the GPU gives the host's prolongation bit for bit, on one
+    level and through a whole BAMG + EMIN set-up."""
+    from test_emin import _level
+
+    A, fc, nc, tv, P = _level(ne=10, seed=3)
+    T = H.emin_pattern(A, P, fc, tv.shape[1])
+    Ph, ith = H.emin_improve(A, P, T, fc, tv, H.SetupParams())
+    Pg, itg = H.emin_improve(A, P, T, fc, tv, H.SetupParams(gpu_emin=True))
+    print(f"EMIN level n={A.nrows}: CG {ith} host / {itg} GPU", flush=True)
+    assert ith == itg
+    assert np.array_equal(Ph.rowptr, Pg.rowptr) and np.array_equal(Ph.col, Pg.col)
+    assert np.array_equal(Ph.val, Pg.val)
+    _, _, hh = _elasticity(16)
+    _, _, hg = _elasticity(16, gpu_emin=True)
+    assert hh.nlev == hg.nlev
+    for a, b in zip(hh.levels, hg.levels):
+        assert np.array_equal(a.A.val, b.A.val)
+        if a.P is not None:
+            assert np.array_equal(a.P.col, b.P.col) and np.array_equal(a.P.val, b.P.val)
