@@ -225,3 +225,84 @@ def test_emin_needs_the_bamg_test_space():
     A, _ = H.bc_fix(A, 1)
     with pytest.raises(Exception):
         H.build(A, H.SetupParams(prolong="exti", prol_emin="emin", nlev_max=2, max_coarse=50))
+
+
+def _ops_struct(open_fn, apply_fn, close_fn):
+    import ctypes as C
+
+    OPEN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                       C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_void_p))
+    APPLY = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+    CLOSE = C.CFUNCTYPE(None, C.c_void_p)
+    cbs = (OPEN(open_fn), APPLY(apply_fn), CLOSE(close_fn))
+    addr = lambda f: C.cast(f, C.c_void_p).value
+    return H._CEminOps(addr(cbs[0]), addr(cbs[1]), addr(cbs[2]), None), cbs
+
+
+def _improve_with_ops(A, P, T, fc, tv, ops):
+    import ctypes as C
+
+    ha, hp, ht = H._to_handle(A), H._to_handle(P), H._to_handle(T)
+    fc = np.ascontiguousarray(fc, np.int32)
+    tvf = np.asfortranarray(tv, dtype=np.float64)
+    ep = H._cemin(H.SetupParams())
+    out, it = C.c_void_p(), C.c_int32()
+    rc = H._lib().chs_emin_improve_ops(ha.ptr, hp.ptr, ht.ptr, fc.ctypes.data_as(H._I32P), tvf.shape[1],
+                                       tvf.ctypes.data_as(H._F64P), C.cast(C.pointer(ep), C.c_void_p),
+                                       C.cast(C.pointer(ops), C.c_void_p), 0, C.byref(out), C.byref(it))
+    return rc, (H._take(out) if rc == 0 else None), it.value
+
+
+def test_operator_hook():
+    """The CG-operator hook of chs_build (chs_emin_kp_ops): a declining open falls back to the
+    host product (same bits), an external product (here numpy's, same operator, other
+    summation order) gives the same P to rounding, and a failing apply is an error."""
+    import ctypes as C
+
+    A, fc, nc, tv, P = _level(ne=4, seed=5)
+    T = H.emin_pattern(A, P, fc, tv.shape[1])
+    Ph, ith = H.emin_improve(A, P, T, fc, tv)
+    calls = {"open": 0, "apply": 0, "close": 0}
+    state = {}
+
+    def open_decline(ctx, n, ncc, rpA, colA, valA, rpT, colT, handle):
+        calls["open"] += 1
+        return 1
+
+    def open_np(ctx, n, ncc, rpA, colA, valA, rpT, colT, handle):
+        calls["open"] += 1
+        nnz, nt = rpA[n], rpT[n]
+        Am = sp.csr_matrix((np.ctypeslib.as_array(valA, (nnz,)).copy(), np.ctypeslib.as_array(colA, (nnz,)).copy(),
+                            np.ctypeslib.as_array(rpA, (n + 1,)).copy()), shape=(n, n))
+        tr, tc = np.ctypeslib.as_array(rpT, (n + 1,)).copy(), np.ctypeslib.as_array(colT, (nt,)).copy()
+        state["A"], state["rows"], state["cols"], state["shape"] = Am, np.repeat(np.arange(n), np.diff(tr)), tc, (n, ncc)
+        handle[0] = C.c_void_p(1)
+        return 0
+
+    def apply_np(h, x, y):
+        calls["apply"] += 1
+        nt = len(state["cols"])
+        X = sp.csr_matrix((np.ctypeslib.as_array(x, (nt,)).copy(), (state["rows"], state["cols"])), shape=state["shape"])
+        Y = (state["A"] @ X).tocsr()
+        np.ctypeslib.as_array(y, (nt,))[:] = np.asarray(Y[state["rows"], state["cols"]]).ravel()
+        return 0
+
+    def apply_fail(h, x, y):
+        calls["apply"] += 1
+        return -1
+
+    def close(h):
+        calls["close"] += 1
+
+    ops, keep = _ops_struct(open_decline, apply_np, close)
+    rc, Pd, itd = _improve_with_ops(A, P, T, fc, tv, ops)
+    assert rc == 0 and itd == ith and calls["open"] == 1 and calls["apply"] == 0 and calls["close"] == 0
+    assert np.array_equal(Pd.rowptr, Ph.rowptr) and np.array_equal(Pd.col, Ph.col) and np.array_equal(Pd.val, Ph.val)
+    ops, keep = _ops_struct(open_np, apply_np, close)
+    rc, Pn, itn = _improve_with_ops(A, P, T, fc, tv, ops)
+    assert rc == 0 and itn == ith and calls["apply"] == ith + 1 and calls["close"] == 1
+    assert np.array_equal(Pn.col, Ph.col)
+    np.testing.assert_allclose(Pn.val, Ph.val, rtol=0, atol=1e-12 * np.abs(Ph.val).max())
+    ops, keep = _ops_struct(open_np, apply_fail, close)
+    rc, _, _ = _improve_with_ops(A, P, T, fc, tv, ops)
+    assert rc != 0 and calls["close"] == 2
