@@ -227,8 +227,15 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
       }
     }
   }
-  // Step 3: sliced ELL.
-  if (!rc && sell_mode != 0 && !(pack_mode == 1 && M.use_pack)) {
+  // Step 3: sliced ELL.  Matrices timed warm (below the cache-cold size) are candidates only
+  // when every row fits one 16-element chunk: longer rows chain dependent chunk loads, which
+  // time well from a warm L2 but stall inside the cycle, where the operands come from
+  // further away (a 5,889-row level of about 70 entries per row took 11-14 us per product
+  // in sliced ELL there against 5 us in the stream layout).
+  int32_t maxrow = 0;
+  for (int32_t i = 0; i < M.nrows; ++i) maxrow = std::max(maxrow, hrp[i + 1] - hrp[i]);
+  const bool sell_ok = cold || maxrow <= 16 || sell_mode == 1;
+  if (!rc && sell_mode != 0 && sell_ok && !(pack_mode == 1 && M.use_pack)) {
     // Unsorted first; when its padding is within 12% of the nonzeros the sorted variants
     // (fewer padded bytes, scattered rows) are not tried.  The builder's choice last.
     std::vector<int> sorts;
