@@ -74,7 +74,7 @@ def load_setup_lib():
             _sig(lib, "chs_fsai_omega", C.c_int, [_VP, _VP, _U64, C.c_int, C.POINTER(_F64), C.POINTER(_F64)])
             _sig(lib, "chs_transpose", C.c_int, [_VP, C.POINTER(_VP)])
             _sig(lib, "chs_galerkin", C.c_int, [_VP, _VP, C.c_int, C.POINTER(_VP)])
-            _sig(lib, "chs_bc_fix", C.c_int, [_VP, _U64, C.POINTER(_VP), C.POINTER(_I32)])
+            _sig(lib, "chs_bc_fix", C.c_int, [_VP, _U64, _I32, C.c_int, C.POINTER(_VP), C.POINTER(_I32)])
             _sig(lib, "chs_build", C.c_int, [_VP, _VP, C.POINTER(_VP)])
             _sig(lib, "chs_sell_apply", C.c_int,
                  [_I32, _I32, _I32P, _I32P, _F64P, _I32, _F64P, _F64P, _I32P, C.POINTER(_I64)])

@@ -30,6 +30,47 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-march=x86-64-v2", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-Wall"]
 
 
+def source_digest(paths: list[str]) -> str:
+    """SHA-256 over the named files' contents (in the given order)."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for p in paths:
+        h.update(os.path.basename(p).encode())
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def setup_sources() -> list[str]:
+    return sorted(glob.glob(os.path.join(HERE, "csrc", "host", "*.cpp")) +
+                  glob.glob(os.path.join(HERE, "csrc", "host", "*.hpp")) +
+                  glob.glob(os.path.join(HERE, "csrc", "common", "*.hpp")) +
+                  [os.path.join(INC, "chronos_setup.h")])
+
+
+def amg_sources() -> list[str]:
+    return sorted(glob.glob(os.path.join(HERE, "csrc", "gpu", "*.hip")) +
+                  glob.glob(os.path.join(HERE, "csrc", "gpu", "*.hpp")) +
+                  glob.glob(os.path.join(HERE, "csrc", "common", "*.hpp")) +
+                  [os.path.join(INC, "chronos_amg.h")])
+
+
+def is_current(lib: str, sources: list[str]) -> bool:
+    """True when lib exists and was built from the sources as they are now (the digest
+    written beside it by the build matches)."""
+    try:
+        with open(lib + ".srchash") as f:
+            return os.path.exists(lib) and f.read().strip() == source_digest(sources)
+    except OSError:
+        return False
+
+
+def _stamp(lib: str, sources: list[str]) -> None:
+    with open(lib + ".srchash", "w") as f:
+        f.write(source_digest(sources) + "\n")
+
+
 def _run(cmd: list[str]) -> None:
     print("+", " ".join(cmd), flush=True)
     p = subprocess.run(cmd, capture_output=True, text=True)
@@ -45,15 +86,27 @@ def build_setup() -> str:
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "host", "*.cpp")))
     _run(["g++", *HOST_FLAGS, f"-I{INC}", f"-I{os.path.join(HERE, 'csrc', 'common')}", "-o", out + ".tmp", *srcs])
     os.replace(out + ".tmp", out)
+    _stamp(out, setup_sources())
     return out
 
 
 def build_amg() -> str:
+    """Each .hip source is compiled to an object of its own (in parallel), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+
     out = os.path.join(HERE, "libchronos_amg.so")
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "gpu", "*.hip")))
-    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{os.path.join(HERE, 'csrc', 'gpu')}",
-          f"-I{os.path.join(HERE, 'csrc', 'common')}", "-o", out + ".tmp", *srcs])
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    incs = [f"-I{INC}", f"-I{os.path.join(HERE, 'csrc', 'gpu')}", f"-I{os.path.join(HERE, 'csrc', 'common')}"]
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    jobs = max(1, min(len(srcs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([HIPCC, *cflags, *incs, "-c", "-o", so[1], so[0]]), zip(srcs, objs)))
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp", *objs])
     os.replace(out + ".tmp", out)
+    _stamp(out, amg_sources())
     return out
 
 

@@ -336,6 +336,7 @@ int ensure_block_capacity(chr_hier &h, int k) {
   if (k <= h.kcap) return CHR_OK;
   HIPC(hipStreamSynchronize(h.stream));
   if (h.pcg) h.pcg->drop_graphs();  // they point at the buffers about to be replaced
+  ++h.buf_gen;
   const size_t s = h.fp32() ? 4 : 8;
   int rc;
   for (size_t l = 0; l < h.lev.size(); ++l) {
@@ -510,17 +511,51 @@ namespace chrrt {
 
 // Tail collapse.  The V-cycle from a small level c down (zero initial guess, smoothing,
 // transfers, the coarsest solve) is a fixed linear map b_c -> x_c.  When n_c is at most
-// CHRONOS_TAIL_ROWS (default 2048; 0 disables), that map is formed once, column by column,
-// by running the cycle's own kernels from level c on the unit vectors, and level c becomes
-// the last level with that matrix applied as its dense FP64 direct solve.  The dozen small
-// kernels of the levels below c (each bound by dependent memory latency) become one GEMV;
-// the result equals the uncollapsed cycle's up to the rounding of one matrix-vector
-// product.  Single-system hierarchies only (batched hierarchies keep every level).
-// b = e_j (length m).
+// CHRONOS_TAIL_ROWS (default 2048; 0 disables), that map is formed once by running the
+// cycle's own kernels from level c on the unit vectors, and level c becomes the last level
+// with that matrix applied as its dense FP64 direct solve.  The dozen small kernels of the
+// levels below c (each bound by dependent memory latency) become one GEMV; the result equals
+// the uncollapsed cycle's up to the rounding of one matrix-vector product.  Single-system
+// hierarchies only (batched hierarchies keep every level).
+//
+// The unit vectors go through the block V-cycle, kTailCols columns per pass (each column of
+// a block cycle equals the single-vector cycle of that column bit for bit), with the stream
+// synchronised and the pass's columns copied out before the next pass is enqueued, so at
+// most one pass (a few dozen kernels) is ever queued.
+constexpr int kTailCols = 16;
+
+// B = the unit vectors e_{j0}, ..., e_{j0+k-1} as a row-major m x k block.
 template <typename T>
-__global__ __launch_bounds__(chr::kBlock) void unit_vector_kernel(int m, int j, T *__restrict__ b) {
-  const int i = blockIdx.x * chr::kBlock + threadIdx.x;
-  if (i < m) b[i] = i == j ? T(1) : T(0);
+__global__ __launch_bounds__(chr::kBlock) void unit_block_kernel(int m, int k, int j0, T *__restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * chr::kBlock + threadIdx.x;
+  if (t >= (int64_t)m * k) return;
+  const int i = (int)(t / k), c = (int)(t % k);
+  b[t] = i == j0 + c ? T(1) : T(0);
+}
+
+template <typename T>
+int tail_columns(chr_hier &h, int32_t c, std::vector<double> &M) {
+  const int32_t m = h.lev[c].n;
+  const int *stop = h.zero.as<int>();
+  DevBuf out;
+  int rc;
+  if ((rc = out.alloc(sizeof(T) * (size_t)m * kTailCols))) return rc;
+  std::vector<T> host((size_t)m * kTailCols);
+  for (int32_t j0 = 0; j0 < m; j0 += kTailCols) {
+    const int k = std::min<int32_t>(kTailCols, m - j0);
+    const unsigned grid = (unsigned)(((int64_t)m * k + chr::kBlock - 1) / chr::kBlock);
+    hipLaunchKernelGGL(unit_block_kernel<T>, dim3(grid), dim3(chr::kBlock), 0, h.stream, m, k, j0,
+                       h.lev[c].b.as<T>());
+    HIPC(hipGetLastError());
+    Cycle<T> cy{h, stop, h.stream, k};
+    HIPC((cy.template level<T, T>(c, h.lev[c].b.as<T>(), out.as<T>())));
+    HIPC(hipMemcpyAsync(host.data(), out.p, sizeof(T) * (size_t)m * k, hipMemcpyDeviceToHost, h.stream));
+    HIPC(hipStreamSynchronize(h.stream));
+    // Row i of the block is (M e_{j0}, ..., M e_{j0+k-1})_i; M is stored row-major.
+    for (int32_t i = 0; i < m; ++i)
+      for (int q = 0; q < k; ++q) M[(size_t)i * m + j0 + q] = (double)host[(size_t)i * k + q];
+  }
+  return CHR_OK;
 }
 
 int collapse_tail(chr_hier &h) {
@@ -537,40 +572,19 @@ int collapse_tail(chr_hier &h) {
   auto &L = h.lev[c];
   const int32_t m = L.n;
   const size_t sz = h.fp32() ? 4 : 8;
-  DevBuf cols;
   int rc;
-  if ((rc = cols.alloc(sz * (size_t)m * m))) return rc;
-  const int *stop = h.zero.as<int>();
-  const unsigned ug = (unsigned)((m + chr::kBlock - 1) / chr::kBlock);
-  for (int32_t j = 0; j < m; ++j) {
-    if (h.fp32()) {
-      hipLaunchKernelGGL(unit_vector_kernel<float>, dim3(ug), dim3(chr::kBlock), 0, h.stream, m, j,
-                         L.b.as<float>());
-      HIPC(hipGetLastError());
-      Cycle<float> cy{h, stop, h.stream, 1};
-      HIPC((cy.level<float, float>(c, L.b.as<float>(), cols.as<float>() + (size_t)j * m)));
-    } else {
-      hipLaunchKernelGGL(unit_vector_kernel<double>, dim3(ug), dim3(chr::kBlock), 0, h.stream, m, j,
-                         L.b.as<double>());
-      HIPC(hipGetLastError());
-      Cycle<double> cy{h, stop, h.stream, 1};
-      HIPC((cy.level<double, double>(c, L.b.as<double>(), cols.as<double>() + (size_t)j * m)));
-    }
-  }
+  // Cycle vectors of levels c and below widened to kTailCols columns for the passes (the
+  // levels below c are dropped afterwards; level c keeps the wider buffers).
   HIPC(hipStreamSynchronize(h.stream));
-  // cols holds column j = M e_j contiguously; the direct solve wants row-major M.
+  if (h.kcap < kTailCols)
+    for (int32_t l = c; l < nlev; ++l) {
+      auto &Ll = h.lev[l];
+      const size_t vb = sz * (size_t)Ll.n * kTailCols;
+      if ((rc = Ll.x.alloc(vb)) || (rc = Ll.r.alloc(vb)) || (rc = Ll.y.alloc(vb)) || (rc = Ll.b.alloc(vb)))
+        return rc;
+    }
   std::vector<double> M((size_t)m * m);
-  if (h.fp32()) {
-    std::vector<float> t((size_t)m * m);
-    HIPC(hipMemcpy(t.data(), cols.p, sizeof(float) * t.size(), hipMemcpyDeviceToHost));
-    for (int32_t j = 0; j < m; ++j)
-      for (int32_t i = 0; i < m; ++i) M[(size_t)i * m + j] = (double)t[(size_t)j * m + i];
-  } else {
-    std::vector<double> t((size_t)m * m);
-    HIPC(hipMemcpy(t.data(), cols.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
-    for (int32_t j = 0; j < m; ++j)
-      for (int32_t i = 0; i < m; ++i) M[(size_t)i * m + j] = t[(size_t)j * m + i];
-  }
+  if ((rc = h.fp32() ? tail_columns<float>(h, c, M) : tail_columns<double>(h, c, M))) return rc;
   // Level c becomes the last level: all its rows are direct, with M as their operator.
   std::vector<int4> cmap((size_t)m, make_int4(0, m, 0, 0));
   if ((rc = L.cmap.alloc(sizeof(int4) * m)) || (rc = h.Ainv.alloc(sizeof(double) * M.size()))) return rc;

@@ -331,6 +331,7 @@ struct chr_batch {
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   int32_t graph_k = 0;
+  uint64_t graph_gen = 0;  // h->buf_gen the graphs were captured with
   void drop_graphs() {
     for (auto &g : exec)
       if (g) {
@@ -525,6 +526,7 @@ int batch_prepare(chr_batch &bt, int k, int itmax) {
     for (auto &e : bt.ev)
       if (!e) HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     bt.kcap = k;
+    bt.itcap = 0;  // the residual histories hold (itmax + 1) * nsys * kcap entries: regrow them
   }
   if (bt.itcap < itmax + 1) {
     bt.drop_graphs();
@@ -669,8 +671,11 @@ extern "C" int chr_batch_pcg(chr_batch *bt, int32_t k, const double *b, double *
                      (const int *)bt->cstart.as<int>(), (const double *)bt->part.as<double>(), bt->a,
                      bt->state.as<BState>(), bt->resv.as<double>());
   HIPC(hipGetLastError());
-  // Graphs of one batch iteration (p buffers alternate).
-  if (bt->graph_k != k || !bt->exec[0] || !bt->exec[1]) {
+  if (bt->resv.bytes < sizeof(double) * (size_t)(itmax + 1) * Q || bt->itcap < itmax + 1 || bt->kcap < k)
+    return fail(CHR_EHIP, "chr_batch_pcg: work buffers smaller than the solve (internal error)");
+  // Graphs of one batch iteration (p buffers alternate); they embed the cycle vectors'
+  // addresses, so a reallocation of those (a wider chr_batch_vcycle call) invalidates them.
+  if (bt->graph_k != k || bt->graph_gen != h.buf_gen || !bt->exec[0] || !bt->exec[1]) {
     bt->drop_graphs();
     for (int parity = 0; parity < 2; ++parity) {
       hipGraph_t g = nullptr;
@@ -686,6 +691,7 @@ extern "C" int chr_batch_pcg(chr_batch *bt, int32_t k, const double *b, double *
       if (e != hipSuccess) return fail(CHR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
     bt->graph_k = k;
+    bt->graph_gen = h.buf_gen;
   }
   HIPC(hipMemcpyAsync(&bt->pinned[1], bt->state.p, sizeof(BState), hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));

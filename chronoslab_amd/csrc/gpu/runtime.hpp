@@ -470,6 +470,9 @@ struct chr_hier {
   int32_t nc = 0;  // rows solved directly, over all levels
   DevBuf Ainv;     // packed FP64 dense inverses of the direct blocks (row-major each)
   int32_t kcap = 1;   // columns the cycle vectors are allocated for
+  // Incremented whenever the cycle vectors are reallocated: graphs captured by other objects
+  // (the batch's) embed their addresses and must be rebuilt when it changes.
+  uint64_t buf_gen = 0;
   DevBuf io_b, io_x;  // FP64 staging for host-pointer and block calls (n_0 x kcap)
   chr_mat outer;     // level-0 operator in FP64 (default outer operator of PCG)
   DevBuf zero;       // an int 0: the stop flag of V-cycles outside a PCG solve

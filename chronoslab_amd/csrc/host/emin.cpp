@@ -471,7 +471,7 @@ int improve(int32_t n, int32_t nc, const int32_t *arp, const int32_t *aci, const
   for (int64_t k = 0; k < nt; ++k) v[k] -= f[k];
   orth(v.data(), res.data());
   const double res0 = std::sqrt(det_dot(nt, res.data(), res.data(), nthreads));
-  bool done = prm.itmax <= 0 || res0 < 1e-15;
+  bool done = kp_fail || prm.itmax <= 0 || res0 < 1e-15;
   double gamma_old = 0.0, de0 = 0.0;
   while (!done) {
     ++iter;
@@ -486,6 +486,7 @@ int improve(int32_t n, int32_t nc, const int32_t *arp, const int32_t *aci, const
     }
     gamma_old = gamma;
     kp(pdir.data(), v.data());
+    if (kp_fail) break;  // the external product failed: the result is discarded below
     orth(v.data(), qkp.data());
     const double denom = det_dot(nt, qkp.data(), pdir.data(), nthreads);
     if (std::fabs(denom) < 1e-18) break;

@@ -469,8 +469,8 @@ static void transpose_into(const chs_mat &M, chs_mat &T) {
     }
 }
 
-static void spmv(const chs_mat &M, const double *x, double *y) {
-#pragma omp parallel for schedule(static)
+static void spmv(const chs_mat &M, const double *x, double *y, int nthreads = 0) {
+#pragma omp parallel for schedule(static) num_threads(nthreads_or_default(nthreads))
   for (int32_t i = 0; i < M.nrows; ++i) {
     double s = 0.0;
     for (int32_t e = M.rp[i]; e < M.rp[i + 1]; ++e) s += M.v[e] * x[M.ci[e]];
@@ -1379,7 +1379,8 @@ double det_dot(int32_t n, const double *x, const double *y, std::vector<double> 
 }
 
 template <class Op>
-double lanczos_max(int32_t n, Op op, uint64_t seed, int maxit, double tol) {
+double lanczos_max(int32_t n, Op op, uint64_t seed, int maxit, double tol, int nthreads) {
+  nthreads = nthreads_or_default(nthreads);
   maxit = std::max(2, std::min<int>(maxit, n));
   std::vector<std::vector<double>> V;
   std::vector<double> a, b, v(n), w(n), part;
@@ -1398,7 +1399,7 @@ double lanczos_max(int32_t n, Op op, uint64_t seed, int maxit, double tol) {
     for (int pass = 0; pass < 2; ++pass)
       for (auto &u : V) {
         const double d = det_dot(n, w.data(), u.data(), part);
-#pragma omp parallel for
+#pragma omp parallel for num_threads(nthreads)
         for (int32_t i = 0; i < n; ++i) w[i] -= d * u[i];
       }
     const double beta = std::sqrt(det_dot(n, w.data(), w.data(), part));
@@ -1407,7 +1408,7 @@ double lanczos_max(int32_t n, Op op, uint64_t seed, int maxit, double tol) {
     lam_old = lam;
     if (beta <= 1e-14 * std::fabs(lam)) break;
     b.push_back(beta);
-#pragma omp parallel for
+#pragma omp parallel for num_threads(nthreads)
     for (int32_t i = 0; i < n; ++i) v[i] = w[i] / beta;
   }
   return lam;
@@ -1424,28 +1425,37 @@ extern "C" int chs_fsai_omega(const chs_mat *Ap, const chs_mat *Gp, uint64_t see
   const int32_t n = Ap->nrows;
   std::vector<double> t1(n), t2(n);
   auto op = [&](const double *x, double *y) {
-    spmv(GT, x, t1.data());
-    spmv(*Ap, t1.data(), t2.data());
-    spmv(*Gp, t2.data(), y);
+    spmv(GT, x, t1.data(), nthreads);
+    spmv(*Ap, t1.data(), t2.data(), nthreads);
+    spmv(*Gp, t2.data(), y, nthreads);
   };
-  double lam = lanczos_max(n, op, seed, 40, 1e-7);
+  double lam = lanczos_max(n, op, seed, 40, 1e-7, nthreads);
   if (lambda) *lambda = lam;
   if (omega) *omega = std::min(1.0, 1.9 / lam);
   return CHS_OK;
 }
 
-extern "C" int chs_bc_fix(const chs_mat *Ap, uint64_t seed, chs_mat **Afix, int32_t *nfixed) {
+// Dirichlet rows of the input (rows assembled as unit rows) are rescaled to the spectrum of
+// A: lambda_max(A) by Lanczos (the reference calls eigs(A,1,'lm') with tolerance 1e-3), then
+//   mode CHS_BC_DIAG: diagonal entries equal to 1 become lambda_max/10 (computeAMG.m:5-10);
+//   mode CHS_BC_ALL:  every stored entry equal to 1 becomes lambda_max/10
+//                     (sources/Drivers/driver_AMG/aspAMG_main.m:89-90, A(A==1) = lmax/10).
+// The Lanczos dot products use fixed blocks, so the result is the same bits for any thread
+// count.
+extern "C" int chs_bc_fix(const chs_mat *Ap, uint64_t seed, int32_t mode, int nthreads, chs_mat **Afix,
+                          int32_t *nfixed) {
   if (!Ap || !Afix) return fail(CHS_EARG, "null");
+  if (mode != CHS_BC_DIAG && mode != CHS_BC_ALL) return fail(CHS_EARG, "chs_bc_fix: unknown mode");
   auto *m = new (std::nothrow) chs_mat(*Ap);
   if (!m) return fail(CHS_EALLOC, "alloc");
   int32_t cnt = 0;
   std::vector<int32_t> where;
   for (int32_t i = 0; i < m->nrows; ++i)
     for (int32_t e = m->rp[i]; e < m->rp[i + 1]; ++e)
-      if (m->ci[e] == i && m->v[e] == 1.0) where.push_back(e);
+      if ((mode == CHS_BC_ALL || m->ci[e] == i) && m->v[e] == 1.0) where.push_back(e);
   if (!where.empty()) {
-    auto op = [&](const double *x, double *y) { spmv(*Ap, x, y); };
-    double lmax = lanczos_max(m->nrows, op, seed, 40, 1e-6);
+    auto op = [&](const double *x, double *y) { spmv(*Ap, x, y, nthreads); };
+    double lmax = lanczos_max(m->nrows, op, seed, 40, 1e-6, nthreads);
     for (int32_t e : where) m->v[e] = lmax / 10.0;
     cnt = (int32_t)where.size();
   }

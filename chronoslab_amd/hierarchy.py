@@ -342,13 +342,59 @@ def transpose(M: Csr) -> Csr:
     return _take(out)
 
 
-def bc_fix(A: Csr, seed: int = 1) -> tuple[Csr, int]:
-    """computeAMG.m:5-10: unit diagonal entries become lambda_max(A)/10."""
+BC_DIAG = 0
+BC_ALL = 1
+
+
+def bc_fix(A: Csr, seed: int = 1, mode: str = "diag", nthreads: int = 0) -> tuple[Csr, int]:
+    """Dirichlet-row rescaling by lambda_max(A) (Lanczos); returns (A_fixed, entries changed).
+
+    mode "diag": diagonal entries equal to 1 become lambda_max/10 (sources/utils/computeAMG.m:5-10);
+    mode "all": every stored entry equal to 1 does, as the driver's A(A==1) = lmax/10
+    (sources/Drivers/driver_AMG/aspAMG_main.m:89-90)."""
+    m = {"diag": BC_DIAG, "all": BC_ALL}[mode]
     h = _to_handle(A)
     out = C.c_void_p()
     nf = C.c_int32()
-    _check(_lib().chs_bc_fix(h.ptr, C.c_uint64(seed), C.byref(out), C.byref(nf)))
+    _check(_lib().chs_bc_fix(h.ptr, C.c_uint64(seed), m, nthreads, C.byref(out), C.byref(nf)))
     return _take(out), nf.value
+
+
+RHS_MODES = ("unit_sol", "unit_rhs", "rand_sol", "rand_rhs", "rhs_in")
+
+
+def rhs_build(A: Csr, mode: str, seed: int = 1, rhs: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray | None]:
+    """The driver's right-hand-side modes (sources/Drivers/driver_AMG/aspAMG_main.m:119-134);
+    returns (rhs, exact solution or None).
+
+      unit_sol  rhs = A 1                          (solution 1)
+      unit_rhs  rhs = 1
+      rand_sol  rhs = A u, u ~ U[0,1)               (solution u)
+      rand_rhs  rhs_i = u_i 10^(6 (v_i - 0.5)), u, v ~ U[0,1)
+      rhs_in    rhs as given (an error when it is missing)
+
+    MATLAB's rand stream is replaced by the shared counter RNG: u is stream 30 and v stream 31
+    of `seed`.  The matrix is the one the solver sees (after the Dirichlet rescaling)."""
+    mode = mode.lower()
+    n = A.nrows
+    if mode == "unit_sol":
+        x = np.ones(n)
+        return A.matvec(x), x
+    if mode == "unit_rhs":
+        return np.ones(n), None
+    if mode == "rand_sol":
+        x = rng_uniform(seed, 30, n)
+        return A.matvec(x), x
+    if mode == "rand_rhs":
+        return rng_uniform(seed, 30, n) * 10.0 ** (6.0 * (rng_uniform(seed, 31, n) - 0.5)), None
+    if mode == "rhs_in":
+        if rhs is None:
+            raise ValueError("Missing the expected right-hand side")
+        r = np.asarray(rhs, np.float64)
+        if r.shape[0] != n:
+            raise ValueError("right-hand side length differs from the matrix size")
+        return r, None
+    raise ValueError(f"Wrong value for rhs_build: {mode}")
 
 
 @dataclasses.dataclass

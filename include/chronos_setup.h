@@ -36,7 +36,8 @@
  *                             compute_local_fsai.cpp:6-116, cpt_afsai_coef.cpp:316-554
  *   chs_fsai_omega            sources/Preconditioner/AMG/smoother/smoother.m:72-81
  *   chs_galerkin              sources/Preconditioner/AMG/aspAMG/cpt_aspAMG.m:277
- *   chs_bc_fix                sources/utils/computeAMG.m:5-10
+ *   chs_bc_fix                sources/utils/computeAMG.m:5-10,
+ *                             sources/Drivers/driver_AMG/aspAMG_main.m:89-90
  *   chs_gen_rbm               sources/utils/mk_rbm_3d.m:1-27
  */
 #ifndef CHRONOS_SETUP_H
@@ -275,8 +276,14 @@ int chs_fsai_omega(const chs_mat *A, const chs_mat *G, uint64_t seed, int nthrea
 /* Transpose and Galerkin product P^T (A P) (exact zeros dropped). */
 int chs_transpose(const chs_mat *M, chs_mat **MT);
 int chs_galerkin(const chs_mat *A, const chs_mat *P, int nthreads, chs_mat **Ac);
-/* computeAMG.m:5-10 rule: diagonal entries equal to 1 become lambda_max(A)/10. */
-int chs_bc_fix(const chs_mat *A, uint64_t seed, chs_mat **Afix, int32_t *nfixed);
+/* Dirichlet-row rescaling with lambda_max(A) (Lanczos, start vector from seed):
+ * CHS_BC_DIAG: diagonal entries equal to 1 become lambda_max/10 (sources/utils/computeAMG.m:5-10);
+ * CHS_BC_ALL: every stored entry equal to 1 becomes lambda_max/10
+ * (sources/Drivers/driver_AMG/aspAMG_main.m:89-90).  nfixed = entries changed.  nthreads <= 0:
+ * the OpenMP default; the result does not depend on it. */
+enum { CHS_BC_DIAG = 0, CHS_BC_ALL = 1 };
+int chs_bc_fix(const chs_mat *A, uint64_t seed, int32_t mode, int nthreads, chs_mat **Afix,
+               int32_t *nfixed);
 
 /* ---- the full hierarchy ------------------------------------------------------------- */
 /* Builds levels l = 0..nlev-1.  Level l < nlev-1 holds A_l, P_l, G_l, omega_l; the last

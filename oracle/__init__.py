@@ -37,8 +37,35 @@ _ref = None
 _ref_exti = None
 
 
+def sources() -> list:
+    return [os.path.join(HERE, f) for f in ("oracle.c", "oracle.h", "Makefile")]
+
+
+def is_current() -> bool:
+    """liboracle.so exists and was built from oracle.c / oracle.h as they are now."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for p in sources():
+        with open(p, "rb") as f:
+            h.update(f.read())
+    try:
+        with open(LIB + ".srchash") as f:
+            return os.path.exists(LIB) and f.read().strip() == h.hexdigest()
+    except OSError:
+        return False
+
+
 def build(reference: bool = True) -> None:
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    import hashlib
+
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle.so"], check=True)
+    h = hashlib.sha256()
+    for p in sources():
+        with open(p, "rb") as f:
+            h.update(f.read())
+    with open(LIB + ".srchash", "w") as f:
+        f.write(h.hexdigest() + "\n")
     if reference and os.path.isdir("/root/reference"):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 

@@ -100,3 +100,32 @@ def test_batch_config_48_systems_8_rhs(gpu):
           flush=True)
     assert np.all(flags == 0)
     assert worst <= 1e-8
+
+
+def test_batch_wider_block_after_narrow_solve(gpu):
+    """A batch solved with k = 1 and then with k = 8 at the same itmax: the residual
+    histories are regrown with the block width, and every solve equals a fresh batch's."""
+    mats, hiers = _systems([12, 15, 10], 1.0, 90)
+    bs = gpu.BatchSolver(hiers, "fp64")
+    B8 = _stack(bs.sizes, 8, 5)
+    bs.pcg(np.asfortranarray(B8[:, :1]), itmax=60, tol=1e-8)
+    X, iters, flags, resv = bs.pcg(B8, itmax=60, tol=1e-8)
+    X2, iters2, flags2, resv2 = gpu.BatchSolver(hiers, "fp64").pcg(B8, itmax=60, tol=1e-8)
+    np.testing.assert_array_equal(iters, iters2)
+    np.testing.assert_array_equal(flags, flags2)
+    np.testing.assert_array_equal(resv, resv2)
+    np.testing.assert_array_equal(X, X2)
+
+
+def test_batch_graphs_rebuilt_after_wider_vcycle(gpu):
+    """pcg(k=2), then a V-cycle with k=8 (which reallocates the cycle vectors the PCG graphs
+    were captured with), then pcg(k=2) again: the second solve equals the first."""
+    mats, hiers = _systems([12, 15], 1.0, 95)
+    bs = gpu.BatchSolver(hiers, "fp64")
+    B2 = _stack(bs.sizes, 2, 6)
+    X1, it1, fl1, rv1 = bs.pcg(B2, itmax=80, tol=1e-8)
+    bs.vcycle(_stack(bs.sizes, 8, 7))
+    X2, it2, fl2, rv2 = bs.pcg(B2, itmax=80, tol=1e-8)
+    np.testing.assert_array_equal(it1, it2)
+    np.testing.assert_array_equal(X1, X2)
+    np.testing.assert_array_equal(rv1, rv2)
