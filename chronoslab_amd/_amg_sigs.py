@@ -36,6 +36,7 @@ EXPORTS = {
     "chr_mat_destroy": (C.c_int, [_VP]),
     "chr_spmv": (C.c_int, [_VP, _VP, _VP, _I32]),
     "chr_pcg_solve": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _F64, _I32, _I32P, _F64P, _I32P]),
+    "chr_pcg_solve_noalloc": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _F64, _I32P, _F64P, _I32P]),
     "chr_lobpcg": (C.c_int, [_VP, _VP, _I32, _I32, _VP, _I64, _I32, _I32, _I32, _F64, _I32, _I32, _I32P, _F64P,
                              _F64P, _F64P, _I32P]),
     "chr_fv_grad": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _I32]),

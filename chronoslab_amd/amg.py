@@ -232,6 +232,17 @@ class AMGPreconditioner:
                "chr_pcg_solve")
         return it.value, resvec[: it.value + 1], fl.value
 
+    def pcg_noalloc_device(self, b_ptr: int, x_ptr: int, itmax: int = 1000, tol: float = 1e-8,
+                           A: DeviceMatrix | None = None):
+        """chr_pcg_solve_noalloc: PCG_CJ on device pointers with the library's fixed scratch
+        array as working memory (no device allocation); returns (iter, resvec, flag)."""
+        it, fl = C.c_int32(), C.c_int32()
+        resvec = np.zeros(itmax + 1)
+        _check(_lib().chr_pcg_solve_noalloc(self.ptr, A.ptr if A is not None else None, b_ptr, x_ptr, itmax, tol,
+                                            C.byref(it), resvec.ctypes.data_as(_F64P), C.byref(fl)),
+               "chr_pcg_solve_noalloc")
+        return it.value, resvec[: it.value + 1], fl.value
+
     def time_vcycle(self, reps: int = 20) -> float:
         ms = C.c_double()
         _check(_lib().chr_time_vcycle(self.ptr, reps, C.byref(ms)), "chr_time_vcycle")

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -402,9 +403,14 @@ using namespace chrrt;
 // ---------------------------------------------------------------------------------------
 extern "C" const char *chr_last_error(void) { return g_err.c_str(); }
 
+namespace chrrt {
+int scratch_setup(void **out);
+}
+
 extern "C" int chr_init(int32_t device) {
   HIPC(hipSetDevice(device));
-  return CHR_OK;
+  void *p = nullptr;
+  return scratch_setup(&p);
 }
 
 namespace chrrt {
@@ -413,6 +419,7 @@ namespace chrrt {
 int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out, int tune_k) {
   auto h = std::make_unique<chr_hier>();
   h->precision = precision;
+  HIPC(hipGetDevice(&h->device));
   HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   int rc;
   if ((rc = h->zero.alloc(sizeof(int)))) return rc;
@@ -780,23 +787,22 @@ hipError_t pcg_finalize(int kind, int count, double *part, chr::PcgState *st, do
 }
 
 template <int PER, typename TI, bool FORMED>
-hipError_t launch_pcg_p_f(const DevCsr &A, PcgWork &w, const double *pold, double *pnew,
+hipError_t launch_pcg_p_f(const DevCsr &A, const PcgPtrs &w, const double *pold, double *pnew,
                           const chr::PcgState *st, hipStream_t s) {
   hipLaunchKernelGGL((chr::pcg_p_stream_kernel<PER, TI, FORMED>), dim3(A.nblk), dim3(chr::kBlock), 0, s,
                      A.nblk, A.blk.as<int4>(), A.rp.as<int>(), A.ci.as<TI>(), A.val.as<double>(),
-                     (const double *)w.z.as<double>(), pold, pnew, w.ap.as<double>(),
-                     (const double *)w.r.as<double>(), w.part_p.as<double>(), st);
+                     (const double *)w.z, pold, pnew, w.ap, (const double *)w.r, w.part_p, st);
   return hipGetLastError();
 }
 
 template <int PER, typename TI>
-hipError_t launch_pcg_p_t(const DevCsr &A, PcgWork &w, const double *pold, double *pnew,
+hipError_t launch_pcg_p_t(const DevCsr &A, const PcgPtrs &w, const double *pold, double *pnew,
                           const chr::PcgState *st, bool formed, hipStream_t s) {
   return formed ? launch_pcg_p_f<PER, TI, true>(A, w, pold, pnew, st, s)
                 : launch_pcg_p_f<PER, TI, false>(A, w, pold, pnew, st, s);
 }
 
-hipError_t launch_pcg_p(const DevCsr &A, PcgWork &w, const double *pold, double *pnew,
+hipError_t launch_pcg_p(const DevCsr &A, const PcgPtrs &w, const double *pold, double *pnew,
                         const chr::PcgState *st, bool formed, hipStream_t s) {
   switch (A.per * 2 + (A.idx16 ? 1 : 0)) {
     case 8: return launch_pcg_p_t<4, int>(A, w, pold, pnew, st, formed, s);
@@ -817,37 +823,59 @@ bool pcg_formed(const DevCsr &) {
   return true;
 }
 
-// One PCG_CJ iteration (PCG_CJ.m:88-120), p read from pold and written to pnew.
-hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, PcgWork &w, int parity) {
-  auto *st = w.state.as<chr::PcgState>();
+// One PCG_CJ iteration (PCG_CJ.m:88-120), p read from p0 (parity 0) or p1 and written to
+// the other.  state_dst (may be null): host memory that receives the state at the end.
+hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, const PcgPtrs &w, int parity,
+                            chr::PcgState *state_dst) {
+  chr::PcgState *st = w.st;
   const int *stop = &st->stop;
-  double *pold = parity ? w.p1.as<double>() : w.p0.as<double>();
-  double *pnew = parity ? w.p0.as<double>() : w.p1.as<double>();
+  double *pold = parity ? w.p1 : w.p0;
+  double *pnew = parity ? w.p0 : w.p1;
   hipError_t e;
-  float *r32 = h.fp32() ? w.r32.as<float>() : nullptr;
-  if ((e = enqueue_vcycle(h, w.r.as<double>(), w.z.as<double>(), stop, 1, r32))) return e;
-  double *part = w.part.as<double>(), *resvec = w.resvec.as<double>();
+  if ((e = enqueue_vcycle(h, w.r, w.z, stop, 1, w.r32))) return e;
   hipLaunchKernelGGL(chr::pcg_zap_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
-                     (const double *)w.z.as<double>(), (const double *)w.ap.as<double>(), part,
-                     (const chr::PcgState *)st);
+                     (const double *)w.z, (const double *)w.ap, w.part, (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
-  if ((e = pcg_finalize(chr::kFinBeta, (int)w.nblk, part, st, resvec, h.stream))) return e;
+  if ((e = pcg_finalize(chr::kFinBeta, (int)w.nblk, w.part, st, w.resvec, h.stream))) return e;
   const bool formed = pcg_formed(A);
   if (formed) {
     hipLaunchKernelGGL(chr::pcg_pnew_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
-                       (const double *)w.z.as<double>(), (const double *)pold, pnew,
-                       (const chr::PcgState *)st);
+                       (const double *)w.z, (const double *)pold, pnew, (const chr::PcgState *)st);
     if ((e = hipGetLastError())) return e;
   }
   if ((e = launch_pcg_p(A, w, pold, pnew, st, formed, h.stream))) return e;
-  if ((e = pcg_finalize(chr::kFinAlpha, A.nblk, w.part_p.as<double>(), st, resvec, h.stream))) return e;
+  if ((e = pcg_finalize(chr::kFinAlpha, A.nblk, w.part_p, st, w.resvec, h.stream))) return e;
   hipLaunchKernelGGL(chr::pcg_update_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
-                     (const double *)pnew, (const double *)w.ap.as<double>(), w.x.as<double>(),
-                     w.r.as<double>(), r32, part, (const chr::PcgState *)st);
+                     (const double *)pnew, (const double *)w.ap, w.x, w.r, w.r32, w.part,
+                     (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
-  if ((e = pcg_finalize(chr::kFinUpdate, (int)w.nblk, part, st, resvec, h.stream))) return e;
-  return hipMemcpyAsync(&w.pinned[parity], st, sizeof(chr::PcgState), hipMemcpyDeviceToHost,
-                        h.stream);
+  if ((e = pcg_finalize(chr::kFinUpdate, (int)w.nblk, w.part, st, w.resvec, h.stream))) return e;
+  if (!state_dst) return hipSuccess;
+  return hipMemcpyAsync(state_dst, st, sizeof(chr::PcgState), hipMemcpyDeviceToHost, h.stream);
+}
+
+// r = b - A x, the FP32 copy of r, the norms and the initial scalars (PCG_CJ.m:5-30); the
+// state must hold tol and itmax, and p0, p1, Ap must be zero.
+hipError_t enqueue_pcg_init(chr_hier &h, const DevCsr &A, const PcgPtrs &w, const double *b) {
+  hipError_t e;
+  if ((e = launch_spmv(A, (const double *)w.x, chr::EpiResidual<double, double>{b, w.r}, h.zero.as<int>(),
+                       h.stream)))
+    return e;
+  if (w.r32) {
+    hipLaunchKernelGGL((chr::convert_kernel<double, float>), dim3((w.n + chr::kBlock - 1) / chr::kBlock),
+                       dim3(chr::kBlock), 0, h.stream, w.n, (const double *)w.r, w.r32);
+    if ((e = hipGetLastError())) return e;
+  }
+  hipLaunchKernelGGL(chr::pcg_norms_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n, b,
+                     (const double *)w.r, w.part);
+  if ((e = hipGetLastError())) return e;
+  return pcg_finalize(chr::kFinInit, (int)w.nblk, w.part, w.st, w.resvec, h.stream);
+}
+
+// Blocks of the PCG vector kernels: enough to fill 256 CUs several times, few enough for the
+// single-block finalisation of their partial sums.
+unsigned pcg_nblk(int32_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, ((int64_t)n + 1023) / 1024));
 }
 
 int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
@@ -863,9 +891,7 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
         (rc = w.p0.alloc(vb)) || (rc = w.p1.alloc(vb)) || (rc = w.ap.alloc(vb)) ||
         (rc = w.r32.alloc(h.fp32() ? 4 * (size_t)n : 0)))
       return rc;
-    // Fixed grid: enough blocks to fill 256 CUs several times, small enough for the
-    // single-block finalisation of the partial sums.
-    w.nblk = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, ((int64_t)n + 1023) / 1024));
+    w.nblk = pcg_nblk(n);
     if ((rc = w.part.alloc(sizeof(double) * 2 * w.nblk))) return rc;
     if ((rc = w.state.alloc(sizeof(chr::PcgState)))) return rc;
     if (!w.pinned) HIPC(hipHostMalloc((void **)&w.pinned, 2 * sizeof(chr::PcgState), hipHostMallocDefault));
@@ -899,7 +925,7 @@ int pcg_graphs(chr_hier &h, const chr_mat *Akey, const DevCsr &A) {
     }
     hipGraph_t g = nullptr;
     HIPC(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_pcg_iter(h, A, w, parity);
+    hipError_t e = enqueue_pcg_iter(h, A, w.ptrs(h.fp32()), parity, &w.pinned[parity]);
     hipError_t e2 = hipStreamEndCapture(h.stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
@@ -946,18 +972,7 @@ extern "C" int chr_pcg_solve(chr_hier *h, chr_mat *Aop, const double *b, double 
   st0.tol = tol;
   st0.itmax = itmax;
   HIPC(hipMemcpyAsync(w.state.p, &st0, sizeof(st0), hipMemcpyHostToDevice, h->stream));
-  HIPC(launch_spmv(A, w.x.as<double>(), chr::EpiResidual<double, double>{db, w.r.as<double>()},
-                   h->zero.as<int>(), h->stream));
-  if (h->fp32()) {
-    hipLaunchKernelGGL((chr::convert_kernel<double, float>), dim3((n + chr::kBlock - 1) / chr::kBlock),
-                       dim3(chr::kBlock), 0, h->stream, n, (const double *)w.r.as<double>(), w.r32.as<float>());
-    HIPC(hipGetLastError());
-  }
-  hipLaunchKernelGGL(chr::pcg_norms_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h->stream, n, db,
-                     (const double *)w.r.as<double>(), w.part.as<double>());
-  HIPC(hipGetLastError());
-  HIPC(pcg_finalize(chr::kFinInit, (int)w.nblk, w.part.as<double>(), w.state.as<chr::PcgState>(),
-                    w.resvec.as<double>(), h->stream));
+  HIPC(enqueue_pcg_init(*h, A, w.ptrs(h->fp32()), db));
   HIPC(hipMemcpyAsync(&w.pinned[1], w.state.p, sizeof(chr::PcgState), hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
   chr::PcgState fin = w.pinned[1];
@@ -984,6 +999,140 @@ extern "C" int chr_pcg_solve(chr_hier *h, chr_mat *Aop, const double *b, double 
   HIPC(hipMemcpyAsync(x, w.x.p, vb, out_kind, h->stream));
   if (resvec) HIPC(hipMemcpyAsync(resvec, w.resvec.p, sizeof(double) * (fin.iter + 1), hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
+  if (iter) *iter = fin.iter;
+  if (flag) *flag = fin.resiter > tol ? 1 : 0;
+  if (fin.breakdown) return fail(CHR_EBREAKDOWN, "PCG breakdown: p^T A p is zero or not finite");
+  return CHR_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// PCG_CJ without allocation (chr_pcg_solve_noalloc)
+// ---------------------------------------------------------------------------------------
+// The solver's working vectors (r, z, p0, p1, Ap, and the FP32 copy of r), its partial sums,
+// residual history and state live in one fixed device array per GPU instead of buffers the
+// library allocates.  In host code `scratch` names a host-side stand-in: its device address
+// is looked up once per device with hipGetSymbolAddress (chr_init, or the first call on a
+// device) and only that address is ever passed to kernels.
+namespace chrrt {
+
+constexpr size_t kScratchBytes = CHR_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// Device address of the current device's scratch array.
+int scratch_setup(void **out) {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CHR_EARG, "scratch: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CHR_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return CHR_OK;
+}
+
+template <typename T>
+__global__ __launch_bounds__(chr::kBlock) void fill_kernel(int64_t n, T v, T *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * chr::kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * chr::kBlock)
+    out[i] = v;
+}
+
+__global__ void pcg_state_init_kernel(chr::PcgState *st, double tol, int itmax) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    chr::PcgState z{};
+    z.tol = tol;
+    z.itmax = itmax;
+    *st = z;
+  }
+}
+
+template <typename T>
+hipError_t fill(T *p, int64_t n, T v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<int64_t>(4096, (n + chr::kBlock - 1) / chr::kBlock);
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(grid), dim3(chr::kBlock), 0, s, n, v, p);
+  return hipGetLastError();
+}
+
+}  // namespace chrrt
+
+extern "C" int chr_pcg_solve_noalloc(chr_hier *h, chr_mat *Aop, const double *b, double *x, int32_t itmax,
+                                     double tol, int32_t *iter, double *resvec, int32_t *flag) {
+  if (!h || !b || !x) return fail(CHR_EARG, "chr_pcg_solve_noalloc: null argument");
+  if (itmax < 0) return fail(CHR_EARG, "chr_pcg_solve_noalloc: itmax < 0");
+  const DevCsr &A = (Aop ? Aop : &h->outer)->A;
+  if (A.nrows != h->lev[0].n || A.ncols != A.nrows)
+    return fail(CHR_EARG, "chr_pcg_solve_noalloc: operator size differs from the hierarchy");
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  if (dev != h->device) return fail(CHR_EARG, "chr_pcg_solve_noalloc: call it on the hierarchy's device");
+  void *base = nullptr;
+  int rc;
+  if ((rc = scratch_setup(&base))) return rc;
+  // Carve the scratch array: 256-byte aligned pieces, every size checked before any work.
+  const int32_t n = A.nrows;
+  const unsigned nblk = pcg_nblk(n);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = (off + bytes + 255) / 256 * 256;
+    return at;
+  };
+  const size_t vb = 8 * (size_t)n;
+  const size_t o_r = take(vb), o_z = take(vb), o_p0 = take(vb), o_p1 = take(vb), o_ap = take(vb);
+  const size_t o_r32 = take(h->fp32() ? 4 * (size_t)n : 0);
+  const size_t o_part = take(sizeof(double) * 2 * nblk);
+  const size_t o_partp = take(sizeof(double) * 2 * (size_t)std::max(A.nblk, 1));
+  const size_t o_res = take(sizeof(double) * ((size_t)itmax + 1));
+  const size_t o_st = take(sizeof(chr::PcgState));
+  if (off > kScratchBytes)
+    return fail(CHR_EARG, "chr_pcg_solve_noalloc: the solve needs " + std::to_string(off) +
+                              " bytes of scratch, more than its " + std::to_string(kScratchBytes) +
+                              " (5 n doubles, 4 n bytes in the FP32 cycle, partial sums, itmax + 1 doubles)");
+  char *sb = static_cast<char *>(base);
+  PcgPtrs w;
+  w.n = n;
+  w.nblk = nblk;
+  w.x = x;  // iterated in place
+  w.r = reinterpret_cast<double *>(sb + o_r);
+  w.z = reinterpret_cast<double *>(sb + o_z);
+  w.p0 = reinterpret_cast<double *>(sb + o_p0);
+  w.p1 = reinterpret_cast<double *>(sb + o_p1);
+  w.ap = reinterpret_cast<double *>(sb + o_ap);
+  w.r32 = h->fp32() ? reinterpret_cast<float *>(sb + o_r32) : nullptr;
+  w.part = reinterpret_cast<double *>(sb + o_part);
+  w.part_p = reinterpret_cast<double *>(sb + o_partp);
+  w.resvec = reinterpret_cast<double *>(sb + o_res);
+  w.st = reinterpret_cast<chr::PcgState *>(sb + o_st);
+  hipStream_t s = h->stream;
+  // The array keeps the previous call's contents: p0, p1, Ap and the state start from zero.
+  HIPC(fill(w.p0, n, 0.0, s));
+  HIPC(fill(w.p1, n, 0.0, s));
+  HIPC(fill(w.ap, n, 0.0, s));
+  hipLaunchKernelGGL(pcg_state_init_kernel, dim3(1), dim3(64), 0, s, w.st, tol, itmax);
+  HIPC(hipGetLastError());
+  HIPC(enqueue_pcg_init(*h, A, w, b));
+  chr::PcgState fin{};
+  HIPC(hipMemcpyAsync(&fin, w.st, sizeof(fin), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  // One iteration at a time, the state read back after each (kernels of an iteration that
+  // starts after convergence would exit at once; none is enqueued).
+  for (int32_t it = 1; it <= itmax && !fin.stop; ++it) {
+    HIPC(enqueue_pcg_iter(*h, A, w, (it - 1) & 1, nullptr));
+    HIPC(hipMemcpyAsync(&fin, w.st, sizeof(fin), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+  }
+  if (fin.bnorm == 0.0) HIPC(fill(x, n, 0.0, s));
+  if (resvec) HIPC(hipMemcpyAsync(resvec, w.resvec, sizeof(double) * (fin.iter + 1), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
   if (iter) *iter = fin.iter;
   if (flag) *flag = fin.resiter > tol ? 1 : 0;
   if (fin.breakdown) return fail(CHR_EBREAKDOWN, "PCG breakdown: p^T A p is zero or not finite");

@@ -421,6 +421,17 @@ struct chr_mat {
   DevBuf zero;  // an int 0: the stop flag of products outside a PCG solve
 };
 
+// The device buffers of one PCG_CJ solve, as plain pointers (the solver's kernels take these,
+// whoever owns the memory).
+struct PcgPtrs {
+  int32_t n = 0;
+  unsigned nblk = 0;  // blocks of the vector kernels (partials in part: 2 per block)
+  double *x = nullptr, *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *ap = nullptr;
+  double *part = nullptr, *part_p = nullptr, *resvec = nullptr;
+  float *r32 = nullptr;  // FP32 copy of r (FP32 cycle only)
+  chr::PcgState *st = nullptr;
+};
+
 struct PcgWork {
   int32_t n = 0, itmax_cap = 0;
   DevBuf x, r, z, p0, p1, ap, part, part_p, state, resvec;
@@ -431,6 +442,23 @@ struct PcgWork {
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   uint64_t graph_uid = 0;  // uid of the outer operator the cached graphs were built for
   size_t part_p_cap = 0;   // doubles in part_p
+  PcgPtrs ptrs(bool fp32) const {
+    PcgPtrs q;
+    q.n = n;
+    q.nblk = nblk;
+    q.x = x.as<double>();
+    q.r = r.as<double>();
+    q.z = z.as<double>();
+    q.p0 = p0.as<double>();
+    q.p1 = p1.as<double>();
+    q.ap = ap.as<double>();
+    q.part = part.as<double>();
+    q.part_p = part_p.as<double>();
+    q.resvec = resvec.as<double>();
+    q.r32 = fp32 ? r32.as<float>() : nullptr;
+    q.st = state.as<chr::PcgState>();
+    return q;
+  }
   void drop_graphs() {     // the graphs hold buffer addresses: drop them before any realloc
     for (auto &g : exec)
       if (g) {
@@ -466,6 +494,7 @@ struct chr_hier {
     int64_t coarse_m2 = 0;  // sum of m^2 over the level's direct blocks
   };
   int32_t precision = CHR_FP64;
+  int device = 0;  // HIP device the hierarchy was created on
   std::vector<Level> lev;
   int32_t nc = 0;  // rows solved directly, over all levels
   DevBuf Ainv;     // packed FP64 dense inverses of the direct blocks (row-major each)

@@ -34,6 +34,7 @@
 #ifndef CHRONOS_AMG_H
 #define CHRONOS_AMG_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -120,6 +121,24 @@ int chr_spmv(chr_mat *A, const double *x, double *y, int32_t ptr_kind);
  * entries (may be NULL).  flag = 0 converged, 1 not converged (PCG_CJ.m:123-125). */
 int chr_pcg_solve(chr_hier *h, chr_mat *A, const double *b, double *x, int32_t itmax,
                   double tol, int32_t ptr_kind, int32_t *iter, double *resvec, int32_t *flag);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory once they
+ * run (serving loops, harnesses that forbid hipMalloc after start-up): chr_pcg_solve with
+ * device pointers whose working memory - r, z, the two p buffers, Ap, the FP32 copy of r in
+ * the FP32 cycle, the partial sums, the residual history and the solver state - lives in a
+ * fixed device array of CHR_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load until
+ * the process ends.  The call makes no hipMalloc / hipFree and instantiates no graph: its
+ * kernels are enqueued on the handle's stream one iteration at a time, with a wait for the
+ * stop flag after each.  b, x: device pointers (x holds x0 on entry and is updated in
+ * place); resvec (host, itmax+1 entries, may be NULL), iter and flag as for chr_pcg_solve,
+ * whose results it equals bit for bit.  A solve needing more than the array (about 40 n
+ * bytes, 44 n in the FP32 cycle, plus 8 (itmax+1) and the partial sums: n up to about 1.5
+ * million) returns CHR_EARG before any work.  Call it on the device the hierarchy was created
+ * on.  Every call on one GPU shares that GPU's array and leaves its contents behind; calls
+ * that overlap on one GPU, through any handle, race. */
+#define CHR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int chr_pcg_solve_noalloc(chr_hier *h, chr_mat *A, const double *b, double *x, int32_t itmax,
+                          double tol, int32_t *iter, double *resvec, int32_t *flag);
 
 /* lobpcg (sources/EigenSolver/lobpcg.m:21-467, B_orthonormalize.m:1-25) with prodB = [] and
  * no constraint block, as sources/Drivers/driver_Eigen/EigDriver.m:202-226 calls it:
