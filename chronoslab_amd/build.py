@@ -81,10 +81,17 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build failed: {' '.join(cmd)}")
 
 
-def build_setup() -> str:
+# AddressSanitizer + UndefinedBehaviorSanitizer build of the host library (CPU checking only;
+# load it with libasan preloaded, e.g. LD_PRELOAD=$(g++ -print-file-name=libasan.so)).
+SANITIZE_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+                  "-fno-sanitize-recover=undefined"]
+
+
+def build_setup(sanitize: bool = False) -> str:
     out = os.path.join(HERE, "libchronos_setup.so")
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "host", "*.cpp")))
-    _run(["g++", *HOST_FLAGS, f"-I{INC}", f"-I{os.path.join(HERE, 'csrc', 'common')}", "-o", out + ".tmp", *srcs])
+    flags = [f for f in HOST_FLAGS if f != "-O3"] + SANITIZE_FLAGS if sanitize else HOST_FLAGS
+    _run(["g++", *flags, f"-I{INC}", f"-I{os.path.join(HERE, 'csrc', 'common')}", "-o", out + ".tmp", *srcs])
     os.replace(out + ".tmp", out)
     _stamp(out, setup_sources())
     return out
@@ -116,4 +123,7 @@ def build_all() -> None:
 
 
 if __name__ == "__main__":
-    build_all()
+    if "--sanitize-host" in sys.argv:
+        build_setup(sanitize=True)
+    else:
+        build_all()

@@ -84,3 +84,29 @@ def test_malformed_files_rejected(tmp_path):
     p.write_bytes(struct.pack("<qq", 2, 3) + struct.pack("<qqd", 1, 1, 1.0))  # truncated
     with pytest.raises(ValueError):
         H.read_ext_bin_mat(str(p))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_reloaded_hierarchy_runs_on_the_gpu(gpu, tmp_path, prec):
+    """A 32^3 hierarchy built once, stored in Ext_bin files and reloaded (the cross-run reuse
+    the format serves, DumpChronos.cpp:27-45): the HIP V-cycle and PCG_CJ on the reloaded
+    hierarchy against the oracle on the original."""
+    A = H.poisson7(32)
+    hier = H.build(A)
+    H.save_hierarchy(str(tmp_path), hier)
+    again = H.load_hierarchy(str(tmp_path))
+    ref = oracle.Hier(hier)
+    b = H.rng_uniform(1, 2, A.nrows, -1.0, 1.0)
+    M = gpu.AMGPreconditioner(again, prec)
+    x, xo = M(b), ref.vcycle(b)
+    assert np.linalg.norm(x - xo) <= (1e-12 if prec == "fp64" else 1e-5) * np.linalg.norm(xo)
+    xs, it, rv, fl = M.pcg(b, itmax=100, tol=1e-8)
+    xr, itr, rvr, flr = ref.pcg(A, b, itmax=100, tol=1e-8)
+    assert fl == 0 and flr == 0
+    if prec == "fp64":
+        assert it == itr
+        assert np.linalg.norm(xs - xr) <= 1e-10 * np.linalg.norm(xr)
+    else:
+        assert abs(it - itr) <= 1
+        assert np.linalg.norm(b - A.matvec(xs)) <= 1e-8 * np.linalg.norm(b)

@@ -129,3 +129,33 @@ def test_batch_graphs_rebuilt_after_wider_vcycle(gpu):
     np.testing.assert_array_equal(it1, it2)
     np.testing.assert_array_equal(X1, X2)
     np.testing.assert_array_equal(rv1, rv2)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_pcg_against_oracle(gpu, prec):
+    """Each (system, column) solve of a batch against the oracle's PCG_CJ (PCG_CJ.m:1-73) on
+    that system's own hierarchy: FP64 cycle the same iteration count and solution within
+    1e-10, FP32 cycle within one iteration and the true residual at the tolerance."""
+    import oracle
+
+    edges = [14, 10, 20, 12, 17]
+    mats, hiers = _systems(edges, 1.5, 110)
+    bs = gpu.BatchSolver(hiers, prec)
+    k = 3
+    B = _stack(bs.sizes, k, 8)
+    X, iters, flags, resv = bs.pcg(B, itmax=200, tol=1e-8)
+    assert np.all(flags == 0)
+    for s, (A, h) in enumerate(zip(mats, hiers)):
+        o0, o1 = bs.offsets[s], bs.offsets[s + 1]
+        ref = oracle.Hier(h)
+        for c in range(k):
+            b = np.ascontiguousarray(B[o0:o1, c])
+            xr, itr, rvr, flr = ref.pcg(A, b, itmax=200, tol=1e-8)
+            x = X[o0:o1, c]
+            if prec == "fp64":
+                assert iters[s, c] == itr, (s, c, iters[s, c], itr)
+                assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
+                np.testing.assert_allclose(resv[s * k + c, : itr + 1], rvr, rtol=1e-7, atol=1e-14)
+            else:
+                assert abs(iters[s, c] - itr) <= 1, (s, c, iters[s, c], itr)
+                assert np.linalg.norm(b - A.matvec(x)) <= 1e-8 * np.linalg.norm(b)
