@@ -23,6 +23,12 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# OpenMP threads of the CPU legs pinned close, one per physical core (read at the first
+# OpenMP use in this process, so set before any native library loads).
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc summary of the roofline kernel (measured HBM bytes per launch).
@@ -45,7 +51,9 @@ def _measured_traffic(n: int):
 
 
 def _cpu_info():
+    """(model name, logical CPUs this process may use, physical cores among them)."""
     model = platform.processor() or "unknown"
+    out = ""
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
         for line in out.splitlines():
@@ -54,35 +62,115 @@ def _cpu_info():
     except OSError:
         pass
     try:
-        cores = len(os.sched_getaffinity(0))
+        cpus = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    return model, cores
+        cpus = list(range(os.cpu_count() or 1))
+    # Physical cores among them: distinct (package, core) pairs from sysfs; where sysfs does not
+    # tell them apart (some virtual machines report one core), lscpu's threads per core.
+    cores = set()
+    for c in cpus:
+        try:
+            t = f"/sys/devices/system/cpu/cpu{c}/topology"
+            cores.add((open(f"{t}/physical_package_id").read().strip(), open(f"{t}/core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    tpc = 1
+    try:
+        for line in out.splitlines():
+            if line.startswith("Thread(s) per core:"):
+                tpc = max(1, int(line.split(":", 1)[1]))
+    except (NameError, ValueError):
+        pass
+    phys = len(cores) if len(cores) * tpc >= len(cpus) else max(1, len(cpus) // tpc)
+    return model, len(cpus), phys
 
 
-def other_configs(np, torch, amg, H, log) -> dict:
-    """LOBPCG (96^3 Poisson, 16 pairs), batching (48 systems x 8 rhs) and the adjoint
-    gradient (96^3), each timed with its inputs resident on the GPU."""
+def _time_cpu(fn, reps: int = 5):
+    """Median wall time of reps calls after one warm-up call."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def vcycle_bytes(hier, s: int, k: int = 1) -> float:
+    """Algorithmic bytes of one V-cycle on an n x k block (AMG_Vcycle.m:1-54, V(1,1), zero
+    initial guess): every matrix streamed once per product as plain CSR (s-byte values, 4-byte
+    columns and row pointers), every vector word read or written once per operation.  Per
+    level l below the last: 2 B(A) + 2 B(G) + 2 B(G^T) + B(P) + B(P^T) + k (18 n_l + 2 n_{l+1}) s;
+    last level: the dense FP64 inverse once, 8 m^2 + 16 m k."""
+    def B(M):
+        return M.nnz * (s + 4) + 4 * (M.nrows + 1)
+
+    tot = 0.0
+    for l, L in enumerate(hier.levels):
+        if L.P is None:
+            m = L.A.nrows
+            tot += 8.0 * m * m + 16.0 * m * k
+            continue
+        GT = 4 * (L.G.ncols + 1) + L.G.nnz * (s + 4)
+        PT = 4 * (L.P.ncols + 1) + L.P.nnz * (s + 4)
+        tot += 2 * B(L.A) + 2 * B(L.G) + 2 * GT + B(L.P) + PT + k * (18 * L.A.nrows + 2 * L.P.ncols) * s
+    return tot
+
+
+def config_lobpcg(np, torch, amg, H, log) -> dict:
+    """LOBPCG: 16 smallest eigenpairs of the 96^3 Poisson matrix, AMG preconditioner."""
     out = {}
     dev = torch.device("cuda:0")
-    # LOBPCG: 16 smallest eigenpairs of the 96^3 Poisson matrix, AMG preconditioner.
     N, k = 96, 16
     A = H.poisson7(N)
-    M = amg.AMGPreconditioner(H.build(A), "fp32")
+    hl = H.build(A)
+    M = amg.AMGPreconditioner(hl, "fp32")
     X0 = np.asfortranarray(np.stack([H.rng_uniform(7, 10 + c, A.nrows) for c in range(k)], axis=1))
-    amg.lobpcg(None, M, X0, itmax=300, tol=1e-8)  # warm-up
-    t = time.perf_counter()
-    it, lam, X, rh, lh, ierr = amg.lobpcg(None, M, X0, itmax=300, tol=1e-8)
-    dt = time.perf_counter() - t
+    X0d = torch.from_numpy(np.ascontiguousarray(X0.T).ravel()).to(dev)  # column-major n x k
+    Xd = torch.empty_like(X0d)
+    ts = []
+    for rep in range(4):  # one warm-up, three timed
+        Xd.copy_(X0d)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        it, lam, rh, lh, ierr = amg.lobpcg_device(None, M, A.nrows, k, Xd.data_ptr(), itmax=300, tol=1e-8)
+        torch.cuda.synchronize()
+        if rep:
+            ts.append(time.perf_counter() - t)
+    dt = float(np.median(ts))
     j = np.arange(1, N + 1)
     l1 = 2 - 2 * np.cos(j * np.pi / (N + 1))
     exact = np.sort((l1[:, None, None] + l1[None, :, None] + l1[None, None, :]).ravel())[:k]
+    # Algorithmic bytes of the iterations (lobpcg.m:141-379), with a = the columns still active
+    # (resnorm >= tol at the previous step): block V-cycle on a columns (FP32), FP64 product
+    # A R (matrix once, a columns in and out), the Rayleigh-Ritz Gram blocks (the six tall
+    # blocks X, R, P and A X, A R, A P read once: 2 (k + 2a) columns), the closing updates
+    # (P and X, then their operator sides: 2 (3a + 2k) columns) and the residual (3k columns).
+    n8 = A.nrows * 8.0
+    BA = A.nnz * 12.0 + 4.0 * (A.nrows + 1)
+    it_bytes = 0.0
+    for i in range(1, it + 1):
+        a = int(np.sum(rh[i - 1] >= 1e-8)) if i - 1 < rh.shape[0] else k
+        a = max(a, 1)
+        it_bytes += vcycle_bytes(hl, 4, a) + BA + 2 * a * n8 + 2 * (k + 2 * a) * n8 + 2 * (3 * a + 2 * k) * n8 + 3 * k * n8
+    gbs = it_bytes / dt / 1e9
     out["lobpcg_96cube_16"] = {"seconds": dt, "iterations": it, "ierr": ierr,
                                "max_eig_rel_err_vs_closed_form": float(np.max(np.abs(lam - exact) / exact)),
-                               "note": "FP32 V-cycle preconditioner, FP64 Rayleigh-Ritz; host X0/X copies included"}
-    log(f"[bench] LOBPCG 96^3 x16: {dt:.3f} s, {it} it, max eig err {out['lobpcg_96cube_16']['max_eig_rel_err_vs_closed_form']:.2e}")
+                               "algorithmic_GB": it_bytes / 1e9, "GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS,
+                               "note": "FP32 V-cycle preconditioner, FP64 Rayleigh-Ritz; X0 and X resident on the "
+                                       "GPU; median of 3 after a warm-up; GB/s = algorithmic bytes of the iterations "
+                                       "(active-column count per step) / whole solve time"}
+    log(f"[bench] LOBPCG 96^3 x16: {dt:.3f} s, {it} it, max eig err "
+        f"{out['lobpcg_96cube_16']['max_eig_rel_err_vs_closed_form']:.2e}, {gbs:.0f} GB/s algorithmic "
+        f"({gbs / HBM_PEAK_GBS:.2f} of HBM)")
     del M
-    # Batching: 48 heterogeneous-diffusion systems, edge U{16..64}, ln k ~ N(0, 1.5^2), 8 rhs.
+    return out["lobpcg_96cube_16"]
+
+
+def config_batch(np, torch, amg, H, log) -> dict:
+    """Batching: 48 heterogeneous-diffusion systems, edge U{16..64}, ln k ~ N(0, 1.5^2), 8 rhs."""
+    out = {}
+    dev = torch.device("cuda:0")
     rng = np.random.default_rng(48)
     edges = rng.integers(16, 65, size=48)
     mats, hiers = [], []
@@ -111,13 +199,30 @@ def other_configs(np, torch, amg, H, log) -> dict:
         o0, o1 = bs.offsets[s_], bs.offsets[s_ + 1]
         R = B[o0:o1] - Ai.to_scipy() @ X[o0:o1]
         worst = max(worst, float((np.linalg.norm(R, axis=0) / np.linalg.norm(B[o0:o1], axis=0)).max()))
-    out["batch_48x8"] = {"seconds": float(np.median(ts)), "solves": 48 * kk, "rows": bs.n,
+    # Algorithmic bytes of one batch iteration: every system's block V-cycle on 8 columns (FP32)
+    # and its outer FP64 product and vector work (PCG_CJ.m:38-58: 18 n words per column); a
+    # converged solve is frozen but its rows are still streamed, so every iteration of the
+    # batch moves the whole batch.
+    bt_bytes = sum(vcycle_bytes(h_, 4, kk) + A_.nnz * 12.0 + 4.0 * (A_.nrows + 1) + 18.0 * A_.nrows * kk * 8
+                   for h_, A_ in zip(hiers, mats))
+    t_b = float(np.median(ts))
+    gbs = bt_bytes * int(iters.max()) / t_b / 1e9
+    out["batch_48x8"] = {"seconds": t_b, "solves": 48 * kk, "rows": bs.n,
                          "iterations_min_max": [int(iters.min()), int(iters.max())],
                          "all_converged": bool(np.all(flags == 0)), "worst_true_rel_residual": worst,
-                         "note": "FP32 V-cycle, one graph per batch iteration, inputs resident on the GPU"}
-    log(f"[bench] batch 48x8: {np.median(ts) * 1e3:.1f} ms, iters {iters.min()}..{iters.max()}, worst res {worst:.2e}")
+                         "bytes_per_batch_iteration": bt_bytes, "GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS,
+                         "note": "FP32 V-cycle, one graph per batch iteration, inputs resident on the GPU; GB/s = "
+                                 "algorithmic bytes per batch iteration x iterations of the longest solve / solve time"}
+    log(f"[bench] batch 48x8: {t_b * 1e3:.1f} ms, iters {iters.min()}..{iters.max()}, worst res {worst:.2e}, "
+        f"{gbs:.0f} GB/s algorithmic ({gbs / HBM_PEAK_GBS:.2f} of HBM)")
     del bs, hiers, mats
-    # Adjoint: 96^3, ln k ~ N(0,1); forward FP32-cycle PCG to 1e-10, adjoint and gradient.
+    return out["batch_48x8"]
+
+
+def config_adjoint(np, torch, amg, H, log) -> dict:
+    """Adjoint: 96^3, ln k ~ N(0,1); forward FP32-cycle PCG to 1e-10, adjoint and gradient."""
+    out = {}
+    dev = torch.device("cuda:0")
     from chronoslab_amd.autograd import FVDiffusionSolve
     import oracle as O  # the FP64 CPU gradient the GPU gradient is checked against
 
@@ -149,8 +254,18 @@ def other_configs(np, torch, amg, H, log) -> dict:
                                      "PCG (same hierarchy) + fused dL/d(ln k) kernel"}
     log(f"[bench] adjoint 96^3: forward {t_fwd:.2f}s (incl. setup), backward {t_bwd * 1e3:.1f} ms, grad err {rel:.2e}")
     del oh, Ak
-    out["elasticity_64cube"] = elasticity(np, torch, amg, H, O, log)
-    return out
+    return out["adjoint_96cube"]
+
+
+def other_configs(np, torch, amg, H, log) -> dict:
+    """LOBPCG (96^3 Poisson, 16 pairs), batching (48 systems x 8 rhs), the adjoint gradient
+    (96^3) and the elasticity system, each timed with its inputs resident on the GPU."""
+    import oracle as O
+
+    return {"lobpcg_96cube_16": config_lobpcg(np, torch, amg, H, log),
+            "batch_48x8": config_batch(np, torch, amg, H, log),
+            "adjoint_96cube": config_adjoint(np, torch, amg, H, log),
+            "elasticity_64cube": elasticity(np, torch, amg, H, O, log)}
 
 
 def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) -> dict:
@@ -194,7 +309,7 @@ def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) 
         log(f"[bench] elasticity {prec}: {np.mean(ts) * 1e3:.1f} ms, {it} it, V-cycle {vms:.3f} ms "
             f"({st['vcycle_csr_bytes'] / vms / 1e6:.0f} GB/s algorithmic)")
         del M
-    model, cores = _cpu_info()
+    model, logical, cores = _cpu_info()
     O.set_threads(cores)
     t = time.perf_counter()
     xo, ito, _, flo = O.Hier(hier).pcg(Afix, rhs, itmax=500, tol=1e-8)
@@ -203,7 +318,7 @@ def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) 
     for prec, r in res.items():
         xs = r.pop("x")
         r["err_vs_fp64_oracle"] = float(np.linalg.norm(xs - xo) / np.linalg.norm(xo))
-    log(f"[bench] elasticity oracle: {t_cpu:.2f} s ({ito} it) on {cores} threads")
+    log(f"[bench] elasticity oracle: {t_cpu:.2f} s ({ito} it) on {cores} threads (one per physical core)")
     return {"n": A.nrows, "nnz": A.nnz, "levels": hier.nlev, "setup_s": t_setup,
             "level_rows": [l.A.nrows for l in hier.levels], "level_nnz": [l.A.nnz for l in hier.levels],
             "solve": res,
@@ -211,6 +326,57 @@ def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) 
             "cpu_oracle_s": t_cpu, "cpu_oracle_iterations": ito, "cpu_threads": cores,
             "note": "BAMG + EMIN prolongation, aFSAI; host coarsening, interpolation and EMIN "
                     "constraints, GPU EMIN operator, aFSAI, omega and Galerkin products"}
+
+
+def config_poisson32(np, torch, amg, H, log) -> dict:
+    """The reference's CPU-runnable case: 7-point Poisson on 32^3 (32,768 unknowns, Dirichlet
+    boundary eliminated), RHS U[-1,1] from the seed, CLA + PMIS + classical prolongation, aFSAI
+    smoother, PCG_CJ to 1e-8 in FP64.  The oracle (the C restatement of the reference's MATLAB
+    path) is timed on the host's physical cores, median of 5, and on one thread; the GPU solves
+    beside it with b resident."""
+    import oracle
+
+    dev = torch.device("cuda:0")
+    A = H.poisson7(32)
+    hier = H.build(A)
+    bh = H.rng_uniform(1, 2, A.nrows, -1.0, 1.0)
+    model, logical, cores = _cpu_info()
+    oh = oracle.Hier(hier)
+    box = {}
+
+    def solve():
+        box["o"] = oh.pcg(A, bh, itmax=500, tol=1e-8)
+
+    oracle.set_threads(cores)
+    t_cpu = _time_cpu(solve, 5)
+    oracle.set_threads(1)
+    t_one = _time_cpu(solve, 5)
+    oracle.set_threads(0)
+    xo, ito, _, _ = box["o"]
+    b = torch.from_numpy(bh).to(dev)
+    x = torch.zeros_like(b)
+    gpu = {}
+    for prec in ("fp64", "fp32"):
+        M = amg.AMGPreconditioner(hier, prec)
+
+        def gsolve():
+            x.zero_()
+            torch.cuda.synchronize()
+            box["g"] = M.pcg_device(b.data_ptr(), x.data_ptr(), itmax=500, tol=1e-8)
+            torch.cuda.synchronize()
+
+        t = _time_cpu(gsolve, 5)
+        it, _, fl = box["g"]
+        xs = x.cpu().numpy()
+        gpu[prec] = {"seconds": t, "iterations": it, "flag": fl,
+                     "err_vs_fp64_oracle": float(np.linalg.norm(xs - xo) / np.linalg.norm(xo))}
+        del M
+    log(f"[bench] poisson 32^3: oracle {t_cpu * 1e3:.1f} ms on {cores} cores ({t_one * 1e3:.1f} ms on 1), {ito} it; "
+        f"GPU fp64 {gpu['fp64']['seconds'] * 1e3:.2f} ms, fp32 {gpu['fp32']['seconds'] * 1e3:.2f} ms")
+    return {"n": A.nrows, "nnz": A.nnz, "levels": hier.nlev, "cpu_oracle_s": t_cpu, "cpu_oracle_one_thread_s": t_one,
+            "cpu_cores": cores, "cpu_model": model, "cpu_iterations": ito, "gpu": gpu,
+            "note": "median of 5 after a warm-up for every figure; OpenMP one thread per physical core, "
+                    "OMP_PROC_BIND=close"}
 
 
 def main() -> int:
@@ -228,6 +394,8 @@ def main() -> int:
                     help="skip the LOBPCG, batching and adjoint configurations")
     ap.add_argument("--only-elasticity", action="store_true",
                     help="only the 64^3 elasticity configuration (BAMG + EMIN set-up)")
+    ap.add_argument("--only-config", choices=["lobpcg", "batch", "adjoint", "poisson32"],
+                    help="only the named configuration (its JSON object on one line)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run 8 device-to-device copies of 512 MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -255,6 +423,11 @@ def main() -> int:
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     amg.init(0)
+    if args.only_config:
+        fn = {"lobpcg": config_lobpcg, "batch": config_batch, "adjoint": config_adjoint,
+              "poisson32": config_poisson32}[args.only_config]
+        print(json.dumps({args.only_config: fn(np, torch, amg, H, log)}), flush=True)
+        return 0
     if args.only_elasticity:
         import oracle
 
@@ -341,32 +514,39 @@ def main() -> int:
     if not args.no_cpu:
         import oracle
 
-        model, cores = _cpu_info()
+        model, logical, cores = _cpu_info()
         oh = oracle.Hier(hier)
+        res_o = {}
+
+        def solve_oracle():
+            res_o["out"] = oh.pcg(A, b_host, itmax=500, tol=args.tol)
+
         oracle.set_threads(cores)
-        oh.pcg(A, b_host, itmax=500, tol=args.tol)  # warm-up
-        ts = []
-        for _ in range(3):
-            t = time.perf_counter()
-            xo, ito, rvo, flo = oh.pcg(A, b_host, itmax=500, tol=args.tol)
-            ts.append(time.perf_counter() - t)
+        t_med = _time_cpu(solve_oracle, 5)
+        xo, ito, rvo, flo = res_o["out"]
         oracle.set_threads(1)
         t = time.perf_counter()
         oh.pcg(A, b_host, itmax=500, tol=args.tol)
         t1 = time.perf_counter() - t
         oracle.set_threads(0)
+        ts = [t_med]
         for prec, r in res.items():
             err_vs_oracle[prec] = float(np.linalg.norm(r["x"] - xo) / np.linalg.norm(xo))
-        cpu = {"value": float(np.median(ts)), "unit": "s", "cores": cores, "kind": "port",
+        cpu = {"value": t_med, "unit": "s", "cores": cores, "kind": "port",
                "sample": (f"full oracle PCG_CJ solve of the same {N}^3 system to {args.tol:g} "
-                          f"({ito} iterations), median of 3 after a warm-up, OMP threads = {cores} "
-                          f"({model}); 1-thread solve {t1:.2f} s"),
+                          f"({ito} iterations), median of 5 after a warm-up, {cores} OpenMP threads, one per "
+                          f"physical core (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                          f"OMP_PLACES={os.environ.get('OMP_PLACES')}; {logical} logical CPUs available; "
+                          f"{model}); 1-thread solve {t1:.2f} s.  The reference's own path is MATLAB, "
+                          f"which cannot run here; the oracle is its C restatement"),
                "one_thread_s": t1, "iters": ito}
         log(f"[bench] cpu oracle: {np.median(ts):.3f} s ({ito} it) on {cores} threads; 1 thread {t1:.2f} s")
 
     extra = {}
+    if not args.no_cpu:
+        extra["poisson_32cube_cpu_path"] = config_poisson32(np, torch, amg, H, log)
     if not args.only_c2:
-        extra = other_configs(np, torch, amg, H, log)
+        extra.update(other_configs(np, torch, amg, H, log))
 
     main_prec = "fp32" if "fp32" in res else "fp64"
     r = res[main_prec]

@@ -338,6 +338,26 @@ def lobpcg(prodA, prodM: AMGPreconditioner | None, X0: np.ndarray, largest: bool
     return niter.value, lam, X, rh[:k], lh[:k], ierr.value
 
 
+def lobpcg_device(prodA, prodM: AMGPreconditioner | None, n: int, neig: int, x_ptr: int, largest: bool = False,
+                  reslambda_check: bool = False, itmax: int = 100, tol: float = 1e-8, restart_control: int = 7):
+    """lobpcg as above on a device block: x_ptr holds X0 (n x neig, column-major FP64) on entry
+    and the eigenvectors on exit.  Returns (niter, lambda, resnorm_hist, lambda_hist, ierr)."""
+    if isinstance(prodA, H.Csr):
+        prodA = DeviceMatrix(prodA)
+    if prodA is None and prodM is None:
+        raise ValueError("lobpcg needs prodA or prodM")
+    niter, ierr = C.c_int32(), C.c_int32()
+    lam = np.zeros(neig)
+    rh = np.zeros((itmax + 1, neig), order="F")
+    lh = np.zeros((itmax + 1, neig), order="F")
+    _check(_lib().chr_lobpcg(prodM.ptr if prodM is not None else None, prodA.ptr if prodA is not None else None,
+                             n, neig, x_ptr, n, int(largest), int(reslambda_check), itmax, tol, restart_control,
+                             PTR_DEVICE, C.byref(niter), lam.ctypes.data_as(_F64P), rh.ctypes.data_as(_F64P),
+                             lh.ctypes.data_as(_F64P), C.byref(ierr)), "chr_lobpcg")
+    k = niter.value + 1
+    return niter.value, lam, rh[:k], lh[:k], ierr.value
+
+
 def computeAMG(A: H.Csr, param: H.SetupParams | None = None, precision="fp64", seed: int = 1):
     """computeAMG.m:1-51: BC rule (unit diagonal -> lambda_max/10), setup, upload.
 
