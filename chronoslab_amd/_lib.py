@@ -71,6 +71,7 @@ def load_setup_lib():
             _sig(lib, "chs_prol_classical", C.c_int, [_VP, _I8P, _I32P, _I32, C.c_int, C.POINTER(_VP)])
             _sig(lib, "chs_prol_exti", C.c_int, [_VP, _I8P, _I32P, _I32, C.c_int, C.POINTER(_VP)])
             _sig(lib, "chs_afsai", C.c_int, [_VP, _I32, _I32, _F64, C.c_int, C.POINTER(_VP), C.POINTER(_I32)])
+            _sig(lib, "chs_fsai_nan_repair", C.c_int, [_VP, _VP, C.POINTER(_I32)])
             _sig(lib, "chs_fsai_omega", C.c_int, [_VP, _VP, _U64, C.c_int, C.POINTER(_F64), C.POINTER(_F64)])
             _sig(lib, "chs_transpose", C.c_int, [_VP, C.POINTER(_VP)])
             _sig(lib, "chs_galerkin", C.c_int, [_VP, _VP, C.c_int, C.POINTER(_VP)])

@@ -119,7 +119,7 @@ int DevPack::build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *
 // CHRONOS_SELL_SORT=0|8|10|12|14 fixes its sort window (0: unsorted); CHRONOS_PACK=0
 // excludes and CHRONOS_PACK=1 forces the packed-code layout (when sliced ELL is not forced).
 int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
-                const int *zero, bool tune_per, const DevBuf *flush, int tune_k) {
+                const int *zero, bool tune_per, const DevBuf *flush, int tune_k, bool stream_only) {
   M.use_sell = false;
   M.sell.release();
   M.use_pack = false;
@@ -204,7 +204,7 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
       if (best_per != M.per) rc = M.layout(best_per, hrp, hci);
     }
   }
-  if (kt > 1) return rc;  // block products use the stream blocks only
+  if (kt > 1 || stream_only) return rc;  // block products use the stream blocks only
   // Step 2: packed codes (the stream kernel's blocks with 2-byte codes).
   // (Only where 16-bit block offsets do not apply: otherwise the stream layout's index is
   // already 2 bytes.)
@@ -370,12 +370,17 @@ void fill_stats(chr_hier &h) {
       // Per level: 2 A, 2 G, 2 G^T, P, P^T, 18 n_l + 2 n_{l+1} vector words (AMG_Vcycle.m).
       const double nc = h.lev[l + 1].n;
       const double vec = 18.0 * L.nd * s + 2.0 * nc * s;
-      b += 2 * L.A.stream_bytes() + 2 * L.G.stream_bytes() + 2 * L.GT.stream_bytes() +
-           L.P.stream_bytes() + L.PT.stream_bytes() + vec;
+      // Stored bytes of the products the cycle actually runs (composites where fused).
+      const double pre = L.fuse_s ? L.S.stream_bytes() : L.G.stream_bytes() + L.GT.stream_bytes();
+      const double down = L.fuse_r ? L.R.stream_bytes() : L.A.stream_bytes() + L.PT.stream_bytes();
+      const double up = !L.fuse_up ? L.P.stream_bytes() + L.A.stream_bytes() + L.G.stream_bytes() + L.GT.stream_bytes()
+                        : L.U.stream_bytes() + L.W.stream_bytes() + (L.fuse_s ? 0.0 : L.G.stream_bytes());
+      b += pre + down + up + vec;
       bc += 2 * L.A.csr_bytes() + 2 * L.G.csr_bytes() + 2 * L.GT.csr_bytes() + L.P.csr_bytes() +
             L.PT.csr_bytes() + vec;
       dev += L.A.device_bytes() + L.G.device_bytes() + L.GT.device_bytes() + L.P.device_bytes() +
-             L.PT.device_bytes() + L.b.bytes + L.x.bytes + L.r.bytes + L.y.bytes;
+             L.PT.device_bytes() + L.R.device_bytes() + L.U.device_bytes() + L.W.device_bytes() +
+             L.S.device_bytes() + L.b.bytes + L.x.bytes + L.r.bytes + L.y.bytes;
     }
     if (l < 16) st.level_bytes[l] = bc;
     total += b;
@@ -415,6 +420,131 @@ extern "C" int chr_init(int32_t device) {
 
 namespace chrrt {
 
+// Owned host CSR (FP64).
+struct HostMat {
+  int32_t nrows = 0, ncols = 0;
+  std::vector<int32_t> rp{0}, ci;
+  std::vector<double> v;
+  chr_csr view() const { return {nrows, ncols, rp.data(), ci.data(), v.data()}; }
+};
+
+// C = L R, row by row (Gustavson): row i of C accumulates s * (row j of R) for the entries
+// (i, j, s) of L in L's column order; columns sorted.  Exact zeros are kept.
+HostMat spgemm_host(const chr_csr &L, const chr_csr &R) {
+  HostMat C;
+  C.nrows = L.nrows;
+  C.ncols = R.ncols;
+  C.rp.assign((size_t)L.nrows + 1, 0);
+  std::vector<int32_t> mark((size_t)R.ncols, -1), cols;
+  std::vector<double> acc((size_t)R.ncols, 0.0);
+  for (int32_t i = 0; i < L.nrows; ++i) {
+    cols.clear();
+    for (int32_t e = L.rowptr[i]; e < L.rowptr[i + 1]; ++e) {
+      const int32_t j = L.col[e];
+      const double a = L.val[e];
+      for (int32_t f = R.rowptr[j]; f < R.rowptr[j + 1]; ++f) {
+        const int32_t c = R.col[f];
+        if (mark[c] != i) {
+          mark[c] = i;
+          acc[c] = 0.0;
+          cols.push_back(c);
+        }
+        acc[c] += a * R.val[f];
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    for (int32_t c : cols) {
+      C.ci.push_back(c);
+      C.v.push_back(acc[c]);
+    }
+    C.rp[i + 1] = (int32_t)C.ci.size();
+  }
+  return C;
+}
+
+// [s1 M1 | s2 M2] (same row count) with interleaved columns: M1's column j becomes 2j and
+// M2's column j becomes 2j + 1 (chr::XTwo); each row's entries in ascending column order.
+HostMat hcat_host(const chr_csr &M1, double s1, const chr_csr &M2, double s2) {
+  HostMat C;
+  C.nrows = M1.nrows;
+  C.ncols = 2 * std::max(M1.ncols, M2.ncols);
+  C.rp.assign((size_t)M1.nrows + 1, 0);
+  const int64_t nnz = (int64_t)M1.rowptr[M1.nrows] + M2.rowptr[M2.nrows];
+  C.ci.reserve(nnz);
+  C.v.reserve(nnz);
+  for (int32_t i = 0; i < M1.nrows; ++i) {
+    int32_t e1 = M1.rowptr[i], e2 = M2.rowptr[i];
+    const int32_t z1 = M1.rowptr[i + 1], z2 = M2.rowptr[i + 1];
+    while (e1 < z1 || e2 < z2) {  // merge by interleaved column
+      if (e2 >= z2 || (e1 < z1 && 2 * M1.col[e1] < 2 * M2.col[e2] + 1)) {
+        C.ci.push_back(2 * M1.col[e1]);
+        C.v.push_back(s1 * M1.val[e1++]);
+      } else {
+        C.ci.push_back(2 * M2.col[e2] + 1);
+        C.v.push_back(s2 * M2.val[e2++]);
+      }
+    }
+    C.rp[i + 1] = (int32_t)C.ci.size();
+  }
+  return C;
+}
+
+// Composite operators of level L (chr_hier::Level).  The residual-and-restriction operator R
+// is formed on every level >= 1 where P^T A has no more nonzeros than A and R keeps 16-bit
+// column offsets (it then streams fewer bytes than A and P^T, and saves a kernel and the
+// residual vector's round trip), and on every small level.  On small levels (A with fewer than CHRONOS_FUSE_NNZ nonzeros,
+// default 4,000,000: their products are bound by latency, not bytes) the up path and the
+// pre-smoothing are fused too (U, W, S).  CHRONOS_FUSE=0 disables all of it.
+int build_composites(chr_hier::Level &L, const LevelIn &in, const std::vector<int32_t> &ptrp,
+                     const std::vector<int32_t> &ptci, const std::vector<double> &ptv,
+                     const std::vector<int32_t> &gtrp, const std::vector<int32_t> &gtci,
+                     const std::vector<double> &gtv, bool f32, hipStream_t s, const int *zf) {
+  if (const char *e = std::getenv("CHRONOS_FUSE"); e && std::atoi(e) == 0) return CHR_OK;
+  const char *env_nnz = std::getenv("CHRONOS_FUSE_NNZ");
+  const int64_t small_nnz = env_nnz ? std::atoll(env_nnz) : 4000000;
+  const int64_t nnzA = in.A.rowptr[in.A.nrows];
+  const bool small = nnzA < small_nnz;
+  const chr_csr PT{in.P.ncols, in.P.nrows, ptrp.data(), ptci.data(), ptv.data()};
+  HostMat PtA = spgemm_host(PT, in.A);
+  if (!small && (int64_t)PtA.ci.size() > nnzA) return CHR_OK;
+  int rc;
+  auto upload = [&](DevCsr &M, const HostMat &H) -> int {
+    int r;
+    if ((r = M.upload(H.nrows, H.ncols, H.rp.data(), H.ci.data(), H.v.data(), f32))) return r;
+    return tune_layout(M, H.rp.data(), H.ci.data(), H.v.data(), s, zf, true, nullptr, 1, true);
+  };
+  HostMat R = hcat_host(PtA.view(), -1.0, PT, 1.0);
+  PtA = HostMat();
+  if ((rc = upload(L.R, R))) return rc;
+  if (!small && !L.R.idx16) {  // 4-byte columns: more bytes than A and P^T in their layouts
+    L.R = DevCsr();
+    return CHR_OK;
+  }
+  L.fuse_r = true;
+  if (!small) return CHR_OK;
+  const chr_csr GT{in.G.ncols, in.G.nrows, gtrp.data(), gtci.data(), gtv.data()};
+  // CHRONOS_FUSE_S=0: the up path fused with W = [P | omega G^T] and the smoothing sweeps as
+  // two products (for comparison).
+  const char *env_s = std::getenv("CHRONOS_FUSE_S");
+  const bool with_s = !env_s || std::atoi(env_s) != 0;
+  HostMat AP = spgemm_host(in.A, in.P);
+  HostMat U = hcat_host(AP.view(), 1.0, in.A, 1.0);
+  AP = HostMat();
+  if ((rc = upload(L.U, U))) return rc;
+  if (with_s) {
+    HostMat S = spgemm_host(GT, in.G);
+    for (double &v : S.v) v *= in.omega;
+    HostMat W = hcat_host(in.P, 1.0, S.view(), 1.0);
+    if ((rc = upload(L.S, S)) || (rc = upload(L.W, W))) return rc;
+  } else {
+    HostMat W = hcat_host(in.P, 1.0, GT, in.omega);
+    if ((rc = upload(L.W, W))) return rc;
+  }
+  L.fuse_s = with_s;
+  L.fuse_up = true;
+  return CHR_OK;
+}
+
 // Builds a hierarchy on the device from per-level host descriptions (see chr_hier::Level).
 int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out, int tune_k) {
   auto h = std::make_unique<chr_hier>();
@@ -453,15 +583,19 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
         return rc;
       if ((rc = L.P.upload(in.P, f32)) || (rc = tune_layout(L.P, in.P.rowptr, in.P.col, in.P.val, h->stream, zf, true, fl, tune_k)))
         return rc;
-      std::vector<int32_t> trp, tci;
-      std::vector<double> tv;
+      std::vector<int32_t> trp, tci, prp, pci;
+      std::vector<double> tv, pv;
       transpose_host(in.G, trp, tci, tv);
       if ((rc = L.GT.upload(in.G.ncols, in.G.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
           (rc = tune_layout(L.GT, trp.data(), tci.data(), tv.data(), h->stream, zf, true, fl, tune_k)))
         return rc;
-      transpose_host(in.P, trp, tci, tv);
-      if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, trp.data(), tci.data(), tv.data(), f32)) ||
-          (rc = tune_layout(L.PT, trp.data(), tci.data(), tv.data(), h->stream, zf, true, fl, tune_k)))
+      transpose_host(in.P, prp, pci, pv);
+      if ((rc = L.PT.upload(in.P.ncols, in.P.nrows, prp.data(), pci.data(), pv.data(), f32)) ||
+          (rc = tune_layout(L.PT, prp.data(), pci.data(), pv.data(), h->stream, zf, true, fl, tune_k)))
+        return rc;
+      // Composite operators: single systems (every row smoothed), levels >= 1, V(1,1).
+      if (l >= 1 && in.nd == in.n && tune_k == 1 && in.nupre == 1 && in.nupost == 1 &&
+          (rc = build_composites(L, in, prp, pci, pv, trp, tci, tv, f32, h->stream, zf)))
         return rc;
       if (!in.omega_rows.empty()) {
         if ((rc = L.omega_rows.alloc(sizeof(double) * in.nd))) return rc;
@@ -604,6 +738,11 @@ int collapse_tail(chr_hier &h) {
   L.GT = DevCsr();
   L.P = DevCsr();
   L.PT = DevCsr();
+  L.R = DevCsr();
+  L.U = DevCsr();
+  L.W = DevCsr();
+  L.S = DevCsr();
+  L.fuse_r = L.fuse_up = L.fuse_s = false;
   h.lev.resize((size_t)c + 1);
   h.nc = m;
   fill_stats(h);
@@ -675,8 +814,10 @@ extern "C" int chr_hier_layout(const chr_hier *h, int32_t level, char which, int
     return fail(CHR_EARG, "chr_hier_layout: level without smoother/transfers");
   const auto &L = h->lev[level];
   const DevCsr *M = which == 'A' ? &L.A : which == 'G' ? &L.G : which == 'T' ? &L.GT
-                  : which == 'P' ? &L.P : which == 'R' ? &L.PT : nullptr;
-  if (!M) return fail(CHR_EARG, "chr_hier_layout: which must be A, G, T, P or R");
+                  : which == 'P' ? &L.P : which == 'R' ? &L.PT
+                  : which == 'r' && L.fuse_r ? &L.R : which == 'u' && L.fuse_up ? &L.U
+                  : which == 'w' && L.fuse_up ? &L.W : which == 's' && L.fuse_s ? &L.S : nullptr;
+  if (!M) return fail(CHR_EARG, "chr_hier_layout: which must be A, G, T, P or R, or a composite (r, u, w, s) the level has");
   if (per) *per = M->use_sell ? 0 : M->use_pack ? -M->pack.per : M->per;
   if (code_bits) *code_bits = M->use_sell ? M->sell.cb : (M->use_pack || M->idx16) ? 16 : 32;
   if (nnz) *nnz = M->nnz;

@@ -203,21 +203,48 @@ __device__ __forceinline__ void stream_products(int e0, int e1, int base, const 
   }
 }
 
-template <typename TA, typename TX>
+// The operand of a product: one vector (XOne), or two for a composite operator (XTwo) whose
+// even columns 2j index row j of vector a and odd columns 2j + 1 row j of vector b (the two
+// column ranges interleaved, so that a row block of the operator spans twice the columns of
+// either part and keeps 16-bit column offsets where the parts do).  Both vectors are
+// row-major n x k blocks; row(j, k) is the address of operand row j.
+template <typename TX>
+struct XOne {
+  using elem = TX;
+  const TX *a;
+  __device__ __forceinline__ const TX *row(int j, int k) const { return a + (size_t)j * k; }
+};
+// b is held as a byte offset from a: the address stays one kernel-argument pointer plus an
+// integer, which the compiler keeps in global (not flat) loads; a select between two
+// pointers would turn every gather into a flat load.
+template <typename TX>
+struct XTwo {
+  using elem = TX;
+  const TX *a;
+  int64_t db;  // (char *)b - (char *)a
+  __host__ __device__ XTwo(const TX *a_, const TX *b_)
+      : a(a_), db(reinterpret_cast<const char *>(b_) - reinterpret_cast<const char *>(a_)) {}
+  __device__ __forceinline__ const TX *row(int j, int k) const {
+    const char *base = reinterpret_cast<const char *>(a) + ((j & 1) ? db : 0);
+    return reinterpret_cast<const TX *>(base) + (size_t)(j >> 1) * k;
+  }
+};
+
+template <typename TA, typename XS>
 struct ColLoad {
-  const TX *__restrict__ x;
-  __device__ __forceinline__ TA operator()(int j) const { return (TA)x[j]; }
+  XS x;
+  __device__ __forceinline__ TA operator()(int j) const { return (TA)*x.row(j, 1); }
 };
 
 // stop: device flag (never null); the kernel does nothing once it is set.  It is read
 // together with the block table so that it adds no latency of its own.
-template <int PER, typename TV, typename TI, typename TX, typename Epi>
+template <int PER, typename TV, typename TI, typename XS, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
                                                              const int4 *__restrict__ blk,
                                                              const int *__restrict__ rp,
                                                              const TI *__restrict__ ci,
                                                              const TV *__restrict__ val,
-                                                             const TX *__restrict__ x, Epi epi,
+                                                             XS x, Epi epi,
                                                              const int *__restrict__ stop) {
   using TA = typename Acc<TV>::type;
   __shared__ TA prod[PER * kBlock];
@@ -227,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   const int r0 = lo.x, r1 = hi.x, e0 = lo.y, e1 = hi.y, base = lo.z;
   const int tid = threadIdx.x;
   if (e1 - e0 > PER * kBlock) {  // one long row: strided sum + block reduction
-    TA s = strided_row_sum<TA>(e0, e1, [&](int e) { return (TA)val[e] * (TA)x[base + (int)ci[e]]; });
+    TA s = strided_row_sum<TA>(e0, e1, [&](int e) { return (TA)val[e] * (TA)*x.row(base + (int)ci[e], 1); });
     s = block_sum_t<TA>(s, prod);
     if (tid == 0) epi(r0, (double)s, epi.pre(r0));
     return;
@@ -237,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   const int rc = min(r, r1 - 1);
   const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
   const double pv = epi.pre(rc);
-  stream_products<PER, TA>(e0, e1, base, ci, val, ColLoad<TA, TX>{x}, prod);
+  stream_products<PER, TA>(e0, e1, base, ci, val, ColLoad<TA, XS>{x}, prod);
   __syncthreads();
   if (r < r1) epi(r, (double)row_sum<TA>(prod, ra, rz), pv);
 }
@@ -465,12 +492,12 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
 // lanes straight from memory, in the same order.  Epilogues see flat indices i*k + c.
 // ---------------------------------------------------------------------------------------
 
-template <int KL, typename TV, typename TI, typename TX, typename Epi>
+template <int KL, typename TV, typename TI, typename XS, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__restrict__ blk,
                                                       const int *__restrict__ rp,
                                                       const TI *__restrict__ ci,
                                                       const TV *__restrict__ val,
-                                                      const TX *__restrict__ x, int k, int cap, Epi epi,
+                                                      XS x, int k, int cap, Epi epi,
                                                       const int *__restrict__ stop) {
   using TA = typename Acc<TV>::type;
   extern __shared__ __align__(16) char smem[];
@@ -485,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__re
     const int i = r0 * k + c;
     const double pv = epi.pre(i);
     TA s = 0;
-    for (int e = rp[r0]; e < rp[r0 + 1]; ++e) s = s + mul_rn((TA)val[e], (TA)x[(base + (int)ci[e]) * k + c]);
+    for (int e = rp[r0]; e < rp[r0 + 1]; ++e) s = s + mul_rn((TA)val[e], (TA)x.row(base + (int)ci[e], k)[c]);
     epi(i, (double)s, pv);
     return;
   }
@@ -512,7 +539,7 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__re
       for (int u = 0; u < 16; ++u) {
         const int qq = q + min(u, n - 1);
         vv[u] = sv[qq];
-        xv[u] = (TA)x[(base + (int)sc[qq]) * k + c];
+        xv[u] = (TA)x.row(base + (int)sc[qq], k)[c];
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u)
@@ -552,14 +579,15 @@ struct Quad<double> {
   }
 };
 
-template <int KL, typename TV, typename TI, typename TX, typename Epi>
+template <int KL, typename TV, typename TI, typename XS, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmm4_kernel(int nblk, const int4 *__restrict__ blk,
                                                        const int *__restrict__ rp,
                                                        const TI *__restrict__ ci,
                                                        const TV *__restrict__ val,
-                                                       const TX *__restrict__ x, int k, int cap, Epi epi,
+                                                       XS x, int k, int cap, Epi epi,
                                                        const int *__restrict__ stop) {
   static_assert(KL % 4 == 0, "four columns per lane");
+  using TX = typename XS::elem;
   using TA = typename Acc<TV>::type;
   constexpr int LR = KL / 4, U = Quad<TX>::kUnroll;
   extern __shared__ __align__(16) char smem[];
@@ -574,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void spmm4_kernel(int nblk, const int4 *__r
     const int i = r0 * k + c;
     const double pv = epi.pre(i);
     TA s = 0;
-    for (int e = rp[r0]; e < rp[r0 + 1]; ++e) s = s + mul_rn((TA)val[e], (TA)x[(base + (int)ci[e]) * k + c]);
+    for (int e = rp[r0]; e < rp[r0 + 1]; ++e) s = s + mul_rn((TA)val[e], (TA)x.row(base + (int)ci[e], k)[c]);
     epi(i, (double)s, pv);
     return;
   }
@@ -603,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void spmm4_kernel(int nblk, const int4 *__r
       for (int u = 0; u < U; ++u) {
         const int qq = q + min(u, n - 1);
         vv[u] = sv[qq];
-        Quad<TX>::load(x + (size_t)(base + (int)sc[qq]) * k + c0, xv[u]);
+        Quad<TX>::load(x.row(base + (int)sc[qq], k) + c0, xv[u]);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)

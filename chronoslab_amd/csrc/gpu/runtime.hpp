@@ -307,30 +307,39 @@ hipError_t launch_pack_p(const DevCsr &M, const TX *x, Epi epi, const int *stop,
   return hipGetLastError();
 }
 
-template <int PER, typename TV, typename TX, typename Epi>
-hipError_t launch_spmv_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+template <int PER, typename TV, typename XS, typename Epi>
+hipError_t launch_spmv_p(const DevCsr &M, XS x, Epi epi, const int *stop, hipStream_t s) {
   if (M.idx16)
-    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, uint16_t, TX, Epi>), dim3(M.nblk),
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, uint16_t, XS, Epi>), dim3(M.nblk),
                        dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
                        M.ci.as<uint16_t>(), M.val.as<TV>(), x, epi, stop);
   else
-    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, int, TX, Epi>), dim3(M.nblk),
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, int, XS, Epi>), dim3(M.nblk),
                        dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
                        M.ci.as<int>(), M.val.as<TV>(), x, epi, stop);
   return hipGetLastError();
 }
 
-template <typename TV, typename TX, typename Epi>
-hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
-                         hipStream_t s) {
+template <typename XS>
+struct is_x_one : std::false_type {};
+template <typename TX>
+struct is_x_one<chr::XOne<TX>> : std::true_type {};
+
+template <typename TV, typename XS, typename Epi>
+hipError_t launch_spmv_t(const DevCsr &M, XS x, Epi epi, const int *stop, hipStream_t s) {
   if (M.nrows == 0) return hipSuccess;
-  if (M.use_sell) return launch_sell<TV>(M.sell, x, epi, stop, s);
-  if (M.use_pack) {
-    switch (M.pack.per) {
-      case 4: return launch_pack_p<4, TV>(M, x, epi, stop, s);
-      case 16: return launch_pack_p<16, TV>(M, x, epi, stop, s);
-      default: return launch_pack_p<8, TV>(M, x, epi, stop, s);
+  if constexpr (is_x_one<XS>::value) {
+    if (M.use_sell) return launch_sell<TV>(M.sell, x.a, epi, stop, s);
+    if (M.use_pack) {
+      switch (M.pack.per) {
+        case 4: return launch_pack_p<4, TV>(M, x.a, epi, stop, s);
+        case 16: return launch_pack_p<16, TV>(M, x.a, epi, stop, s);
+        default: return launch_pack_p<8, TV>(M, x.a, epi, stop, s);
+      }
     }
+  } else {
+    // Composite operators (two operand vectors) are kept in the stream layout only.
+    if (M.use_sell || M.use_pack) return hipErrorInvalidValue;
   }
   switch (M.per) {
     case 4: return launch_spmv_p<4, TV>(M, x, epi, stop, s);
@@ -339,43 +348,62 @@ hipError_t launch_spmv_t(const DevCsr &M, const TX *x, Epi epi, const int *stop,
   }
 }
 
-template <typename TX, typename Epi>
-hipError_t launch_spmv(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+template <typename TX>
+chr::XOne<TX> xsrc(const TX *x) {
+  return chr::XOne<TX>{x};
+}
+template <typename TX>
+chr::XTwo<TX> xsrc(chr::XTwo<TX> x) {
+  return x;
+}
+
+template <typename XP, typename Epi>
+hipError_t launch_spmv(const DevCsr &M, XP xp, Epi epi, const int *stop, hipStream_t s) {
+  const auto x = xsrc(xp);
   return M.fp32 ? launch_spmv_t<float>(M, x, epi, stop, s)
                 : launch_spmv_t<double>(M, x, epi, stop, s);
 }
 
-template <int KL, typename TV, typename TX, typename Epi>
-hipError_t launch_spmm_kl(const DevCsr &M, const TX *x, int k, Epi epi, const int *stop,
+template <typename TX>
+bool aligned16(const chr::XOne<TX> &x) {
+  return reinterpret_cast<uintptr_t>(x.a) % 16 == 0;
+}
+template <typename TX>
+bool aligned16(const chr::XTwo<TX> &x) {
+  return reinterpret_cast<uintptr_t>(x.a) % 16 == 0 && x.db % 16 == 0;
+}
+
+template <int KL, typename TV, typename XS, typename Epi>
+hipError_t launch_spmm_kl(const DevCsr &M, XS x, int k, Epi epi, const int *stop,
                           hipStream_t s) {
   const int cap = M.per * chr::kBlock;  // nonzeros of a stream block (longer: one long row)
   if constexpr (KL >= 4) {
     // Four columns per lane when k allows it and X takes 16-byte loads (same bits).
-    if (k % 4 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 && !std::getenv("CHRONOS_SPMM_SCALAR")) {
+    if (k % 4 == 0 && aligned16(x) && !std::getenv("CHRONOS_SPMM_SCALAR")) {
       if (M.idx16)
-        hipLaunchKernelGGL((chr::spmm4_kernel<KL, TV, uint16_t, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
+        hipLaunchKernelGGL((chr::spmm4_kernel<KL, TV, uint16_t, XS, Epi>), dim3(M.nblk), dim3(chr::kBlock),
                            (size_t)cap * (sizeof(TV) + sizeof(uint16_t)), s, M.nblk, M.blk.as<int4>(),
                            M.rp.as<int>(), M.ci.as<uint16_t>(), M.val.as<TV>(), x, k, cap, epi, stop);
       else
-        hipLaunchKernelGGL((chr::spmm4_kernel<KL, TV, int, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
+        hipLaunchKernelGGL((chr::spmm4_kernel<KL, TV, int, XS, Epi>), dim3(M.nblk), dim3(chr::kBlock),
                            (size_t)cap * (sizeof(TV) + sizeof(int)), s, M.nblk, M.blk.as<int4>(),
                            M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, k, cap, epi, stop);
       return hipGetLastError();
     }
   }
   if (M.idx16)
-    hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, uint16_t, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
+    hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, uint16_t, XS, Epi>), dim3(M.nblk), dim3(chr::kBlock),
                        (size_t)cap * (sizeof(TV) + sizeof(uint16_t)), s, M.nblk, M.blk.as<int4>(),
                        M.rp.as<int>(), M.ci.as<uint16_t>(), M.val.as<TV>(), x, k, cap, epi, stop);
   else
-    hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, int, TX, Epi>), dim3(M.nblk), dim3(chr::kBlock),
+    hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, int, XS, Epi>), dim3(M.nblk), dim3(chr::kBlock),
                        (size_t)cap * (sizeof(TV) + sizeof(int)), s, M.nblk, M.blk.as<int4>(),
                        M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, k, cap, epi, stop);
   return hipGetLastError();
 }
 
-template <typename TV, typename TX, typename Epi>
-hipError_t launch_spmm_t(const DevCsr &M, const TX *x, int k, Epi epi, const int *stop,
+template <typename TV, typename XS, typename Epi>
+hipError_t launch_spmm_t(const DevCsr &M, XS x, int k, Epi epi, const int *stop,
                          hipStream_t s) {
   if (k <= 2) return launch_spmm_kl<2, TV>(M, x, k, epi, stop, s);
   if (k <= 4) return launch_spmm_kl<4, TV>(M, x, k, epi, stop, s);
@@ -384,11 +412,13 @@ hipError_t launch_spmm_t(const DevCsr &M, const TX *x, int k, Epi epi, const int
 }
 
 // Y = M X on row-major n x k blocks (k = 1: the stream kernel; 2 <= k <= 16: the block
-// kernel, whose columns equal the stream kernel's results bit for bit).
-template <typename TX, typename Epi>
-hipError_t launch_op(const DevCsr &M, const TX *x, int k, Epi epi, const int *stop, hipStream_t s) {
-  if (k == 1) return launch_spmv(M, x, epi, stop, s);
+// kernel, whose columns equal the stream kernel's results bit for bit).  x: a vector
+// pointer, or chr::XTwo for a composite operator over two vectors.
+template <typename XP, typename Epi>
+hipError_t launch_op(const DevCsr &M, XP xp, int k, Epi epi, const int *stop, hipStream_t s) {
+  if (k == 1) return launch_spmv(M, xp, epi, stop, s);
   if (M.nrows == 0) return hipSuccess;
+  const auto x = xsrc(xp);
   return M.fp32 ? launch_spmm_t<float>(M, x, k, epi, stop, s)
                 : launch_spmm_t<double>(M, x, k, epi, stop, s);
 }
@@ -400,8 +430,11 @@ constexpr int kMaxBlockCols = 16;
 // the Infinity Cache before cache-cold timings (NULL: every candidate timed back to back).
 // tune_k > 1: the matrix serves block products of that many columns, so the stream blocks'
 // PER is timed with those and the single-vector layouts (packed, sliced ELL) are not built.
+// stream_only: only the stream layout's PER is tuned (composite operators, whose products
+// read two operand vectors, run in the stream layout).
 int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double *hv, hipStream_t s,
-                const int *zero, bool tune_per = true, const DevBuf *flush = nullptr, int tune_k = 1);
+                const int *zero, bool tune_per = true, const DevBuf *flush = nullptr, int tune_k = 1,
+                bool stream_only = false);
 
 uint64_t next_uid();
 
@@ -492,6 +525,19 @@ struct chr_hier {
     DevBuf cmap;        // int4 per direct row: {block first row, m, inverse offset, 0}
     int32_t ncoarse_blocks = 0;
     int64_t coarse_m2 = 0;  // sum of m^2 over the level's direct blocks
+    // Composite operators of a fused level (levels >= 1 of a single-system hierarchy with
+    // one pre- and one post-sweep), formed on the host at upload.  [M1 | M2] applied to
+    // [u; v] is one operator over two vectors, its columns interleaved (M1's column j is 2j,
+    // M2's is 2j + 1; chr::XTwo):
+    //   R = [-P^T A | P^T]          b_c = R [x; b]  (residual + restriction)
+    //   U = [A P | A]               r = b - U [x_c; x]  (prolongation + residual; x itself is
+    //                               updated by W's epilogue)
+    //   W = [P | omega G^T]         out = x + W [x_c; G r]
+    //    or [P | S]                 with S below:  out = x + W [x_c; r]
+    //   S = omega G^T G             x = S b  (pre-smoothing in one product)
+    // They are the same linear maps as the unfused sequence; only the rounding differs.
+    DevCsr R, U, W, S;
+    bool fuse_r = false, fuse_up = false, fuse_s = false;
   };
   int32_t precision = CHR_FP64;
   int device = 0;  // HIP device the hierarchy was created on
@@ -533,8 +579,8 @@ struct Cycle {
   // gathered bytes).
   const float *b0f = nullptr;
 
-  template <typename TX, typename Epi>
-  hipError_t op(const DevCsr &M, const TX *x, Epi epi) {
+  template <typename XP, typename Epi>
+  hipError_t op(const DevCsr &M, XP x, Epi epi) {
     return launch_op(M, x, k, epi, stop, s);
   }
 
@@ -560,6 +606,8 @@ struct Cycle {
     return op(L.GT, y, chr::EpiAxpy<T, TO>{x, out, L.omega, L.omega_rows.as<double>(), k});
   }
 
+  hipError_t fused_level(int l, const T *b, T *out);
+
   template <typename TB, typename TO>
   hipError_t level(int l, const TB *b, TO *out) {
     chr_hier::Level &L = h.lev[l];
@@ -575,6 +623,9 @@ struct Cycle {
     }
     if (L.nd == 0) return hipSuccess;
     T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
+    if constexpr (std::is_same<TB, T>::value && std::is_same<TO, T>::value) {
+      if (L.fuse_r) return fused_level(l, b, out);
+    }
     // Pre-smoothing from a zero initial guess, then nupre-1 further sweeps.
     if (l == 0 && b0f && k == 1 && std::is_same<T, float>::value)
       e = op(L.G, b0f, chr::EpiStore<T>{y});
@@ -604,6 +655,34 @@ struct Cycle {
     return post_sweep(L, b, out);
   }
 };
+
+// The fused sequence of a level with composite operators (see chr_hier::Level), one pre- and
+// one post-sweep, zero initial guess: 4 products with S, 5 without, instead of 8.
+template <typename T>
+hipError_t Cycle<T>::fused_level(int l, const T *b, T *out) {
+  chr_hier::Level &L = h.lev[l];
+  chr_hier::Level &C = h.lev[l + 1];
+  T *x = L.x.as<T>(), *r = L.r.as<T>(), *y = L.y.as<T>();
+  T *bc = C.b.as<T>(), *xc = C.x.as<T>();
+  hipError_t e;
+  if (L.fuse_s) {
+    e = op(L.S, b, chr::EpiStore<T>{x});
+  } else {
+    if ((e = op(L.G, b, chr::EpiStore<T>{y}))) return e;
+    e = op(L.GT, y, chr::EpiScale<T>{x, L.omega, nullptr, k});
+  }
+  if (e) return e;
+  if ((e = op(L.R, chr::XTwo<T>{x, b}, chr::EpiStore<T>{bc}))) return e;
+  if ((e = level<T, T>(l + 1, bc, xc))) return e;
+  if (!L.fuse_up) {
+    if ((e = op(L.P, xc, chr::EpiAxpy<T, T>{x, x, 1.0}))) return e;
+    return post_sweep(L, b, out);
+  }
+  if ((e = op(L.U, chr::XTwo<T>{xc, x}, chr::EpiResidual<T, T>{b, r}))) return e;
+  if (L.fuse_s) return op(L.W, chr::XTwo<T>{xc, r}, chr::EpiAxpy<T, T>{x, out, 1.0});
+  if ((e = op(L.G, r, chr::EpiStore<T>{y}))) return e;
+  return op(L.W, chr::XTwo<T>{xc, y}, chr::EpiAxpy<T, T>{x, out, 1.0});
+}
 
 // Host description of one level for build_hier (views into caller-owned CSR arrays).
 struct LevelIn {

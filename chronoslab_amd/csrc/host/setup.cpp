@@ -1464,6 +1464,48 @@ extern "C" int chs_bc_fix(const chs_mat *Ap, uint64_t seed, int32_t mode, int nt
   return CHS_OK;
 }
 
+// The aFSAI factor's repair for very ill-conditioned rows (smoother.m:59-71): every row whose
+// diagonal entry is NaN becomes the single entry 1/sqrt(a_ii) on the diagonal.
+extern "C" int chs_fsai_nan_repair(const chs_mat *Ap, chs_mat *Gp, int32_t *nfixed) {
+  if (!Ap || !Gp || Ap->nrows != Gp->nrows) return fail(CHS_EARG, "chs_fsai_nan_repair: arguments");
+  const chs_mat &A = *Ap;
+  chs_mat &G = *Gp;
+  std::vector<int32_t> bad;
+  for (int32_t i = 0; i < G.nrows; ++i)
+    for (int32_t e = G.rp[i]; e < G.rp[i + 1]; ++e)
+      if (G.ci[e] == i && std::isnan(G.v[e])) {
+        bad.push_back(i);
+        break;
+      }
+  if (nfixed) *nfixed = (int32_t)bad.size();
+  if (bad.empty()) return CHS_OK;
+  std::vector<int32_t> rp((size_t)G.nrows + 1, 0), ci;
+  std::vector<double> v;
+  ci.reserve(G.ci.size());
+  v.reserve(G.v.size());
+  size_t q = 0;
+  for (int32_t i = 0; i < G.nrows; ++i) {
+    if (q < bad.size() && bad[q] == i) {
+      double aii = 0.0;
+      for (int32_t e = A.rp[i]; e < A.rp[i + 1]; ++e)
+        if (A.ci[e] == i) aii = A.v[e];
+      ci.push_back(i);
+      v.push_back(1.0 / std::sqrt(aii));
+      ++q;
+    } else {
+      for (int32_t e = G.rp[i]; e < G.rp[i + 1]; ++e) {
+        ci.push_back(G.ci[e]);
+        v.push_back(G.v[e]);
+      }
+    }
+    rp[i + 1] = (int32_t)ci.size();
+  }
+  G.rp.swap(rp);
+  G.ci.swap(ci);
+  G.v.swap(v);
+  return CHS_OK;
+}
+
 // ----------------------------------------------------------------------------------------
 // Jacobi smoother factor: G = diag(1/sqrt(a_ii)) (smoother.m:112-132).
 // ----------------------------------------------------------------------------------------
@@ -1726,6 +1768,9 @@ extern "C" int chs_build(const chs_mat *A0, const chs_params *pp, chs_hier **out
         if (!pd) return fail(CHS_ENOTSPD, "afsai_sym: matrix deemed indefinite at level " +
                                               std::to_string(level));
       }
+      int32_t nfix = 0;
+      int rc = chs_fsai_nan_repair(&A, &L.G, &nfix);
+      if (rc) return rc;
     }
     if (p.omega_fn) {
       int rc = p.omega_fn(p.omega_ctx, n, A.rp.data(), A.ci.data(), A.v.data(), L.G.rp.data(),

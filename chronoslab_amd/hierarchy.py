@@ -321,6 +321,15 @@ def afsai(A: Csr, nstep: int = 15, step_size: int = 1, eps: float = 1e-3, nthrea
     return _take(out), bool(pd.value)
 
 
+def fsai_nan_repair(A: Csr, G: Csr) -> tuple[Csr, int]:
+    """smoother.m:59-71: rows of G with a NaN diagonal become diag(1/sqrt(a_ii)); returns
+    (repaired G, rows repaired)."""
+    ha, hg = _to_handle(A), _to_handle(G)
+    nf = C.c_int32()
+    _check(_lib().chs_fsai_nan_repair(ha.ptr, hg.ptr, C.byref(nf)))
+    return _from_ptr(hg.ptr), nf.value
+
+
 def fsai_omega(A: Csr, G: Csr, seed: int = 1, nthreads: int = 0) -> tuple[float, float]:
     ha, hg = _to_handle(A), _to_handle(G)
     om, lam = C.c_double(), C.c_double()

@@ -97,7 +97,9 @@ int chr_hier_destroy(chr_hier *h);
 int chr_hier_get_stats(const chr_hier *h, chr_hier_stats *st);
 
 /* Layout the autotuner chose for the single-vector products of one level matrix (which =
- * 'A','G','T','P','R'): per = nonzeros per thread of the CSR-stream kernel, minus that for
+ * 'A','G','T','P','R' = P^T, or a composite operator of a fused level: 'r' = [-P^T A | P^T],
+ * 'u' = [A P | A], 'w' = [P | omega G^T G] or [P | omega G^T], 's' = omega G^T G; CHR_EARG
+ * when the level has no such composite): per = nonzeros per thread of the CSR-stream kernel, minus that for
  * the packed-code stream kernel, or 0 for the sliced-ELL kernel; code_bits = bits per
  * stored column code (stream: 16 for block offsets, 32 for columns; packed: 16; sliced
  * ELL: 8 for dictionary codes, 16 for row-relative offsets, 32 for columns); nnz =

@@ -270,6 +270,10 @@ int chs_prol_exti(const chs_mat *A, const int8_t *strong, const int32_t *fcnode,
 /* Adaptive symmetric FSAI (lower-triangular G, diagonal last in every row). */
 int chs_afsai(const chs_mat *A, int32_t nstep, int32_t step_size, double eps,
               int nthreads, chs_mat **G, int32_t *is_posdef);
+/* The factor repair of sources/Preconditioner/AMG/smoother/smoother.m:59-71: every row of G
+ * whose diagonal entry is NaN becomes the single diagonal entry 1/sqrt(a_ii) (in place);
+ * nfixed = rows repaired.  chs_build applies it to every aFSAI factor. */
+int chs_fsai_nan_repair(const chs_mat *A, chs_mat *G, int32_t *nfixed);
 /* omega = min(1, 1.9 / lambda_max(G A G^T)) by Lanczos; lambda returned too. */
 int chs_fsai_omega(const chs_mat *A, const chs_mat *G, uint64_t seed, int nthreads,
                    double *omega, double *lambda);

@@ -24,9 +24,11 @@ def _stack(sizes, k, seed):
 
 @pytest.fixture
 def no_tail(monkeypatch):
-    # Batched hierarchies keep every level; the single-system hierarchies they are compared
-    # with bit for bit must not collapse their tails either.
+    # Batched hierarchies keep every level and every product of the V(1,1) sequence; the
+    # single-system hierarchies they are compared with bit for bit must neither collapse
+    # their tails nor fuse their coarse levels into composite operators.
     monkeypatch.setenv("CHRONOS_TAIL_ROWS", "0")
+    monkeypatch.setenv("CHRONOS_FUSE", "0")
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])

@@ -110,3 +110,38 @@ def test_tail_collapse(gpu, monkeypatch, prec):
     _, itf, _, flf = Mfull.pcg(b, itmax=100, tol=1e-8)
     _, itt, _, flt = Mtail.pcg(b, itmax=100, tol=1e-8)
     assert flf == flt == 0 and abs(itf - itt) <= (0 if prec == "fp64" else 1)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fused_levels_against_unfused_and_oracle(gpu, monkeypatch, prec):
+    """Levels >= 1 with composite operators ([-P^T A | P^T], [A P | A], [P | omega G^T G],
+    omega G^T G: the same linear maps as the V(1,1) sequence, other roundings) against the
+    unfused cycle and the oracle: FP64 within 1e-13 of the unfused cycle and 1e-12 of the
+    oracle with equal PCG iterations; FP32 within 1e-5 and one iteration."""
+    import ctypes as C
+
+    import oracle
+
+    A, _ = H.heterogeneous_diffusion(24, 2.0, 5)
+    hier = H.build(A)
+    b = H.rng_uniform(5, 2, A.nrows, -1.0, 1.0)
+    fused = gpu.AMGPreconditioner(hier, prec)
+    monkeypatch.setenv("CHRONOS_FUSE", "0")
+    plain = gpu.AMGPreconditioner(hier, prec)
+    monkeypatch.delenv("CHRONOS_FUSE")
+    # Every level >= 1 below the tail is small here: all four composites are formed.
+    per = C.c_int32()
+    nd = fused.stats()["nlev"] - 1
+    assert nd >= 2
+    for l in range(1, nd):
+        for w in "ruws":
+            assert gpu._lib().chr_hier_layout(fused.ptr, l, w.encode(), C.byref(per), None, None, None) == 0
+        assert gpu._lib().chr_hier_layout(plain.ptr, l, b"r", C.byref(per), None, None, None) != 0
+    xf, xp, xo = fused(b), plain(b), oracle.Hier(hier).vcycle(b)
+    tol_p, tol_o = (1e-13, 1e-12) if prec == "fp64" else (1e-5, 1e-5)
+    assert np.linalg.norm(xf - xp) <= tol_p * np.linalg.norm(xp)
+    assert np.linalg.norm(xf - xo) <= tol_o * np.linalg.norm(xo)
+    _, itf, _, flf = fused.pcg(b, itmax=200, tol=1e-8)
+    _, itp, _, flp = plain.pcg(b, itmax=200, tol=1e-8)
+    assert flf == flp == 0
+    assert itf == itp if prec == "fp64" else abs(itf - itp) <= 1

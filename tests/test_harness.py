@@ -99,3 +99,32 @@ def test_rhs_rand_rhs_spans_six_decades():
     assert e.min() >= -3.0 and e.max() < 3.0
     assert scipy.stats.kstest((e + 3.0) / 6.0, "uniform").pvalue > 1e-3
     assert abs(np.corrcoef(u, e)[0, 1]) < 0.02
+
+
+def test_fsai_nan_diagonal_repair():
+    """smoother.m:59-71: rows of the aFSAI factor whose diagonal is NaN become the single
+    entry 1/sqrt(a_ii); the other rows are untouched.  Invariant kept: diag(G A G^T) = 1 on
+    every row (the aFSAI closed form, and (1/sqrt(a_ii))^2 a_ii = 1 on a repaired row)."""
+    A = H.heterogeneous_diffusion(5, 1.0, 4)[0]
+    G, pd = H.afsai(A)
+    assert pd
+    bad = [3, 40, 77]
+    v = G.val.copy()
+    for i in bad:
+        v[G.rowptr[i + 1] - 1] = np.nan  # the diagonal is each row's last entry
+    Gn = H.Csr(G.nrows, G.ncols, G.rowptr, G.col, v)
+    Gr, nf = H.fsai_nan_repair(A, Gn)
+    assert nf == len(bad)
+    a = A.to_scipy()
+    for i in range(G.nrows):
+        lo, hi = Gr.rowptr[i], Gr.rowptr[i + 1]
+        if i in bad:
+            assert list(Gr.col[lo:hi]) == [i] and Gr.val[lo] == 1.0 / np.sqrt(a[i, i])
+        else:
+            g0, g1 = G.rowptr[i], G.rowptr[i + 1]
+            assert np.array_equal(Gr.col[lo:hi], G.col[g0:g1]) and np.array_equal(Gr.val[lo:hi], G.val[g0:g1])
+    d = (Gr.to_scipy() @ a @ Gr.to_scipy().T).diagonal()
+    np.testing.assert_allclose(d, 1.0, rtol=1e-12)
+    # A factor without NaN comes back unchanged.
+    G2, n2 = H.fsai_nan_repair(A, G)
+    assert n2 == 0 and np.array_equal(G2.val, G.val)
