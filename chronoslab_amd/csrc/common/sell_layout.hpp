@@ -66,16 +66,18 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
   // gathers and stores) close together; large ones cut the padding.
   std::vector<int32_t> ident((size_t)nslots, -1);
   for (int32_t r = 0; r < n; ++r) ident[(size_t)r] = r;
+  // Rows of each sort window ordered by decreasing length, ties in row order (a stable
+  // counting sort: lengths are at most kMaxRow).
   auto sorted_map = [&](int ss) {
     std::vector<int32_t> m((size_t)nslots, -1);
     const int64_t SW = (int64_t)1 << ss;
-    std::vector<int32_t> rows;
+    std::vector<int32_t> start(kMaxRow + 2);
     for (int64_t w0 = 0; w0 < n; w0 += SW) {
       const int64_t w1 = std::min<int64_t>(w0 + SW, n);
-      rows.clear();
-      for (int64_t r = w0; r < w1; ++r) rows.push_back((int32_t)r);
-      std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return rlen(a) > rlen(b); });
-      std::copy(rows.begin(), rows.end(), m.begin() + w0);
+      std::fill(start.begin(), start.end(), 0);
+      for (int64_t r = w0; r < w1; ++r) ++start[(size_t)(kMaxRow - rlen((int32_t)r)) + 1];
+      for (int k = 0; k <= kMaxRow; ++k) start[(size_t)k + 1] += start[(size_t)k];
+      for (int64_t r = w0; r < w1; ++r) m[(size_t)(w0 + start[(size_t)(kMaxRow - rlen((int32_t)r))]++)] = (int32_t)r;
     }
     return m;
   };

@@ -15,10 +15,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -33,6 +35,18 @@ namespace chrrt {
 
 thread_local std::string g_err;
 
+// CHRONOS_UPLOAD_TRACE=1: per-matrix upload and autotuning times on stderr.
+double now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+bool upload_trace() {
+  static const bool on = [] {
+    const char *e = std::getenv("CHRONOS_UPLOAD_TRACE");
+    return e && std::atoi(e) != 0;
+  }();
+  return on;
+}
+
 uint64_t next_uid() {
   static std::atomic<uint64_t> counter{0};
   return ++counter;
@@ -40,11 +54,16 @@ uint64_t next_uid() {
 
 int DevSell::build(int32_t n, int32_t nc, const int32_t *hrp, const int32_t *hci,
                    const double *hv, bool fp32, int sort_shift) {
+  chrsell::Layout L;
+  chrsell::build(n, nc, hrp, hci, hv, fp32 ? 4 : 8, L, sort_shift);
+  return adopt(L, fp32);
+}
+
+int DevSell::adopt(const chrsell::Layout &L, bool fp32) {
   release();
   elems = 0;
-  nrows = n;
-  chrsell::Layout L;
-  if (!chrsell::build(n, nc, hrp, hci, hv, fp32 ? 4 : 8, L, sort_shift)) return CHR_OK;
+  nrows = L.nrows;
+  if (!L.cb) return CHR_OK;
   const size_t vb = fp32 ? 4 : 8, cbytes = (size_t)L.cb / 8;
   int rc;
   if ((rc = sp.alloc(sizeof(int32_t) * L.sp.size())) || (rc = val.alloc(vb * (size_t)L.elems)) ||
@@ -181,6 +200,7 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     if (e) return fail(CHR_EHIP, std::string("autotune: ") + hipGetErrorString(e));
     return CHR_OK;
   };
+  const double t_tune0 = now_seconds();
   // Step 1: stream layout.
   float stream_ms = -1.f;
   if (fixed_per) {
@@ -204,6 +224,7 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
       if (best_per != M.per) rc = M.layout(best_per, hrp, hci);
     }
   }
+  const double t_tune1 = now_seconds();
   if (kt > 1 || stream_only) return rc;  // block products use the stream blocks only
   // Step 2: packed codes (the stream kernel's blocks with 2-byte codes).
   // (Only where 16-bit block offsets do not apply: otherwise the stream layout's index is
@@ -247,8 +268,23 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     float best_ms = sell_mode == 1 ? 1e30f : 0.97f * stream_ms;
     int best_sort = -2, last_sort = -2;  // -2: stream layout
     std::vector<std::pair<int, int>> seen;  // (sorted, sshift) of the layouts built so far
+    // Host layouts: the first candidate alone (an unsorted layout that pads little ends the
+    // search), the others built concurrently once they are needed.
+    auto host_build = [&](int so) {
+      chrsell::Layout Ls;
+      chrsell::build(M.nrows, M.ncols, hrp, hci, hv, M.fp32 ? 4 : 8, Ls, so);
+      return Ls;
+    };
+    std::vector<chrsell::Layout> layouts(sorts.size());
     for (size_t c = 0; c < sorts.size() && !rc; ++c) {
-      if ((rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32, sorts[c]))) break;
+      if (c == 1) {
+        std::vector<std::future<chrsell::Layout>> rest;
+        for (size_t q = 1; q < sorts.size(); ++q) rest.push_back(std::async(std::launch::async, host_build, sorts[q]));
+        for (size_t q = 1; q < sorts.size(); ++q) layouts[q] = rest[q - 1].get();
+      } else if (c == 0) {
+        layouts[0] = host_build(sorts[0]);
+      }
+      if ((rc = M.sell.adopt(layouts[c], M.fp32))) break;
       if (!M.sell.cb) continue;  // not a candidate with this sort window
       last_sort = sorts[c];
       const bool tight = sorts[c] == 0 && (double)M.sell.elems <= 1.12 * (double)M.nnz;
@@ -267,7 +303,12 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
       if (tight) break;
     }
     if (!rc && best_sort != -2) {
-      if (best_sort != last_sort) rc = M.sell.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32, best_sort);
+      if (best_sort != last_sort)
+        for (size_t c = 0; c < sorts.size(); ++c)
+          if (sorts[c] == best_sort) {
+            rc = M.sell.adopt(layouts[c], M.fp32);
+            break;
+          }
       M.use_sell = !rc && M.sell.cb != 0;
       if (M.use_sell) {
         M.use_pack = false;
@@ -283,6 +324,10 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     M.use_pack = false;
     M.pack.release();
   }
+  if (upload_trace())
+    std::fprintf(stderr, "[upload] tune %d x %d nnz %lld: stream %.3f s, pack+sell %.3f s -> %s\n", M.nrows, M.ncols,
+                 (long long)M.nnz, t_tune1 - t_tune0, now_seconds() - t_tune1,
+                 M.use_sell ? "sell" : M.use_pack ? "pack" : "stream");
   return rc;
 }
 
@@ -489,6 +534,50 @@ HostMat hcat_host(const chr_csr &M1, double s1, const chr_csr &M2, double s2) {
   return C;
 }
 
+// Whether R = [-P^T A | P^T] (interleaved columns) can keep 16-bit column offsets, judged
+// before forming it: row c of R reaches the columns of A's rows i in P^T's row c, so its
+// interleaved columns lie in [2 lo_c, 2 hi_c + 1] with lo_c, hi_c the extreme columns of
+// those rows; rows are cut into blocks as the stream layout's smallest candidate does (at
+// most 1024 entries and 256 rows, with row lengths bounded by the sum of those A rows), and
+// every block must span fewer than 65536 columns.
+bool restriction_fits_16bit(const chr_csr &PT, const chr_csr &A) {
+  std::vector<int32_t> amin((size_t)A.nrows), amax((size_t)A.nrows);
+  for (int32_t i = 0; i < A.nrows; ++i) {
+    int32_t lo = i, hi = i;
+    for (int32_t e = A.rowptr[i]; e < A.rowptr[i + 1]; ++e) {
+      lo = std::min(lo, A.col[e]);
+      hi = std::max(hi, A.col[e]);
+    }
+    amin[(size_t)i] = lo;
+    amax[(size_t)i] = hi;
+  }
+  int64_t blo = INT64_MAX, bhi = INT64_MIN, cnt = 0;
+  int32_t rows = 0;
+  for (int32_t c = 0; c < PT.nrows; ++c) {
+    int64_t lo = INT64_MAX, hi = INT64_MIN, len = 0;
+    for (int32_t e = PT.rowptr[c]; e < PT.rowptr[c + 1]; ++e) {
+      const int32_t i = PT.col[e];
+      lo = std::min<int64_t>(lo, amin[(size_t)i]);
+      hi = std::max<int64_t>(hi, amax[(size_t)i]);
+      len += 1 + A.rowptr[i + 1] - A.rowptr[i];
+    }
+    if (rows > 0 && (rows == chr::kBlock || cnt + len > 4 * chr::kBlock)) {
+      blo = INT64_MAX;
+      bhi = INT64_MIN;
+      rows = 0;
+      cnt = 0;
+    }
+    ++rows;
+    cnt += len;
+    if (len > 0) {
+      blo = std::min(blo, 2 * lo);
+      bhi = std::max(bhi, 2 * hi + 1);
+      if (bhi - blo > 65535) return false;
+    }
+  }
+  return true;
+}
+
 // Composite operators of level L (chr_hier::Level).  The residual-and-restriction operator R
 // is formed on every level >= 1 where P^T A has no more nonzeros than A and R keeps 16-bit
 // column offsets (it then streams fewer bytes than A and P^T, and saves a kernel and the
@@ -505,6 +594,7 @@ int build_composites(chr_hier::Level &L, const LevelIn &in, const std::vector<in
   const int64_t nnzA = in.A.rowptr[in.A.nrows];
   const bool small = nnzA < small_nnz;
   const chr_csr PT{in.P.ncols, in.P.nrows, ptrp.data(), ptci.data(), ptv.data()};
+  if (!small && !restriction_fits_16bit(PT, in.A)) return CHR_OK;
   HostMat PtA = spgemm_host(PT, in.A);
   if (!small && (int64_t)PtA.ci.size() > nnzA) return CHR_OK;
   int rc;
@@ -568,9 +658,11 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
   const DevBuf *fl = flush.p ? &flush : nullptr;
   h->lev.resize(nlev);
   std::vector<double> inv_all;  // packed inverses of every direct block
+  const double t_all = now_seconds();
   for (int32_t l = 0; l < nlev; ++l) {
     const LevelIn &in = lv[l];
     auto &L = h->lev[l];
+    const double t_lev = now_seconds();
     L.n = in.n;
     L.nd = in.nd;
     L.omega = in.omega;
@@ -594,9 +686,13 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
           (rc = tune_layout(L.PT, prp.data(), pci.data(), pv.data(), h->stream, zf, true, fl, tune_k)))
         return rc;
       // Composite operators: single systems (every row smoothed), levels >= 1, V(1,1).
+      const double tc = now_seconds();
       if (l >= 1 && in.nd == in.n && tune_k == 1 && in.nupre == 1 && in.nupost == 1 &&
           (rc = build_composites(L, in, prp, pci, pv, trp, tci, tv, f32, h->stream, zf)))
         return rc;
+      if (upload_trace())
+        std::fprintf(stderr, "[upload] level %d: composites %.3f s (r %d u %d s %d)\n", l, now_seconds() - tc,
+                     (int)L.fuse_r, (int)L.fuse_up, (int)L.fuse_s);
       if (!in.omega_rows.empty()) {
         if ((rc = L.omega_rows.alloc(sizeof(double) * in.nd))) return rc;
         HIPC(hipMemcpy(L.omega_rows.p, in.omega_rows.data(), sizeof(double) * in.nd, hipMemcpyHostToDevice));
@@ -627,6 +723,7 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
     }
     if ((rc = L.x.alloc(s * L.n)) || (rc = L.r.alloc(s * L.n)) || (rc = L.y.alloc(s * L.n))) return rc;
     if (l > 0 && (rc = L.b.alloc(s * L.n))) return rc;
+    if (upload_trace()) std::fprintf(stderr, "[upload] level %d (%d rows): %.3f s\n", l, L.n, now_seconds() - t_lev);
   }
   if (!inv_all.empty()) {
     if ((rc = h->Ainv.alloc(sizeof(double) * inv_all.size()))) return rc;
@@ -642,6 +739,7 @@ int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer
   if ((rc = h->io_b.alloc(8 * (size_t)n0))) return rc;
   if ((rc = h->io_x.alloc(8 * (size_t)n0))) return rc;
   fill_stats(*h);
+  if (upload_trace()) std::fprintf(stderr, "[upload] hierarchy: %.3f s\n", now_seconds() - t_all);
   *out = h.release();
   return CHR_OK;
 }
@@ -787,7 +885,10 @@ extern "C" int chr_hier_create(int32_t nlev, const chr_level *levels, int32_t pr
   }
   int rc = build_hier(lv, precision, levels[0].A, out);
   if (rc) return rc;
-  if ((rc = collapse_tail(**out))) {
+  const double t_tail = now_seconds();
+  rc = collapse_tail(**out);
+  if (upload_trace()) std::fprintf(stderr, "[upload] tail collapse: %.3f s\n", now_seconds() - t_tail);
+  if (rc) {
     chr_hier_destroy(*out);
     *out = nullptr;
   }

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "sell_layout.hpp"
 
 namespace chrrt {
 
@@ -140,6 +141,8 @@ struct DevSell {
   // returns CHR_OK) when the matrix is not a candidate.
   int build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *hci, const double *hv,
             bool fp32, int sort_shift = -1);
+  // Uploads a layout built on the host (cb = 0: not a candidate, nothing uploaded).
+  int adopt(const chrsell::Layout &L, bool fp32);
   void release() {
     sp = DevBuf();
     perm = DevBuf();
