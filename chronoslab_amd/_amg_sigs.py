@@ -37,8 +37,8 @@ EXPORTS = {
     "chr_spmv": (C.c_int, [_VP, _VP, _VP, _I32]),
     "chr_pcg_solve": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _F64, _I32, _I32P, _F64P, _I32P]),
     "chr_pcg_solve_noalloc": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _F64, _I32P, _F64P, _I32P]),
-    "chr_lobpcg": (C.c_int, [_VP, _VP, _I32, _I32, _VP, _I64, _I32, _I32, _I32, _F64, _I32, _I32, _I32P, _F64P,
-                             _F64P, _F64P, _I32P]),
+    "chr_lobpcg": (C.c_int, [_VP, _VP, _I32, _I32, _VP, _I64, _VP, _I32, _I64, _I32, _I32, _I32, _F64, _I32, _I32,
+                             _I32P, _F64P, _F64P, _F64P, _I32P]),
     "chr_fv_grad": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _I32]),
     "chr_batch_create": (C.c_int, [_I32, _I32P, C.POINTER(C.POINTER(ChrLevel)), _I32, C.POINTER(_VP)]),
     "chr_batch_destroy": (C.c_int, [_VP]),

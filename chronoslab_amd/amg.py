@@ -316,24 +316,29 @@ class BatchSolver:
 
 
 def lobpcg(prodA, prodM: AMGPreconditioner | None, X0: np.ndarray, largest: bool = False,
-           reslambda_check: bool = False, itmax: int = 100, tol: float = 1e-8, restart_control: int = 7):
-    """[niter, lambda, X, resnorm_vec, lambda_vec, ierr] = lobpcg(prodA, [], prodM, [], X0, ...)
+           reslambda_check: bool = False, itmax: int = 100, tol: float = 1e-8, restart_control: int = 7,
+           Y: np.ndarray | None = None):
+    """[niter, lambda, X, resnorm_vec, lambda_vec, ierr] = lobpcg(prodA, [], prodM, Y, X0, ...)
     (sources/EigenSolver/lobpcg.m:21-23) on the GPU.  prodA: DeviceMatrix, host Csr or None
-    (the preconditioner's level-0 operator); prodM: the uploaded hierarchy or None."""
+    (the preconditioner's level-0 operator); prodM: the uploaded hierarchy or None; Y: the
+    constraint block (n x m, m <= 48) or None."""
     if isinstance(prodA, H.Csr):
         prodA = DeviceMatrix(prodA)
     if prodA is None and prodM is None:
         raise ValueError("lobpcg needs prodA or prodM")
     X = np.asfortranarray(X0, dtype=np.float64).copy(order="F")
     n, neig = X.shape
+    Yf = None if Y is None else np.asfortranarray(Y, dtype=np.float64)
+    m = 0 if Yf is None else Yf.shape[1]
     niter, ierr = C.c_int32(), C.c_int32()
     lam = np.zeros(neig)
     rh = np.zeros((itmax + 1, neig), order="F")
     lh = np.zeros((itmax + 1, neig), order="F")
     _check(_lib().chr_lobpcg(prodM.ptr if prodM is not None else None, prodA.ptr if prodA is not None else None,
-                             n, neig, X.ctypes.data, n, int(largest), int(reslambda_check), itmax, tol,
-                             restart_control, PTR_HOST, C.byref(niter), lam.ctypes.data_as(_F64P),
-                             rh.ctypes.data_as(_F64P), lh.ctypes.data_as(_F64P), C.byref(ierr)), "chr_lobpcg")
+                             n, neig, X.ctypes.data, n, Yf.ctypes.data if Yf is not None else None, m, n,
+                             int(largest), int(reslambda_check), itmax, tol, restart_control, PTR_HOST,
+                             C.byref(niter), lam.ctypes.data_as(_F64P), rh.ctypes.data_as(_F64P),
+                             lh.ctypes.data_as(_F64P), C.byref(ierr)), "chr_lobpcg")
     k = niter.value + 1
     return niter.value, lam, X, rh[:k], lh[:k], ierr.value
 
@@ -351,9 +356,9 @@ def lobpcg_device(prodA, prodM: AMGPreconditioner | None, n: int, neig: int, x_p
     rh = np.zeros((itmax + 1, neig), order="F")
     lh = np.zeros((itmax + 1, neig), order="F")
     _check(_lib().chr_lobpcg(prodM.ptr if prodM is not None else None, prodA.ptr if prodA is not None else None,
-                             n, neig, x_ptr, n, int(largest), int(reslambda_check), itmax, tol, restart_control,
-                             PTR_DEVICE, C.byref(niter), lam.ctypes.data_as(_F64P), rh.ctypes.data_as(_F64P),
-                             lh.ctypes.data_as(_F64P), C.byref(ierr)), "chr_lobpcg")
+                             n, neig, x_ptr, n, None, 0, n, int(largest), int(reslambda_check), itmax, tol,
+                             restart_control, PTR_DEVICE, C.byref(niter), lam.ctypes.data_as(_F64P),
+                             rh.ctypes.data_as(_F64P), lh.ctypes.data_as(_F64P), C.byref(ierr)), "chr_lobpcg")
     k = niter.value + 1
     return niter.value, lam, rh[:k], lh[:k], ierr.value
 

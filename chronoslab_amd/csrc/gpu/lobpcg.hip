@@ -1,5 +1,5 @@
-// LOBPCG (sources/EigenSolver/lobpcg.m:21-467 with B_orthonormalize.m:1-25, prodB = []
-// and no constraint block, as EigDriver.m:202-226 calls it) on the GPU.
+// LOBPCG (sources/EigenSolver/lobpcg.m:21-467 with B_orthonormalize.m:1-25, prodB = [] and
+// an optional constraint block Y, as EigDriver.m:202-226 calls it) on the GPU.
 //
 // The n x k blocks live in HBM as row-major FP64 blocks.  Products with A are block SpMMs,
 // the preconditioner is the block V-cycle, and the Gram matrices are tall-skinny products
@@ -234,6 +234,89 @@ struct Lob {
     HIPC(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToDevice, s));
     return CHR_OK;
   }
+  // Constraint block Y (lobpcg.m:64-75 with prodB = []): row-major chunks of at most 16
+  // columns, and Uy (column-major upper) with Uy^T Uy = Y^T Y.
+  std::vector<DevBuf> ych;
+  std::vector<int> ycw;
+  int ym = 0;
+  std::vector<double> Uy;
+  // Y: device, column-major n x m with leading dimension ld.  Returns CHR_EARG when Y^T Y is
+  // not positive definite (MATLAB's chol fails there).
+  int set_constraint(const double *Y, int64_t ld, int m) {
+    ym = m;
+    ych.clear();
+    ycw.clear();
+    for (int c0 = 0; c0 < m; c0 += 16) {
+      const int w = std::min(16, m - c0);
+      ych.emplace_back();
+      int rc;
+      if ((rc = ych.back().alloc(sizeof(double) * (size_t)n * w))) return rc;
+      hipLaunchKernelGGL(chr::cm_to_rm_kernel, dim3(grid((int64_t)n * w)), dim3(chr::kBlock), 0, s, n, w,
+                         Y + (size_t)c0 * ld, ld, ych.back().as<double>());
+      HIPC(hipGetLastError());
+      ycw.push_back(w);
+    }
+    const int nc = (int)ych.size();
+    std::vector<int> slot((size_t)nc * nc, -1);
+    for (int a = 0; a < nc; ++a)
+      for (int b = a; b < nc; ++b)
+        if ((slot[(size_t)a * nc + b] = gram_launch(ych[a].as<double>(), ycw[a], ych[b].as<double>(), ycw[b])) < 0)
+          return CHR_EARG;
+    int rc;
+    if ((rc = fetch())) return rc;
+    std::vector<double> M((size_t)m * m), g;
+    for (int a = 0, ra = 0; a < nc; ra += ycw[a++])
+      for (int b = a, rb = ra; b < nc; rb += ycw[b++]) {
+        get(slot[(size_t)a * nc + b], ycw[a], ycw[b], g);
+        for (int q = 0; q < ycw[b]; ++q)
+          for (int p = 0; p < ycw[a]; ++p) {
+            M[(size_t)(ra + p) + (size_t)(rb + q) * m] = g[p + (size_t)q * ycw[a]];
+            M[(size_t)(rb + q) + (size_t)(ra + p) * m] = g[p + (size_t)q * ycw[a]];
+          }
+      }
+    if (!chrdense::chol_lower(m, M.data())) return fail(CHR_EARG, "chr_lobpcg: the constraint block Y is rank deficient");
+    Uy.assign((size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i <= j; ++i) Uy[i + (size_t)j * m] = M[j + (size_t)i * m];
+    return CHR_OK;
+  }
+  // X <- X - Y Uy^-1 Uy^-T (Y^T X) for an n x k block (orthog_Y of lobpcg.m:74).
+  int orthog(double *X, int k) {
+    if (ym == 0) return CHR_OK;
+    const int nc = (int)ych.size();
+    std::vector<int> slot(nc);
+    for (int c = 0; c < nc; ++c)
+      if ((slot[c] = gram_launch(ych[c].as<double>(), ycw[c], X, k)) < 0) return CHR_EARG;
+    int rc;
+    if ((rc = fetch())) return rc;
+    std::vector<double> C((size_t)ym * k), g;
+    for (int c = 0, r0 = 0; c < nc; r0 += ycw[c++]) {
+      get(slot[c], ycw[c], k, g);
+      for (int q = 0; q < k; ++q)
+        for (int p = 0; p < ycw[c]; ++p) C[(size_t)(r0 + p) + (size_t)q * ym] = g[p + (size_t)q * ycw[c]];
+    }
+    for (int q = 0; q < k; ++q) {
+      double *cq = C.data() + (size_t)q * ym;
+      for (int i = 0; i < ym; ++i) {  // Uy^T z = c
+        double t = cq[i];
+        for (int j = 0; j < i; ++j) t -= Uy[j + (size_t)i * ym] * cq[j];
+        cq[i] = t / Uy[i + (size_t)i * ym];
+      }
+      for (int i = ym - 1; i >= 0; --i) {  // Uy w = z
+        double t = cq[i];
+        for (int j = i + 1; j < ym; ++j) t -= Uy[i + (size_t)j * ym] * cq[j];
+        cq[i] = t / Uy[i + (size_t)i * ym];
+      }
+    }
+    std::vector<double> Wc;
+    for (int c = 0, r0 = 0; c < nc; r0 += ycw[c++]) {
+      Wc.assign((size_t)ycw[c] * k, 0.0);
+      for (int q = 0; q < k; ++q)
+        for (int p = 0; p < ycw[c]; ++p) Wc[p + (size_t)q * ycw[c]] = -C[(size_t)(r0 + p) + (size_t)q * ym];
+      if ((rc = gemm(ych[c].as<double>(), ycw[c], Wc.data(), k, 1.0, X))) return rc;
+    }
+    return CHR_OK;
+  }
   // B_orthonormalize.m with prodB = []: X <- X U^-1 with U^T U = X^T X (formed in tmp,
   // then the pointers swap); returns 1 when cond(X^T X) > 1/(k eps) (the reference's
   // failure), U (column-major upper) otherwise.
@@ -260,12 +343,14 @@ struct Lob {
 }  // namespace
 
 extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, double *X, int64_t ldx,
-                          int32_t largest, int32_t reslambda_check, int32_t itmax, double tol,
-                          int32_t restart_control, int32_t ptr_kind, int32_t *niter_out,
-                          double *lambda_out, double *resnorm_hist, double *lambda_hist,
+                          const double *Y, int32_t m_y, int64_t ldy, int32_t largest, int32_t reslambda_check,
+                          int32_t itmax, double tol, int32_t restart_control, int32_t ptr_kind,
+                          int32_t *niter_out, double *lambda_out, double *resnorm_hist, double *lambda_hist,
                           int32_t *ierr_out) {
   if (!X || neig < 1 || neig > kMaxBlockCols || itmax < 2 || ldx < n || (int64_t)n < 3 * (int64_t)neig)
     return fail(CHR_EARG, "chr_lobpcg: need 1 <= neig <= 16, itmax >= 2, n >= 3*neig, ldx >= n");
+  if (m_y < 0 || m_y > 48 || (m_y > 0 && (!Y || ldy < n)))
+    return fail(CHR_EARG, "chr_lobpcg: the constraint block needs 0 <= m <= 48 columns and ldy >= n");
   if (!h && !Aop) return fail(CHR_EARG, "chr_lobpcg: need a hierarchy or an operator");
   const DevCsr *A = Aop ? &Aop->A : &h->outer.A;
   if (A->nrows != n || A->ncols != n || (h && h->lev[0].n != n))
@@ -329,6 +414,18 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     return 0;
   };
   if ((rc = L.copy(dBest, dX, neig))) return rc;
+  if (m_y > 0) {  // the constraint block, then X0 projected off it (lobpcg.m:64-75)
+    DevBuf ydev;
+    const double *ysrc = Y;
+    int64_t ly = ldy;
+    if (ptr_kind == CHR_PTR_HOST) {
+      if ((rc = ydev.alloc(sizeof(double) * (size_t)n * m_y))) return rc;
+      HIPC(copy_cols(ydev.p, 8 * (size_t)n, Y, 8 * (size_t)ldy, 8 * (size_t)n, m_y, hipMemcpyHostToDevice, s));
+      ysrc = ydev.as<double>();
+      ly = n;
+    }
+    if ((rc = L.set_constraint(ysrc, ly, m_y)) || (rc = L.orthog(dX, neig))) return rc;
+  }
   if ((rc = L.borth(dX, neig, U, dT, failed))) return rc;
   if (failed) {
     ierr = 1;
@@ -373,6 +470,7 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
       if ((rc = L.gather(dP, neig, act, dAPa)) || (rc = L.gather(dAP, neig, act, dAAP))) return rc;
     }
     if ((rc = L.applyM(dAR, na, dT))) return rc;
+    if ((rc = L.orthog(dAR, na))) return rc;  // lobpcg.m:228-230
     // AR <- AR - X (X^T AR)
     std::vector<double> xar;
     if ((rc = L.gram(dX, neig, dAR, na, xar))) return rc;
@@ -505,6 +603,7 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     }
     // Final Rayleigh-Ritz without preconditioning (lobpcg.m:420-459).
     ++iter;
+    if ((rc = L.orthog(dX, neig))) return rc;  // lobpcg.m:384-387
     if ((rc = L.applyA(dX, neig, dAX))) return rc;
     int r = rayleigh_ritz_X();
     if (r < 0) return r;

@@ -142,18 +142,21 @@ int chr_pcg_solve(chr_hier *h, chr_mat *A, const double *b, double *x, int32_t i
 int chr_pcg_solve_noalloc(chr_hier *h, chr_mat *A, const double *b, double *x, int32_t itmax,
                           double tol, int32_t *iter, double *resvec, int32_t *flag);
 
-/* lobpcg (sources/EigenSolver/lobpcg.m:21-467, B_orthonormalize.m:1-25) with prodB = [] and
- * no constraint block, as sources/Drivers/driver_Eigen/EigDriver.m:202-226 calls it:
- * prodA = A (NULL: the level-0 operator of h), prodM = V-cycle of h (h NULL: no
- * preconditioner, A required).  X: n x neig column-major (ldx), X0 on entry and the
- * eigenvectors on exit (neig <= 16).  lambda[neig]; resnorm_hist and lambda_hist, when not
+/* lobpcg (sources/EigenSolver/lobpcg.m:21-467, B_orthonormalize.m:1-25) with prodB = [], as
+ * sources/Drivers/driver_Eigen/EigDriver.m:202-226 calls it (the reference has no caller with
+ * a non-empty prodB): prodA = A (NULL: the level-0 operator of h), prodM = V-cycle of h (h
+ * NULL: no preconditioner, A required).  X: n x neig column-major (ldx), X0 on entry and the
+ * eigenvectors on exit (neig <= 16).  Y: the constraint block blockY, n x m column-major
+ * (ldy), m <= 48 (m = 0 or Y NULL: none): X0, the preconditioned residuals and the final X
+ * are projected off it as lobpcg.m:64-75,228-230,384-387 do; CHR_EARG when Y^T Y is not
+ * positive definite.  lambda[neig]; resnorm_hist and lambda_hist, when not
  * NULL, are (itmax+1) x neig column-major histories.  ierr: 0 converged, -1 not converged
  * (X = best iterate), 1 X0 or a Rayleigh-Ritz step rank deficient, 2 B-orthonormalisation
  * of the residual block failed (lobpcg.m:86-89,242-246,401-414). */
 int chr_lobpcg(chr_hier *h, chr_mat *A, int32_t n, int32_t neig, double *X, int64_t ldx,
-               int32_t largest, int32_t reslambda_check, int32_t itmax, double tol,
-               int32_t restart_control, int32_t ptr_kind, int32_t *niter, double *lambda,
-               double *resnorm_hist, double *lambda_hist, int32_t *ierr);
+               const double *Y, int32_t m, int64_t ldy, int32_t largest, int32_t reslambda_check,
+               int32_t itmax, double tol, int32_t restart_control, int32_t ptr_kind, int32_t *niter,
+               double *lambda, double *resnorm_hist, double *lambda_hist, int32_t *ierr);
 
 /* Adjoint gradient of a scalar loss for the cell-centred finite-volume diffusion operator
  * (harmonic face transmissibilities T_ij = 2 k_i k_j/(k_i+k_j) = -A_ij, Dirichlet terms

@@ -83,9 +83,9 @@ def lib():
         L.orc_vcycle.argtypes = [C.c_void_p, C.c_int32, _F64P, C.c_int64, _F64P, C.c_int64]
         L.orc_pcg.argtypes = [C.c_void_p, C.POINTER(OrcCsr), _F64P, _F64P, C.c_int32, C.c_double,
                               _I32P, _F64P, _I32P]
-        L.orc_lobpcg.argtypes = [C.POINTER(OrcCsr), C.c_void_p, _F64P, C.c_int32, C.c_int32, C.c_int32,
-                                 C.c_int32, C.c_int32, C.c_double, C.c_int32, _I32P, _F64P, _F64P, _F64P,
-                                 _I32P]
+        L.orc_lobpcg.argtypes = [C.POINTER(OrcCsr), C.c_void_p, _F64P, _F64P, C.c_int32, C.c_int32, C.c_int32,
+                                 C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, _I32P, _F64P, _F64P,
+                                 _F64P, _I32P]
         L.orc_fv_grad.argtypes = [C.c_int32, _F64P, _F64P, _F64P, _F64P]
         L.orc_sygv.argtypes = [C.c_int32, _F64P, _F64P, _F64P, _F64P]
         _lib = L
@@ -170,16 +170,20 @@ def spmv(A, x: np.ndarray) -> np.ndarray:
 
 
 def lobpcg(A, hier: "Hier | None", X0: np.ndarray, largest: bool = False, reslambda_check: bool = False,
-           itmax: int = 100, tol: float = 1e-8, restart_control: int = 7):
+           itmax: int = 100, tol: float = 1e-8, restart_control: int = 7, Y: np.ndarray | None = None):
+    """lobpcg(prodA, [], prodM, Y, X0, ...); Y: the constraint block (n x m) or None."""
     keep = []
     Ac = _csr(A, keep)
     X = np.asfortranarray(X0, dtype=np.float64).copy(order="F")
     n, neig = X.shape
+    Yf = None if Y is None else np.asfortranarray(Y, dtype=np.float64)
+    m = 0 if Yf is None else Yf.shape[1]
     niter, ierr = C.c_int32(), C.c_int32()
     lam = np.zeros(neig)
     rh = np.zeros((itmax + 1, neig), order="F")
     lh = np.zeros((itmax + 1, neig), order="F")
-    rc = lib().orc_lobpcg(C.byref(Ac), hier.ptr if hier is not None else None, _p(X, _F64P), n, neig,
+    rc = lib().orc_lobpcg(C.byref(Ac), hier.ptr if hier is not None else None, _p(X, _F64P),
+                          _p(Yf, _F64P) if Yf is not None else None, m, n, neig,
                           int(largest), int(reslambda_check), itmax, tol, restart_control, C.byref(niter),
                           _p(lam, _F64P), _p(rh, _F64P), _p(lh, _F64P), C.byref(ierr))
     if rc != 0:

@@ -523,13 +523,53 @@ static void apply_M_block(const orc_hier *h, int64_t n, int32_t k, double *X, do
   }
 }
 
-/* lobpcg.m:21-467 without constraints Y and with prodB = []. */
-int orc_lobpcg(const orc_csr *A, const orc_hier *h, double *X, int32_t n_, int32_t neig,
-               int32_t largest, int32_t reslambda_check, int32_t itmax, double tol,
-               int32_t restart_control, int32_t *niter_out, double *lambda_out,
+/* The constraint projector of lobpcg.m:64-75 with prodB = []: with U^T U = Y^T Y (chol),
+ * X <- X - Y U^-1 U^-T (Y^T X).  Y: n x m column-major, Uy: m x m column-major upper. */
+static void orthog_y(int64_t n, int32_t m, const double *Y, const double *Uy, int32_t k, double *X,
+                     double *tmp) {
+  double *C = (double *)malloc(sizeof(double) * m * k);
+  gemm_tn(n, m, Y, k, X, C);
+  for (int32_t j = 0; j < k; ++j) {
+    double *c = C + (int64_t)j * m;
+    for (int32_t i = 0; i < m; ++i) { /* U^T z = c */
+      double s = c[i];
+      for (int32_t q = 0; q < i; ++q) s -= Uy[q + i * m] * c[q];
+      c[i] = s / Uy[i + i * m];
+    }
+    for (int32_t i = m - 1; i >= 0; --i) { /* U w = z */
+      double s = c[i];
+      for (int32_t q = i + 1; q < m; ++q) s -= Uy[i + q * m] * c[q];
+      c[i] = s / Uy[i + i * m];
+    }
+  }
+  gemm_nn(n, m, Y, k, C, tmp);
+  for (int64_t i = 0; i < n * k; ++i) X[i] -= tmp[i];
+  free(C);
+}
+
+/* lobpcg.m:21-467 with prodB = []; Y (n x m column-major, m = 0: none) is the constraint
+ * block: X0, the preconditioned residuals and the final X are projected B-orthogonally to it
+ * (lobpcg.m:64-75, 228-230, 384-387). */
+int orc_lobpcg(const orc_csr *A, const orc_hier *h, double *X, const double *Y, int32_t m_y,
+               int32_t n_, int32_t neig, int32_t largest, int32_t reslambda_check, int32_t itmax,
+               double tol, int32_t restart_control, int32_t *niter_out, double *lambda_out,
                double *resnorm_hist, double *lambda_hist, int32_t *ierr_out) {
   const int64_t n = n_;
-  if (!A || !X || neig < 1 || itmax < 2 || n < 3 * (int64_t)neig) return -1;
+  if (!A || !X || neig < 1 || itmax < 2 || n < 3 * (int64_t)neig || m_y < 0 || (m_y > 0 && !Y)) return -1;
+  double *Uy = NULL;
+  if (m_y > 0) { /* YBY = Y^T Y; U_YBY = chol(YBY) (upper) */
+    Uy = (double *)malloc(sizeof(double) * m_y * m_y);
+    double *M = (double *)malloc(sizeof(double) * m_y * m_y);
+    gemm_tn(n, m_y, Y, m_y, Y, M);
+    if (chol_dense(m_y, M)) {
+      free(M);
+      free(Uy);
+      return -2;
+    }
+    for (int32_t j = 0; j < m_y; ++j)
+      for (int32_t i = 0; i < m_y; ++i) Uy[i + j * m_y] = (i <= j) ? M[j + i * m_y] : 0.0;
+    free(M);
+  }
   const size_t blk = sizeof(double) * n * neig;
   double *AX = (double *)malloc(blk), *P = (double *)calloc(n * neig, sizeof(double));
   double *AP = (double *)calloc(n * neig, sizeof(double)), *R = (double *)malloc(blk);
@@ -555,6 +595,7 @@ int orc_lobpcg(const orc_csr *A, const orc_hier *h, double *X, int32_t n_, int32
 #define HIST(hist, it, j) if (hist) (hist)[(int64_t)(j) * (itmax + 1) + (it)]
   double smallest = DBL_MAX;
   memcpy(best, X, blk);
+  if (m_y > 0) orthog_y(n, m_y, Y, Uy, neig, X, tmp);
   if (b_orthonormalize(n, neig, X, U)) {
     ierr = 1;
     goto done;
@@ -604,6 +645,7 @@ int orc_lobpcg(const orc_csr *A, const orc_hier *h, double *X, int32_t n_, int32
       }
     }
     apply_M_block(h, n, na, AR, vtmp);
+    if (m_y > 0) orthog_y(n, m_y, Y, Uy, na, AR, tmp);
     /* AR <- AR - X (X^T AR) */
     gemm_tn(n, neig, X, na, AR, Csm);
     gemm_nn(n, neig, X, na, Csm, tmp);
@@ -703,6 +745,7 @@ int orc_lobpcg(const orc_csr *A, const orc_hier *h, double *X, int32_t n_, int32
   }
   /* Final Rayleigh-Ritz without preconditioning (lobpcg.m:420-459). */
   ++iter;
+  if (m_y > 0) orthog_y(n, m_y, Y, Uy, neig, X, tmp);
   apply_A_block(A, n, neig, X, AX);
   gemm_tn(n, neig, X, neig, AX, gA);
   symmetrize(neig, gA);
@@ -724,6 +767,7 @@ done:
   if (ierr_out) *ierr_out = ierr;
   free(AX); free(P); free(AP); free(R); free(AR); free(AAR); free(APa); free(AAP);
   free(best); free(tmp); free(vtmp); free(gA); free(gB); free(E); free(wall); free(Csm); free(U);
+  free(Uy);
 #undef HIST
   return 0;
 }

@@ -10,7 +10,8 @@
  *                  closures of cpt_aspAMG.m:301-302 and the coarsest solve of :103-131
  *   orc_pcg        sources/LinearSolver/PCG_CJ.m:1-73
  *   orc_lobpcg     sources/EigenSolver/lobpcg.m:21-467 + B_orthonormalize.m:1-25
- *                  (prodB = [], as in sources/Drivers/driver_Eigen/EigDriver.m:202-226)
+ *                  (prodB = [] and an optional constraint block Y, as
+ *                  sources/Drivers/driver_Eigen/EigDriver.m:202-226 calls it)
  *   orc_fv_grad    no reference counterpart: gradient of 0.5||x - x_obs||^2 with respect to
  *                  ln k for the finite-volume diffusion operator, from the adjoint field.
  * All dense vectors are column-major n x k blocks (MATLAB layout).
@@ -48,9 +49,11 @@ int orc_pcg(const orc_hier *h, const orc_csr *A, const double *b, double *x, int
             double tol, int32_t *iter, double *resvec, int32_t *flag);
 /* prodM = V-cycle of h, or the identity when h is NULL.  X (n x neig) holds X0 on entry
  * and the eigenvectors on exit; lambda[neig]; histories may be NULL. */
-int orc_lobpcg(const orc_csr *A, const orc_hier *h, double *X, int32_t n, int32_t neig,
-               int32_t largest, int32_t reslambda_check, int32_t itmax, double tol,
-               int32_t restart_control, int32_t *niter, double *lambda,
+/* Y: the constraint block, n x m column-major (m = 0: none); returns -2 when Y^T Y is not
+ * positive definite. */
+int orc_lobpcg(const orc_csr *A, const orc_hier *h, double *X, const double *Y, int32_t m,
+               int32_t n, int32_t neig, int32_t largest, int32_t reslambda_check, int32_t itmax,
+               double tol, int32_t restart_control, int32_t *niter, double *lambda,
                double *resnorm_hist, double *lambda_hist, int32_t *ierr);
 /* dL/d(ln k) for the 7-point finite-volume operator: A_ij = -2 k_i k_j/(k_i+k_j), plus
  * Dirichlet terms 2 k_i on the x-faces, with the adjoint lam solving A lam = x - x_obs. */

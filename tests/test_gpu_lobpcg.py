@@ -89,3 +89,25 @@ def test_lobpcg_96cube_16_pairs(gpu):
     print(f"96^3 LOBPCG: {it} iterations, {dt:.2f} s, ierr {ierr}, max eig rel err {err.max():.2e}", flush=True)
     assert ierr == 0
     assert err.max() <= 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lobpcg_constraint_block_against_oracle(gpu, prec):
+    """The constraint block of EigDriver.m:213-223 (blockY, lobpcg.m:64-75,228-230,384-387):
+    Y spans the eigenvectors of the four smallest eigenvalues of the 12^3 Poisson matrix, so
+    the constrained solve finds the next three (closed form, numpy.linalg.eigh), with X
+    orthogonal to Y; eigenvalues within 1e-10 of the oracle's and iterations within one."""
+    A = H.poisson7(12)
+    w, V = np.linalg.eigh(A.to_scipy().toarray())
+    Y = V[:, :4]
+    M = gpu.AMGPreconditioner(H.build(A), prec)
+    X0 = x0(A.nrows, 3, 21)
+    it, lam, X, rh, lh, ierr = gpu.lobpcg(None, M, X0, itmax=200, tol=1e-9, Y=Y)
+    ito, lamo, Xo, rho, lho, ierro = oracle.lobpcg(A, oracle.Hier(M.hier), X0, itmax=200, tol=1e-9, Y=Y)
+    assert ierr == 0 and ierro == 0
+    np.testing.assert_allclose(lam, w[4:7], rtol=1e-9)
+    np.testing.assert_allclose(lam, lamo, rtol=1e-10)
+    assert abs(it - ito) <= 1
+    assert np.abs(Y.T @ X).max() <= 1e-9
+    with pytest.raises(RuntimeError, match="rank deficient"):
+        gpu.lobpcg(None, M, X0, itmax=20, tol=1e-9, Y=np.stack([Y[:, 0], Y[:, 0]], 1))

@@ -181,3 +181,25 @@ def test_fv_grad_central_differences():
         e[c] = h
         fd = (loss(lnk + e) - loss(lnk - e)) / (2 * h)
         assert g[c] == pytest.approx(fd, rel=1e-5, abs=1e-12)
+
+
+def test_lobpcg_constraint_block_gives_the_next_eigenpairs():
+    """With Y spanning the eigenvectors of the four smallest eigenvalues of the 8^3 Poisson
+    matrix (lambda_1 and the threefold lambda_2..4, so Y spans an invariant subspace), the
+    constrained LOBPCG (lobpcg.m:64-75,228-230,384-387) finds the next eigenvalues
+    (numpy.linalg.eigh: the threefold lambda_5..7) and returns X orthogonal to Y."""
+    A = H.poisson7(8)
+    w, V = np.linalg.eigh(A.to_scipy().toarray())
+    Y = V[:, :4]
+    X0 = np.asfortranarray(np.stack([H.rng_uniform(3, 50 + c, A.nrows) for c in range(3)], 1))
+    it, lam, X, rh, lh, ierr = oracle.lobpcg(A, None, X0, itmax=200, tol=1e-9, Y=Y)
+    assert ierr == 0
+    np.testing.assert_allclose(lam, w[4:7], rtol=1e-10)
+    assert np.abs(Y.T @ X).max() <= 1e-10
+    # Without the constraint the same start finds the smallest three.
+    _, lam0, _, _, _, ierr0 = oracle.lobpcg(A, None, X0, itmax=200, tol=1e-9)
+    assert ierr0 == 0
+    np.testing.assert_allclose(lam0, w[:3], rtol=1e-10)
+    # A rank-deficient Y (Y^T Y singular) is refused.
+    with pytest.raises(RuntimeError):
+        oracle.lobpcg(A, None, X0, itmax=20, tol=1e-9, Y=np.stack([Y[:, 0], Y[:, 0]], 1))
