@@ -458,6 +458,88 @@ class SetupParams:
     nl_agg: int = 0
 
 
+def _parm_lines(path: str, kinds: str) -> list:
+    """The first token of each of the first len(kinds) lines of a parameter file, as a number
+    ('f') or a string ('s'); the rest of each line is a comment (textscan of fgetl)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    if len(lines) < len(kinds):
+        raise ValueError(f"{path}: expected {len(kinds)} parameter lines")
+    out = []
+    for ln, k in zip(lines, kinds):
+        tok = ln.split()
+        if not tok:
+            raise ValueError(f"{path}: empty parameter line")
+        out.append(float(tok[0]) if k == "f" else tok[0])
+    return out
+
+
+def read_default_params(directory: str, fnames: str = "aspAMG.fnames") -> SetupParams:
+    """readDefaultParams (sources/utils/readDefaultParams.m:1-30): the file list (one name per
+    line: AMG, SMOOTH, TSPACE, COARSEN, PROLONG, FILTER) and the readers read_amg.m,
+    read_smoother.m, read_tspace.m, read_coarsen.m, read_prolong.m and read_filter.m, into
+    SetupParams.  The reference overwrites nthread/np with maxNumCompThreads; here the OpenMP
+    default applies (nthreads = 0).  Options outside the restated set-up are rejected:
+    smoother afsai_nsy, strength AFF/CLASNY/ALG, HYBC prolongation, test-space methods other
+    than NONE, EMIN variants other than prec_emin 1 / solv_emin 4, updateCF other than 0, and
+    active filters (prolongation filter weight below 100, coarse-operator filter tau above 0)."""
+    with open(os.path.join(directory, fnames)) as f:
+        names = [ln.split()[0] for ln in f.read().splitlines()[:6]]
+    if len(names) < 6:
+        raise ValueError(f"{fnames}: expected six parameter file names")
+    path = lambda i: os.path.join(directory, names[i])
+    nlev_max, max_coarse = _parm_lines(path(0), "ff")
+    _nthread, nupre, nupost, nstep, step_size, eps, method = _parm_lines(path(1), "ffffffs")
+    ntv, _it_tv, _tol_tv, tv_method, _init = _parm_lines(path(2), "fffsf")
+    soc_type, tau, nl_agg = _parm_lines(path(3), "sff")
+    pr = _parm_lines(path(4), "ssffffffffffffffffffff")
+    _np, filt_wgt, _filt_tol, filt_tau, _min_patt = _parm_lines(path(5), "fffff")
+    proltype, prol_emin = pr[0].upper(), pr[1].upper()
+    if proltype not in ("BAMG", "EXTI", "CLAS", "HYBC") or prol_emin not in ("EMIN", "SMOOTH", "NONE"):
+        raise ValueError(f"Wrong proltype flag: {pr[0]} / {pr[1]}")  # read_prolong.m:31-38
+    (itmax_vol, tol_vol, dist_min, dist_max, maxcond, maxrownrm, eps_prol, update_cf, patt_pow, patt_tau,
+     nnzr_max, itmax_emin, energ_tol, condmax_emin, prec_emin, solv_emin) = pr[3:19]
+    if prol_emin == "EMIN" and not (1 <= prec_emin <= 3 and 1 <= solv_emin <= 4):
+        raise ValueError(f"Wrong prec_emin / solv_emin: {prec_emin:g} / {solv_emin:g}")  # read_prolong.m:41-56
+    tv_ok = ("none", "smoothing", "srqcg", "lobpcg", "lanczos", "ng-srqcg", "ng-lanczos", "arnoldi",
+             "block_arnoldi")
+    if tv_method.lower() not in tv_ok:
+        raise ValueError("Not existing method")  # read_tspace.m:12-25
+    unsupported = []
+    if method.lower() not in ("afsai_sym", "jacobi"):
+        unsupported.append(f"smoother {method}")
+    if soc_type.upper() not in ("CLA", "DOM"):
+        unsupported.append(f"strength {soc_type}")
+    if proltype == "HYBC":
+        unsupported.append("HYBC prolongation")
+    if tv_method.lower() != "none":
+        unsupported.append(f"test-space method {tv_method}")
+    if prol_emin == "EMIN" and (prec_emin != 1 or solv_emin != 4):
+        unsupported.append("EMIN other than prec_emin 1 / solv_emin 4")
+    if update_cf != 0:
+        unsupported.append("updateCF")
+    if filt_wgt < 100 or filt_tau > 0:
+        unsupported.append("prolongation / coarse-operator filtering")
+    if unsupported:
+        raise ValueError("parameter set outside the restated set-up: " + ", ".join(unsupported))
+    return SetupParams(nlev_max=int(nlev_max), max_coarse=int(max_coarse), soc_tau=float(tau), nstep=int(nstep),
+                       step_size=int(step_size), fsai_eps=float(eps), nupre=int(nupre), nupost=int(nupost),
+                       smoother=method.lower(), prolong=proltype.lower(), soc=soc_type.lower(), ntv=int(ntv),
+                       itmax_vol=int(itmax_vol), tol_vol=float(tol_vol), dist_min=int(dist_min),
+                       dist_max=int(dist_max), maxcond=float(maxcond), maxrownrm=float(maxrownrm),
+                       eps_prol=float(eps_prol), prol_emin=prol_emin.lower(), patt_pow=int(patt_pow),
+                       patt_tau=float(patt_tau), nnzr_max=int(nnzr_max), itmax_emin=int(itmax_emin),
+                       energ_tol=float(energ_tol), condmax_emin=float(condmax_emin), nl_agg=int(nl_agg))
+
+
+def read_general(path: str) -> dict:
+    """The driver's parm_GENERAL (sources/Drivers/driver_AMG/aspAMG_main.m): ascii_input,
+    rhs_build, symmetric, solv_method, itmax, tol, restart."""
+    a, rhs, sym, solver, itmax, tol, restart = _parm_lines(path, "ssssfff")
+    return {"ascii_input": a.lower() == "true", "rhs_build": rhs.lower(), "symmetric": sym.lower() == "true",
+            "solv_method": solver.upper(), "itmax": int(itmax), "tol": float(tol), "restart": int(restart)}
+
+
 class _CBamg(C.Structure):
     _fields_ = [("itmax_vol", C.c_int32), ("tol_vol", C.c_double), ("dist_min", C.c_int32),
                 ("dist_max", C.c_int32), ("maxcond", C.c_double), ("maxrownrm", C.c_double),
