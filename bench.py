@@ -211,7 +211,7 @@ def config_batch(np, torch, amg, H, log) -> dict:
                          "iterations_min_max": [int(iters.min()), int(iters.max())],
                          "all_converged": bool(np.all(flags == 0)), "worst_true_rel_residual": worst,
                          "bytes_per_batch_iteration": bt_bytes, "GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS,
-                         "note": "FP32 V-cycle, one graph per batch iteration, inputs resident on the GPU; GB/s = "
+                         "note": "FP32 V-cycle, the batch solve one launch of a graph of 16 iterations, inputs resident on the GPU; GB/s = "
                                  "algorithmic bytes per batch iteration x iterations of the longest solve / solve time"}
     log(f"[bench] batch 48x8: {t_b * 1e3:.1f} ms, iters {iters.min()}..{iters.max()}, worst res {worst:.2e}, "
         f"{gbs:.0f} GB/s algorithmic ({gbs / HBM_PEAK_GBS:.2f} of HBM)")

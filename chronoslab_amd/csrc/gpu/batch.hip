@@ -1,5 +1,5 @@
 // Batched AMG-PCG: many independent systems of uneven size, each with a block of k right-
-// hand sides, advanced together in one captured graph per iteration.
+// hand sides, advanced together by one captured graph of several iterations.
 //
 // The systems' hierarchies are merged level by level into one block-diagonal hierarchy
 // (systems sorted by depth, deepest first).  On level l the systems that still descend
@@ -328,22 +328,19 @@ struct chr_batch {
   DevBuf x, r, z, p0, p1, ap, b, part, state, arr, resv;
   BArrays a{};
   BState *pinned = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  int32_t graph_k = 0;
-  uint64_t graph_gen = 0;  // h->buf_gen the graphs were captured with
+  // The solve's graph: graph_iters consecutive batch iterations (an even number, so that the
+  // p buffers are back in place for a further launch).
+  hipGraphExec_t exec = nullptr;
+  int32_t graph_k = 0, graph_iters = 0;
+  uint64_t graph_gen = 0;  // h->buf_gen the graph was captured with
   void drop_graphs() {
-    for (auto &g : exec)
-      if (g) {
-        (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
     graph_k = 0;
+    graph_iters = 0;
   }
   ~chr_batch() {
     drop_graphs();
-    for (auto &e : ev)
-      if (e) (void)hipEventDestroy(e);
     if (pinned) (void)hipHostFree(pinned);
     delete h;
   }
@@ -523,8 +520,6 @@ int batch_prepare(chr_batch &bt, int k, int itmax) {
     int *ip = reinterpret_cast<int *>(d + 6 * (size_t)Q);
     bt.a = BArrays{d, d + Q, d + 2 * Q, d + 3 * Q, d + 4 * Q, d + 5 * Q, ip, ip + Q, ip + 2 * Q};
     if (!bt.pinned) HIPC(hipHostMalloc((void **)&bt.pinned, 2 * sizeof(BState), hipHostMallocDefault));
-    for (auto &e : bt.ev)
-      if (!e) HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     bt.kcap = k;
     bt.itcap = 0;  // the residual histories hold (itmax + 1) * nsys * kcap entries: regrow them
   }
@@ -569,7 +564,9 @@ hipError_t launch_bp(chr_batch &bt, int k, const double *pold, double *pnew, hip
   return hipGetLastError();
 }
 
-hipError_t batch_iter(chr_batch &bt, int k, int parity) {
+// One batch iteration, p read from p0 (parity 0) or p1; with copy_state the device state is
+// copied to pinned host memory at the end.
+hipError_t batch_iter(chr_batch &bt, int k, int parity, bool copy_state) {
   chr_hier &h = *bt.h;
   hipStream_t s = h.stream;
   auto *st = bt.state.as<BState>();
@@ -602,7 +599,8 @@ hipError_t batch_iter(chr_batch &bt, int k, int parity) {
                      (const int *)bt.cstart.as<int>(), (const double *)part, bt.a, st, resv);
   if ((e = hipGetLastError())) return e;
   (void)Q;
-  return hipMemcpyAsync(&bt.pinned[parity], st, sizeof(BState), hipMemcpyDeviceToHost, s);
+  if (!copy_state) return hipSuccess;
+  return hipMemcpyAsync(&bt.pinned[0], st, sizeof(BState), hipMemcpyDeviceToHost, s);
 }
 
 }  // namespace
@@ -673,38 +671,39 @@ extern "C" int chr_batch_pcg(chr_batch *bt, int32_t k, const double *b, double *
   HIPC(hipGetLastError());
   if (bt->resv.bytes < sizeof(double) * (size_t)(itmax + 1) * Q || bt->itcap < itmax + 1 || bt->kcap < k)
     return fail(CHR_EHIP, "chr_batch_pcg: work buffers smaller than the solve (internal error)");
-  // Graphs of one batch iteration (p buffers alternate); they embed the cycle vectors'
-  // addresses, so a reallocation of those (a wider chr_batch_vcycle call) invalidates them.
-  if (bt->graph_k != k || bt->graph_gen != h.buf_gen || !bt->exec[0] || !bt->exec[1]) {
+  // One graph holds graph_iters batch iterations (p buffers alternating) and ends with the
+  // state copy; it embeds the cycle vectors' addresses, so a reallocation of those (a wider
+  // chr_batch_vcycle call) invalidates it.  A solve is one launch of it when it converges
+  // within graph_iters iterations; kernels of iterations after every solve has stopped exit
+  // at once.  graph_iters: CHRONOS_BATCH_GRAPH_ITERS (default 16), rounded up to an even
+  // number, at most itmax rounded up likewise.
+  const char *env_gi = std::getenv("CHRONOS_BATCH_GRAPH_ITERS");
+  int gi = env_gi ? std::max(1, std::atoi(env_gi)) : 16;
+  gi = std::min(gi, std::max(itmax, 1));
+  gi += gi & 1;
+  if (bt->graph_k != k || bt->graph_iters != gi || bt->graph_gen != h.buf_gen || !bt->exec) {
     bt->drop_graphs();
-    for (int parity = 0; parity < 2; ++parity) {
-      hipGraph_t g = nullptr;
-      HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      hipError_t e = batch_iter(*bt, k, parity);
-      hipError_t e2 = hipStreamEndCapture(s, &g);
-      if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return fail(CHR_EHIP, std::string("batch graph capture: ") + hipGetErrorString(e ? e : e2));
-      }
-      e = hipGraphInstantiate(&bt->exec[parity], g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return fail(CHR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    hipGraph_t g = nullptr;
+    HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < gi && e == hipSuccess; ++i) e = batch_iter(*bt, k, i & 1, i == gi - 1);
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(CHR_EHIP, std::string("batch graph capture: ") + hipGetErrorString(e ? e : e2));
     }
+    e = hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(CHR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     bt->graph_k = k;
+    bt->graph_iters = gi;
     bt->graph_gen = h.buf_gen;
   }
-  HIPC(hipMemcpyAsync(&bt->pinned[1], bt->state.p, sizeof(BState), hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(&bt->pinned[0], bt->state.p, sizeof(BState), hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));
-  if (!bt->pinned[1].stop) {
-    for (int32_t it = 1; it <= itmax; ++it) {
-      const int parity = (it - 1) & 1;
-      HIPC(hipGraphLaunch(bt->exec[parity], s));
-      HIPC(hipEventRecord(bt->ev[parity], s));
-      if (it >= 2) {
-        HIPC(hipEventSynchronize(bt->ev[parity ^ 1]));
-        if (bt->pinned[parity ^ 1].stop) break;
-      }
-    }
+  for (int32_t done = 0; !bt->pinned[0].stop && done < itmax; done += gi) {
+    HIPC(hipGraphLaunch(bt->exec, s));
+    HIPC(hipStreamSynchronize(s));
   }
   // Outputs.
   double *xd = ptr_kind == CHR_PTR_HOST ? tmp.as<double>() : x;

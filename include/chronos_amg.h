@@ -181,11 +181,13 @@ chr_hier *chr_batch_hier(chr_batch *bt);
  * (sum of n_s rows) x k, column-major with leading dimension = total rows (k <= 16). */
 int chr_batch_vcycle(chr_batch *bt, int32_t k, const double *b, double *x, int32_t ptr_kind);
 /* PCG_CJ (sources/LinearSolver/PCG_CJ.m:1-73) for every system and every column of its
- * right-hand-side block, all advancing in one captured graph per iteration; each
- * (system, column) keeps its own scalars and stops at its own tolerance.  b, x as for
- * chr_batch_vcycle (x holds x0).  iters and flags (nsys*k, index s*k + c) receive the
- * iteration count and flag (0 converged, 1 not converged, -1 breakdown); resvec, when not
- * NULL, (itmax+1) x (nsys*k) column-major.  At most 1024 solves per call. */
+ * right-hand-side block, all advancing together: one launch of a captured graph of 16
+ * consecutive iterations (CHRONOS_BATCH_GRAPH_ITERS) solves the batch when every solve
+ * converges within them, further launches only otherwise; each (system, column) keeps its
+ * own scalars and stops at its own tolerance.  b, x as for chr_batch_vcycle (x holds x0).
+ * iters and flags (nsys*k, index s*k + c) receive the iteration count and flag (0
+ * converged, 1 not converged, -1 breakdown); resvec, when not NULL, (itmax+1) x (nsys*k)
+ * column-major.  At most 1024 solves per call. */
 int chr_batch_pcg(chr_batch *bt, int32_t k, const double *b, double *x, int32_t itmax,
                   double tol, int32_t ptr_kind, int32_t *iters, int32_t *flags, double *resvec);
 

@@ -79,7 +79,7 @@ def test_batch_zero_rhs_column_and_itmax(gpu):
 
 def test_batch_config_48_systems_8_rhs(gpu):
     """48 heterogeneous-diffusion systems, edge U{16..64}, ln k ~ N(0, 1.5^2), 8 random
-    right-hand sides each, FP32 V-cycle, PCG_CJ to 1e-8 in one graph per iteration."""
+    right-hand sides each, FP32 V-cycle, PCG_CJ to 1e-8 in one launch of a 16-iteration graph."""
     rng = np.random.default_rng(48)
     edges = rng.integers(16, 65, size=48)
     t = time.perf_counter()
@@ -161,3 +161,22 @@ def test_batch_pcg_against_oracle(gpu, prec):
             else:
                 assert abs(iters[s, c] - itr) <= 1, (s, c, iters[s, c], itr)
                 assert np.linalg.norm(b - A.matvec(x)) <= 1e-8 * np.linalg.norm(b)
+
+
+def test_batch_graph_length_does_not_change_results(gpu, monkeypatch):
+    """The batch solve as one launch of a 16-iteration graph, as launches of 2-iteration
+    graphs, and with itmax cutting a launch short: the same iterations, histories and
+    solutions bit for bit (kernels of iterations after every solve stopped exit at once)."""
+    mats, hiers = _systems([12, 15, 10], 1.0, 120)
+    B = _stack([h.levels[0].A.nrows for h in hiers], 4, 9)
+    out = {}
+    for gi in ("16", "2"):
+        monkeypatch.setenv("CHRONOS_BATCH_GRAPH_ITERS", gi)
+        out[gi] = gpu.BatchSolver(hiers, "fp64").pcg(B, itmax=60, tol=1e-8)
+    for a, b in zip(out["16"], out["2"]):
+        np.testing.assert_array_equal(a, b)
+    monkeypatch.setenv("CHRONOS_BATCH_GRAPH_ITERS", "16")
+    X, iters, flags, _ = gpu.BatchSolver(hiers, "fp64").pcg(B, itmax=5, tol=1e-8)
+    # The 10^3 system is one direct level: one iteration solves it.
+    assert np.all(iters[:2] == 5) and np.all(flags[:2] == 1)
+    assert np.all(iters[2] == 1) and np.all(flags[2] == 0)
