@@ -1022,9 +1022,9 @@ extern "C" int chr_spmv(chr_mat *A, const double *x, double *y, int32_t ptr_kind
 namespace {
 
 hipError_t pcg_finalize(int kind, int count, double *part, chr::PcgState *st, double *resvec,
-                        hipStream_t s) {
+                        hipStream_t s, chr::PcgState *host_out = nullptr) {
   hipLaunchKernelGGL(chr::pcg_finalize_kernel, dim3(1), dim3(chr::kFinBlock), 0, s, kind, count,
-                     (const double *)part, st, resvec);
+                     (const double *)part, st, resvec, host_out);
   return hipGetLastError();
 }
 
@@ -1066,7 +1066,8 @@ bool pcg_formed(const DevCsr &) {
 }
 
 // One PCG_CJ iteration (PCG_CJ.m:88-120), p read from p0 (parity 0) or p1 and written to
-// the other.  state_dst (may be null): host memory that receives the state at the end.
+// the other.  state_dst (may be null): the device address of host-mapped pinned memory that
+// the closing finalisation writes the state to.
 hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, const PcgPtrs &w, int parity,
                             chr::PcgState *state_dst) {
   chr::PcgState *st = w.st;
@@ -1091,9 +1092,7 @@ hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, const PcgPtrs &w, int 
                      (const double *)pnew, (const double *)w.ap, w.x, w.r, w.r32, w.part,
                      (const chr::PcgState *)st);
   if ((e = hipGetLastError())) return e;
-  if ((e = pcg_finalize(chr::kFinUpdate, (int)w.nblk, w.part, st, w.resvec, h.stream))) return e;
-  if (!state_dst) return hipSuccess;
-  return hipMemcpyAsync(state_dst, st, sizeof(chr::PcgState), hipMemcpyDeviceToHost, h.stream);
+  return pcg_finalize(chr::kFinUpdate, (int)w.nblk, w.part, st, w.resvec, h.stream, state_dst);
 }
 
 // r = b - A x, the FP32 copy of r, the norms and the initial scalars (PCG_CJ.m:5-30); the
@@ -1136,7 +1135,10 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
     w.nblk = pcg_nblk(n);
     if ((rc = w.part.alloc(sizeof(double) * 2 * w.nblk))) return rc;
     if ((rc = w.state.alloc(sizeof(chr::PcgState)))) return rc;
-    if (!w.pinned) HIPC(hipHostMalloc((void **)&w.pinned, 2 * sizeof(chr::PcgState), hipHostMallocDefault));
+    if (!w.pinned) {
+      HIPC(hipHostMalloc((void **)&w.pinned, 2 * sizeof(chr::PcgState), hipHostMallocDefault));
+      HIPC(hipHostGetDevicePointer((void **)&w.pinned_dev, w.pinned, 0));
+    }
     for (auto &e : w.ev)
       if (!e) HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
@@ -1167,7 +1169,7 @@ int pcg_graphs(chr_hier &h, const chr_mat *Akey, const DevCsr &A) {
     }
     hipGraph_t g = nullptr;
     HIPC(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_pcg_iter(h, A, w.ptrs(h.fp32()), parity, &w.pinned[parity]);
+    hipError_t e = enqueue_pcg_iter(h, A, w.ptrs(h.fp32()), parity, w.pinned_dev + parity);
     hipError_t e2 = hipStreamEndCapture(h.stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);

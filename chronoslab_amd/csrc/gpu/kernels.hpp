@@ -1087,12 +1087,21 @@ __device__ __forceinline__ void fin_sum(const double *part, int count, double *l
 enum { kFinInit = 0, kFinBeta = 1, kFinAlpha = 2, kFinUpdate = 3 };
 
 // Scalar updates of PCG_CJ.m after each reduction (one workgroup).
+// host_out (may be null): host-mapped pinned memory that receives the state when the kernel
+// ends (the iteration's closing finalisation), stopped or not, so that no copy follows.
 static __global__ __launch_bounds__(kFinBlock) void pcg_finalize_kernel(int kind, int count,
                                                                  const double *__restrict__ part,
                                                                  PcgState *st,
-                                                                 double *__restrict__ resvec) {
+                                                                 double *__restrict__ resvec,
+                                                                 PcgState *host_out = nullptr) {
   __shared__ double lds[2 * (kFinBlock / kWave)];
-  if (kind != kFinInit && st->stop) return;
+  if (kind != kFinInit && st->stop) {
+    if (host_out && threadIdx.x == 0) {
+      *host_out = *st;
+      __threadfence_system();
+    }
+    return;
+  }
   if (kind == kFinBeta && st->iter == 0) {
     if (threadIdx.x == 0) st->beta = 0.0;  // first iteration: p = z (PCG_CJ.m:95-96)
     return;
@@ -1142,6 +1151,10 @@ static __global__ __launch_bounds__(kFinBlock) void pcg_finalize_kernel(int kind
       st->resiter = sqrt(s0) / st->bnorm;
       resvec[it] = st->resiter;
       if (st->resiter < st->tol || it >= st->itmax) st->stop = 1;
+      if (host_out) {
+        *host_out = *st;
+        __threadfence_system();
+      }
     }
   }
 }

@@ -473,7 +473,8 @@ struct PcgWork {
   DevBuf x, r, z, p0, p1, ap, part, part_p, state, resvec;
   DevBuf r32;  // FP32 copy of r (FP32 cycle): the V-cycle's first smoother operand
   unsigned nblk = 0;
-  chr::PcgState *pinned = nullptr;  // two slots
+  chr::PcgState *pinned = nullptr;      // two slots of host-mapped pinned memory
+  chr::PcgState *pinned_dev = nullptr;  // their device address (written by the finalize kernel)
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   uint64_t graph_uid = 0;  // uid of the outer operator the cached graphs were built for
