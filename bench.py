@@ -27,6 +27,13 @@ sys.path.insert(0, ROOT)
 # OpenMP use in this process, so set before any native library loads).
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# The CPUs this process may use, read before any OpenMP runtime starts: with OMP_PROC_BIND
+# set, the runtime binds the initial thread to its first place, after which the main
+# thread's affinity mask holds that place alone.
+try:
+    _AFFINITY = sorted(os.sched_getaffinity(0))
+except AttributeError:
+    _AFFINITY = list(range(os.cpu_count() or 1))
 
 import numpy as np  # noqa: E402
 
@@ -61,10 +68,7 @@ def _cpu_info():
                 model = line.split(":", 1)[1].strip()
     except OSError:
         pass
-    try:
-        cpus = sorted(os.sched_getaffinity(0))
-    except AttributeError:
-        cpus = list(range(os.cpu_count() or 1))
+    cpus = _AFFINITY
     # Physical cores among them: distinct (package, core) pairs from sysfs; where sysfs does not
     # tell them apart (some virtual machines report one core), lscpu's threads per core.
     cores = set()

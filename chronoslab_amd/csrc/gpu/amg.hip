@@ -198,6 +198,14 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
       if (!e) e = hipEventElapsedTime(&ms, ev[10], ev[11]);
     }
     if (e) return fail(CHR_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+    if (upload_trace()) {
+      if (M.use_sell)
+        std::fprintf(stderr, "[upload]   %d rows: sell cb %d sort %d (shift %d) elems/nnz %.3f: %.4f ms\n", M.nrows,
+                     M.sell.cb, (int)M.sell.sorted, M.sell.sshift, (double)M.sell.elems / (double)M.nnz, ms / 5);
+      else
+        std::fprintf(stderr, "[upload]   %d rows: %s PER %d: %.4f ms\n", M.nrows, M.use_pack ? "pack" : "stream",
+                     M.per, ms / 5);
+    }
     return CHR_OK;
   };
   const double t_tune0 = now_seconds();

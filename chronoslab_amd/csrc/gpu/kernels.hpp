@@ -693,88 +693,102 @@ static __global__ __launch_bounds__(kBlock) void rm_to_cm_kernel(int n, int k, c
   dst[c * ld + i] = src[i * k + c];
 }
 
-// Per-block partial Gram products: part[g][p*b + q] = sum over the block's rows of
-// X[i*a + p] * Y[i*b + q] in row order (a, b <= 16).  Rows are split in fixed chunks, so
-// the partials (and the fixed-order reduction that follows) do not depend on timing.  Row
-// tiles of X and Y are staged in LDS with coalesced loads; thread p*b + q then walks the
-// tile's rows.
-constexpr int kGramTile = 64;
-static __global__ __launch_bounds__(kBlock) void gram_partial_kernel(int n, int a, const double *__restrict__ X,
-                                                              int b, const double *__restrict__ Y,
-                                                              int rows_per_block,
-                                                              double *__restrict__ part) {
-  __shared__ double xs[kGramTile * 16], ys[kGramTile * 16];
-  const int t = threadIdx.x;
-  const int ab = a * b;
-  const int p = t / b, q = t % b;
-  const int i0 = blockIdx.x * rows_per_block, i1 = min(n, i0 + rows_per_block);
-  double s = 0.0;
-  for (int r0 = i0; r0 < i1; r0 += kGramTile) {
-    const int rows = min(kGramTile, i1 - r0);
-    for (int e = t; e < rows * a; e += kBlock) xs[e] = X[(size_t)r0 * a + e];
-    for (int e = t; e < rows * b; e += kBlock) ys[e] = Y[(size_t)r0 * b + e];
-    __syncthreads();
-    if (t < ab)
-      for (int r = 0; r < rows; ++r) s += xs[r * a + p] * ys[r * b + q];
-    __syncthreads();
-  }
-  if (t < ab) part[(size_t)blockIdx.x * ab + t] = s;
+// ---------------------------------------------------------------------------------------
+// Tall-skinny FP64 products of LOBPCG on the matrix cores (v_mfma_f64_16x16x4f64).  Lane
+// layout of that instruction (checked on gfx950): A[i][k] at lane i + 16 k and B[k][j] at
+// lane j + 16 k, one double each; D[i][j] at lane j + 16 (i % 4), element i / 4.
+// ---------------------------------------------------------------------------------------
+typedef double mf64x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ mf64x4 mfma_f64_16x16x4(double a, double b, mf64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
-// Several Gram products of up to six row-major blocks in one pass: block b (n x w[b],
-// w[b] <= 16) is staged once per 64-row tile, and pair k contributes the entries
-// part[g][po[k] + p*w[pj[k]] + q] = sum over the part's rows of blk[pi[k]][i][p] *
-// blk[pj[k]][i][q], in row order with the same row split as gram_partial_kernel, so every
-// entry equals that kernel's partial bit for bit.
-constexpr int kGramMaxBlocks = 6, kGramMaxPairs = 12, kGramMaxOut = 12;  // 12 * kBlock outputs
+
+// Gram products X_k^T Y_k of row-major n x w blocks (w <= 16).  The operands are numbered
+// in order of first appearance; three fixed tables of (i, j) pairs, one nibble per pair:
+//   table 0: one product, blocks 0 and 1;
+//   table 1: the six products of a Rayleigh-Ritz step after a restart, over the blocks
+//            X, AX, AAR, AR: X'AX, X'AAR, AR'AAR, X'X, X'AR, AR'AR;
+//   table 2: table 1 and the six products with AAP and APa: X'AAP, AR'AAP, APa'AAP,
+//            X'APa, AR'APa, APa'APa;
+//   table 3: R^T R of the residual block R = AX - X diag(lam), formed on the fly from
+//            blocks 0 (AX) and 1 (X) and never stored (the product's operand is block 0).
+constexpr int kGramMaxBlocks = 6, kGramMaxPairs = 12, kGramMaxOut = kGramMaxPairs * 256;
+constexpr uint64_t kGramPi = 0x530530300300ull, kGramPj = 0x555444330221ull;
+__host__ __device__ constexpr int gram_npairs(int t) { return t == 0 || t == 3 ? 1 : t == 1 ? 6 : 12; }
+__host__ __device__ constexpr int gram_nblocks(int t) { return t == 0 || t == 3 ? 2 : t == 1 ? 4 : 6; }
+__host__ __device__ constexpr int gram_pi(int t, int k) {
+  return t == 0 || t == 3 ? 0 : (int)((kGramPi >> (4 * k)) & 15);
+}
+__host__ __device__ constexpr int gram_pj(int t, int k) {
+  return t == 0 ? 1 : t == 3 ? 0 : (int)((kGramPj >> (4 * k)) & 15);
+}
 struct GramSet {
   const double *blk[kGramMaxBlocks];
   int w[kGramMaxBlocks];
-  int loff[kGramMaxBlocks];  // first LDS column of block b (prefix sum of the widths)
-  int nblk;
-  int pi[kGramMaxPairs], pj[kGramMaxPairs], po[kGramMaxPairs + 1];
-  int npair;
+  int po[kGramMaxPairs + 1];  // first output entry of each product (row-major w_i x w_j)
+  const double *lam;          // table 3: the k eigenvalues
 };
-static __global__ __launch_bounds__(kBlock) void gram_multi_partial_kernel(int n, GramSet gs, int rows_per_block,
-                                                                    double *__restrict__ part) {
-  __shared__ double tile[kGramTile * kGramMaxBlocks * 16];
-  const int t = threadIdx.x, total = gs.po[gs.npair];
-  const int i0 = blockIdx.x * rows_per_block, i1 = min(n, i0 + rows_per_block);
-  double acc[kGramMaxOut];
-  int ax[kGramMaxOut], wx[kGramMaxOut], ay[kGramMaxOut], wy[kGramMaxOut];
+
+// Per-workgroup partial Gram products: wave v of the grid takes rows [v R, v R + R) (R a
+// multiple of 4, fixed by the launch), four rows per matrix instruction, one accumulator
+// per product; the workgroup's four waves are then added in wave order and written to
+// part[g][po[k] + p w_j + q].  The row split and the order of the additions are fixed, so
+// the partials do not depend on timing, and a product's partials are the same whichever
+// table it is computed in.
+template <int T>
+__global__ __launch_bounds__(kBlock) void gram_mfma_kernel(int n, GramSet gs, int rows_per_wave,
+                                                           double *__restrict__ part) {
+  constexpr int NP = gram_npairs(T), NB = gram_nblocks(T), NW = kBlock / kWave;
+  constexpr int UNR = NP == 1 ? 8 : NP <= 6 ? 4 : 2;  // row steps whose loads are issued together
+  __shared__ double red[NW][kWave * 4];
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  const int64_t v = (int64_t)blockIdx.x * NW + wv;
+  const int i0 = (int)min<int64_t>(n, v * rows_per_wave), i1 = min(n, i0 + rows_per_wave);
+  const int rsub = lane / 16, csub = lane % 16;
+  mf64x4 acc[NP];
 #pragma unroll
-  for (int m = 0; m < kGramMaxOut; ++m) {
-    acc[m] = 0.0;
-    ax[m] = ay[m] = 0;
-    wx[m] = wy[m] = 0;
-    const int o = t + m * kBlock;
-    if (o < total) {
-      int k = 0;
-      while (o >= gs.po[k + 1]) ++k;
-      const int bx = gs.pi[k], by = gs.pj[k], local = o - gs.po[k];
-      wx[m] = gs.w[bx];
-      wy[m] = gs.w[by];
-      ax[m] = kGramTile * gs.loff[bx] + local / wy[m];
-      ay[m] = kGramTile * gs.loff[by] + local % wy[m];
+  for (int k = 0; k < NP; ++k) acc[k] = mf64x4{0.0, 0.0, 0.0, 0.0};
+  const double *src[NB];
+  int wd[NB];
+  bool col_ok[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    wd[b] = gs.w[b];
+    col_ok[b] = csub < wd[b];
+    src[b] = gs.blk[b] + csub;
+  }
+  const double lam_c = (T == 3 && col_ok[0]) ? gs.lam[csub] : 0.0;
+  // UNR row steps per round: all their loads first, then their instructions in row order
+  // (steps past the wave's rows load zeros, which leave the accumulators unchanged).
+  for (int r = i0; r < i1; r += 4 * UNR) {
+    double f[UNR][NB];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int row = r + 4 * u + rsub;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) f[u][b] = (row < i1 && col_ok[b]) ? src[b][(size_t)row * wd[b]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      if constexpr (T == 3) f[u][0] = f[u][0] - f[u][1] * lam_c;  // R = AX - X diag(lam), as block_resid_gather_kernel
+#pragma unroll
+      for (int k = 0; k < NP; ++k) acc[k] = mfma_f64_16x16x4(f[u][gram_pi(T, k)], f[u][gram_pj(T, k)], acc[k]);
     }
   }
-  for (int r0 = i0; r0 < i1; r0 += kGramTile) {
-    const int rows = min(kGramTile, i1 - r0);
-    for (int b = 0; b < gs.nblk; ++b) {
-      const double *src = gs.blk[b] + (size_t)r0 * gs.w[b];
-      double *dst = tile + kGramTile * gs.loff[b];
-      for (int e = t; e < rows * gs.w[b]; e += kBlock) dst[e] = src[e];
-    }
-    __syncthreads();
+  const int total = gs.po[NP];
+  const int t = threadIdx.x, tl = t / 4, te = t % 4;
+  const int p = tl / 16 + 4 * te, q = tl % 16;  // entry held by element te of lane tl
 #pragma unroll
-    for (int m = 0; m < kGramMaxOut; ++m)
-      if (wx[m] > 0)
-        for (int r = 0; r < rows; ++r) acc[m] += tile[ax[m] + r * wx[m]] * tile[ay[m] + r * wy[m]];
-    __syncthreads();
-  }
+  for (int k = 0; k < NP; ++k) {
 #pragma unroll
-  for (int m = 0; m < kGramMaxOut; ++m) {
-    const int o = t + m * kBlock;
-    if (o < total) part[(size_t)blockIdx.x * total + o] = acc[m];
+    for (int e = 0; e < 4; ++e) red[wv][lane * 4 + e] = acc[k][e];
+    __syncthreads();
+    double s = red[0][t];
+#pragma unroll
+    for (int u = 1; u < NW; ++u) s += red[u][t];
+    const int wa = gs.w[gram_pi(T, k)], wb = gs.w[gram_pj(T, k)];
+    if (p < wa && q < wb) part[(size_t)blockIdx.x * total + gs.po[k] + p * wb + q] = s;
+    __syncthreads();
   }
 }
 
@@ -791,90 +805,126 @@ static __global__ __launch_bounds__(kBlock) void gram_reduce_kernel(int nparts, 
   if (lane == 0) out[t] = s;  // row-major a x b: out[p*b + q]
 }
 
-// Z (n x b) = beta * Z + X (n x a) * Cm, Cm column-major a x b (a, b <= 48).  Cm is staged
-// in LDS once per workgroup and X in tiles of 32 rows; output (i, q) sums over p in order.
-constexpr int kGemmTile = 32;
-static __global__ __launch_bounds__(kBlock) void block_gemm_kernel(int n, int a, const double *__restrict__ X,
-                                                            int b, const double *__restrict__ Cm,
-                                                            double beta, double *__restrict__ Z) {
-  __shared__ double cs[48 * 48], xs[kGemmTile * 48];
-  const int t = threadIdx.x;
-  for (int e = t; e < a * b; e += kBlock) cs[e] = Cm[e];
-  __syncthreads();
-  for (int r0 = blockIdx.x * kGemmTile; r0 < n; r0 += gridDim.x * kGemmTile) {
-    const int rows = min(kGemmTile, n - r0);
-    for (int e = t; e < rows * a; e += kBlock) xs[e] = X[(size_t)r0 * a + e];
-    __syncthreads();
-    for (int o = t; o < rows * b; o += kBlock) {
-      const int r = o / b, q = o % b;
-      double s = 0.0;
-      for (int p = 0; p < a; ++p) s += xs[r * a + p] * cs[p + q * a];
-      const size_t zi = (size_t)(r0 + r) * b + q;
-      Z[zi] = (beta != 0.0 ? beta * Z[zi] : 0.0) + s;
+// Z (n x b) = beta * Z + X (n x a) * C with C(p, q) = cscale * Cm[p * csp + q * csq]
+// (column-major a x b: csp = 1, csq = a; row-major: csp = b, csq = 1), a <= 4 KS,
+// b <= 16 CT, Z not aliasing X.  Each wave takes 16-row tiles; the coefficient fragments of
+// the KS k-steps and CT column tiles stay in registers.  (LOBPCG's blocks are at most 16
+// wide: KS = 4, CT = 1.)
+template <int KS, int CT>
+__global__ __launch_bounds__(kBlock) void tsgemm_mfma_kernel(int n, int a, const double *__restrict__ X, int b,
+                                                             const double *__restrict__ Cm, int csp, int csq,
+                                                             double cscale, double beta, double *__restrict__ Z) {
+  const int lane = threadIdx.x % kWave, hi = lane / 16, lo = lane % 16;
+  double cf[KS][CT];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int kk = 4 * s + hi, j = 16 * c + lo;
+      cf[s][c] = (kk < a && j < b) ? cscale * Cm[(size_t)kk * csp + (size_t)j * csq] : 0.0;
     }
-    __syncthreads();
+  const int64_t ntile = ((int64_t)n + 15) / 16;
+  const int64_t nwave = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t tile = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; tile < ntile; tile += nwave) {
+    const int64_t r0 = tile * 16;
+    const int64_t ar = r0 + lo;  // the A-fragment row of this lane
+    mf64x4 acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = mf64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int kk = 4 * s + hi;
+      const double av = (ar < n && kk < a) ? X[ar * a + kk] : 0.0;
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[c] = mfma_f64_16x16x4(av, cf[s][c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int j = 16 * c + lo;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = r0 + hi + 4 * e;
+        if (row < n && j < b) {
+          const size_t zi = (size_t)row * b + j;
+          Z[zi] = (beta != 0.0 ? beta * Z[zi] : 0.0) + acc[c][e];
+        }
+      }
+    }
   }
 }
 
 // The closing update of a LOBPCG iteration (lobpcg.m:389-400) in one pass, for the
 // search-direction side (R, Pa, X) or the operator side (AR, APa, AX):
 //   Pout = R ER (+ Pa EP),   Xout = Pout + X EX,
-// R and Pa n x na, X n x k, ER and EP column-major na x k, EX k x k (na, k <= 16).  Each
-// product sums over its inner index in order (as block_gemm_kernel does) and the terms are
-// added as ((R ER + Pa EP) + X EX), the order of separate products followed by a sum.  X and Xout may alias: a tile's rows are staged before
-// any of its outputs is written.
-static __global__ __launch_bounds__(kBlock) void lob_update_kernel(int n, int na, int k, const double *__restrict__ R,
-                                                            const double *__restrict__ Pa,
-                                                            const double *X, const double *__restrict__ ER,
-                                                            const double *__restrict__ EP,
-                                                            const double *__restrict__ EX,
-                                                            double *__restrict__ Pout, double *Xout) {
-  __shared__ double cr[16 * 16], cp[16 * 16], cx[16 * 16], rs[kGemmTile * 16], ps[kGemmTile * 16],
-      xs[kGemmTile * 16];
-  const int t = threadIdx.x;
-  for (int e = t; e < na * k; e += kBlock) {
-    cr[e] = ER[e];
-    if (Pa) cp[e] = EP[e];
+// R and Pa n x na, X n x k, ER and EP column-major na x k, EX k x k (na, k <= 16), on the
+// matrix cores: R ER and Pa EP accumulate in one accumulator, X EX in another.  X and Xout
+// may alias: a wave reads all of its 16-row tile before writing it, and no two waves share
+// a tile.
+static __global__ __launch_bounds__(kBlock) void lob_update_mfma_kernel(int n, int na, int k, const double *__restrict__ R,
+                                                                 const double *__restrict__ Pa, const double *X,
+                                                                 const double *__restrict__ ER,
+                                                                 const double *__restrict__ EP,
+                                                                 const double *__restrict__ EX,
+                                                                 double *__restrict__ Pout, double *Xout) {
+  const int lane = threadIdx.x % kWave, hi = lane / 16, lo = lane % 16;
+  double cr[4], cp[4], cx[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int kk = 4 * s + hi;
+    cr[s] = (kk < na && lo < k) ? ER[kk + (size_t)lo * na] : 0.0;
+    cp[s] = (Pa && kk < na && lo < k) ? EP[kk + (size_t)lo * na] : 0.0;
+    cx[s] = (kk < k && lo < k) ? EX[kk + (size_t)lo * k] : 0.0;
   }
-  for (int e = t; e < k * k; e += kBlock) cx[e] = EX[e];
-  __syncthreads();
-  for (int r0 = blockIdx.x * kGemmTile; r0 < n; r0 += gridDim.x * kGemmTile) {
-    const int rows = min(kGemmTile, n - r0);
-    for (int e = t; e < rows * na; e += kBlock) {
-      rs[e] = R[(size_t)r0 * na + e];
-      if (Pa) ps[e] = Pa[(size_t)r0 * na + e];
+  const int sr = (na + 3) / 4, sx = (k + 3) / 4;
+  const int64_t ntile = ((int64_t)n + 15) / 16;
+  const int64_t nwave = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t tile = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; tile < ntile; tile += nwave) {
+    const int64_t r0 = tile * 16, ar = r0 + lo;
+    const bool rok = ar < n;
+    double fr[4], fp[4], fx[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kk = 4 * s + hi;
+      fr[s] = (s < sr && rok && kk < na) ? R[ar * na + kk] : 0.0;
+      fp[s] = (Pa && s < sr && rok && kk < na) ? Pa[ar * na + kk] : 0.0;
+      fx[s] = (s < sx && rok && kk < k) ? X[ar * k + kk] : 0.0;
     }
-    for (int e = t; e < rows * k; e += kBlock) xs[e] = X[(size_t)r0 * k + e];
-    __syncthreads();
-    for (int o = t; o < rows * k; o += kBlock) {
-      const int r = o / k, q = o % k;
-      double s1 = 0.0;
-      for (int p = 0; p < na; ++p) s1 += rs[r * na + p] * cr[p + q * na];
-      double pv = 0.0 + s1;
-      if (Pa) {
-        double s2 = 0.0;
-        for (int p = 0; p < na; ++p) s2 += ps[r * na + p] * cp[p + q * na];
-        pv = 1.0 * pv + s2;
+    mf64x4 p = {0.0, 0.0, 0.0, 0.0}, x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (s < sr) p = mfma_f64_16x16x4(fr[s], cr[s], p);
+    if (Pa)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (s < sr) p = mfma_f64_16x16x4(fp[s], cp[s], p);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (s < sx) x = mfma_f64_16x16x4(fx[s], cx[s], x);
+    if (lo < k)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = r0 + hi + 4 * e;
+        if (row < n) {
+          const size_t zi = (size_t)row * k + lo;
+          Pout[zi] = p[e];
+          Xout[zi] = p[e] + x[e];
+        }
       }
-      double s3 = 0.0;
-      for (int p = 0; p < k; ++p) s3 += xs[r * k + p] * cx[p + q * k];
-      const double tv = 0.0 + s3;
-      const size_t zi = (size_t)(r0 + r) * k + q;
-      Pout[zi] = pv;
-      Xout[zi] = pv + tv;
-    }
-    __syncthreads();
   }
 }
 
-// R = AX - X diag(lam)   (n x k)
-static __global__ __launch_bounds__(kBlock) void block_resid_kernel(int n, int k, const double *__restrict__ AX,
-                                                             const double *__restrict__ X,
-                                                             const double *__restrict__ lam,
-                                                             double *__restrict__ R) {
+// Y (n x m) = columns idx[0..m) of the residual block R = AX - X diag(lam) (n x k): the
+// product rounded, then the difference (as gram_mfma_kernel<3> forms R).
+static __global__ __launch_bounds__(kBlock) void block_resid_gather_kernel(int n, int k, const double *__restrict__ AX,
+                                                                    const double *__restrict__ X,
+                                                                    const double *__restrict__ lam, int m,
+                                                                    const int *__restrict__ idx,
+                                                                    double *__restrict__ Y) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= (int64_t)n * k) return;
-  R[t] = AX[t] - X[t] * lam[t % k];
+  if (t >= (int64_t)n * m) return;
+  const int64_t i = t / m;
+  const int c = idx[t % m];
+  Y[t] = AX[i * k + c] - X[i * k + c] * lam[c];
 }
 
 // Y (n x m) = columns idx[0..m) of X (n x k).

@@ -24,7 +24,7 @@ struct Lob {
   hipStream_t s;
   int n;
   DevBuf part, gout, stage_dev;
-  int nparts = 1, rows_per_part = 1;
+  int nparts = 1, rows_per_wave = 4;
   // Small host operands (Rayleigh-Ritz coefficients, eigenvalues, column lists) go through a
   // pinned staging area, each piece copied asynchronously to its own slot of stage_dev.
   // Slots are reused only after a stream synchronisation (fetch), so no pending copy ever
@@ -41,10 +41,13 @@ struct Lob {
     if (hout) (void)hipHostFree(hout);
   }
   int init() {
+    // Gram partials: nparts workgroups of four waves, each wave a run of rows_per_wave rows
+    // (a multiple of 4, the rows of one matrix instruction).
     nparts = std::max(1, std::min(1024, (n + 255) / 256));
-    rows_per_part = (n + nparts - 1) / nparts;
+    const int waves = nparts * (chr::kBlock / chr::kWave);
+    rows_per_wave = ((n + waves - 1) / waves + 3) / 4 * 4;
     int rc;
-    if ((rc = part.alloc(sizeof(double) * (size_t)nparts * chr::kGramMaxOut * chr::kBlock))) return rc;
+    if ((rc = part.alloc(sizeof(double) * (size_t)nparts * chr::kGramMaxOut))) return rc;
     if ((rc = gout.alloc(sizeof(double) * kSlots * kSlot))) return rc;
     if ((rc = stage_dev.alloc(sizeof(double) * kStage))) return rc;
     HIPC(hipHostMalloc((void **)&hstage, sizeof(double) * kStage, hipHostMallocDefault));
@@ -52,6 +55,12 @@ struct Lob {
     return CHR_OK;
   }
   static unsigned grid(int64_t cnt) { return (unsigned)((cnt + chr::kBlock - 1) / chr::kBlock); }
+  // Workgroups of the 16-row-tile kernels: one tile per wave, at most 2048 workgroups (the
+  // waves then stride over the tiles).
+  unsigned tile_grid() const {
+    const int64_t tiles = ((int64_t)n + 15) / 16, per = chr::kBlock / chr::kWave;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (tiles + per - 1) / per));
+  }
 
   // Copies bytes from src to a fresh staging slot and returns its device address (null on error).
   const void *stage(const void *src, size_t bytes) {
@@ -74,8 +83,15 @@ struct Lob {
       fail(CHR_EARG, "lobpcg: Gram block wider than 16 or too many pending Gram matrices");
       return -1;
     }
-    hipLaunchKernelGGL(chr::gram_partial_kernel, dim3(nparts), dim3(chr::kBlock), 0, s, n, a, X, b, Y,
-                       rows_per_part, part.as<double>());
+    chr::GramSet gs{};
+    gs.blk[0] = X;
+    gs.blk[1] = Y;
+    gs.w[0] = a;
+    gs.w[1] = b;
+    gs.po[0] = 0;
+    gs.po[1] = a * b;
+    hipLaunchKernelGGL(chr::gram_mfma_kernel<0>, dim3(nparts), dim3(chr::kBlock), 0, s, n, gs, rows_per_wave,
+                       part.as<double>());
     const unsigned rgrid = (unsigned)((a * b * chr::kWave + chr::kBlock - 1) / chr::kBlock);
     hipLaunchKernelGGL(chr::gram_reduce_kernel, dim3(rgrid), dim3(chr::kBlock), 0, s, nparts, a * b,
                        (const double *)part.as<double>(), gout.as<double>() + (size_t)nslot * kSlot);
@@ -86,9 +102,9 @@ struct Lob {
     return nslot++;
   }
   // Enqueues the Gram products X_k^T Y_k of pairs (X_k, a_k, Y_k, b_k) in one pass over the
-  // distinct blocks (at most six, widths <= 16; gram_multi_partial_kernel), each entry equal
-  // to the one gram_launch would give.  Returns the first slot, or -1 on error; product k
-  // starts offs[k] entries after that slot's start, row-major a_k x b_k.
+  // distinct blocks (gram_mfma_kernel; the pairs must form one of its Rayleigh-Ritz tables),
+  // each entry equal to the one gram_launch would give.  Returns the first slot, or -1 on
+  // error; product k starts offs[k] entries after that slot's start, row-major a_k x b_k.
   struct GramPair {
     const double *X;
     int a;
@@ -97,42 +113,45 @@ struct Lob {
   };
   int gram_multi_launch(const std::vector<GramPair> &pairs, std::vector<int> &offs) {
     chr::GramSet gs{};
+    int nblk = 0;
     auto id = [&](const double *p, int w) -> int {
-      for (int b = 0; b < gs.nblk; ++b)
+      for (int b = 0; b < nblk; ++b)
         if (gs.blk[b] == p) return gs.w[b] == w ? b : -1;
-      if (gs.nblk == chr::kGramMaxBlocks || w < 1 || w > 16) return -1;
-      gs.blk[gs.nblk] = p;
-      gs.w[gs.nblk] = w;
-      gs.loff[gs.nblk] = gs.nblk ? gs.loff[gs.nblk - 1] + gs.w[gs.nblk - 1] : 0;
-      return gs.nblk++;
+      if (nblk == chr::kGramMaxBlocks || w < 1 || w > 16) return -1;
+      gs.blk[nblk] = p;
+      gs.w[nblk] = w;
+      return nblk++;
     };
     if (pairs.empty() || (int)pairs.size() > chr::kGramMaxPairs) {
       fail(CHR_EARG, "lobpcg: Gram batch size");
       return -1;
     }
     offs.clear();
+    const int np = (int)pairs.size();
+    const int table = np == chr::gram_npairs(1) ? 1 : np == chr::gram_npairs(2) ? 2 : -1;
     int total = 0;
-    for (size_t k = 0; k < pairs.size(); ++k) {
+    for (int k = 0; k < np; ++k) {
       const int bx = id(pairs[k].X, pairs[k].a), by = id(pairs[k].Y, pairs[k].b);
-      if (bx < 0 || by < 0) {
-        fail(CHR_EARG, "lobpcg: Gram batch operands (at most six blocks of width <= 16)");
+      if (bx < 0 || by < 0 || table < 0 || bx != chr::gram_pi(table, k) || by != chr::gram_pj(table, k)) {
+        fail(CHR_EARG, "lobpcg: Gram batch is not a Rayleigh-Ritz table of blocks of width <= 16");
         return -1;
       }
-      gs.pi[k] = bx;
-      gs.pj[k] = by;
       gs.po[k] = total;
       offs.push_back(total);
       total += pairs[k].a * pairs[k].b;
     }
-    gs.npair = (int)pairs.size();
-    gs.po[gs.npair] = total;
+    gs.po[np] = total;
     const int slots = (total + kSlot - 1) / kSlot;
-    if (total > chr::kGramMaxOut * chr::kBlock || nslot + slots > kSlots) {
+    if (total > chr::kGramMaxOut || nslot + slots > kSlots) {
       fail(CHR_EARG, "lobpcg: Gram batch too large");
       return -1;
     }
-    hipLaunchKernelGGL(chr::gram_multi_partial_kernel, dim3(nparts), dim3(chr::kBlock), 0, s, n, gs,
-                       rows_per_part, part.as<double>());
+    if (table == 1)
+      hipLaunchKernelGGL(chr::gram_mfma_kernel<1>, dim3(nparts), dim3(chr::kBlock), 0, s, n, gs, rows_per_wave,
+                         part.as<double>());
+    else
+      hipLaunchKernelGGL(chr::gram_mfma_kernel<2>, dim3(nparts), dim3(chr::kBlock), 0, s, n, gs, rows_per_wave,
+                         part.as<double>());
     const unsigned rgrid = (unsigned)(((int64_t)total * chr::kWave + chr::kBlock - 1) / chr::kBlock);
     hipLaunchKernelGGL(chr::gram_reduce_kernel, dim3(rgrid), dim3(chr::kBlock), 0, s, nparts, total,
                        (const double *)part.as<double>(), gout.as<double>() + (size_t)nslot * kSlot);
@@ -177,12 +196,20 @@ struct Lob {
   }
   // Z (n x b) = beta Z + X (n x a) Cm (column-major a x b, host).
   int gemm(const double *X, int a, const double *Cm, int b, double beta, double *Z) {
-    if (a < 1 || b < 1 || a > 48 || b > 48) return fail(CHR_EARG, "lobpcg: block product wider than 48");
+    if (a < 1 || b < 1 || a > 16 || b > 16) return fail(CHR_EARG, "lobpcg: block product wider than 16");
     const double *cdev = static_cast<const double *>(stage(Cm, sizeof(double) * a * b));
     if (!cdev) return fail(CHR_EHIP, "lobpcg: staging a coefficient block");
-    const unsigned ggrid = (unsigned)std::min<int64_t>(2048, ((int64_t)n + chr::kGemmTile - 1) / chr::kGemmTile);
-    hipLaunchKernelGGL(chr::block_gemm_kernel, dim3(ggrid), dim3(chr::kBlock), 0, s, n, a,
-                       X, b, cdev, beta, Z);
+    hipLaunchKernelGGL((chr::tsgemm_mfma_kernel<4, 1>), dim3(tile_grid()), dim3(chr::kBlock), 0, s, n, a, X, b,
+                       cdev, 1, a, 1.0, beta, Z);
+    HIPC(hipGetLastError());
+    return CHR_OK;
+  }
+  // Z (n x b) = beta Z + X (n x a) (cscale G), G the a x b Gram product in the device slot
+  // (row-major, as gram_launch leaves it): no host round trip.
+  int gemm_slot(const double *X, int a, int slot, int b, double cscale, double beta, double *Z) {
+    if (a < 1 || b < 1 || a > 16 || b > 16) return fail(CHR_EARG, "lobpcg: block product wider than 16");
+    hipLaunchKernelGGL((chr::tsgemm_mfma_kernel<4, 1>), dim3(tile_grid()), dim3(chr::kBlock), 0, s, n, a, X, b,
+                       (const double *)(gout.as<double>() + (size_t)slot * kSlot), b, 1, cscale, beta, Z);
     HIPC(hipGetLastError());
     return CHR_OK;
   }
@@ -198,32 +225,72 @@ struct Lob {
     std::swap(X, tmp);
     return CHR_OK;
   }
-  // Pout = R ER (+ Pa EP), X <- Pout + X EX in one pass (lob_update_kernel); Pa null on a
+  // Pout = R ER (+ Pa EP), X <- Pout + X EX in one pass (lob_update_mfma_kernel); Pa null on a
   // restart.  ER, EP column-major na x k, EX k x k (host).
-  int update(const double *R, const double *Pa, int na, double *X, int k, const double *ER,
-             const double *EP, const double *EX, double *Pout) {
+  // The update coefficients [ER | EP | EX] (column-major na x k, na x k, k x k) staged as one
+  // block for both sides of the update; null on error.
+  const double *stage_update(const double *ER, const double *EP, const double *EX, int na, int k) {
+    coef.resize((size_t)(2 * na + k) * k);
+    std::copy(ER, ER + (size_t)na * k, coef.begin());
+    std::copy(EP, EP + (size_t)na * k, coef.begin() + (size_t)na * k);
+    std::copy(EX, EX + (size_t)k * k, coef.begin() + (size_t)2 * na * k);
+    return static_cast<const double *>(stage(coef.data(), sizeof(double) * coef.size()));
+  }
+  std::vector<double> coef;
+  // Pout = R ER (+ Pa EP), Xout = Pout + X EX in one pass (lob_update_mfma_kernel; Xout may
+  // be X); Pa null on a restart.  cdev: the coefficients from stage_update.
+  int update(const double *R, const double *Pa, int na, const double *X, int k, const double *cdev, double *Pout,
+             double *Xout) {
     if (na < 1 || k < 1 || na > 16 || k > 16) return fail(CHR_EARG, "lobpcg: update block wider than 16");
-    const double *er = static_cast<const double *>(stage(ER, sizeof(double) * na * k));
-    const double *ep = Pa ? static_cast<const double *>(stage(EP, sizeof(double) * na * k)) : er;
-    const double *ex = static_cast<const double *>(stage(EX, sizeof(double) * k * k));
-    if (!er || !ep || !ex) return fail(CHR_EHIP, "lobpcg: staging the update coefficients");
-    const unsigned ggrid = (unsigned)std::min<int64_t>(2048, ((int64_t)n + chr::kGemmTile - 1) / chr::kGemmTile);
-    hipLaunchKernelGGL(chr::lob_update_kernel, dim3(ggrid), dim3(chr::kBlock), 0, s, n, na, k, R, Pa,
-                       (const double *)X, er, Pa ? ep : nullptr, ex, Pout, X);
+    if (!cdev) return fail(CHR_EHIP, "lobpcg: staging the update coefficients");
+    const double *er = cdev, *ep = cdev + (size_t)na * k, *ex = cdev + (size_t)2 * na * k;
+    hipLaunchKernelGGL(chr::lob_update_mfma_kernel, dim3(tile_grid()), dim3(chr::kBlock), 0, s, n, na, k, R, Pa,
+                       X, er, Pa ? ep : nullptr, ex, Pout, Xout);
     HIPC(hipGetLastError());
     return CHR_OK;
   }
-  int resid(const double *AX, const double *X, const double *lam_host, int k, double *R) {
+  // Enqueues the Gram product R^T R of the residual block R = AX - X diag(lam) (n x k, never
+  // stored) into the next slot; returns the slot, or -1 on error.
+  int resid_norms(const double *AX, const double *X, const double *lam_host, int k) {
     const double *ldev = static_cast<const double *>(stage(lam_host, sizeof(double) * k));
-    if (!ldev) return fail(CHR_EHIP, "lobpcg: staging eigenvalues");
-    hipLaunchKernelGGL(chr::block_resid_kernel, dim3(grid((int64_t)n * k)), dim3(chr::kBlock), 0, s, n, k,
-                       AX, X, ldev, R);
+    if (!ldev || k < 1 || k > 16 || nslot >= kSlots) {
+      fail(CHR_EARG, "lobpcg: residual norms");
+      return -1;
+    }
+    chr::GramSet gs{};
+    gs.blk[0] = AX;
+    gs.blk[1] = X;
+    gs.w[0] = gs.w[1] = k;
+    gs.po[0] = 0;
+    gs.po[1] = k * k;
+    gs.lam = ldev;
+    hipLaunchKernelGGL(chr::gram_mfma_kernel<3>, dim3(nparts), dim3(chr::kBlock), 0, s, n, gs, rows_per_wave,
+                       part.as<double>());
+    const unsigned rgrid = (unsigned)((k * k * chr::kWave + chr::kBlock - 1) / chr::kBlock);
+    hipLaunchKernelGGL(chr::gram_reduce_kernel, dim3(rgrid), dim3(chr::kBlock), 0, s, nparts, k * k,
+                       (const double *)part.as<double>(), gout.as<double>() + (size_t)nslot * kSlot);
+    if (hipGetLastError() != hipSuccess) {
+      fail(CHR_EHIP, "lobpcg: residual norms launch");
+      return -1;
+    }
+    return nslot++;
+  }
+  // Y (n x m) = the columns cols of the residual block AX - X diag(lam) (n x k).
+  int resid_gather(const double *AX, const double *X, const double *lam_host, int k, const std::vector<int> &cols,
+                   double *Y) {
+    const int m = (int)cols.size();
+    std::vector<double> st((size_t)k + ((size_t)m + 1) / 2);
+    std::copy(lam_host, lam_host + k, st.begin());
+    std::memcpy(st.data() + k, cols.data(), sizeof(int) * m);
+    const double *ldev = static_cast<const double *>(stage(st.data(), sizeof(double) * st.size()));
+    if (!ldev) return fail(CHR_EHIP, "lobpcg: staging eigenvalues and a column list");
+    hipLaunchKernelGGL(chr::block_resid_gather_kernel, dim3(grid((int64_t)n * m)), dim3(chr::kBlock), 0, s, n, k,
+                       AX, X, ldev, m, reinterpret_cast<const int *>(ldev + k), Y);
     HIPC(hipGetLastError());
     return CHR_OK;
   }
-  int gather(const double *X, int k, const std::vector<int> &cols, double *Y) {
-    const int m = (int)cols.size();
-    const int *idev = static_cast<const int *>(stage(cols.data(), sizeof(int) * m));
+  // Y (n x m) = the columns idev[0..m) of X (n x k); idev staged by stage().
+  int gather(const double *X, int k, const int *idev, int m, double *Y) {
     if (!idev) return fail(CHR_EHIP, "lobpcg: staging a column list");
     hipLaunchKernelGGL(chr::block_gather_kernel, dim3(grid((int64_t)n * m)), dim3(chr::kBlock), 0, s, n, k,
                        X, m, idev, Y);
@@ -364,11 +431,11 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
   Lob L{h, A, zero.as<int>(), s, n};
   if ((rc = L.init())) return rc;
   const size_t blk = sizeof(double) * (size_t)n * neig;
-  DevBuf bX, bAX, bP, bAP, bR, bAR, bAAR, bAPa, bAAP, bBest, bT;
-  for (DevBuf *b : {&bX, &bAX, &bP, &bAP, &bR, &bAR, &bAAR, &bAPa, &bAAP, &bBest, &bT})
+  DevBuf bX, bAX, bP, bAP, bAR, bAAR, bAPa, bAAP, bBest, bT;
+  for (DevBuf *b : {&bX, &bAX, &bP, &bAP, &bAR, &bAAR, &bAPa, &bAAP, &bBest, &bT})
     if ((rc = b->alloc(blk))) return rc;
   double *dX = bX.as<double>(), *dAX = bAX.as<double>(), *dP = bP.as<double>(), *dAP = bAP.as<double>();
-  double *dR = bR.as<double>(), *dAR = bAR.as<double>(), *dAAR = bAAR.as<double>();
+  double *dAR = bAR.as<double>(), *dAAR = bAAR.as<double>();
   double *dAPa = bAPa.as<double>(), *dAAP = bAAP.as<double>(), *dBest = bBest.as<double>();
   double *dT = bT.as<double>();
   HIPC(hipMemsetAsync(dP, 0, blk, s));
@@ -436,11 +503,28 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     if (r == 1) ierr = 1;
     for (int j = 0; j < neig; ++j) hist(lambda_hist, 0, j, lam[j]);
   }
+  // X buffers: the current block, the best one (best_blockX of lobpcg.m, which aliases the
+  // current one until the next update writes a fresh buffer) and a spare; no block copies.
+  DevBuf bSpare;
+  if ((rc = bSpare.alloc(blk))) return rc;
+  double *const xpool[3] = {dX, dBest, bSpare.as<double>()};
+  auto xfree = [&]() {  // a pool buffer that is neither the current nor the best block
+    for (double *p : xpool)
+      if (p != dX && p != dBest) return p;
+    return (double *)nullptr;
+  };
+  // R^T R of R = AX - X diag(lam), fetched (R itself is never stored).
+  auto resid_norms = [&](std::vector<double> &rr) -> int {
+    const int sl = L.resid_norms(dAX, dX, lam.data(), neig);
+    if (sl < 0) return CHR_EARG;
+    if (int r = L.fetch()) return r;
+    L.get(sl, neig, neig, rr);
+    return CHR_OK;
+  };
   while (!ierr && iter < itmax - 1) {
     ++iter;
-    if ((rc = L.resid(dAX, dX, lam.data(), neig, dR))) return rc;
     std::vector<double> rr;
-    if ((rc = L.gram(dR, neig, dR, neig, rr))) return rc;
+    if ((rc = resid_norms(rr))) return rc;
     double avg = 0.0;
     for (int j = 0; j < neig; ++j) {
       resn[j] = std::sqrt(rr[j + (size_t)j * neig]) / lam[j];
@@ -450,12 +534,13 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
       lam_old[j] = lam[j];
     }
     avg /= neig;
+    bool recompute_ax = false;
     if (avg < smallest) {
       smallest = avg;
-      if ((rc = L.copy(dBest, dX, neig))) return rc;
+      dBest = dX;
     } else if (avg > std::ldexp(1.0, restart_control) * smallest) {
       forced = 1;
-      if ((rc = L.applyA(dX, neig, dAX))) return rc;
+      recompute_ax = true;
     }
     act.clear();
     for (int j = 0; j < neig; ++j) {
@@ -463,19 +548,22 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
       if (!conv) act.push_back(j);
     }
     n_active = (int)act.size();
+    // The active residual columns come from the residual of this step's AX (lobpcg.m:145-149,
+    // 200), formed before a forced restart recomputes AX (lobpcg.m:166-172).
+    if (n_active > 0 && (rc = L.resid_gather(dAX, dX, lam.data(), neig, act, dAR))) return rc;
+    if (recompute_ax && (rc = L.applyA(dX, neig, dAX))) return rc;
     if (n_active == 0) break;
     const int na = n_active;
-    if ((rc = L.gather(dR, neig, act, dAR))) return rc;
+    const int *act_dev = static_cast<const int *>(L.stage(act.data(), sizeof(int) * na));
     if (iter > 1) {
-      if ((rc = L.gather(dP, neig, act, dAPa)) || (rc = L.gather(dAP, neig, act, dAAP))) return rc;
+      if ((rc = L.gather(dP, neig, act_dev, na, dAPa)) || (rc = L.gather(dAP, neig, act_dev, na, dAAP))) return rc;
     }
     if ((rc = L.applyM(dAR, na, dT))) return rc;
     if ((rc = L.orthog(dAR, na))) return rc;  // lobpcg.m:228-230
-    // AR <- AR - X (X^T AR)
-    std::vector<double> xar;
-    if ((rc = L.gram(dX, neig, dAR, na, xar))) return rc;
-    for (double &v : xar) v = -v;
-    if ((rc = L.gemm(dX, neig, xar.data(), na, 1.0, dAR))) return rc;
+    // AR <- AR - X (X^T AR), the coefficients read on the device
+    const int sxar = L.gram_launch(dX, neig, dAR, na);
+    if (sxar < 0) return CHR_EARG;
+    if ((rc = L.gemm_slot(dX, neig, sxar, na, -1.0, 1.0, dAR))) return rc;
     if ((rc = L.borth(dAR, na, U, dT, failed))) return rc;
     if (failed) {
       ierr = 2;
@@ -579,16 +667,18 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
         for (int i = 0; i < na; ++i) EP[i + (size_t)j * na] = e[neig + na + i];
     }
     // P = AR ER + APa EP, X = P + X EX; AP and AX alike (lobpcg.m:389-400).
-    if ((rc = L.update(dAR, restart ? nullptr : dAPa, na, dX, neig, ER.data(), EP.data(), EX.data(), dP)) ||
-        (rc = L.update(dAAR, restart ? nullptr : dAAP, na, dAX, neig, ER.data(), EP.data(), EX.data(), dAP)))
+    const double *cdev = L.stage_update(ER.data(), EP.data(), EX.data(), na, neig);
+    double *xout = dX == dBest ? xfree() : dX;  // keep the best block intact
+    if ((rc = L.update(dAR, restart ? nullptr : dAPa, na, dX, neig, cdev, dP, xout)) ||
+        (rc = L.update(dAAR, restart ? nullptr : dAAP, na, dAX, neig, cdev, dAP, dAX)))
       return rc;
+    dX = xout;
   }
   if (ierr != 1) {
     if (n_active > 0) {
       ++iter;
-      if ((rc = L.resid(dAX, dX, lam.data(), neig, dR))) return rc;
       std::vector<double> rr;
-      if ((rc = L.gram(dR, neig, dR, neig, rr))) return rc;
+      if ((rc = resid_norms(rr))) return rc;
       for (int j = 0; j < neig; ++j) {
         resn[j] = std::sqrt(rr[j + (size_t)j * neig]) / lam[j];
         hist(resnorm_hist, iter - 1, j, resn[j]);
@@ -599,7 +689,7 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     for (int j = 0; j < neig; ++j) mx = std::max(mx, reslambda_check ? relDelta[j] : resn[j]);
     if (mx > tol) {
       ierr = -1;
-      if ((rc = L.copy(dX, dBest, neig))) return rc;
+      dX = dBest;
     }
     // Final Rayleigh-Ritz without preconditioning (lobpcg.m:420-459).
     ++iter;
@@ -610,9 +700,8 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     if (r == 1) {
       ierr = 1;
     } else {
-      if ((rc = L.resid(dAX, dX, lam.data(), neig, dR))) return rc;
       std::vector<double> rr;
-      if ((rc = L.gram(dR, neig, dR, neig, rr))) return rc;
+      if ((rc = resid_norms(rr))) return rc;
       for (int j = 0; j < neig; ++j) {
         resn[j] = std::sqrt(rr[j + (size_t)j * neig]) / lam[j];
         hist(resnorm_hist, iter - 1, j, resn[j]);

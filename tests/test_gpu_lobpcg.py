@@ -42,6 +42,24 @@ def test_lobpcg_matches_oracle(gpu, prec):
     assert np.linalg.norm(R, axis=0).max() <= 1e-6 * lam.max()
 
 
+def test_lobpcg_gram_table_equals_single_products(gpu, monkeypatch):
+    """The Rayleigh-Ritz Gram blocks computed in one pass over the blocks (a table of the
+    matrix-core Gram kernel) give the same bits as one pass per product
+    (CHRONOS_GRAM_MULTI=0): same iterations, eigenvalues and eigenvectors.  n = 13^3 and five
+    pairs, so neither the rows nor the columns fill whole 16 x 4 instruction tiles."""
+    A = H.heterogeneous_diffusion(13, 1.0, 7)[0]
+    h = H.build(A)
+    M = gpu.AMGPreconditioner(h, "fp64")
+    X0 = x0(A.nrows, 5)
+    r1 = gpu.lobpcg(None, M, X0, itmax=60, tol=1e-9)
+    monkeypatch.setenv("CHRONOS_GRAM_MULTI", "0")
+    r2 = gpu.lobpcg(None, M, X0, itmax=60, tol=1e-9)
+    print(f"iterations {r1[0]} / {r2[0]}, ierr {r1[5]} / {r2[5]}", flush=True)
+    assert r1[5] == 0 and r1[0] == r2[0]
+    np.testing.assert_array_equal(r1[1], r2[1])
+    np.testing.assert_array_equal(r1[2], r2[2])
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_lobpcg_poisson_closed_form(gpu, prec):
     """16^3 Poisson: eigenvalues within 1e-8 of the closed form.  The spectrum has exactly
