@@ -126,7 +126,9 @@ int DevPack::build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *
 //     span too many columns for 16-bit offsets;
 //  3. the sliced-ELL layout, when the matrix admits it: unsorted, then (unless unsorted
 //     pads by at most 12%) sorted in 256-row windows and with the builder's own choice of
-//     sort window.
+//     sort window; the sorted ones are skipped when the unsorted layout pads past the
+//     builder's limit or when its time, scaled down by its padding, does not beat the best
+//     candidate so far (CHRONOS_SELL_PRUNE=0 tries them anyway).
 // Each candidate is timed with HIP events, 2 warm-up and 5 timed launches; a matrix of 32 MB
 // or more is timed cache-cold (a 512 MiB device copy before every timed launch, as the
 // other products of a V-cycle evict it), smaller ones back to back.  The fastest candidate
@@ -268,6 +270,8 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
   if (!rc && sell_mode != 0 && sell_ok && !(pack_mode == 1 && M.use_pack)) {
     // Unsorted first; when its padding is within 12% of the nonzeros the sorted variants
     // (fewer padded bytes, scattered rows) are not tried.  The builder's choice last.
+    const char *env_prune = std::getenv("CHRONOS_SELL_PRUNE");
+    const bool prune = !env_sort && !(env_prune && std::atoi(env_prune) == 0);
     std::vector<int> sorts;
     if (env_sort) sorts.push_back(std::atoi(env_sort));
     else sorts = {0, 8, -1};
@@ -293,7 +297,15 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
         layouts[0] = host_build(sorts[0]);
       }
       if ((rc = M.sell.adopt(layouts[c], M.fp32))) break;
-      if (!M.sell.cb) continue;  // not a candidate with this sort window
+      if (!M.sell.cb) {
+        // Not a candidate with this sort window.  An unsorted layout padded past the limit
+        // means rows of very uneven length inside a slice; sorting them apart scatters the
+        // slice's gathers, which measured slower than the packed layout on every such matrix
+        // (level-0 G^T at 128^3: 88-149 us sorted against 69 us packed), so the sorted
+        // layouts are not tried either.
+        if (sorts[c] == 0 && prune) break;
+        continue;
+      }
       last_sort = sorts[c];
       const bool tight = sorts[c] == 0 && (double)M.sell.elems <= 1.12 * (double)M.nnz;
       const std::pair<int, int> key{M.sell.sorted ? 1 : 0, M.sell.sorted ? M.sell.sshift : 0};
@@ -309,6 +321,9 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
         if (sell_mode == 1) break;
       }
       if (tight) break;
+      // Sorting removes at most the unsorted layout's padding; when the unsorted time scaled
+      // down by that padding still does not beat the best so far, no sort window can.
+      if (!rc && prune && sorts[c] == 0 && ms * (double)M.nnz / (double)M.sell.elems >= best_ms) break;
     }
     if (!rc && best_sort != -2) {
       if (best_sort != last_sort)
@@ -333,9 +348,10 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     M.pack.release();
   }
   if (upload_trace())
-    std::fprintf(stderr, "[upload] tune %d x %d nnz %lld: stream %.3f s, pack+sell %.3f s -> %s\n", M.nrows, M.ncols,
-                 (long long)M.nnz, t_tune1 - t_tune0, now_seconds() - t_tune1,
-                 M.use_sell ? "sell" : M.use_pack ? "pack" : "stream");
+    std::fprintf(stderr, "[upload] tune %d x %d nnz %lld: stream %.3f s, pack+sell %.3f s -> %s%s PER %d\n", M.nrows,
+                 M.ncols, (long long)M.nnz, t_tune1 - t_tune0, now_seconds() - t_tune1,
+                 M.use_sell ? "sell" : M.use_pack ? "pack" : "stream", M.use_sell && M.sell.sorted ? " sorted" : "",
+                 M.per);
   return rc;
 }
 
