@@ -711,22 +711,36 @@ __device__ __forceinline__ mf64x4 mfma_f64_16x16x4(double a, double b, mf64x4 c)
 //   table 2: table 1 and the six products with AAP and APa: X'AAP, AR'AAP, APa'AAP,
 //            X'APa, AR'APa, APa'APa;
 //   table 3: R^T R of the residual block R = AX - X diag(lam), formed on the fly from
-//            blocks 0 (AX) and 1 (X) and never stored (the product's operand is block 0).
+//            blocks 0 (AX) and 1 (X) and never stored (the product's operand is block 0);
+//   tables 4 and 5: the Rayleigh-Ritz products when the narrow blocks (AR, AAR, APa, AAP,
+//            na <= 8 columns each) fit side by side in 16 columns: blocks 0 (X) and 1 (AX)
+//            and packed operands of up to four narrow blocks in slots of sw = 4 or 8 columns
+//            (columns [s sw, s sw + na) hold segment s, the rest are zero).  Table 4, one
+//            packed block (2): X'AX, X'X, X'P, P'P; table 5, two (2 = [AR | AAR],
+//            3 = [APa | AAP]): X'AX, X'X, X'P2, X'P3, P2'P2, P2'P3, P3'P3.  An entry of a
+//            packed product is the same matrix-instruction sum as in the unpacked one.
 constexpr int kGramMaxBlocks = 6, kGramMaxPairs = 12, kGramMaxOut = kGramMaxPairs * 256;
 constexpr uint64_t kGramPi = 0x530530300300ull, kGramPj = 0x555444330221ull;
-__host__ __device__ constexpr int gram_npairs(int t) { return t == 0 || t == 3 ? 1 : t == 1 ? 6 : 12; }
-__host__ __device__ constexpr int gram_nblocks(int t) { return t == 0 || t == 3 ? 2 : t == 1 ? 4 : 6; }
+constexpr uint64_t kGramPi4 = 0x2000ull, kGramPj4 = 0x2201ull, kGramPi5 = 0x3220000ull, kGramPj5 = 0x3323201ull;
+__host__ __device__ constexpr int gram_npairs(int t) {
+  return t == 0 || t == 3 ? 1 : t == 1 ? 6 : t == 2 ? 12 : t == 4 ? 4 : 7;
+}
+__host__ __device__ constexpr int gram_nblocks(int t) {
+  return t == 0 || t == 3 ? 2 : t == 1 ? 4 : t == 2 ? 6 : t == 4 ? 3 : 4;
+}
 __host__ __device__ constexpr int gram_pi(int t, int k) {
-  return t == 0 || t == 3 ? 0 : (int)((kGramPi >> (4 * k)) & 15);
+  return t == 0 || t == 3 ? 0 : (int)(((t == 4 ? kGramPi4 : t == 5 ? kGramPi5 : kGramPi) >> (4 * k)) & 15);
 }
 __host__ __device__ constexpr int gram_pj(int t, int k) {
-  return t == 0 ? 1 : t == 3 ? 0 : (int)((kGramPj >> (4 * k)) & 15);
+  return t == 0 ? 1 : t == 3 ? 0 : (int)(((t == 4 ? kGramPj4 : t == 5 ? kGramPj5 : kGramPj) >> (4 * k)) & 15);
 }
 struct GramSet {
   const double *blk[kGramMaxBlocks];
-  int w[kGramMaxBlocks];
+  int w[kGramMaxBlocks];      // widths (packed blocks: 16 or the slots used)
   int po[kGramMaxPairs + 1];  // first output entry of each product (row-major w_i x w_j)
   const double *lam;          // table 3: the k eigenvalues
+  const double *seg[2][4];    // tables 4, 5: segments of packed blocks 2 and 3
+  int nseg[2], sw, na;        // segments per packed block, slot width, segment width
 };
 
 // Per-workgroup partial Gram products: wave v of the grid takes rows [v R, v R + R) (R a
@@ -753,9 +767,16 @@ __global__ __launch_bounds__(kBlock) void gram_mfma_kernel(int n, GramSet gs, in
   bool col_ok[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    wd[b] = gs.w[b];
-    col_ok[b] = csub < wd[b];
-    src[b] = gs.blk[b] + csub;
+    if (T >= 4 && b >= 2) {  // packed: segment csub / sw, its column csub % sw
+      const int sg = csub / gs.sw, c = csub % gs.sw;
+      col_ok[b] = sg < gs.nseg[b - 2] && c < gs.na;
+      wd[b] = gs.na;
+      src[b] = col_ok[b] ? gs.seg[b - 2][sg] + c : gs.blk[0];
+    } else {
+      wd[b] = gs.w[b];
+      col_ok[b] = csub < wd[b];
+      src[b] = gs.blk[b] + csub;
+    }
   }
   const double lam_c = (T == 3 && col_ok[0]) ? gs.lam[csub] : 0.0;
   // UNR row steps per round: all their loads first, then their instructions in row order

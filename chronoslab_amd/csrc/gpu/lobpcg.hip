@@ -163,6 +163,99 @@ struct Lob {
     nslot += slots;
     return base;
   }
+  // The Rayleigh-Ritz Gram products with the narrow blocks packed side by side (tables 4
+  // and 5 of gram_mfma_kernel; na <= 8): operands X and AX (k columns) and narrow = {AR,
+  // AAR} after a restart, {AR, AAR, APa, AAP} otherwise.  Records where each operand sits
+  // (block and first column) for get_packed.  Returns the first slot, or -1 on error.
+  struct Packed {
+    int table = 0, np = 0;
+    const double *ptr[6] = {};
+    int blk[6] = {}, off[6] = {}, nptr = 0;
+    int w[4] = {}, po[8] = {};
+  };
+  int gram_packed_launch(const double *X, const double *AX, const std::vector<const double *> &narrow, int k,
+                         int na, Packed &pk) {
+    const int nn = (int)narrow.size();
+    if (k < 1 || k > 16 || na < 1 || na > 8 || (nn != 2 && nn != 4)) {
+      fail(CHR_EARG, "lobpcg: packed Gram operands");
+      return -1;
+    }
+    chr::GramSet gs{};
+    pk = Packed();
+    gs.blk[0] = X;
+    gs.blk[1] = AX;
+    gs.w[0] = gs.w[1] = k;
+    gs.na = na;
+    gs.sw = na <= 4 ? 4 : 8;
+    pk.ptr[0] = X;
+    pk.ptr[1] = AX;
+    pk.blk[1] = 1;
+    pk.nptr = 2;
+    const int per_blk = 16 / gs.sw;  // segments a packed block holds
+    const int npk = (nn + per_blk - 1) / per_blk;
+    pk.table = npk == 1 ? 4 : 5;
+    for (int s = 0; s < nn; ++s) {
+      const int b = s / per_blk, sg = s % per_blk;
+      gs.seg[b][sg] = narrow[s];
+      gs.nseg[b] = sg + 1;
+      pk.ptr[pk.nptr] = narrow[s];
+      pk.blk[pk.nptr] = 2 + b;
+      pk.off[pk.nptr++] = sg * gs.sw;
+    }
+    for (int b = 0; b < npk; ++b) {
+      gs.blk[2 + b] = narrow[(size_t)b * per_blk];  // not read: packed blocks load through seg
+      gs.w[2 + b] = gs.nseg[b] * gs.sw;
+    }
+    pk.np = chr::gram_npairs(pk.table);
+    int total = 0;
+    for (int q = 0; q < pk.np; ++q) {
+      gs.po[q] = pk.po[q] = total;
+      total += gs.w[chr::gram_pi(pk.table, q)] * gs.w[chr::gram_pj(pk.table, q)];
+    }
+    gs.po[pk.np] = pk.po[pk.np] = total;
+    for (int b = 0; b < 4; ++b) pk.w[b] = gs.w[b];
+    const int slots = (total + kSlot - 1) / kSlot;
+    if (total > chr::kGramMaxOut || nslot + slots > kSlots) {
+      fail(CHR_EARG, "lobpcg: Gram batch too large");
+      return -1;
+    }
+    if (pk.table == 4)
+      hipLaunchKernelGGL(chr::gram_mfma_kernel<4>, dim3(nparts), dim3(chr::kBlock), 0, s, n, gs, rows_per_wave,
+                         part.as<double>());
+    else
+      hipLaunchKernelGGL(chr::gram_mfma_kernel<5>, dim3(nparts), dim3(chr::kBlock), 0, s, n, gs, rows_per_wave,
+                         part.as<double>());
+    const unsigned rgrid = (unsigned)(((int64_t)total * chr::kWave + chr::kBlock - 1) / chr::kBlock);
+    hipLaunchKernelGGL(chr::gram_reduce_kernel, dim3(rgrid), dim3(chr::kBlock), 0, s, nparts, total,
+                       (const double *)part.as<double>(), gout.as<double>() + (size_t)nslot * kSlot);
+    if (hipGetLastError() != hipSuccess) {
+      fail(CHR_EHIP, "lobpcg: packed Gram launch");
+      return -1;
+    }
+    const int base = nslot;
+    nslot += slots;
+    return base;
+  }
+  // out (column-major a x b) = Xp^T Yp from the fetched packed products at slot base.
+  bool get_packed(int base, const Packed &pk, const double *Xp, int a, const double *Yp, int b,
+                  std::vector<double> &out) const {
+    int ix = -1, iy = -1;
+    for (int i = 0; i < pk.nptr; ++i) {
+      if (pk.ptr[i] == Xp) ix = i;
+      if (pk.ptr[i] == Yp) iy = i;
+    }
+    if (ix < 0 || iy < 0) return false;
+    const int u = pk.blk[ix], v = pk.blk[iy];
+    for (int q = 0; q < pk.np; ++q)
+      if (chr::gram_pi(pk.table, q) == u && chr::gram_pj(pk.table, q) == v) {
+        const double *g = hout + (size_t)base * kSlot + pk.po[q];
+        out.assign((size_t)a * b, 0.0);
+        for (int i = 0; i < a; ++i)
+          for (int j = 0; j < b; ++j) out[i + (size_t)j * a] = g[(size_t)(pk.off[ix] + i) * pk.w[v] + pk.off[iy] + j];
+        return true;
+      }
+    return false;
+  }
   // out (column-major a x b) = the fetched product at offset off of slot base.
   void get_at(int base, int off, int a, int b, std::vector<double> &out) const {
     const double *g = hout + (size_t)base * kSlot + off;
@@ -623,7 +716,18 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     for (const auto &bk : blocks) pairs.push_back({bk.X, bk.a, bk.Y, bk.b});
     std::vector<int> offs;
     int base;
-    if (const char *env = std::getenv("CHRONOS_GRAM_MULTI"); env && std::atoi(env) == 0) {
+    const char *env_multi = std::getenv("CHRONOS_GRAM_MULTI");
+    const bool one_pass = !(env_multi && std::atoi(env_multi) == 0);
+    Lob::Packed pk;
+    const bool packed = one_pass && na <= 8;
+    if (packed) {
+      std::vector<const double *> narrow = {dAR, dAAR};
+      if (!restart) {
+        narrow.push_back(dAPa);
+        narrow.push_back(dAAP);
+      }
+      if ((base = L.gram_packed_launch(dX, dAX, narrow, neig, na, pk)) < 0) return CHR_EARG;
+    } else if (!one_pass) {
       // one pass per product (same entries)
       base = -1;
       offs.clear();
@@ -641,7 +745,11 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     std::vector<double> g;
     for (size_t k = 0; k < blocks.size(); ++k) {
       const auto &bk = blocks[k];
-      L.get_at(base, offs[k], bk.a, bk.b, g);
+      if (packed) {
+        if (!L.get_packed(base, pk, bk.X, bk.a, bk.Y, bk.b, g)) return fail(CHR_EARG, "lobpcg: packed Gram lookup");
+      } else {
+        L.get_at(base, offs[k], bk.a, bk.b, g);
+      }
       if (bk.diag) {
         chrdense::symmetrize(bk.a, g.data());
         put(bk.toA ? gA : gB, bk.r0, bk.c0, bk.a, bk.b, g, false);
