@@ -186,7 +186,8 @@ class AMGPreconditioner:
     def layout(self, stored_bytes: bool = False) -> list:
         """Per level, the layout chosen for the single-vector products of A, G, G^T, P, P^T:
         ("sell", code bits) for the sliced-ELL kernel, (PER, code bits) for the stream kernel,
-        (("pack", PER), 16) for the packed-code stream kernel,
+        (("pack", PER), 16) for the packed-code stream kernel, ("dia", 0) for the symmetric
+        diagonal kernel,
         with the bytes one product reads of the stored matrix appended when stored_bytes."""
         out = []
         for l in range(self.stats()["nlev"] - 1):  # device levels (the tail may be collapsed)
@@ -195,7 +196,8 @@ class AMGPreconditioner:
                 per, bits, nnz, sb = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
                 _check(_lib().chr_hier_layout(self.ptr, l, w.encode(), C.byref(per), C.byref(bits),
                                               C.byref(nnz), C.byref(sb)))
-                kind = "sell" if per.value == 0 else ("pack", -per.value) if per.value < 0 else per.value
+                kind = (("dia" if bits.value == 0 else "sell") if per.value == 0
+                        else ("pack", -per.value) if per.value < 0 else per.value)
                 row[w] = (kind, bits.value) + ((sb.value,) if stored_bytes else ())
             out.append(row)
         return out

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "runtime.hpp"
+#include "dia_layout.hpp"
 #include "pack_layout.hpp"
 #include "sell_layout.hpp"
 
@@ -100,6 +101,28 @@ int DevSell::adopt(const chrsell::Layout &L, bool fp32) {
   return CHR_OK;
 }
 
+int DevDia::build(int32_t n_, int32_t ncols, const int32_t *hrp, const int32_t *hci, const double *hv,
+                  bool fp32) {
+  release();
+  n = n_;
+  chrdia::Layout L;
+  if (!chrdia::build(n_, ncols, hrp, hci, hv, L)) return CHR_OK;
+  const size_t cnt = (size_t)L.m * n_;
+  int rc;
+  if ((rc = val.alloc((fp32 ? 4 : 8) * cnt))) return rc;
+  if (fp32) {
+    std::vector<float> f(L.val.begin(), L.val.end());
+    HIPC(hipMemcpy(val.p, f.data(), 4 * cnt, hipMemcpyHostToDevice));
+  } else {
+    HIPC(hipMemcpy(val.p, L.val.data(), 8 * cnt, hipMemcpyHostToDevice));
+  }
+  m = L.m;
+  offs = chr::DiaOffs{};
+  offs.m = L.m;
+  for (int k = 0; k < L.m; ++k) offs.off[k] = L.off[k];
+  return CHR_OK;
+}
+
 int DevPack::build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *hci, int per_) {
   release();
   chrpack::Layout L;
@@ -145,6 +168,9 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
   M.sell.release();
   M.use_pack = false;
   M.pack.release();
+  M.use_dia = false;
+  M.dia_pcg = false;
+  M.dia.release();
   if (M.nnz == 0) return CHR_OK;
   const char *env_pack = std::getenv("CHRONOS_PACK");
   const int pack_mode = env_pack ? std::atoi(env_pack) : -1;  // -1 timed, 0 off, 1 forced
@@ -201,7 +227,9 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     }
     if (e) return fail(CHR_EHIP, std::string("autotune: ") + hipGetErrorString(e));
     if (upload_trace()) {
-      if (M.use_sell)
+      if (M.use_dia)
+        std::fprintf(stderr, "[upload]   %d rows: dia %d diagonals: %.4f ms\n", M.nrows, M.dia.m, ms / 5);
+      else if (M.use_sell)
         std::fprintf(stderr, "[upload]   %d rows: sell cb %d sort %d (shift %d) elems/nnz %.3f: %.4f ms\n", M.nrows,
                      M.sell.cb, (int)M.sell.sorted, M.sell.sshift, (double)M.sell.elems / (double)M.nnz, ms / 5);
       else
@@ -341,17 +369,51 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
       M.sell.release();
     }
   }
+  // Step 4: the symmetric diagonal layout, when the matrix admits it (a few diagonals,
+  // values symmetric bit for bit).  For the outer operator of PCG (tune_per = false) it also
+  // carries the solver's p-product whenever it is built, a fixed rule, so that the solver's
+  // rounding (its dot products are summed per row block of the product's layout) never
+  // depends on a timing or on the single-vector layout.  CHRONOS_DIA=0 excludes it,
+  // CHRONOS_DIA=1 forces it for single-vector products; a forced sliced-ELL or packed
+  // layout keeps the single-vector products off it unless CHRONOS_DIA=1.
+  const char *env_dia = std::getenv("CHRONOS_DIA");
+  const int dia_mode = env_dia ? std::atoi(env_dia) : -1;
+  if (!rc && dia_mode != 0 && !(rc = M.dia.build(M.nrows, M.ncols, hrp, hci, hv, M.fp32)) && M.dia.m) {
+    M.dia_pcg = !tune_per;
+    if (dia_mode == 1) {
+      M.use_dia = true;
+    } else if (sell_mode != 1 && pack_mode != 1) {
+      float cur_ms = 0.f, ms = 0.f;
+      rc = time_it(cur_ms);
+      if (!rc) {
+        M.use_dia = true;
+        rc = time_it(ms);
+        M.use_dia = !rc && ms < 0.97f * cur_ms;
+      }
+    }
+    if (M.use_dia) {
+      M.use_sell = false;
+      M.sell.release();
+      M.use_pack = false;
+      M.pack.release();
+    } else if (!M.dia_pcg) {
+      M.dia.release();
+    }
+  }
   if (rc) {
     M.use_sell = false;
     M.sell.release();
     M.use_pack = false;
     M.pack.release();
+    M.use_dia = false;
+    M.dia_pcg = false;
+    M.dia.release();
   }
   if (upload_trace())
-    std::fprintf(stderr, "[upload] tune %d x %d nnz %lld: stream %.3f s, pack+sell %.3f s -> %s%s PER %d\n", M.nrows,
-                 M.ncols, (long long)M.nnz, t_tune1 - t_tune0, now_seconds() - t_tune1,
-                 M.use_sell ? "sell" : M.use_pack ? "pack" : "stream", M.use_sell && M.sell.sorted ? " sorted" : "",
-                 M.per);
+    std::fprintf(stderr, "[upload] tune %d x %d nnz %lld: stream %.3f s, pack+sell+dia %.3f s -> %s%s PER %d%s\n",
+                 M.nrows, M.ncols, (long long)M.nnz, t_tune1 - t_tune0, now_seconds() - t_tune1,
+                 M.use_dia ? "dia" : M.use_sell ? "sell" : M.use_pack ? "pack" : "stream",
+                 M.use_sell && M.sell.sorted ? " sorted" : "", M.per, M.dia_pcg ? " (PCG product: dia)" : "");
   return rc;
 }
 
@@ -943,8 +1005,8 @@ extern "C" int chr_hier_layout(const chr_hier *h, int32_t level, char which, int
                   : which == 'r' && L.fuse_r ? &L.R : which == 'u' && L.fuse_up ? &L.U
                   : which == 'w' && L.fuse_up ? &L.W : which == 's' && L.fuse_s ? &L.S : nullptr;
   if (!M) return fail(CHR_EARG, "chr_hier_layout: which must be A, G, T, P or R, or a composite (r, u, w, s) the level has");
-  if (per) *per = M->use_sell ? 0 : M->use_pack ? -M->pack.per : M->per;
-  if (code_bits) *code_bits = M->use_sell ? M->sell.cb : (M->use_pack || M->idx16) ? 16 : 32;
+  if (per) *per = (M->use_sell || M->use_dia) ? 0 : M->use_pack ? -M->pack.per : M->per;
+  if (code_bits) *code_bits = M->use_dia ? 0 : M->use_sell ? M->sell.cb : (M->use_pack || M->idx16) ? 16 : 32;
   if (nnz) *nnz = M->nnz;
   if (stored_bytes) *stored_bytes = M->stream_bytes();
   return CHR_OK;
@@ -1068,8 +1130,26 @@ hipError_t launch_pcg_p_t(const DevCsr &A, const PcgPtrs &w, const double *pold,
                 : launch_pcg_p_f<PER, TI, false>(A, w, pold, pnew, st, s);
 }
 
+template <int MAXM>
+hipError_t launch_pcg_p_dia(const DevDia &D, const PcgPtrs &w, const double *pnew, const chr::PcgState *st,
+                            hipStream_t s) {
+  hipLaunchKernelGGL((chr::pcg_p_dia_kernel<MAXM>), dim3(D.grid()), dim3(chr::kBlock), 0, s, D.n, D.offs,
+                     D.val.as<double>(), pnew, w.ap, (const double *)w.r, w.part_p, st);
+  return hipGetLastError();
+}
+
+// Row blocks of the PCG p-product, each with two partial sums.
+int pcg_p_parts(const DevCsr &A) { return A.dia_pcg ? (int)A.dia.grid() : std::max(A.nblk, 1); }
+
 hipError_t launch_pcg_p(const DevCsr &A, const PcgPtrs &w, const double *pold, double *pnew,
                         const chr::PcgState *st, bool formed, hipStream_t s) {
+  if (A.dia_pcg) {
+    // p_new is formed by its own pass before this product (pcg_pnew_kernel).
+    if (!formed) return hipErrorInvalidValue;
+    if (A.dia.m <= 4) return launch_pcg_p_dia<4>(A.dia, w, pnew, st, s);
+    if (A.dia.m <= 8) return launch_pcg_p_dia<8>(A.dia, w, pnew, st, s);
+    return launch_pcg_p_dia<16>(A.dia, w, pnew, st, s);
+  }
   switch (A.per * 2 + (A.idx16 ? 1 : 0)) {
     case 8: return launch_pcg_p_t<4, int>(A, w, pold, pnew, st, formed, s);
     case 9: return launch_pcg_p_t<4, uint16_t>(A, w, pold, pnew, st, formed, s);
@@ -1084,7 +1164,8 @@ hipError_t launch_pcg_p(const DevCsr &A, const PcgPtrs &w, const double *pold, d
 // operand per nonzero instead of two (same bits either way; measured faster on the 7-point
 // diffusion rows and on the 81-entry elasticity rows alike).  CHRONOS_PCG_FORMED=0 selects
 // the single-pass form.
-bool pcg_formed(const DevCsr &) {
+bool pcg_formed(const DevCsr &A) {
+  if (A.dia_pcg) return true;  // the diagonal product gathers p_new alone
   if (const char *e = std::getenv("CHRONOS_PCG_FORMED")) return std::atoi(e) != 0;
   return true;
 }
@@ -1111,7 +1192,7 @@ hipError_t enqueue_pcg_iter(chr_hier &h, const DevCsr &A, const PcgPtrs &w, int 
     if ((e = hipGetLastError())) return e;
   }
   if ((e = launch_pcg_p(A, w, pold, pnew, st, formed, h.stream))) return e;
-  if ((e = pcg_finalize(chr::kFinAlpha, A.nblk, w.part_p, st, w.resvec, h.stream))) return e;
+  if ((e = pcg_finalize(chr::kFinAlpha, pcg_p_parts(A), w.part_p, st, w.resvec, h.stream))) return e;
   hipLaunchKernelGGL(chr::pcg_update_kernel, dim3(w.nblk), dim3(chr::kBlock), 0, h.stream, w.n,
                      (const double *)pnew, (const double *)w.ap, w.x, w.r, w.r32, w.part,
                      (const chr::PcgState *)st);
@@ -1168,7 +1249,7 @@ int pcg_prepare(chr_hier &h, const DevCsr &A, int32_t itmax) {
   }
   // p-SpMV partials: two per row block of the outer operator (its layout may differ from
   // one operator to the next even at equal n).
-  const size_t need = 2 * (size_t)std::max(A.nblk, 1);
+  const size_t need = 2 * (size_t)pcg_p_parts(A);
   if (w.part_p_cap < need) {
     w.drop_graphs();
     if ((rc = w.part_p.alloc(sizeof(double) * need))) return rc;
@@ -1358,7 +1439,7 @@ extern "C" int chr_pcg_solve_noalloc(chr_hier *h, chr_mat *Aop, const double *b,
   const size_t o_r = take(vb), o_z = take(vb), o_p0 = take(vb), o_p1 = take(vb), o_ap = take(vb);
   const size_t o_r32 = take(h->fp32() ? 4 * (size_t)n : 0);
   const size_t o_part = take(sizeof(double) * 2 * nblk);
-  const size_t o_partp = take(sizeof(double) * 2 * (size_t)std::max(A.nblk, 1));
+  const size_t o_partp = take(sizeof(double) * 2 * (size_t)pcg_p_parts(A));
   const size_t o_res = take(sizeof(double) * ((size_t)itmax + 1));
   const size_t o_st = take(sizeof(chr::PcgState));
   if (off > kScratchBytes)

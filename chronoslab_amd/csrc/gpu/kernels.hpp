@@ -481,6 +481,80 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
 }
 
 // ---------------------------------------------------------------------------------------
+// Symmetric diagonal layout (dia_layout.hpp): m stored diagonals at offsets off[0] = 0 <
+// off[1] < ... < off[m-1], diagonal k at val + k n holding A(i, i + off[k]); the lower
+// triangle is read from the mirrored upper entry, A(i, i - off[k]) = val[k n + i - off[k]].
+// One row per thread; every load is a contiguous run of the wave's rows (no column
+// indices, no gathers).  Row i is summed in column order (lower diagonals from the
+// farthest, the main diagonal, upper diagonals from the nearest) with the product rounded
+// before the add and zero (absent) entries skipped: the sequential CSR order, so the same
+// bits as the other layouts.
+// ---------------------------------------------------------------------------------------
+struct DiaOffs {
+  int m;
+  int off[16];
+};
+
+// Loads of row i (clamped into [0, n)): lower/upper values and operands of diagonals
+// 1..MAXM-1 (those >= m and those outside the matrix read as zero values), and the main
+// diagonal.
+template <int MAXM, typename TV, typename TA, typename XF>
+struct DiaRow {
+  TV lo[MAXM], up[MAXM], dg;
+  TA xl[MAXM], xu[MAXM], xd;
+  __device__ __forceinline__ void load(int i, int n, const DiaOffs &d, const TV *__restrict__ val, XF xf) {
+#pragma unroll
+    for (int k = 1; k < MAXM; ++k) {
+      lo[k] = up[k] = TV(0);
+      xl[k] = xu[k] = TA(0);
+      if (k < d.m) {
+        const int jl = i - d.off[k], ju = i + d.off[k];
+        const int jlc = max(jl, 0), juc = min(ju, n - 1);
+        const TV a = val[(size_t)k * n + jlc];
+        lo[k] = jl >= 0 ? a : TV(0);
+        xl[k] = (TA)xf(jlc);
+        up[k] = val[(size_t)k * n + i];
+        xu[k] = (TA)xf(juc);
+      }
+    }
+    dg = val[i];
+    xd = (TA)xf(i);
+  }
+  __device__ __forceinline__ TA sum() const {
+    TA s = 0;
+#pragma unroll
+    for (int k = MAXM - 1; k >= 1; --k)
+      if (lo[k] != TV(0)) s = s + mul_rn((TA)lo[k], xl[k]);
+    if (dg != TV(0)) s = s + mul_rn((TA)dg, xd);
+#pragma unroll
+    for (int k = 1; k < MAXM; ++k)
+      if (up[k] != TV(0)) s = s + mul_rn((TA)up[k], xu[k]);
+    return s;
+  }
+};
+
+template <typename TX>
+struct VecLoad {
+  const TX *__restrict__ x;
+  __device__ __forceinline__ TX operator()(int j) const { return x[j]; }
+};
+
+template <int MAXM, typename TV, typename TX, typename Epi>
+__global__ __launch_bounds__(kBlock) void spmv_dia_kernel(int n, DiaOffs d, const TV *__restrict__ val,
+                                                          const TX *__restrict__ x, Epi epi,
+                                                          const int *__restrict__ stop) {
+  using TA = typename Acc<TV>::type;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i = b * kBlock + (int)threadIdx.x;
+  const int ic = min(i, n - 1);
+  DiaRow<MAXM, TV, TA, VecLoad<TX>> row;
+  row.load(ic, n, d, val, VecLoad<TX>{x});
+  const double pv = epi.pre(ic);
+  if (*stop || i >= n) return;
+  epi(i, (double)row.sum(), pv);
+}
+
+// ---------------------------------------------------------------------------------------
 // Block product Y = M X for row-major n x k blocks (element (i, c) at i*k + c, k <= KL), on
 // the stream layout's row blocks.  The block's values and column indices are staged in LDS
 // with coalesced loads (cap = the layout's PER * kBlock nonzeros, in dynamic LDS); then each
@@ -657,16 +731,41 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(int nrows, int nd,
                                                              const TX *__restrict__ b,
                                                              TX *__restrict__ x,
                                                              const int *__restrict__ stop, int k) {
-  if (stop && *stop) return;
   const int t = (blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int lane = threadIdx.x % kWave;
   const int c = blockIdx.y;
   if (t >= nrows) return;
   const int4 info = cmap[t];
+  if (stop && *stop) return;
   const int row = nd + t, m = info.y, r0 = info.x;
   const double *a = Ainv + (size_t)info.z + (size_t)(row - r0) * m;
+  // Each lane adds a[j] b[j] for j = lane, lane + 64, ... in order; the loads of sixteen
+  // steps are issued before their adds (one memory round trip per 1024 columns).
+  constexpr int kAhead = 16;
   double acc = 0.0;
-  for (int j = lane; j < m; j += kWave) acc += a[j] * (double)b[(size_t)(r0 + j) * k + c];
+  int j = lane;
+  for (; j + (kAhead - 1) * kWave < m; j += kAhead * kWave) {
+    double av[kAhead], bv[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+      av[u] = a[j + u * kWave];
+      bv[u] = (double)b[(size_t)(r0 + j + u * kWave) * k + c];
+    }
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) acc += av[u] * bv[u];
+  }
+  if (j < m) {
+    double av[kAhead], bv[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+      const int jj = min(j + u * kWave, m - 1);
+      av[u] = a[jj];
+      bv[u] = (double)b[(size_t)(r0 + jj) * k + c];
+    }
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u)
+      if (j + u * kWave < m) acc += av[u] * bv[u];
+  }
   acc = group_sum<kWave>(acc);
   if (lane == 0) x[(size_t)row * k + c] = (TX)acc;
 }
@@ -1085,6 +1184,38 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
   pap = block_sum_t<double>(pap, prod);
   prr = block_sum_t<double>(prr, prod);
   if (tid == 0) {
+    part[b] = pap;
+    part[nblk + b] = prr;
+  }
+}
+
+// Ap = A p in the symmetric diagonal layout (spmv_dia_kernel's loads and summation order)
+// and per-block partials of p.Ap and p.r, one block of kBlock rows per workgroup (grid =
+// the blocks, partials at part[b] and part[nblk + b] for the row block b).
+template <int MAXM>
+__global__ __launch_bounds__(kBlock) void pcg_p_dia_kernel(int n, DiaOffs d, const double *__restrict__ val,
+                                                           const double *__restrict__ p, double *__restrict__ ap,
+                                                           const double *__restrict__ r,
+                                                           double *__restrict__ part, const PcgState *st) {
+  __shared__ double lds[kBlock / kWave];
+  const int nblk = (int)gridDim.x;
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int i = b * kBlock + (int)threadIdx.x;
+  const int ic = min(i, n - 1);
+  DiaRow<MAXM, double, double, VecLoad<double>> row;
+  row.load(ic, n, d, val, VecLoad<double>{p});
+  const double ri = r[ic];
+  if (st->stop) return;
+  double pap = 0.0, prr = 0.0;
+  if (i < n) {
+    const double s = row.sum();
+    ap[i] = s;
+    pap = row.xd * s;
+    prr = row.xd * ri;
+  }
+  pap = block_sum_t<double>(pap, lds);
+  prr = block_sum_t<double>(prr, lds);
+  if (threadIdx.x == 0) {
     part[b] = pap;
     part[nblk + b] = prr;
   }

@@ -157,6 +157,23 @@ struct DevSell {
   }
 };
 
+// Symmetric diagonal copy of a square matrix (dia_layout.hpp) for single-vector products
+// (spmv_dia_kernel) and the PCG operator product (pcg_p_dia_kernel).
+struct DevDia {
+  int32_t n = 0, m = 0;  // m = 0: not built
+  chr::DiaOffs offs{};
+  DevBuf val;
+  // Builds the layout from host CSR and uploads it; leaves m = 0 (and returns CHR_OK) when
+  // the matrix is not a candidate.
+  int build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *hci, const double *hv, bool fp32);
+  void release() {
+    val = DevBuf();
+    m = 0;
+  }
+  double bytes(bool fp32) const { return (double)m * n * (fp32 ? 4 : 8); }
+  unsigned grid() const { return (unsigned)((n + chr::kBlock - 1) / chr::kBlock); }
+};
+
 // Packed-code index copy for single-vector stream products (spmv_pack_kernel): block table
 // with dictionary windows, one 16-bit code per nonzero, per-window dictionaries.  Values
 // and row pointers are the DevCsr's own (same nonzero order).
@@ -188,6 +205,9 @@ struct DevCsr {
   bool use_sell = false;
   DevPack pack;        // packed-code index copy, used for single-vector products when use_pack
   bool use_pack = false;
+  DevDia dia;          // symmetric diagonal copy, used for single-vector products when use_dia
+  bool use_dia = false;
+  bool dia_pcg = false;  // the PCG operator product runs on dia (a fixed rule, never timed)
   int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
              bool as_fp32, int per_ = 8) {
     nrows = nr;
@@ -263,6 +283,7 @@ struct DevCsr {
   // Bytes of one single-vector product as stored: values + column codes + row pointers and
   // block table (stream layout) or slice offsets, permutation and dictionaries (sliced ELL).
   double stream_bytes() const {
+    if (use_dia) return dia.bytes(fp32);
     if (use_sell) return sell.bytes(fp32);
     if (use_pack)
       return (double)nnz * ((fp32 ? 4 : 8) + 2) + 4.0 * (nrows + 1) + 16.0 * (pack.nblk + 1) + (double)pack.dict.bytes;
@@ -273,9 +294,23 @@ struct DevCsr {
   double device_bytes() const {
     return (double)(rp.bytes + ci.bytes + val.bytes + blk.bytes + sell.sp.bytes + sell.perm.bytes +
                     sell.val.bytes + sell.code.bytes + sell.dict.bytes + pack.blk.bytes + pack.code.bytes +
-                    pack.dict.bytes);
+                    pack.dict.bytes + dia.val.bytes);
   }
 };
+
+template <int MAXM, typename TV, typename TX, typename Epi>
+hipError_t launch_dia_m(const DevDia &D, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  hipLaunchKernelGGL((chr::spmv_dia_kernel<MAXM, TV, TX, Epi>), dim3(D.grid()), dim3(chr::kBlock), 0, s, D.n,
+                     D.offs, D.val.as<TV>(), x, epi, stop);
+  return hipGetLastError();
+}
+
+template <typename TV, typename TX, typename Epi>
+hipError_t launch_dia(const DevDia &D, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  if (D.m <= 4) return launch_dia_m<4, TV>(D, x, epi, stop, s);
+  if (D.m <= 8) return launch_dia_m<8, TV>(D, x, epi, stop, s);
+  return launch_dia_m<16, TV>(D, x, epi, stop, s);
+}
 
 template <int CB, int U, typename TV, typename TX, typename Epi>
 hipError_t launch_sell_cb(const DevSell &S, const TX *x, Epi epi, const int *stop, hipStream_t s) {
@@ -332,6 +367,7 @@ template <typename TV, typename XS, typename Epi>
 hipError_t launch_spmv_t(const DevCsr &M, XS x, Epi epi, const int *stop, hipStream_t s) {
   if (M.nrows == 0) return hipSuccess;
   if constexpr (is_x_one<XS>::value) {
+    if (M.use_dia) return launch_dia<TV>(M.dia, x.a, epi, stop, s);
     if (M.use_sell) return launch_sell<TV>(M.sell, x.a, epi, stop, s);
     if (M.use_pack) {
       switch (M.pack.per) {
@@ -342,7 +378,7 @@ hipError_t launch_spmv_t(const DevCsr &M, XS x, Epi epi, const int *stop, hipStr
     }
   } else {
     // Composite operators (two operand vectors) are kept in the stream layout only.
-    if (M.use_sell || M.use_pack) return hipErrorInvalidValue;
+    if (M.use_sell || M.use_pack || M.use_dia) return hipErrorInvalidValue;
   }
   switch (M.per) {
     case 4: return launch_spmv_p<4, TV>(M, x, epi, stop, s);

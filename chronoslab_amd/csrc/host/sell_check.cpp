@@ -1,6 +1,7 @@
-// Host replays of the GPU library's sliced-ELL product (spmv_sell_kernel) and packed-code
-// stream product (spmv_pack_kernel): each builds the layout with the same builder
-// (sell_layout.hpp, pack_layout.hpp) and walks it with the kernel's own index arithmetic,
+// Host replays of the GPU library's sliced-ELL product (spmv_sell_kernel), packed-code
+// stream product (spmv_pack_kernel) and symmetric diagonal product (spmv_dia_kernel): each
+// builds the layout with the same builder (sell_layout.hpp, pack_layout.hpp,
+// dia_layout.hpp) and walks it with the kernel's own index arithmetic,
 // workgroup by workgroup and lane by lane, checking every load against the buffers the
 // kernel would read.  It lets the layout and the kernel's addressing be
 // verified without a GPU; the product it returns has the kernel's summation order and
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "chronos_setup.h"
+#include "dia_layout.hpp"
 #include "pack_layout.hpp"
 #include "sell_layout.hpp"
 
@@ -203,4 +205,58 @@ extern "C" int chs_pack_apply(int32_t n, int32_t ncols, const int32_t *rowptr, c
     return replay_pack<float>(L, rowptr, vf.data(), nnz, xf.data(), y);
   }
   return replay_pack<double>(L, rowptr, val, nnz, x, y);
+}
+
+namespace {
+
+// spmv_dia_kernel row by row: the loads of DiaRow::load (indices clamped into [0, n), a
+// lower entry outside the matrix read as zero), then DiaRow::sum's order and rounding.
+template <typename TA>
+int replay_dia(const chrdia::Layout &L, const std::vector<double> &val, const double *x, double *y) {
+  const int32_t n = L.n;
+  const int64_t nv = (int64_t)val.size();
+  if (L.m < 1 || L.m > chrdia::kMaxDiag || L.off[0] != 0) return -31;
+  for (int32_t i = 0; i < n; ++i) {
+    TA lo[chrdia::kMaxDiag], up[chrdia::kMaxDiag], xl[chrdia::kMaxDiag], xu[chrdia::kMaxDiag];
+    for (int k = 1; k < L.m; ++k) {
+      if (L.off[k] <= L.off[k - 1]) return -32;
+      const int64_t jl = (int64_t)i - L.off[k], ju = (int64_t)i + L.off[k];
+      const int64_t jlc = std::max<int64_t>(jl, 0), juc = std::min<int64_t>(ju, n - 1);
+      const int64_t el = (int64_t)k * n + jlc, eu = (int64_t)k * n + i;
+      if (el < 0 || el >= nv || eu >= nv) return -33;
+      lo[k] = jl >= 0 ? (TA)val[(size_t)el] : TA(0);
+      up[k] = (TA)val[(size_t)eu];
+      xl[k] = (TA)x[jlc];
+      xu[k] = (TA)x[juc];
+      if (ju >= n && up[k] != TA(0)) return -34;  // an upper entry past the last column
+    }
+    TA s = 0;
+    for (int k = L.m - 1; k >= 1; --k)
+      if (lo[k] != TA(0)) s = s + lo[k] * xl[k];
+    if ((TA)val[(size_t)i] != TA(0)) s = s + (TA)val[(size_t)i] * (TA)x[i];
+    for (int k = 1; k < L.m; ++k)
+      if (up[k] != TA(0)) s = s + up[k] * xu[k];
+    y[i] = (double)s;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int chs_dia_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                             const double *val, int32_t fp32, const double *x, double *y, int32_t *ndiag,
+                             int32_t *offsets) {
+  chrdia::Layout L;
+  if (!chrdia::build(n, ncols, rowptr, col, val, L)) return 1;
+  if (ndiag) *ndiag = L.m;
+  if (offsets)
+    for (int k = 0; k < L.m; ++k) offsets[k] = L.off[k];
+  if (fp32) {
+    // The kernel reads FP32 values and an FP32 vector.
+    std::vector<double> vf(L.val.size()), xf((size_t)n);
+    for (size_t e = 0; e < vf.size(); ++e) vf[e] = (double)(float)L.val[e];
+    for (int32_t i = 0; i < n; ++i) xf[(size_t)i] = (double)(float)x[i];
+    return replay_dia<float>(L, vf, xf.data(), y);
+  }
+  return replay_dia<double>(L, L.val, x, y);
 }

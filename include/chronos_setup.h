@@ -313,6 +313,16 @@ int chs_sell_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_
 int chs_pack_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
                    const double *val, int32_t per, int32_t fp32, const double *x, double *y,
                    int32_t *nblk);
+/* Host replay of the GPU library's symmetric diagonal product (no reference counterpart):
+ * builds the layout the GPU library would upload (the main and upper diagonals of a matrix
+ * with at most 16 of them, values symmetric bit for bit, no stored zeros, sorted rows) and
+ * walks it with the kernel's index arithmetic and summation order; y = A x (fp32 != 0:
+ * FP32 values, vector, products and sums).  ndiag receives the stored diagonal count and
+ * offsets (room for 16) their offsets.  Returns 0, 1 when the matrix is not a candidate,
+ * or a negative code naming the first out-of-range access. */
+int chs_dia_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                  const double *val, int32_t fp32, const double *x, double *y, int32_t *ndiag,
+                  int32_t *offsets);
 int chs_hier_nlev(const chs_hier *h);
 /* which: 'A', 'P' or 'G'.  Returned matrix is borrowed (owned by the hierarchy). */
 const chs_mat *chs_hier_mat(const chs_hier *h, int32_t level, char which);
