@@ -1131,10 +1131,14 @@ hipError_t launch_pcg_p_t(const DevCsr &A, const PcgPtrs &w, const double *pold,
 }
 
 template <int MAXM>
-hipError_t launch_pcg_p_dia(const DevDia &D, const PcgPtrs &w, const double *pnew, const chr::PcgState *st,
-                            hipStream_t s) {
-  hipLaunchKernelGGL((chr::pcg_p_dia_kernel<MAXM>), dim3(D.grid()), dim3(chr::kBlock), 0, s, D.n, D.offs,
-                     D.val.as<double>(), pnew, w.ap, (const double *)w.r, w.part_p, st);
+hipError_t launch_pcg_p_dia(const DevDia &D, const PcgPtrs &w, const double *pold, double *pnew,
+                            const chr::PcgState *st, bool formed, hipStream_t s) {
+  if (formed)
+    hipLaunchKernelGGL((chr::pcg_p_dia_kernel<MAXM, true>), dim3(D.grid()), dim3(chr::kBlock), 0, s, D.n, D.offs,
+                       D.val.as<double>(), (const double *)w.z, pold, pnew, w.ap, (const double *)w.r, w.part_p, st);
+  else
+    hipLaunchKernelGGL((chr::pcg_p_dia_kernel<MAXM, false>), dim3(D.grid()), dim3(chr::kBlock), 0, s, D.n, D.offs,
+                       D.val.as<double>(), (const double *)w.z, pold, pnew, w.ap, (const double *)w.r, w.part_p, st);
   return hipGetLastError();
 }
 
@@ -1144,11 +1148,9 @@ int pcg_p_parts(const DevCsr &A) { return A.dia_pcg ? (int)A.dia.grid() : std::m
 hipError_t launch_pcg_p(const DevCsr &A, const PcgPtrs &w, const double *pold, double *pnew,
                         const chr::PcgState *st, bool formed, hipStream_t s) {
   if (A.dia_pcg) {
-    // p_new is formed by its own pass before this product (pcg_pnew_kernel).
-    if (!formed) return hipErrorInvalidValue;
-    if (A.dia.m <= 4) return launch_pcg_p_dia<4>(A.dia, w, pnew, st, s);
-    if (A.dia.m <= 8) return launch_pcg_p_dia<8>(A.dia, w, pnew, st, s);
-    return launch_pcg_p_dia<16>(A.dia, w, pnew, st, s);
+    if (A.dia.m <= 4) return launch_pcg_p_dia<4>(A.dia, w, pold, pnew, st, formed, s);
+    if (A.dia.m <= 8) return launch_pcg_p_dia<8>(A.dia, w, pold, pnew, st, formed, s);
+    return launch_pcg_p_dia<16>(A.dia, w, pold, pnew, st, formed, s);
   }
   switch (A.per * 2 + (A.idx16 ? 1 : 0)) {
     case 8: return launch_pcg_p_t<4, int>(A, w, pold, pnew, st, formed, s);
@@ -1160,14 +1162,14 @@ hipError_t launch_pcg_p(const DevCsr &A, const PcgPtrs &w, const double *pold, d
   }
 }
 
-// p = z + beta p_old is formed in a pass of its own, so that the product gathers one
-// operand per nonzero instead of two (same bits either way; measured faster on the 7-point
-// diffusion rows and on the 81-entry elasticity rows alike).  CHRONOS_PCG_FORMED=0 selects
-// the single-pass form.
+// In the stream layout p = z + beta p_old is formed in a pass of its own, so that the
+// product gathers one operand per nonzero instead of two (same bits either way; measured
+// faster on the 7-point diffusion rows and on the 81-entry elasticity rows alike).  The
+// diagonal layout reads its operands as contiguous runs, so it forms p inside the product
+// and saves the pass.  CHRONOS_PCG_FORMED=0|1 selects the single-pass or two-pass form.
 bool pcg_formed(const DevCsr &A) {
-  if (A.dia_pcg) return true;  // the diagonal product gathers p_new alone
   if (const char *e = std::getenv("CHRONOS_PCG_FORMED")) return std::atoi(e) != 0;
-  return true;
+  return !A.dia_pcg;
 }
 
 // One PCG_CJ iteration (PCG_CJ.m:88-120), p read from p0 (parity 0) or p1 and written to

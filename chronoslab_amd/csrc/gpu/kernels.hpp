@@ -1191,10 +1191,15 @@ __global__ __launch_bounds__(kBlock) void pcg_p_stream_kernel(
 
 // Ap = A p in the symmetric diagonal layout (spmv_dia_kernel's loads and summation order)
 // and per-block partials of p.Ap and p.r, one block of kBlock rows per workgroup (grid =
-// the blocks, partials at part[b] and part[nblk + b] for the row block b).
-template <int MAXM>
+// the blocks, partials at part[b] and part[nblk + b] for the row block b).  FORMED: p_new
+// was formed by pcg_pnew_kernel and is the operand; otherwise the kernel reads z and p_old
+// at the same positions, forms p = z + beta p_old there with pcg_pnew_kernel's expression
+// (same bits) and writes p_new for its own rows.
+template <int MAXM, bool FORMED>
 __global__ __launch_bounds__(kBlock) void pcg_p_dia_kernel(int n, DiaOffs d, const double *__restrict__ val,
-                                                           const double *__restrict__ p, double *__restrict__ ap,
+                                                           const double *__restrict__ z,
+                                                           const double *__restrict__ p_old,
+                                                           double *__restrict__ p_new, double *__restrict__ ap,
                                                            const double *__restrict__ r,
                                                            double *__restrict__ part, const PcgState *st) {
   __shared__ double lds[kBlock / kWave];
@@ -1203,12 +1208,34 @@ __global__ __launch_bounds__(kBlock) void pcg_p_dia_kernel(int n, DiaOffs d, con
   const int i = b * kBlock + (int)threadIdx.x;
   const int ic = min(i, n - 1);
   DiaRow<MAXM, double, double, VecLoad<double>> row;
-  row.load(ic, n, d, val, VecLoad<double>{p});
+  row.load(ic, n, d, val, VecLoad<double>{FORMED ? p_new : z});
+  double pl[MAXM], pu[MAXM], pd = 0.0;
+  if constexpr (!FORMED) {
+#pragma unroll
+    for (int k = 1; k < MAXM; ++k) {
+      pl[k] = pu[k] = 0.0;
+      if (k < d.m) {
+        pl[k] = p_old[max(ic - d.off[k], 0)];
+        pu[k] = p_old[min(ic + d.off[k], n - 1)];
+      }
+    }
+    pd = p_old[ic];
+  }
   const double ri = r[ic];
   if (st->stop) return;
+  if constexpr (!FORMED) {
+    const double beta = st->beta;
+#pragma unroll
+    for (int k = 1; k < MAXM; ++k) {
+      row.xl[k] = row.xl[k] + beta * pl[k];
+      row.xu[k] = row.xu[k] + beta * pu[k];
+    }
+    row.xd = row.xd + beta * pd;
+  }
   double pap = 0.0, prr = 0.0;
   if (i < n) {
     const double s = row.sum();
+    if constexpr (!FORMED) p_new[i] = row.xd;
     ap[i] = s;
     pap = row.xd * s;
     prr = row.xd * ri;
