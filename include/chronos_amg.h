@@ -258,7 +258,10 @@ int chr_setup_omega(void *ctx, int32_t n, const int32_t *rowptr_A, const int32_t
                     const double *val_G, uint64_t seed, double *omega, double *lambda);
 
 /* Timing helpers (HIP events on the handle's stream, device-resident data):
- * mean milliseconds of one V-cycle (k = 1) over reps launches after one warm-up. */
+ * mean milliseconds of one V-cycle (k = 1) over reps cycles, captured back to back in a
+ * graph of at most 10 cycles (as the solver's iteration graph holds its cycle) that is
+ * launched until reps cycles have run, after one warm-up launch; the FP32 cycle reads an
+ * FP32 copy of its input in the first smoother product, as the solver does. */
 int chr_time_vcycle(chr_hier *h, int32_t reps, double *ms);
 /* Mean milliseconds of one launch of a level SpMV: which = 'A', 'G', 'T' (G^T), 'P',
  * 'R' (P^T), each launch timed on its own after a 512 MiB device copy has flushed the
