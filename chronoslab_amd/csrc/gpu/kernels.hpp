@@ -566,6 +566,44 @@ __global__ __launch_bounds__(kBlock) void spmv_dia_kernel(int n, DiaOffs d, cons
 // lanes straight from memory, in the same order.  Epilogues see flat indices i*k + c.
 // ---------------------------------------------------------------------------------------
 
+// Copies a stream block's values and column indices [e0, e1), and its row starts rp[r0..r1]
+// as offsets from e0, into LDS: the loads of eight strides are issued before their stores,
+// so a block of up to 8 * kBlock nonzeros costs one memory round trip (a loop of single loads
+// and stores would wait once per stride).
+template <typename TV, typename TI>
+__device__ __forceinline__ void stage_block(int r0, int r1, int e0, int e1, const int *__restrict__ rp,
+                                            const TV *__restrict__ val, const TI *__restrict__ ci, TV *sv,
+                                            TI *sc, int *srp) {
+  const int tid = threadIdx.x;
+  const int nr = r1 - r0;  // <= kBlock rows: nr + 1 row starts
+  const int ra = rp[r0 + min(tid, nr)];
+  const int rb = (tid == 0 && nr >= kBlock) ? rp[r1] : 0;
+  const int cnt = e1 - e0;
+  for (int t0 = 0; t0 < cnt; t0 += 8 * kBlock) {
+    TV v[8];
+    TI c[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = min(t0 + u * kBlock + (int)threadIdx.x, cnt - 1);
+      v[u] = val[e0 + t];
+      c[u] = ci[e0 + t];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * kBlock + (int)threadIdx.x;
+      if (t < cnt) {
+        sv[t] = v[u];
+        sc[t] = c[u];
+      }
+    }
+  }
+  if (tid <= nr) srp[tid] = ra - e0;
+  if (tid == 0 && nr >= kBlock) srp[nr] = rb - e0;
+}
+
+// Bytes of LDS a block product needs beyond its staged values and indices (the row starts).
+constexpr int kStageRpBytes = (kBlock + 1) * 4;
+
 template <int KL, typename TV, typename TI, typename XS, typename Epi>
 __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__restrict__ blk,
                                                       const int *__restrict__ rp,
@@ -592,10 +630,8 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__re
   }
   TV *sv = reinterpret_cast<TV *>(smem);
   TI *sc = reinterpret_cast<TI *>(smem + (size_t)cap * sizeof(TV));
-  for (int t = tid; t < e1 - e0; t += kBlock) {
-    sv[t] = val[e0 + t];
-    sc[t] = ci[e0 + t];
-  }
+  int *srp = reinterpret_cast<int *>(smem + (size_t)cap * (sizeof(TV) + sizeof(TI)));
+  stage_block(r0, r1, e0, e1, rp, val, ci, sv, sc, srp);
   __syncthreads();
   const int npair = (r1 - r0) * KL;
   for (int p = tid; p < npair; p += kBlock) {
@@ -603,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int nblk, const int4 *__re
     if (c >= k) continue;
     const int i = r * k + c;
     const double pv = epi.pre(i);
-    const int ra = rp[r] - e0, rz = rp[r + 1] - e0;
+    const int ra = srp[r - r0], rz = srp[r - r0 + 1];
     TA s = 0;
     for (int q = ra; q < rz; q += 16) {
       const int n = min(16, rz - q);
@@ -682,10 +718,8 @@ __global__ __launch_bounds__(kBlock) void spmm4_kernel(int nblk, const int4 *__r
   }
   TV *sv = reinterpret_cast<TV *>(smem);
   TI *sc = reinterpret_cast<TI *>(smem + (size_t)cap * sizeof(TV));
-  for (int t = tid; t < e1 - e0; t += kBlock) {
-    sv[t] = val[e0 + t];
-    sc[t] = ci[e0 + t];
-  }
+  int *srp = reinterpret_cast<int *>(smem + (size_t)cap * (sizeof(TV) + sizeof(TI)));
+  stage_block(r0, r1, e0, e1, rp, val, ci, sv, sc, srp);
   __syncthreads();
   const int npair = (r1 - r0) * LR;
   for (int p = tid; p < npair; p += kBlock) {
@@ -695,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void spmm4_kernel(int nblk, const int4 *__r
     double pv[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) pv[j] = epi.pre(i0 + j);
-    const int ra = rp[r] - e0, rz = rp[r + 1] - e0;
+    const int ra = srp[r - r0], rz = srp[r - r0 + 1];
     TA s[4] = {0, 0, 0, 0};
     for (int q = ra; q < rz; q += U) {
       const int n = min(U, rz - q);

@@ -421,22 +421,22 @@ hipError_t launch_spmm_kl(const DevCsr &M, XS x, int k, Epi epi, const int *stop
     if (k % 4 == 0 && aligned16(x) && !std::getenv("CHRONOS_SPMM_SCALAR")) {
       if (M.idx16)
         hipLaunchKernelGGL((chr::spmm4_kernel<KL, TV, uint16_t, XS, Epi>), dim3(M.nblk), dim3(chr::kBlock),
-                           (size_t)cap * (sizeof(TV) + sizeof(uint16_t)), s, M.nblk, M.blk.as<int4>(),
+                           (size_t)cap * (sizeof(TV) + sizeof(uint16_t)) + chr::kStageRpBytes, s, M.nblk, M.blk.as<int4>(),
                            M.rp.as<int>(), M.ci.as<uint16_t>(), M.val.as<TV>(), x, k, cap, epi, stop);
       else
         hipLaunchKernelGGL((chr::spmm4_kernel<KL, TV, int, XS, Epi>), dim3(M.nblk), dim3(chr::kBlock),
-                           (size_t)cap * (sizeof(TV) + sizeof(int)), s, M.nblk, M.blk.as<int4>(),
+                           (size_t)cap * (sizeof(TV) + sizeof(int)) + chr::kStageRpBytes, s, M.nblk, M.blk.as<int4>(),
                            M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, k, cap, epi, stop);
       return hipGetLastError();
     }
   }
   if (M.idx16)
     hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, uint16_t, XS, Epi>), dim3(M.nblk), dim3(chr::kBlock),
-                       (size_t)cap * (sizeof(TV) + sizeof(uint16_t)), s, M.nblk, M.blk.as<int4>(),
+                       (size_t)cap * (sizeof(TV) + sizeof(uint16_t)) + chr::kStageRpBytes, s, M.nblk, M.blk.as<int4>(),
                        M.rp.as<int>(), M.ci.as<uint16_t>(), M.val.as<TV>(), x, k, cap, epi, stop);
   else
     hipLaunchKernelGGL((chr::spmm_kernel<KL, TV, int, XS, Epi>), dim3(M.nblk), dim3(chr::kBlock),
-                       (size_t)cap * (sizeof(TV) + sizeof(int)), s, M.nblk, M.blk.as<int4>(),
+                       (size_t)cap * (sizeof(TV) + sizeof(int)) + chr::kStageRpBytes, s, M.nblk, M.blk.as<int4>(),
                        M.rp.as<int>(), M.ci.as<int>(), M.val.as<TV>(), x, k, cap, epi, stop);
   return hipGetLastError();
 }
