@@ -1091,6 +1091,24 @@ static __global__ __launch_bounds__(kBlock) void block_gather_kernel(int n, int 
   Y[t] = X[i * k + idx[t % m]];
 }
 
+// Y (n x kp) = [X | 0] for X (n x k), k <= kp: the block widened with zero columns.
+static __global__ __launch_bounds__(kBlock) void block_pad_kernel(int n, int k, int kp, const double *__restrict__ X,
+                                                           double *__restrict__ Y) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * kp) return;
+  const int64_t i = t / kp;
+  const int c = (int)(t % kp);
+  Y[t] = c < k ? X[i * k + c] : 0.0;
+}
+
+// X (n x k) = the first k columns of Y (n x kp).
+static __global__ __launch_bounds__(kBlock) void block_unpad_kernel(int n, int k, int kp, const double *__restrict__ Y,
+                                                             double *__restrict__ X) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * k) return;
+  X[t] = Y[(t / k) * kp + t % k];
+}
+
 // Strided block column copy with type conversion: dst[i] = src[i] (one column).
 template <typename TS, typename TD>
 __global__ __launch_bounds__(kBlock) void convert_kernel(int n, const TS *__restrict__ src,

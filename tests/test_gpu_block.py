@@ -1,5 +1,7 @@
 """Block V-cycle (n x k right-hand sides) on the GPU: every column equals the single-vector
-V-cycle bit for bit, and the FP64 cycle matches the oracle."""
+V-cycle bit for bit, and the FP64 cycle matches the oracle.  Block widths cover one column
+per lane (k < 4), four columns per lane with 16-byte loads (k a multiple of 4) and four
+columns per lane with single loads and the last group masked (5, 7, 13)."""
 import numpy as np
 import pytest
 
@@ -16,7 +18,7 @@ def hier():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("k", [2, 3, 8, 16])
+@pytest.mark.parametrize("k", [2, 3, 5, 7, 8, 13, 16])
 def test_block_columns_equal_single(gpu, hier, prec, k):
     A, h = hier
     M = gpu.AMGPreconditioner(h, prec)
