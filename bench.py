@@ -323,13 +323,16 @@ def elasticity(np, torch, amg, H, O, log, ne: int = 64, precs=("fp32", "fp64")) 
         xs = r.pop("x")
         r["err_vs_fp64_oracle"] = float(np.linalg.norm(xs - xo) / np.linalg.norm(xo))
     log(f"[bench] elasticity oracle: {t_cpu:.2f} s ({ito} it) on {cores} threads (one per physical core)")
-    return {"n": A.nrows, "nnz": A.nnz, "levels": hier.nlev, "setup_s": t_setup,
+    return {"n": A.nrows, "nnz": A.nnz, "nnz_nonzero": int(np.count_nonzero(A.val)), "levels": hier.nlev,
+            "setup_s": t_setup,
             "level_rows": [l.A.nrows for l in hier.levels], "level_nnz": [l.A.nnz for l in hier.levels],
             "solve": res,
             "speedup_fp32_vs_fp64": (res["fp64"]["mean_s"] / res["fp32"]["mean_s"]) if len(res) == 2 else None,
             "cpu_oracle_s": t_cpu, "cpu_oracle_iterations": ito, "cpu_threads": cores,
             "note": "BAMG + EMIN prolongation, aFSAI; host coarsening, interpolation and EMIN "
-                    "constraints, GPU EMIN operator, aFSAI, omega and Galerkin products"}
+                    "constraints, GPU EMIN operator, aFSAI, omega and Galerkin products; the generated "
+                    "operator stores exact zeros (nnz counts them, nnz_nonzero does not), which the upload "
+                    "drops (same products); vcycle_GBs counts the matrices as uploaded, as MATLAB's nnz would"}
 
 
 def config_poisson32(np, torch, amg, H, log) -> dict:

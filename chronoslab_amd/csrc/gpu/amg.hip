@@ -559,6 +559,38 @@ struct HostMat {
   chr_csr view() const { return {nrows, ncols, rp.data(), ci.data(), v.data()}; }
 };
 
+// Stored zeros are dropped before a matrix is uploaded.  A product gains nothing from them:
+// every kernel starts a row sum at +0 and adds rounded products, a sum that starts at +0 is
+// never -0, and adding a +-0 product leaves any other sum unchanged, so the products keep
+// their bits for finite operands while the kernels stream fewer bytes (the 64^3 elasticity
+// operator stores 13.4 million zeros among its 63.7 million entries).  Rows longer than a
+// stream block are the exception: their strided partial sums regroup when entries drop out,
+// which changes the rounding of those rows only.  CHRONOS_KEEP_ZEROS=1 uploads matrices as
+// given.  Returns false (and leaves out untouched) when M stores no zero.
+bool drop_zeros(const chr_csr &M, HostMat &out) {
+  if (const char *e = std::getenv("CHRONOS_KEEP_ZEROS"); e && std::atoi(e) != 0) return false;
+  const int64_t nnz = M.rowptr[M.nrows];
+  int64_t zeros = 0;
+  for (int64_t e = 0; e < nnz; ++e) zeros += M.val[e] == 0.0;
+  if (zeros == 0) return false;
+  out.nrows = M.nrows;
+  out.ncols = M.ncols;
+  out.rp.assign((size_t)M.nrows + 1, 0);
+  out.ci.clear();
+  out.v.clear();
+  out.ci.reserve((size_t)(nnz - zeros));
+  out.v.reserve((size_t)(nnz - zeros));
+  for (int32_t i = 0; i < M.nrows; ++i) {
+    for (int32_t e = M.rowptr[i]; e < M.rowptr[i + 1]; ++e)
+      if (M.val[e] != 0.0) {
+        out.ci.push_back(M.col[e]);
+        out.v.push_back(M.val[e]);
+      }
+    out.rp[(size_t)i + 1] = (int32_t)out.ci.size();
+  }
+  return true;
+}
+
 // C = L R, row by row (Gustavson): row i of C accumulates s * (row j of R) for the entries
 // (i, j, s) of L in L's column order; columns sorted.  Exact zeros are kept.
 HostMat spgemm_host(const chr_csr &L, const chr_csr &R) {
@@ -722,7 +754,25 @@ int build_composites(chr_hier::Level &L, const LevelIn &in, const std::vector<in
 }
 
 // Builds a hierarchy on the device from per-level host descriptions (see chr_hier::Level).
-int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer, chr_hier **out, int tune_k) {
+int build_hier(std::vector<LevelIn> &lv, int32_t precision, const chr_csr &outer_in, chr_hier **out, int tune_k) {
+  // Matrices without their stored zeros (drop_zeros); the level views point into these.
+  std::vector<HostMat> nz;
+  nz.reserve(3 * lv.size() + 1);
+  auto nonzero = [&](chr_csr &m) {
+    HostMat hm;
+    if (drop_zeros(m, hm)) {
+      nz.push_back(std::move(hm));
+      m = nz.back().view();
+    }
+  };
+  for (LevelIn &in : lv)
+    if (in.nd > 0) {
+      nonzero(in.A);
+      nonzero(in.G);
+      nonzero(in.P);
+    }
+  chr_csr outer = outer_in;
+  nonzero(outer);
   auto h = std::make_unique<chr_hier>();
   h->precision = precision;
   HIPC(hipGetDevice(&h->device));
@@ -1069,8 +1119,10 @@ extern "C" int chr_mat_create(const chr_csr *A, chr_mat **out) {
   auto m = std::make_unique<chr_mat>();
   if ((rc = m->zero.alloc(sizeof(int)))) return rc;
   HIPC(hipMemset(m->zero.p, 0, sizeof(int)));
-  if ((rc = m->A.upload(*A, false)) ||
-      (rc = tune_layout(m->A, A->rowptr, A->col, A->val, nullptr, m->zero.as<int>(), false)))
+  HostMat nzA;
+  const chr_csr Au = drop_zeros(*A, nzA) ? nzA.view() : *A;
+  if ((rc = m->A.upload(Au, false)) ||
+      (rc = tune_layout(m->A, Au.rowptr, Au.col, Au.val, nullptr, m->zero.as<int>(), false)))
     return rc;
   HIPC(hipDeviceSynchronize());
   *out = m.release();
