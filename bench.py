@@ -158,12 +158,31 @@ def config_lobpcg(np, torch, amg, H, log) -> dict:
         a = max(a, 1)
         it_bytes += vcycle_bytes(hl, 4, a) + BA + 2 * a * n8 + 2 * (k + 2 * a) * n8 + 2 * (3 * a + 2 * k) * n8 + 3 * k * n8
     gbs = it_bytes / dt / 1e9
+    # CPU baseline on a bounded sample: the oracle's lobpcg (same matrix, hierarchy, X0; FP64
+    # V-cycle preconditioner) for its first 3 iterations, in which all 16 columns are active.
+    import oracle as O
+
+    _, _, cores = _cpu_info()
+    O.set_threads(cores)
+    ref = O.Hier(hl)
+    O.lobpcg(A, ref, X0, itmax=2, tol=1e-8)  # warm-up (the oracle takes itmax >= 2)
+    tc = []
+    for _ in range(3):
+        t = time.perf_counter()
+        O.lobpcg(A, ref, X0, itmax=3, tol=1e-8)
+        tc.append(time.perf_counter() - t)
+    cpu_it = float(np.median(tc)) / 3
+    del ref
     out["lobpcg_96cube_16"] = {"seconds": dt, "iterations": it, "ierr": ierr,
+                               "cpu_oracle_s_per_iteration": cpu_it, "cpu_threads": cores,
+                               "gpu_s_per_iteration": dt / max(it, 1),
                                "max_eig_rel_err_vs_closed_form": float(np.max(np.abs(lam - exact) / exact)),
                                "algorithmic_GB": it_bytes / 1e9, "GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS,
                                "note": "FP32 V-cycle preconditioner, FP64 Rayleigh-Ritz; X0 and X resident on the "
                                        "GPU; median of 3 after a warm-up; GB/s = algorithmic bytes of the iterations "
-                                       "(active-column count per step) / whole solve time"}
+                                       "(active-column count per step) / whole solve time; CPU: the oracle's lobpcg "
+                                       "(FP64 V-cycle) over its first 3 iterations (all 16 columns active), median of 3, "
+                                       "per iteration; the GPU figure is the whole solve's mean per iteration"}
     log(f"[bench] LOBPCG 96^3 x16: {dt:.3f} s, {it} it, max eig err "
         f"{out['lobpcg_96cube_16']['max_eig_rel_err_vs_closed_form']:.2e}, {gbs:.0f} GB/s algorithmic "
         f"({gbs / HBM_PEAK_GBS:.2f} of HBM)")
@@ -211,12 +230,30 @@ def config_batch(np, torch, amg, H, log) -> dict:
                    for h_, A_ in zip(hiers, mats))
     t_b = float(np.median(ts))
     gbs = bt_bytes * int(iters.max()) / t_b / 1e9
+    # CPU baseline on a bounded sample: the oracle's PCG_CJ (FP64 V-cycle) of every system on
+    # its first right-hand side (48 of the 384 solves), median of 3 after a warm-up.
+    import oracle as O
+
+    _, _, cores = _cpu_info()
+    O.set_threads(cores)
+    refs = [O.Hier(h_) for h_ in hiers]
+    o0s = [bs.offsets[i] for i in range(len(mats) + 1)]
+
+    def cpu_pass():
+        for i_, (r_, A_) in enumerate(zip(refs, mats)):
+            r_.pcg(A_, np.ascontiguousarray(B[o0s[i_]:o0s[i_ + 1], 0]), itmax=300, tol=1e-8)
+
+    t_cpu = _time_cpu(cpu_pass, reps=3)
+    del refs
     out["batch_48x8"] = {"seconds": t_b, "solves": 48 * kk, "rows": bs.n,
                          "iterations_min_max": [int(iters.min()), int(iters.max())],
                          "all_converged": bool(np.all(flags == 0)), "worst_true_rel_residual": worst,
                          "bytes_per_batch_iteration": bt_bytes, "GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS,
+                         "cpu_oracle_s_48_solves": t_cpu, "cpu_threads": cores,
                          "note": "FP32 V-cycle, the batch solve one launch of a graph of 16 iterations, inputs resident on the GPU; GB/s = "
-                                 "algorithmic bytes per batch iteration x iterations of the longest solve / solve time"}
+                                 "algorithmic bytes per batch iteration x iterations of the longest solve / solve time; CPU: the "
+                                 "oracle's PCG_CJ (FP64 V-cycle) of every system on its first right-hand side (48 of the 384 "
+                                 "solves, one after another), median of 3 after a warm-up"}
     log(f"[bench] batch 48x8: {t_b * 1e3:.1f} ms, iters {iters.min()}..{iters.max()}, worst res {worst:.2e}, "
         f"{gbs:.0f} GB/s algorithmic ({gbs / HBM_PEAK_GBS:.2f} of HBM)")
     del bs, hiers, mats
