@@ -203,6 +203,7 @@ struct DevCsr {
   DevBuf ci;           // uint16 offsets when idx16, else int32 columns
   DevSell sell;        // sliced-ELL copy, used for single-vector products when use_sell
   bool use_sell = false;
+  int sell_arg = -2;   // the sort argument the sliced-ELL copy was built with (-2: none)
   DevPack pack;        // packed-code index copy, used for single-vector products when use_pack
   bool use_pack = false;
   DevDia dia;          // symmetric diagonal copy, used for single-vector products when use_dia
