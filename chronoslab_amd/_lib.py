@@ -83,6 +83,8 @@ def load_setup_lib():
                  [_I32, _I32, _I32P, _I32P, _F64P, _I32, _I32, _F64P, _F64P, _I32P])
             _sig(lib, "chs_dia_apply", C.c_int,
                  [_I32, _I32, _I32P, _I32P, _F64P, _I32, _F64P, _F64P, _I32P, _I32P])
+            _sig(lib, "chs_sygv", C.c_int, [_I32, _F64P, _F64P, _F64P, _F64P])
+            _sig(lib, "chs_cond_sym", _F64, [_I32, _F64P])
             _sig(lib, "chs_hier_nlev", C.c_int, [_VP])
             _sig(lib, "chs_hier_mat", _VP, [_VP, _I32, C.c_char])
             _sig(lib, "chs_hier_omega", _F64, [_VP, _I32])

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "chronos_setup.h"
+#include "dense.hpp"
 #include "dia_layout.hpp"
 #include "pack_layout.hpp"
 #include "sell_layout.hpp"
@@ -259,4 +260,15 @@ extern "C" int chs_dia_apply(int32_t n, int32_t ncols, const int32_t *rowptr, co
     return replay_dia<float>(L, vf, xf.data(), y);
   }
   return replay_dia<double>(L, L.val, x, y);
+}
+
+// The host dense routines of the GPU library's LOBPCG (dense.hpp), exported for testing.
+extern "C" int chs_sygv(int32_t m, const double *A, const double *B, double *w, double *V) {
+  if (m < 1 || !A || !B || !w || !V) return -1;
+  return chrdense::sygv(m, A, B, w, V) ? 0 : 1;
+}
+
+extern "C" double chs_cond_sym(int32_t m, const double *M) {
+  if (m < 1 || !M) return -1.0;
+  return chrdense::cond_sym(m, M);
 }

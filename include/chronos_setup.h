@@ -323,6 +323,14 @@ int chs_pack_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_
 int chs_dia_apply(int32_t n, int32_t ncols, const int32_t *rowptr, const int32_t *col,
                   const double *val, int32_t fp32, const double *x, double *y, int32_t *ndiag,
                   int32_t *offsets);
+/* The host dense routines of the GPU library's LOBPCG Rayleigh-Ritz steps, exported for
+ * testing.  chs_sygv: A v = lambda B v for symmetric A and SPD B (order m, column-major),
+ * ascending eigenvalues w and B-orthonormal eigenvectors V (MATLAB's [V, L] = eig(A, B) in
+ * sources/EigenSolver/lobpcg.m:99, :325, :437); returns 0, 1 when B is not positive definite,
+ * -1 on bad arguments.  chs_cond_sym: the 2-norm condition number of a symmetric matrix
+ * (cond in sources/EigenSolver/B_orthonormalize.m:9); -1 on bad arguments. */
+int chs_sygv(int32_t m, const double *A, const double *B, double *w, double *V);
+double chs_cond_sym(int32_t m, const double *M);
 int chs_hier_nlev(const chs_hier *h);
 /* which: 'A', 'P' or 'G'.  Returned matrix is borrowed (owned by the hierarchy). */
 const chs_mat *chs_hier_mat(const chs_hier *h, int32_t level, char which);
