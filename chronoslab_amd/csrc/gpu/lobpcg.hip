@@ -503,17 +503,24 @@ struct Lob {
     std::vector<double> M;
     int rc;
     if ((rc = gram(X, k, X, k, M))) return rc;
-    failed = 0;
+    failed = borth_factor(M, k, U) ? 0 : 1;
+    return failed ? CHR_OK : borth_apply(X, k, U, tmp);
+  }
+  // The host half of borth: U from the Gram matrix M = X^T X (destroyed); false on the
+  // reference's failure (cond(M) > 1/(k eps), or M not positive definite).
+  static bool borth_factor(std::vector<double> &M, int k, std::vector<double> &U) {
     if (chrdense::cond_sym(k, M.data()) > 1.0 / ((double)k * 2.220446049250313e-16) ||
-        !chrdense::chol_lower(k, M.data())) {
-      failed = 1;
-      return CHR_OK;
-    }
+        !chrdense::chol_lower(k, M.data()))
+      return false;
     U.assign((size_t)k * k, 0.0);
     for (int j = 0; j < k; ++j)
       for (int i = 0; i <= j; ++i) U[i + (size_t)j * k] = M[j + (size_t)i * k];
+    return true;
+  }
+  // The device half: X <- X U^-1 (formed in tmp, then the pointers swap).
+  int borth_apply(double *&X, int k, const std::vector<double> &U, double *&tmp) {
     std::vector<double> Ui = chrdense::inv_upper(k, U.data());
-    if ((rc = gemm(X, k, Ui.data(), k, 0.0, tmp))) return rc;
+    if (int rc = gemm(X, k, Ui.data(), k, 0.0, tmp)) return rc;
     std::swap(X, tmp);
     return CHR_OK;
   }
@@ -677,18 +684,30 @@ extern "C" int chr_lobpcg(chr_hier *h, chr_mat *Aop, int32_t n, int32_t neig, do
     const int sxar = L.gram_launch(dX, neig, dAR, na);
     if (sxar < 0) return CHR_EARG;
     if ((rc = L.gemm_slot(dX, neig, sxar, na, -1.0, 1.0, dAR))) return rc;
-    if ((rc = L.borth(dAR, na, U, dT, failed))) return rc;
-    if (failed) {
+    // B-orthonormalization of R, then of P (lobpcg.m:238, :256).  P does not depend on R, so
+    // both Gram matrices are formed and fetched together (one host round trip); the
+    // factorizations and products follow in the reference's order.
+    const int srr = L.gram_launch(dAR, na, dAR, na);
+    const int spp = iter > 1 ? L.gram_launch(dAPa, na, dAPa, na) : 0;
+    if (srr < 0 || spp < 0) return CHR_EARG;
+    if ((rc = L.fetch())) return rc;
+    std::vector<double> grr, gpp;
+    L.get(srr, na, na, grr);
+    if (iter > 1) L.get(spp, na, na, gpp);
+    if (!Lob::borth_factor(grr, na, U)) {
       ierr = 2;
       break;
     }
+    if ((rc = L.borth_apply(dAR, na, U, dT))) return rc;
     if ((rc = L.applyA(dAR, na, dAAR))) return rc;
     if (iter > 1) {
-      if ((rc = L.borth(dAPa, na, U, dT, failed))) return rc;
-      if (failed) {
+      if (!Lob::borth_factor(gpp, na, U)) {
         restart = 1;
       } else {
+        // P <- P U^-1 and AP <- AP U^-1 (lobpcg.m:256-265)
         std::vector<double> Ui = chrdense::inv_upper(na, U.data());
+        if ((rc = L.gemm(dAPa, na, Ui.data(), na, 0.0, dT))) return rc;
+        std::swap(dAPa, dT);
         if ((rc = L.gemm(dAAP, na, Ui.data(), na, 0.0, dT))) return rc;
         std::swap(dAAP, dT);
         restart = forced;
