@@ -129,7 +129,8 @@ int DevPack::build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *
   chrpack::Layout L;
   if (!chrpack::build(n, ncols, hrp, hci, per_, L)) return CHR_OK;
   int rc;
-  if ((rc = blk.alloc(sizeof(int32_t) * L.blk.size())) || (rc = code.alloc(sizeof(uint16_t) * L.code.size())) ||
+  if ((rc = blk.alloc(sizeof(int32_t) * L.blk.size())) ||
+      (rc = code.alloc(sizeof(uint16_t) * (size_t)quad_len((int64_t)L.code.size()))) ||
       (rc = dict.alloc(sizeof(int32_t) * L.dict.size())))
     return rc;
   HIPC(hipMemcpy(blk.p, L.blk.data(), sizeof(int32_t) * L.blk.size(), hipMemcpyHostToDevice));
@@ -173,6 +174,7 @@ static int tune_layout_timed(DevCsr &M, const int32_t *hrp, const int32_t *hci, 
   M.use_dia = false;
   M.dia_pcg = false;
   M.dia.release();
+  M.vec4 = false;
   if (M.nnz == 0) return CHR_OK;
   const char *env_pack = std::getenv("CHRONOS_PACK");
   const int pack_mode = env_pack ? std::atoi(env_pack) : -1;  // -1 timed, 0 off, 1 forced
@@ -235,8 +237,8 @@ static int tune_layout_timed(DevCsr &M, const int32_t *hrp, const int32_t *hci, 
         std::fprintf(stderr, "[upload]   %d rows: sell cb %d sort %d (shift %d) elems/nnz %.3f: %.4f ms\n", M.nrows,
                      M.sell.cb, (int)M.sell.sorted, M.sell.sshift, (double)M.sell.elems / (double)M.nnz, ms / 5);
       else
-        std::fprintf(stderr, "[upload]   %d rows: %s PER %d: %.4f ms\n", M.nrows, M.use_pack ? "pack" : "stream",
-                     M.per, ms / 5);
+        std::fprintf(stderr, "[upload]   %d rows: %s PER %d%s: %.4f ms\n", M.nrows, M.use_pack ? "pack" : "stream",
+                     M.per, M.vec4 ? "" : " (scalar loads)", ms / 5);
     }
     return CHR_OK;
   };
@@ -403,6 +405,26 @@ static int tune_layout_timed(DevCsr &M, const int32_t *hrp, const int32_t *hci, 
       M.dia.release();
     }
   }
+  // Step 5 (FP32 stream and packed layouts): four-element vector loads or one element per
+  // load, the former unless the latter is more than 3% faster.  Timed per matrix: on the
+  // 128^3 diffusion hierarchy the FP32 solve takes 11.98 ms against 12.21 ms with one element
+  // per load, while vector loads on every FP32 matrix slowed the 64^3 elasticity solve (long
+  // rows, 16 nonzeros per lane) from 14.8 to 15.6 ms.  FP64 products load one element per
+  // lane (with vector loads the FP64 diffusion solve took 16.56 against 16.38 ms).
+  // CHRONOS_VEC4=0 turns vector loads off, CHRONOS_VEC4=1 forces them (FP32).
+  if (!rc && M.fp32 && !M.use_sell && !M.use_dia) {
+    if (const char *env_v4 = std::getenv("CHRONOS_VEC4")) {
+      M.vec4 = std::atoi(env_v4) != 0;
+    } else {
+      float ms4 = 0.f, ms1 = 0.f;
+      M.vec4 = true;
+      if (!(rc = time_it(ms4))) {
+        M.vec4 = false;
+        rc = time_it(ms1);
+        M.vec4 = !rc && !(ms1 < 0.97f * ms4);
+      }
+    }
+  }
   if (rc) {
     M.use_sell = false;
     M.sell.release();
@@ -413,10 +435,11 @@ static int tune_layout_timed(DevCsr &M, const int32_t *hrp, const int32_t *hci, 
     M.dia.release();
   }
   if (upload_trace())
-    std::fprintf(stderr, "[upload] tune %d x %d nnz %lld: stream %.3f s, pack+sell+dia %.3f s -> %s%s PER %d%s\n",
+    std::fprintf(stderr, "[upload] tune %d x %d nnz %lld: stream %.3f s, pack+sell+dia %.3f s -> %s%s PER %d%s%s\n",
                  M.nrows, M.ncols, (long long)M.nnz, t_tune1 - t_tune0, now_seconds() - t_tune1,
                  M.use_dia ? "dia" : M.use_sell ? "sell" : M.use_pack ? "pack" : "stream",
-                 M.use_sell && M.sell.sorted ? " sorted" : "", M.per, M.dia_pcg ? " (PCG product: dia)" : "");
+                 M.use_sell && M.sell.sorted ? " sorted" : "", M.per,
+                 M.use_sell || M.use_dia || M.vec4 ? "" : " scalar loads", M.dia_pcg ? " (PCG product: dia)" : "");
   return rc;
 }
 
@@ -431,14 +454,14 @@ static int tune_layout_timed(DevCsr &M, const int32_t *hrp, const int32_t *hci, 
 // serves default tuning only: with any tuner override in the environment the tuner times
 // and records nothing.
 struct LayoutChoice {
-  int per = 8, pack = 0, sell_arg = -2, dia = 0;  // sell_arg: -2 none, else the builder's sort argument
+  int per = 8, pack = 0, sell_arg = -2, dia = 0, vec4 = 1;  // sell_arg: -2 none, else the builder's sort argument
 };
 
 static std::mutex g_cache_mu;
 
 static bool tuner_overridden() {
   for (const char *v : {"CHRONOS_STREAM_PER", "CHRONOS_SELL", "CHRONOS_SELL_SORT", "CHRONOS_SELL_PRUNE",
-                        "CHRONOS_PACK", "CHRONOS_DIA"})
+                        "CHRONOS_PACK", "CHRONOS_DIA", "CHRONOS_VEC4"})
     if (std::getenv(v)) return true;
   return false;
 }
@@ -453,7 +476,8 @@ static std::map<std::string, LayoutChoice> &layout_cache(const std::string &path
     if (FILE *f = std::fopen(path.c_str(), "r")) {
       char key[128];
       LayoutChoice c;
-      while (std::fscanf(f, "%127s %d %d %d %d", key, &c.per, &c.pack, &c.sell_arg, &c.dia) == 5) m[key] = c;
+      while (std::fscanf(f, "%127s %d %d %d %d %d", key, &c.per, &c.pack, &c.sell_arg, &c.dia, &c.vec4) == 6)
+        m[key] = c;
       std::fclose(f);
     }
   }
@@ -490,6 +514,7 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
     const LayoutChoice c = it->second;
     lk.unlock();
     M.use_sell = M.use_pack = M.use_dia = M.dia_pcg = false;
+    M.vec4 = c.vec4 != 0;
     M.sell_arg = -2;
     M.sell.release();
     M.pack.release();
@@ -524,10 +549,11 @@ int tune_layout(DevCsr &M, const int32_t *hrp, const int32_t *hci, const double 
   c.pack = M.use_pack ? 1 : 0;
   c.sell_arg = M.use_sell ? M.sell_arg : -2;
   c.dia = M.use_dia ? 1 : 0;
+  c.vec4 = M.vec4 ? 1 : 0;
   lk.lock();
   layout_cache(path)[key] = c;
   if (FILE *f = std::fopen(path, "a")) {
-    std::fprintf(f, "%s %d %d %d %d\n", key.c_str(), c.per, c.pack, c.sell_arg, c.dia);
+    std::fprintf(f, "%s %d %d %d %d %d\n", key.c_str(), c.per, c.pack, c.sell_arg, c.dia, c.vec4);
     std::fclose(f);
   }
   return CHR_OK;

@@ -203,6 +203,71 @@ __device__ __forceinline__ void stream_products(int e0, int e1, int base, const 
   }
 }
 
+// Four consecutive elements of an index or value array at a 4-element-aligned position, in
+// one 8- or 16-byte vector load (two for FP64).
+template <typename T>
+struct Run4 {
+  T v[4];
+};
+template <typename T>
+__device__ __forceinline__ Run4<T> load_run4(const T *__restrict__ p) {
+  Run4<T> q;
+  if constexpr (sizeof(T) == 2) {
+    const ushort4 u = *reinterpret_cast<const ushort4 *>(p);
+    q.v[0] = (T)u.x, q.v[1] = (T)u.y, q.v[2] = (T)u.z, q.v[3] = (T)u.w;
+  } else if constexpr (std::is_same<T, float>::value) {
+    const float4 u = *reinterpret_cast<const float4 *>(p);
+    q.v[0] = u.x, q.v[1] = u.y, q.v[2] = u.z, q.v[3] = u.w;
+  } else if constexpr (sizeof(T) == 4) {
+    const int4 u = *reinterpret_cast<const int4 *>(p);
+    q.v[0] = (T)u.x, q.v[1] = (T)u.y, q.v[2] = (T)u.z, q.v[3] = (T)u.w;
+  } else {
+    const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+    q.v[0] = a.x, q.v[1] = a.y, q.v[2] = b.x, q.v[3] = b.y;
+  }
+  return q;
+}
+
+// Vector-load form of stream_products: lane tid takes groups of four consecutive nonzeros,
+// group g covering [a + 4 g, a + 4 g + 4) with a = e0 rounded down to a multiple of 4, so each
+// lane issues one index and one value load per group instead of four of each.  Index and
+// value arrays are allocated to a multiple of four elements, so every group is in bounds;
+// elements outside [e0, e1) are neither gathered nor stored.  col(c) maps an index word to
+// its column; before_gather() runs (block-uniformly) between the loads and the gathers.
+// The products and their LDS positions are those of stream_products, so are the row sums.
+template <int PER, typename TA, typename TV, typename TI, typename Col, typename Xf, typename Pre>
+__device__ __forceinline__ void stream_products4(int e0, int e1, const TI *__restrict__ ci,
+                                                 const TV *__restrict__ val, Col col, Xf colval,
+                                                 Pre before_gather, TA *prod) {
+  if (e1 <= e0) return;
+  constexpr int NG = PER / 4 + 1;  // groups per lane: (PER kBlock + 3) / 4 + 1 groups at most
+  const int tid = threadIdx.x, a = e0 & ~3, last = ((e1 - 1 - a) >> 2);
+  Run4<TI> c[NG];
+  Run4<TV> v[NG];
+#pragma unroll
+  for (int j = 0; j < NG; ++j) {
+    const int g = min(j * kBlock + tid, last);
+    c[j] = load_run4(ci + a + 4 * g);
+    v[j] = load_run4(val + a + 4 * g);
+  }
+  before_gather();
+  TA xv[NG][4];
+#pragma unroll
+  for (int j = 0; j < NG; ++j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = a + 4 * (j * kBlock + tid) + q;
+      xv[j][q] = (e >= e0 && e < e1) ? colval(col(c[j].v[q])) : (TA)0;
+    }
+#pragma unroll
+  for (int j = 0; j < NG; ++j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = a + 4 * (j * kBlock + tid) + q;
+      if (e >= e0 && e < e1) prod[e - e0] = (TA)v[j].v[q] * xv[j][q];
+    }
+}
+
 // The operand of a product: one vector (XOne), or two for a composite operator (XTwo) whose
 // even columns 2j index row j of vector a and odd columns 2j + 1 row j of vector b (the two
 // column ranges interleaved, so that a row block of the operator spans twice the columns of
@@ -238,7 +303,7 @@ struct ColLoad {
 
 // stop: device flag (never null); the kernel does nothing once it is set.  It is read
 // together with the block table so that it adds no latency of its own.
-template <int PER, typename TV, typename TI, typename XS, typename Epi>
+template <int PER, typename TV, typename TI, typename XS, typename Epi, bool V4 = false>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
                                                              const int4 *__restrict__ blk,
                                                              const int *__restrict__ rp,
@@ -264,7 +329,11 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   const int rc = min(r, r1 - 1);
   const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
   const double pv = epi.pre(rc);
-  stream_products<PER, TA>(e0, e1, base, ci, val, ColLoad<TA, XS>{x}, prod);
+  if constexpr (V4)
+    stream_products4<PER, TA>(
+        e0, e1, ci, val, [&](TI c) { return base + (int)c; }, ColLoad<TA, XS>{x}, [] {}, prod);
+  else
+    stream_products<PER, TA>(e0, e1, base, ci, val, ColLoad<TA, XS>{x}, prod);
   __syncthreads();
   if (r < r1) epi(r, (double)row_sum<TA>(prod, ra, rz), pv);
 }
@@ -277,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
 // spmv_stream_kernel; the block table's third word is the block's dictionary window, whose
 // 256 offsets are loaded into LDS beside the block's values and codes.
 // ---------------------------------------------------------------------------------------
-template <int PER, typename TV, typename TX, typename Epi>
+template <int PER, typename TV, typename TX, typename Epi, bool V4 = false>
 __global__ __launch_bounds__(kBlock) void spmv_pack_kernel(int nblk, const int4 *__restrict__ blk,
                                                            const int *__restrict__ rp,
                                                            const uint16_t *__restrict__ code,
@@ -308,7 +377,11 @@ __global__ __launch_bounds__(kBlock) void spmv_pack_kernel(int nblk, const int4 
   const int rc = min(r, r1 - 1);
   const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
   const double pv = epi.pre(rc);
-  if (e1 > e0) {
+  if constexpr (V4) {
+    stream_products4<PER, TA>(
+        e0, e1, code, val, [&](uint16_t c) { return r0 + (int)(c >> 8) - ld[c & 255u]; },
+        [&](int j) { return (TA)x[j]; }, [] { __syncthreads(); }, prod);  // sync: dictionary in LDS
+  } else if (e1 > e0) {
     uint32_t c[PER];
     TV v[PER];
     TA xv[PER];

@@ -37,6 +37,10 @@ inline int fail(int code, const std::string &msg) {
   } while (0)
 
 // Owned device allocation.
+// Element count of a CSR value or index array as allocated: a multiple of four (at least
+// four), so that the products' four-element vector loads stay inside the array.
+inline int64_t quad_len(int64_t nnz) { return std::max<int64_t>(4, (nnz + 3) & ~(int64_t)3); }
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
@@ -209,6 +213,10 @@ struct DevCsr {
   DevDia dia;          // symmetric diagonal copy, used for single-vector products when use_dia
   bool use_dia = false;
   bool dia_pcg = false;  // the PCG operator product runs on dia (a fixed rule, never timed)
+  // FP32 stream and packed-code single-vector products stage their nonzeros with
+  // four-element vector loads (else one element per load; FP64 always); same products and
+  // sums either way.  Set by the autotuner.
+  bool vec4 = false;
   int upload(int32_t nr, int32_t nc, const int32_t *hrp, const int32_t *hci, const double *hv,
              bool as_fp32, int per_ = 8) {
     nrows = nr;
@@ -217,7 +225,7 @@ struct DevCsr {
     fp32 = as_fp32;
     int rc;
     if ((rc = rp.alloc(sizeof(int32_t) * (nr + 1)))) return rc;
-    if ((rc = val.alloc((as_fp32 ? 4 : 8) * std::max<int64_t>(nnz, 1)))) return rc;
+    if ((rc = val.alloc((as_fp32 ? 4 : 8) * (size_t)quad_len(nnz)))) return rc;
     HIPC(hipMemcpy(rp.p, hrp, sizeof(int32_t) * (nr + 1), hipMemcpyHostToDevice));
     if (nnz) {
       if (as_fp32) {
@@ -270,10 +278,10 @@ struct DevCsr {
       std::vector<uint16_t> d(nnz);
       for (int32_t b = 0; b < nblk; ++b)
         for (int64_t e = br[b].y; e < br[b + 1].y; ++e) d[e] = (uint16_t)(hci[e] - br[b].z);
-      if ((rc = ci.alloc(sizeof(uint16_t) * nnz))) return rc;
+      if ((rc = ci.alloc(sizeof(uint16_t) * (size_t)quad_len(nnz)))) return rc;
       HIPC(hipMemcpy(ci.p, d.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
     } else {
-      if ((rc = ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)))) return rc;
+      if ((rc = ci.alloc(sizeof(int32_t) * (size_t)quad_len(nnz)))) return rc;
       if (nnz) HIPC(hipMemcpy(ci.p, hci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
     }
     return CHR_OK;
@@ -338,25 +346,39 @@ hipError_t launch_sell(const DevSell &S, const TX *x, Epi epi, const int *stop, 
   return S.unroll == 16 ? launch_sell_u<16, TV>(S, x, epi, stop, s) : launch_sell_u<8, TV>(S, x, epi, stop, s);
 }
 
-template <int PER, typename TV, typename TX, typename Epi>
-hipError_t launch_pack_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
-  hipLaunchKernelGGL((chr::spmv_pack_kernel<PER, TV, TX, Epi>), dim3(M.pack.nblk), dim3(chr::kBlock), 0, s,
+template <int PER, typename TV, typename TX, typename Epi, bool V4>
+hipError_t launch_pack_v(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  hipLaunchKernelGGL((chr::spmv_pack_kernel<PER, TV, TX, Epi, V4>), dim3(M.pack.nblk), dim3(chr::kBlock), 0, s,
                      M.pack.nblk, M.pack.blk.as<int4>(), M.rp.as<int>(), M.pack.code.as<uint16_t>(),
                      M.val.as<TV>(), M.pack.dict.as<int>(), x, epi, stop);
   return hipGetLastError();
 }
 
-template <int PER, typename TV, typename XS, typename Epi>
-hipError_t launch_spmv_p(const DevCsr &M, XS x, Epi epi, const int *stop, hipStream_t s) {
+template <int PER, typename TV, typename TX, typename Epi>
+hipError_t launch_pack_p(const DevCsr &M, const TX *x, Epi epi, const int *stop, hipStream_t s) {
+  if constexpr (std::is_same<TV, float>::value)
+    if (M.vec4) return launch_pack_v<PER, TV, TX, Epi, true>(M, x, epi, stop, s);
+  return launch_pack_v<PER, TV, TX, Epi, false>(M, x, epi, stop, s);
+}
+
+template <int PER, typename TV, typename XS, typename Epi, bool V4>
+hipError_t launch_spmv_v(const DevCsr &M, XS x, Epi epi, const int *stop, hipStream_t s) {
   if (M.idx16)
-    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, uint16_t, XS, Epi>), dim3(M.nblk),
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, uint16_t, XS, Epi, V4>), dim3(M.nblk),
                        dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
                        M.ci.as<uint16_t>(), M.val.as<TV>(), x, epi, stop);
   else
-    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, int, XS, Epi>), dim3(M.nblk),
+    hipLaunchKernelGGL((chr::spmv_stream_kernel<PER, TV, int, XS, Epi, V4>), dim3(M.nblk),
                        dim3(chr::kBlock), 0, s, M.nblk, M.blk.as<int4>(), M.rp.as<int>(),
                        M.ci.as<int>(), M.val.as<TV>(), x, epi, stop);
   return hipGetLastError();
+}
+
+template <int PER, typename TV, typename XS, typename Epi>
+hipError_t launch_spmv_p(const DevCsr &M, XS x, Epi epi, const int *stop, hipStream_t s) {
+  if constexpr (std::is_same<TV, float>::value)
+    if (M.vec4) return launch_spmv_v<PER, TV, XS, Epi, true>(M, x, epi, stop, s);
+  return launch_spmv_v<PER, TV, XS, Epi, false>(M, x, epi, stop, s);
 }
 
 template <typename XS>
