@@ -3,9 +3,12 @@
 // CPU by the host library (chs_sell_apply) with the kernel's own index arithmetic.
 //
 // Layout:
-//   * slots are taken in slices of 64 (one wave, one row per lane); slice s stores element
-//     (k, lane) at sp[s] + 64 k + lane, k < len_s = the longest row of the slice, or of the
-//     whole matrix when `ulen` > 0 (uniform ELL: sp[s] = 64 ulen s, unsorted);
+//   * slots are taken in slices of 64 (one wave, one row per lane) of len_s elements per
+//     lane, len_s = the longest row of the slice (of the whole matrix when `ulen` > 0:
+//     uniform ELL, sp[s] = 64 ulen s, unsorted) rounded up to a multiple of 4.  A lane's
+//     elements come in groups of four consecutive ones, so that the kernel loads a group
+//     with one vector load: element (k, lane) sits at sp[s] + 64 (k & ~3) + 4 lane + (k & 3)
+//     (elem_at below);
 //   * dictionary windows of 2^wshift slots (256 <= 2^wshift <= 16384); when `sorted`, the
 //     rows of each sort window of 2^sshift slots (256 <= 2^sshift <= 2^wshift) sit in its
 //     slots by decreasing length (stable) and perm[slot] is the row's offset from the sort
@@ -26,6 +29,12 @@ namespace chrsell {
 
 constexpr int kSlice = 64;
 constexpr int kMaxRow = 256;
+
+// Offset of element k of a lane's row from the slice start.
+inline int64_t elem_at(int32_t k, int lane) { return (int64_t)kSlice * (k & ~3) + 4 * lane + (k & 3); }
+
+// Elements per lane of a slice whose longest row has len elements.
+inline int32_t slice_len(int32_t len) { return (len + 3) & ~3; }
 
 struct Layout {
   int cb = 0;  // 0: the matrix is not a candidate
@@ -86,7 +95,7 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
     for (int64_t s = 0; s < nslices; ++s) {
       int32_t mx = 0;
       for (int l = 0; l < kSlice; ++l) mx = std::max(mx, rlen(m[(size_t)(s * kSlice + l)]));
-      tot += (int64_t)mx * kSlice;
+      tot += (int64_t)slice_len(mx) * kSlice;
     }
     return tot;
   };
@@ -159,7 +168,7 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
   const int sshift = sorted ? sshifts[(size_t)pick - 1] : ws;
   // Unsorted with every slice padded to the longest row (uniform ELL) when that costs at
   // most 2% more elements: the kernel then computes slice offsets instead of reading them.
-  const int64_t e_uniform = (int64_t)nslices * maxlen * kSlice;
+  const int64_t e_uniform = (int64_t)nslices * slice_len(maxlen) * kSlice;
   const bool uniform = !sorted && (double)e_uniform <= 1.02 * (double)e_ident;
   const int64_t elems = sorted ? e_sorted_at[(size_t)pick - 1] : uniform ? e_uniform : e_ident;
   if (elems > (int64_t)INT32_MAX - kSlice || elems > nnz + nnz / 2) {
@@ -172,7 +181,7 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
   L.nslices = nslices;
   L.wshift = ws;
   L.sshift = sshift;
-  L.ulen = uniform ? maxlen : 0;
+  L.ulen = uniform ? slice_len(maxlen) : 0;
   L.dmin = cb == 16 ? (int32_t)dlo : 0;
   L.elems = elems;
   L.sorted = sorted;
@@ -196,13 +205,13 @@ inline bool build(int32_t n, int32_t ncols, const int32_t *rp, const int32_t *ci
     }
     int32_t mx = 0;
     for (int l = 0; l < kSlice; ++l) mx = std::max(mx, rlen(map[(size_t)(s * kSlice + l)]));
-    if (uniform) mx = maxlen;
+    mx = slice_len(uniform ? maxlen : mx);
     for (int l = 0; l < kSlice; ++l) {
       const int32_t r = map[(size_t)(s * kSlice + l)];
       if (r < 0) continue;
       for (int32_t k = 0; k < rlen(r); ++k) {
         const int32_t e = rp[r] + k;
-        const size_t at = (size_t)(pos + (int64_t)k * kSlice + l);
+        const size_t at = (size_t)(pos + elem_at(k, l));
         L.val[at] = v[e];
         const int64_t d = (int64_t)r - ci[e];
         if (cb == 8) L.c8[at] = (uint8_t)lut[(size_t)(d - dlo)];

@@ -204,7 +204,7 @@ __device__ __forceinline__ void stream_products(int e0, int e1, int base, const 
 }
 
 // Four consecutive elements of an index or value array at a 4-element-aligned position, in
-// one 8- or 16-byte vector load (two for FP64).
+// one 4-, 8- or 16-byte vector load (two for FP64).
 template <typename T>
 struct Run4 {
   T v[4];
@@ -212,7 +212,10 @@ struct Run4 {
 template <typename T>
 __device__ __forceinline__ Run4<T> load_run4(const T *__restrict__ p) {
   Run4<T> q;
-  if constexpr (sizeof(T) == 2) {
+  if constexpr (sizeof(T) == 1) {
+    const uchar4 u = *reinterpret_cast<const uchar4 *>(p);
+    q.v[0] = (T)u.x, q.v[1] = (T)u.y, q.v[2] = (T)u.z, q.v[3] = (T)u.w;
+  } else if constexpr (sizeof(T) == 2) {
     const ushort4 u = *reinterpret_cast<const ushort4 *>(p);
     q.v[0] = (T)u.x, q.v[1] = (T)u.y, q.v[2] = (T)u.z, q.v[3] = (T)u.w;
   } else if constexpr (std::is_same<T, float>::value) {
@@ -408,9 +411,10 @@ __global__ __launch_bounds__(kBlock) void spmv_pack_kernel(int nblk, const int4 
 // ---------------------------------------------------------------------------------------
 // Sliced-ELL SpMV for square matrices with short rows (the smoother factors G, G^T and
 // the fine-level operators).  Rows are taken in slices of 64 slots, one wave per slice and
-// one row per lane; slice s stores its elements column by column, element (k, lane) at
-// sp[s] + 64 k + lane, so every load of a wave is one contiguous 64-element segment and
-// the row sum stays in a register (no LDS staging, no row pointers).  A slice is as long
+// one row per lane; slice s holds len elements per lane (len a multiple of 4) in groups of
+// four, element (k, lane) at sp[s] + 64 (k & ~3) + 4 lane + (k & 3): a lane loads a group
+// with one vector load, and the wave's load is one contiguous 256-element segment.  The
+// row sum stays in a register (no LDS staging, no row pointers).  A slice is as long
 // as its longest row; shorter rows are padded with a PAD code.  Within sort windows of
 // 2^sshift slots the rows may be ordered by decreasing length to cut padding: perm then
 // holds the row of each slot as an offset from the sort window's first row (0xFFFF: empty
@@ -473,17 +477,26 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
   const int q = perm ? (int)perm[slot] : 0;
   int dv = 0;
   if constexpr (CB == 8) dv = dict[(((b * kSlices) * kWave) >> wshift) * 256 + threadIdx.x];
-  const int base = e0 + lane, len = (e1 - e0) / kWave;
-  // First chunk's values and codes (clamped to the row's last element; an empty slice
-  // reads element 0, which every layout has, and uses none of it).
+  const int len = (e1 - e0) / kWave;
+  // A chunk's values and codes, a group of four per vector load (U is a multiple of 4),
+  // the groups past the lane's last clamped to it (an empty slice reads element 0, which
+  // every layout has, and uses none of it); the surplus products are dropped below.
+  auto load_chunk = [&](int ch, TV(&vv)[U], CT(&cc)[U]) {
+#pragma unroll
+    for (int g = 0; g < U / 4; ++g) {
+      const int at = len > 0 ? e0 + kWave * min(ch * U + 4 * g, len - 4) + 4 * lane : 0;
+      const Run4<TV> a = load_run4(val + at);
+      const Run4<CT> b = load_run4(code + at);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        vv[4 * g + q] = a.v[q];
+        cc[4 * g + q] = b.v[q];
+      }
+    }
+  };
   TV v[U];
   CT c[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int e = len > 0 ? base + min(u, len - 1) * kWave : 0;
-    v[u] = val[e];
-    c[u] = code[e];
-  }
+  load_chunk(0, v, c);
   if (*stop) return;
   if constexpr (CB == 8) {
     // The block's slices lie in one window (windows hold a multiple of kBlock slots).
@@ -514,28 +527,14 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
   TA acc = 0;
   const int nch = (len + U - 1) / U;
   for (int ch = 0; ch < nch; ++ch) {
-    if (!kAhead && ch > 0) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int e = base + min(ch * U + u, len - 1) * kWave;
-        v[u] = val[e];
-        c[u] = code[e];
-      }
-    }
+    if (!kAhead && ch > 0) load_chunk(ch, v, c);
     TA xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xv[u] = (TA)x[col_of(c[u])];
     const bool more = kAhead && ch + 1 < nch;  // uniform over the wave (one slice)
     TV vn[U];
     CT cn[U];
-    if (more) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int e = base + min((ch + 1) * U + u, len - 1) * kWave;
-        vn[u] = val[e];
-        cn[u] = code[e];
-      }
-    }
+    if (more) load_chunk(ch + 1, vn, cn);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const TA p = mul_rn((TA)v[u], xv[u]);

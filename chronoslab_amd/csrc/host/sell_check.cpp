@@ -50,14 +50,21 @@ int replay(const chrsell::Layout &L, const double *x, double *y) {
       const int64_t d0 = L.sp[(size_t)s], d1 = L.sp[(size_t)s + 1];
       if (L.ulen > 0 && (d0 != s * L.ulen * chrsell::kSlice || d1 - d0 != L.ulen * chrsell::kSlice)) return -17;
       if ((d1 - d0) % chrsell::kSlice) return -13;
-      const int64_t base = d0 + lane, len = (d1 - d0) / chrsell::kSlice;
+      const int64_t len = (d1 - d0) / chrsell::kSlice;
       TA acc = 0;
       const int kUnroll = L.unroll;
       const int64_t nch = (len + kUnroll - 1) / kUnroll;
       for (int64_t ch = 0; ch < nch; ++ch)
         for (int u = 0; u < kUnroll; ++u) {
           const int64_t k = std::min<int64_t>(ch * kUnroll + u, len - 1);
-          const int64_t e = base + k * chrsell::kSlice;
+          // The kernel loads each group of four with one vector load: the group of chunk
+          // element u, clamped to the lane's last group, starts at d0 + 64 k0 + 4 lane.
+          if (len % 4) return -18;
+          const int64_t k0 = std::min<int64_t>(ch * kUnroll + (u & ~3), len - 4);
+          const int64_t g = len > 0 ? d0 + chrsell::kSlice * k0 + 4 * lane : 0;
+          if (g % 4 != 0 || g + 3 >= L.elems) return -19;
+          const int64_t e = len > 0 ? g + (u & 3) : 0;
+          if (len > 0 && e != d0 + chrsell::elem_at((int32_t)k, lane) && ch * kUnroll + u < len) return -20;
           if (e < 0 || e >= L.elems) return -14;
           bool pad;
           int64_t col;

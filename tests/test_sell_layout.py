@@ -95,10 +95,11 @@ def check(A, rng, fp32_exact=True, expect_cb=None, allow_skip=False):
 
 
 def test_stencil_dictionary_codes():
-    # 7-point Poisson: 7 offsets, 8-bit dictionary codes.
+    # 7-point Poisson: 7 offsets, 8-bit dictionary codes; slices hold a multiple of four
+    # elements per lane, so rows of 7 take 8 slots.
     A = H.poisson7(12)
     info, _ = check(A, np.random.default_rng(0), expect_cb=8)
-    assert info["elems"] <= 1.2 * A.nnz
+    assert info["elems"] <= 1.25 * A.nnz
 
 
 def test_uniform_ell(monkeypatch):
@@ -108,7 +109,7 @@ def test_uniform_ell(monkeypatch):
     monkeypatch.setenv("CHRONOS_SELL_SORT", "0")
     A = H.poisson7(16)
     info, _ = check(A, np.random.default_rng(5), expect_cb=8)
-    assert info["ulen"] == 7 and not info["sorted"] and info["elems"] == info["nslices"] * 7 * 64
+    assert info["ulen"] == 8 and not info["sorted"] and info["elems"] == info["nslices"] * 8 * 64
 
 
 def test_fsai_factor_and_transpose():
@@ -135,8 +136,8 @@ def test_sort_windows(monkeypatch, sort):
     # rows are permuted only inside their sort window, and every row is written once.
     rng = np.random.default_rng(7)
     n = 40000
-    A = random_csr_fast(n, 9, list(range(-300, 300, 7)), rng)
-    keep = rng.random(A.nnz) < 0.7  # ragged rows
+    A = random_csr_fast(n, 12, list(range(-300, 300, 7)), rng)
+    keep = rng.random(A.nnz) < 0.85  # ragged rows (padded by less than half unsorted)
     M = A.to_scipy().tocoo()
     A = H.Csr.from_scipy(sp.csr_matrix((M.data[keep], (M.row[keep], M.col[keep])), shape=(n, n)))
     monkeypatch.setenv("CHRONOS_SELL_SORT", sort)
@@ -177,9 +178,9 @@ def test_many_windows_and_16bit_offsets():
 def test_wide_offsets_full_columns():
     # Offsets spanning more than 65535 with many distinct values: 32-bit columns.
     rng = np.random.default_rng(5)
-    n = 70000
+    n = 140000
     offs = rng.choice(np.arange(-69999, 70000), size=2000, replace=False)
-    A = random_csr_fast(n, 4, offs, rng)
+    A = random_csr_fast(n, 12, offs, rng)
     check(A, rng, fp32_exact=False, expect_cb=32)
 
 
