@@ -156,7 +156,8 @@ int DevPack::build(int32_t n, int32_t ncols, const int32_t *hrp, const int32_t *
 //     candidate so far (CHRONOS_SELL_PRUNE=0 tries them anyway).
 // Each candidate is timed with HIP events, 2 warm-up and 5 timed launches; a matrix of 32 MB
 // or more is timed cache-cold (a 512 MiB device copy before every timed launch, as the
-// other products of a V-cycle evict it), smaller ones back to back.  The fastest candidate
+// other products of a V-cycle evict it; the median launch counts), smaller ones back to
+// back (their total).  The fastest candidate
 // wins, but PER 8 is kept unless another is more than 3% faster, so that near-ties do not
 // flip from run to run.  Every layout sums a row in the same order, so the choice never
 // changes a product.
@@ -216,12 +217,12 @@ static int tune_layout_timed(DevCsr &M, const int32_t *hrp, const int32_t *hci, 
         if (!e) e = hipEventRecord(ev[2 * i + 1], s);
       }
       if (!e) e = hipEventSynchronize(ev[9]);
-      ms = 0.f;
-      for (int i = 0; i < 5 && !e; ++i) {
-        float t = 0.f;
-        e = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
-        ms += t;
-      }
+      // The median launch, times 5 (the scale of the warm sum below): one slow launch (a
+      // clock change, another process) does not decide the layout.
+      float t5[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      for (int i = 0; i < 5 && !e; ++i) e = hipEventElapsedTime(&t5[i], ev[2 * i], ev[2 * i + 1]);
+      std::sort(t5, t5 + 5);
+      ms = 5.f * t5[2];
     } else {
       if (!e) e = hipEventRecord(ev[10], s);
       for (int i = 0; i < 5 && !e; ++i) e = run();
