@@ -44,18 +44,18 @@ inline void symmetrize(int m, double *M) {
     }
 }
 
-// Symmetric eigen-decomposition: S (symmetric) is destroyed; w = eigenvalues (unsorted),
-// Q = orthonormal eigenvectors (not formed when want_q is false).  Householder reduction to
-// tridiagonal form, each reflector applied to the trailing block as the symmetric rank-2
-// update S - v w^T - w v^T (Golub & Van Loan, Alg. 8.3.1), then implicit QR steps with the
-// Wilkinson shift and Givens bulge chasing (Alg. 8.3.3); both accumulate into Q.
-inline void sym_eig(int m, double *S, double *w, double *Q, bool want_q = true) {
-  if (want_q)
+// Householder reduction of a symmetric S (destroyed) to tridiagonal form T, with diagonal
+// d and off-diagonal e[i] = T(i+1, i) (e[m-1] = 0); each reflector H_k = I - beta_k v_k v_k^T
+// (v_k zero above row k+1) is applied to the trailing block as the symmetric rank-2 update
+// S - v w^T - w v^T (Golub & Van Loan, Alg. 8.3.1).  Q (when not null) receives
+// H_0 H_1 ... H_{m-3}.
+inline void tridiagonalize(int m, double *S, double *d, double *e, double *Q) {
+  if (Q)
     for (int j = 0; j < m; ++j)
       for (int i = 0; i < m; ++i) Q[i + j * m] = (i == j) ? 1.0 : 0.0;
   std::vector<double> v(m), p(m), y(m);
   for (int k = 0; k + 2 < m; ++k) {
-    // Reflector H = I - beta v v^T on indices r0 = k+1..m-1 mapping S[r0.., k] onto e_1.
+    // Reflector H = I - beta_k v v^T on indices r0 = k+1..m-1 mapping S[r0.., k] onto e_1.
     const int r0 = k + 1;
     double sig = 0.0;
     for (int i = k + 2; i < m; ++i) sig += S[i + k * m] * S[i + k * m];
@@ -64,8 +64,8 @@ inline void sym_eig(int m, double *S, double *w, double *Q, bool want_q = true) 
     const double v0 = x0 + (x0 >= 0.0 ? alpha : -alpha);
     v[r0] = v0;
     for (int i = k + 2; i < m; ++i) v[i] = S[i + k * m];
-    const double beta = 2.0 / (v0 * v0 + sig);
-    // p = beta S22 v, then w = p - (beta/2)(p^T v) v (kept in p); S22 <- S22 - v w^T - w v^T.
+    const double beta_k = 2.0 / (v0 * v0 + sig);
+    // p = beta_k S22 v, then w = p - (beta_k/2)(p^T v) v (kept in p); S22 <- S22 - v w^T - w v^T.
     for (int i = r0; i < m; ++i) p[i] = 0.0;
     for (int j = r0; j < m; ++j) {
       const double *cj = S + (size_t)j * m;
@@ -74,10 +74,10 @@ inline void sym_eig(int m, double *S, double *w, double *Q, bool want_q = true) 
     }
     double pv = 0.0;
     for (int i = r0; i < m; ++i) {
-      p[i] *= beta;
+      p[i] *= beta_k;
       pv += p[i] * v[i];
     }
-    const double kk = 0.5 * beta * pv;
+    const double kk = 0.5 * beta_k * pv;
     for (int i = r0; i < m; ++i) p[i] -= kk * v[i];
     for (int j = r0; j < m; ++j) {
       double *cj = S + (size_t)j * m;
@@ -87,7 +87,7 @@ inline void sym_eig(int m, double *S, double *w, double *Q, bool want_q = true) 
     const double sub = x0 >= 0.0 ? -alpha : alpha;  // H x = sub e_1
     S[r0 + k * m] = S[k + r0 * m] = sub;
     for (int i = k + 2; i < m; ++i) S[i + k * m] = S[k + i * m] = 0.0;
-    if (want_q) {  // Q <- Q H = Q - (beta Q v) v^T
+    if (Q) {  // Q <- Q H = Q - (beta_k Q v) v^T
       for (int i = 0; i < m; ++i) y[i] = 0.0;
       for (int j = r0; j < m; ++j) {
         const double *qj = Q + (size_t)j * m;
@@ -96,15 +96,20 @@ inline void sym_eig(int m, double *S, double *w, double *Q, bool want_q = true) 
       }
       for (int j = r0; j < m; ++j) {
         double *qj = Q + (size_t)j * m;
-        const double bvj = beta * v[j];
+        const double bvj = beta_k * v[j];
         for (int i = 0; i < m; ++i) qj[i] -= y[i] * bvj;
       }
     }
   }
-  // Tridiagonal T: diagonal d, off-diagonal e[i] = T(i+1, i).
-  std::vector<double> d(m), e(m, 0.0);
   for (int i = 0; i < m; ++i) d[i] = S[i + i * m];
   for (int i = 0; i + 1 < m; ++i) e[i] = 0.5 * (S[i + 1 + i * m] + S[i + (i + 1) * m]);
+  if (m > 0) e[m - 1] = 0.0;
+}
+
+// Implicit QR steps with the Wilkinson shift and Givens bulge chasing (Alg. 8.3.3) on the
+// tridiagonal (d, e), in place: d ends as the eigenvalues (unsorted), e is destroyed; the
+// rotations accumulate into Q when it is not null.
+inline void tridiag_qr(int m, double *d, double *e, double *Q) {
   const double eps = 2.220446049250313e-16;
   int hi = m - 1, guard = 0;
   while (hi > 0 && guard < 60 * m) {
@@ -140,7 +145,7 @@ inline void sym_eig(int m, double *S, double *w, double *Q, bool want_q = true) 
         x = e[k];
         z = bulge;
       }
-      if (want_q) {
+      if (Q) {
         double *qk = Q + (size_t)k * m, *qk1 = qk + m;
         for (int i = 0; i < m; ++i) {
           const double a = qk[i], b1 = qk1[i];
@@ -150,33 +155,69 @@ inline void sym_eig(int m, double *S, double *w, double *Q, bool want_q = true) 
       }
     }
   }
+}
+
+// Symmetric eigen-decomposition: S (symmetric) is destroyed; w = eigenvalues (unsorted),
+// Q = orthonormal eigenvectors (not formed when want_q is false).
+inline void sym_eig(int m, double *S, double *w, double *Q, bool want_q = true) {
+  std::vector<double> d(m), e(m, 0.0);
+  tridiagonalize(m, S, d.data(), e.data(), want_q ? Q : nullptr);
+  tridiag_qr(m, d.data(), e.data(), want_q ? Q : nullptr);
   for (int i = 0; i < m; ++i) w[i] = d[i];
+}
+
+// C <- L^-1 C L^-T (then symmetrized) for the lower Cholesky factor L of B: the standard
+// form of A v = lambda B v.  The forward substitution runs over four columns at a time
+// (four independent chains of divisions and updates); every entry is computed by the same
+// operations in the same order as one column at a time.
+inline void whiten(int m, const double *L, double *C) {
+  int j = 0;
+  for (; j + 4 <= m; j += 4) {  // C = L^-1 C, four columns at a time
+    double *c0 = C + (size_t)j * m, *c1 = c0 + m, *c2 = c1 + m, *c3 = c2 + m;
+    for (int q = 0; q < m; ++q) {
+      const double *lq = L + (size_t)q * m;
+      const double lqq = lq[q];
+      c0[q] /= lqq;
+      c1[q] /= lqq;
+      c2[q] /= lqq;
+      c3[q] /= lqq;
+      const double a0 = c0[q], a1 = c1[q], a2 = c2[q], a3 = c3[q];
+      for (int i = q + 1; i < m; ++i) {
+        const double li = lq[i];
+        c0[i] -= li * a0;
+        c1[i] -= li * a1;
+        c2[i] -= li * a2;
+        c3[i] -= li * a3;
+      }
+    }
+  }
+  for (; j < m; ++j) {
+    double *c = C + (size_t)j * m;
+    for (int q = 0; q < m; ++q) {
+      const double *lq = L + (size_t)q * m;
+      c[q] /= lq[q];
+      const double cq = c[q];
+      for (int i = q + 1; i < m; ++i) c[i] -= lq[i] * cq;
+    }
+  }
+  for (j = 0; j < m; ++j) {  // C = C L^-T: column j from the final columns q < j
+    double *cj = C + (size_t)j * m;
+    for (int q = 0; q < j; ++q) {
+      const double *cq = C + (size_t)q * m;
+      const double ljq = L[j + (size_t)q * m];
+      for (int i = 0; i < m; ++i) cj[i] -= cq[i] * ljq;
+    }
+    const double ljj = L[j + (size_t)j * m];
+    for (int i = 0; i < m; ++i) cj[i] /= ljj;
+  }
+  symmetrize(m, C);
 }
 
 // A v = lambda B v, A symmetric, B SPD: ascending eigenvalues w, B-orthonormal V.
 inline bool sygv(int m, const double *A, const double *B, double *w, double *V) {
   Mat L(B, B + (size_t)m * m), C(A, A + (size_t)m * m), Q((size_t)m * m), wt(m);
   if (!chol_lower(m, L.data())) return false;
-  for (int j = 0; j < m; ++j) {  // C = L^-1 A, one column at a time
-    double *c = C.data() + (size_t)j * m;
-    for (int k = 0; k < m; ++k) {
-      const double *lk = L.data() + (size_t)k * m;
-      c[k] /= lk[k];
-      const double ck = c[k];
-      for (int i = k + 1; i < m; ++i) c[i] -= lk[i] * ck;
-    }
-  }
-  for (int j = 0; j < m; ++j) {  // C = C L^-T: column j from the final columns k < j
-    double *cj = C.data() + (size_t)j * m;
-    for (int k = 0; k < j; ++k) {
-      const double *ck = C.data() + (size_t)k * m;
-      const double ljk = L[j + (size_t)k * m];
-      for (int i = 0; i < m; ++i) cj[i] -= ck[i] * ljk;
-    }
-    const double ljj = L[j + (size_t)j * m];
-    for (int i = 0; i < m; ++i) cj[i] /= ljj;
-  }
-  symmetrize(m, C.data());
+  whiten(m, L.data(), C.data());
   sym_eig(m, C.data(), wt.data(), Q.data());
   std::vector<int> ord(m);
   std::iota(ord.begin(), ord.end(), 0);

@@ -132,3 +132,4 @@ def test_cond_sym_against_numpy(m):
         c = cond_sym(Ms)
         assert c == np.inf or c > 1e14
     assert cond_sym(np.eye(m)) == 1.0
+
