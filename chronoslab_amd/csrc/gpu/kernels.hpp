@@ -206,7 +206,7 @@ __device__ __forceinline__ void stream_products(int e0, int e1, int base, const 
 // Four consecutive elements of an index or value array at a 4-element-aligned position, in
 // one 4-, 8- or 16-byte vector load (two for FP64).
 template <typename T>
-struct Run4 {
+struct __align__(4 * sizeof(T)) Run4 {
   T v[4];
 };
 template <typename T>
@@ -235,9 +235,12 @@ __device__ __forceinline__ Run4<T> load_run4(const T *__restrict__ p) {
 // group g covering [a + 4 g, a + 4 g + 4) with a = e0 rounded down to a multiple of 4, so each
 // lane issues one index and one value load per group instead of four of each.  Index and
 // value arrays are allocated to a multiple of four elements, so every group is in bounds;
-// elements outside [e0, e1) are neither gathered nor stored.  col(c) maps an index word to
-// its column; before_gather() runs (block-uniformly) between the loads and the gathers.
-// The products and their LDS positions are those of stream_products, so are the row sums.
+// elements outside [e0, e1) are not gathered.  col(c) maps an index word to its column;
+// before_gather() runs (block-uniformly) between the loads and the gathers.  The products
+// land in LDS at prod[e - a] (prod 16-byte aligned, room for PER * kBlock + 8 elements), one
+// aligned four-element store per group (zeros outside [e0, e1)), so a row's products start
+// at prod[(e0 - a) + row start - e0]; they are stream_products' products, so the row sums
+// are the same.
 template <int PER, typename TA, typename TV, typename TI, typename Col, typename Xf, typename Pre>
 __device__ __forceinline__ void stream_products4(int e0, int e1, const TI *__restrict__ ci,
                                                  const TV *__restrict__ val, Col col, Xf colval,
@@ -263,12 +266,15 @@ __device__ __forceinline__ void stream_products4(int e0, int e1, const TI *__res
       xv[j][q] = (e >= e0 && e < e1) ? colval(col(c[j].v[q])) : (TA)0;
     }
 #pragma unroll
-  for (int j = 0; j < NG; ++j)
+  for (int j = 0; j < NG; ++j) {
+    const int g = j * kBlock + tid;
+    if (g <= last) {
+      Run4<TA> pq;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = a + 4 * (j * kBlock + tid) + q;
-      if (e >= e0 && e < e1) prod[e - e0] = (TA)v[j].v[q] * xv[j][q];
+      for (int q = 0; q < 4; ++q) pq.v[q] = (TA)v[j].v[q] * xv[j][q];
+      *reinterpret_cast<Run4<TA> *>(prod + 4 * g) = pq;
     }
+  }
 }
 
 // The operand of a product: one vector (XOne), or two for a composite operator (XTwo) whose
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
                                                              XS x, Epi epi,
                                                              const int *__restrict__ stop) {
   using TA = typename Acc<TV>::type;
-  __shared__ TA prod[PER * kBlock];
+  __shared__ __align__(16) TA prod[PER * kBlock + 8];
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 lo = blk[b], hi = blk[b + 1];
   if (*stop) return;
@@ -330,7 +336,8 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int nblk,
   // Row-phase operands first, so their latency overlaps the product phase.
   const int r = r0 + tid;
   const int rc = min(r, r1 - 1);
-  const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
+  const int off = V4 ? e0 & 3 : 0;  // LDS position of the block's first product
+  const int ra = rp[rc] - e0 + off, rz = rp[rc + 1] - e0 + off;
   const double pv = epi.pre(rc);
   if constexpr (V4)
     stream_products4<PER, TA>(
@@ -358,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void spmv_pack_kernel(int nblk, const int4 
                                                            const TX *__restrict__ x, Epi epi,
                                                            const int *__restrict__ stop) {
   using TA = typename Acc<TV>::type;
-  __shared__ TA prod[PER * kBlock];
+  __shared__ __align__(16) TA prod[PER * kBlock + 8];
   __shared__ int ld[256];
   const int b = xcd_block(blockIdx.x, nblk);
   const int4 lo = blk[b], hi = blk[b + 1];
@@ -378,7 +385,8 @@ __global__ __launch_bounds__(kBlock) void spmv_pack_kernel(int nblk, const int4 
   }
   const int r = r0 + tid;
   const int rc = min(r, r1 - 1);
-  const int ra = rp[rc] - e0, rz = rp[rc + 1] - e0;
+  const int off = V4 ? e0 & 3 : 0;  // LDS position of the block's first product
+  const int ra = rp[rc] - e0 + off, rz = rp[rc + 1] - e0 + off;
   const double pv = epi.pre(rc);
   if constexpr (V4) {
     stream_products4<PER, TA>(
