@@ -231,6 +231,40 @@ __device__ __forceinline__ Run4<T> load_run4(const T *__restrict__ p) {
   return q;
 }
 
+// load_run4 with the non-temporal hint (the loads' nt bit), for data a product reads once.
+// The lanes of the loaded vector are copied to scalars before the bit casts: a bit cast taken
+// straight from a lane expression (w.y) yields lane 0 with this compiler.
+template <typename T>
+__device__ __forceinline__ Run4<T> load_run4_nt(const T *__restrict__ p) {
+  typedef unsigned int u1;
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  Run4<T> q;
+  if constexpr (sizeof(T) == 1) {
+    const u1 w = __builtin_nontemporal_load(reinterpret_cast<const u1 *>(p));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q.v[i] = (T)((w >> (8 * i)) & 0xFFu);
+  } else if constexpr (sizeof(T) == 2) {
+    const u2 w = __builtin_nontemporal_load(reinterpret_cast<const u2 *>(p));
+    const unsigned int w0 = w.x, w1 = w.y;
+    q.v[0] = (T)(w0 & 0xFFFFu), q.v[1] = (T)(w0 >> 16), q.v[2] = (T)(w1 & 0xFFFFu), q.v[3] = (T)(w1 >> 16);
+  } else if constexpr (sizeof(T) == 4) {
+    const u4 w = __builtin_nontemporal_load(reinterpret_cast<const u4 *>(p));
+    const unsigned int w0 = w.x, w1 = w.y, w2 = w.z, w3 = w.w;
+    q.v[0] = __builtin_bit_cast(T, w0), q.v[1] = __builtin_bit_cast(T, w1);
+    q.v[2] = __builtin_bit_cast(T, w2), q.v[3] = __builtin_bit_cast(T, w3);
+  } else {
+    const u4 a = __builtin_nontemporal_load(reinterpret_cast<const u4 *>(p));
+    const u4 b = __builtin_nontemporal_load(reinterpret_cast<const u4 *>(p) + 1);
+    const unsigned long long a0 = a.x, a1 = a.y, a2 = a.z, a3 = a.w, b0 = b.x, b1 = b.y, b2 = b.z, b3 = b.w;
+    q.v[0] = __builtin_bit_cast(T, a0 | (a1 << 32));
+    q.v[1] = __builtin_bit_cast(T, a2 | (a3 << 32));
+    q.v[2] = __builtin_bit_cast(T, b0 | (b1 << 32));
+    q.v[3] = __builtin_bit_cast(T, b2 | (b3 << 32));
+  }
+  return q;
+}
+
 // Vector-load form of stream_products: lane tid takes groups of four consecutive nonzeros,
 // group g covering [a + 4 g, a + 4 g + 4) with a = e0 rounded down to a multiple of 4, so each
 // lane issues one index and one value load per group instead of four of each.  Index and
@@ -489,12 +523,16 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
   // A chunk's values and codes, a group of four per vector load (U is a multiple of 4),
   // the groups past the lane's last clamped to it (an empty slice reads element 0, which
   // every layout has, and uses none of it); the surplus products are dropped below.
+  // FP32 values and their codes are loaded with the non-temporal hint: measured on the 128^3
+  // diffusion hierarchy, it takes the cache-cold level-0 G product from 56 to 48 us and the
+  // FP32 V-cycle from 0.530 to 0.510 ms, while with FP64 values it slowed the cycle by 2%.
+  constexpr bool kNT = sizeof(TV) == 4;
   auto load_chunk = [&](int ch, TV(&vv)[U], CT(&cc)[U]) {
 #pragma unroll
     for (int g = 0; g < U / 4; ++g) {
       const int at = len > 0 ? e0 + kWave * min(ch * U + 4 * g, len - 4) + 4 * lane : 0;
-      const Run4<TV> a = load_run4(val + at);
-      const Run4<CT> b = load_run4(code + at);
+      const Run4<TV> a = kNT ? load_run4_nt(val + at) : load_run4(val + at);
+      const Run4<CT> b = kNT ? load_run4_nt(code + at) : load_run4(code + at);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         vv[4 * g + q] = a.v[q];
